@@ -1,0 +1,233 @@
+/*
+ * mray_amd.h -- C-ABI of the MI355X (gfx950) wavefront path-tracing hot path.
+ *
+ * Every entry point takes plain pointers and sizes.  "d" prefixed pointers are
+ * HIP device pointers, "h" prefixed ones are host pointers, "stream" is a
+ * hipStream_t passed as void* (NULL = the null stream).  Every call returns
+ * MRAY_OK (0) or a non-zero MRayStatus; mray_last_error() gives the message.
+ *
+ * Reference interfaces replaced (yalcinerbora/mray, paths relative to Source/):
+ *   scene / accelerator construction  TracerI::CommitSurfaces            Core/TracerI.h:272-280
+ *                                     AcceleratorGroupLBVH::Construct     Tracer/AcceleratorLBVH.hpp:433-582
+ *   closest / any hit casting         BaseAcceleratorI::CastRays / CastVisibilityRays
+ *                                                                         Tracer/AcceleratorC.h:375-430
+ *   renderer start / one bounce       TracerI::StartRender / DoRenderWork Core/TracerI.h:345-363
+ *                                     PathTracerRendererT::StartRender    TracerDLL/PathTracerRenderer.cu:1182-1355
+ *                                     PathTracerRendererT::DoThroughputSingleTileRender
+ *                                                                         TracerDLL/PathTracerRenderer.cu:1000-1074
+ *   film read-back                    RenderImage::TransferToHost         Tracer/RenderImage.cpp:163
+ */
+#ifndef MRAY_AMD_H
+#define MRAY_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum MRayStatus
+{
+    MRAY_OK              = 0,
+    MRAY_ERR_INVALID     = 1,   /* bad argument (the reference throws MRayError) */
+    MRAY_ERR_HIP         = 2,   /* a HIP runtime call failed                     */
+    MRAY_ERR_OOM         = 3,   /* device allocation failed                      */
+    MRAY_ERR_UNSUPPORTED = 4    /* valid in the reference, not built here        */
+} MRayStatus;
+
+/* ---------------------------------------------------------------------- */
+/* Scene description (host memory).  Mirrors the reference's default types: */
+/* Triangle primitive group, Lambert / Reflect / Refract / Unreal materials, */
+/* PrimLight<Triangle> area lights, Null or Skysphere boundary light and a   */
+/* Pinhole camera (Tracer/TracerDLL/TracerTypeGenInput.txt).                  */
+/* ---------------------------------------------------------------------- */
+enum
+{
+    MRAY_MAT_LAMBERT = 0,   /* MaterialsDefault.hpp:6-131   */
+    MRAY_MAT_REFLECT = 1,   /* MaterialsDefault.hpp:133-230 */
+    MRAY_MAT_REFRACT = 2,   /* MaterialsDefault.hpp:232-473 */
+    MRAY_MAT_UNREAL  = 3    /* MaterialsDefault.hpp:475-784 */
+};
+
+enum
+{
+    MRAY_BOUNDARY_NULL             = 0,   /* LightNull                       */
+    MRAY_BOUNDARY_SKYSPHERE_COOCTA = 1    /* LightSkysphere<CoOcta>, constant */
+};
+
+/* Surface key of a triangle: a material index, or a light index with this flag. */
+#define MRAY_LIGHT_FLAG 0x80000000u
+
+typedef struct MRayMaterial
+{
+    uint32_t type;
+    float    albedo[3];
+    float    roughness;
+    float    metallic;
+    float    specular;
+    float    cauchyFront[3];
+    float    cauchyBack[3];
+} MRayMaterial;
+
+typedef struct MRayCamera
+{
+    float position[3];
+    float gaze[3];       /* look-at point                         */
+    float up[3];
+    float fov[2];        /* (fovX, fovY), radians                 */
+    float nearFar[2];
+} MRayCamera;
+
+typedef struct MRaySceneDesc
+{
+    uint32_t            vertexCount;
+    uint32_t            triangleCount;
+    const float*        hPositions;     /* [vertexCount][3]                        */
+    const float*        hTbnQuats;      /* [vertexCount][4] (w,x,y,z) world->tangent */
+    const uint32_t*     hIndices;       /* [triangleCount][3]                      */
+    const uint32_t*     hSurfaceKeys;   /* [triangleCount] material or light key    */
+    const uint8_t*      hCullBackFace;  /* [triangleCount]                          */
+    uint32_t            materialCount;
+    const MRayMaterial* hMaterials;
+    uint32_t            lightCount;     /* emissive triangles (boundary excluded)   */
+    const uint32_t*     hLightTriangles;/* [lightCount] triangle of light i         */
+    const float*        hLightRadiance; /* [lightCount][3] linear RGB               */
+    const uint8_t*      hLightTwoSided; /* [lightCount]                             */
+    uint32_t            boundaryType;
+    float               boundaryRadiance[3];
+    float               sceneDiameter;
+    MRayCamera          camera;
+} MRaySceneDesc;
+
+enum
+{
+    MRAY_SAMPLE_PURE    = 0,  /* "Pure"                    */
+    MRAY_SAMPLE_NEE     = 1,  /* "WithNextEventEstimation" */
+    MRAY_SAMPLE_NEE_MIS = 2   /* "WithNEEAndMIS"           */
+};
+
+typedef struct MRayRenderParams
+{
+    uint32_t width;
+    uint32_t height;
+    uint32_t totalSPP;
+    uint32_t rrRange[2];      /* russianRouletteRange (start, max depth)  */
+    uint32_t sampleMode;      /* MRAY_SAMPLE_*                            */
+    uint32_t spectral;        /* 0: PathTracerRGB, 1: PathTracerSpectral   */
+    uint32_t maxRayCount;     /* path slots; 0 = width*height             */
+    uint64_t seed;
+    float    filterRadius;    /* Gaussian film filter radius (default 1)  */
+    uint32_t reserved;
+} MRayRenderParams;
+
+/* Per-iteration analytics (Common/AnalyticStructs.h:33-52). */
+typedef struct MRayRenderStats
+{
+    uint64_t deadPaths;       /* paths completed in this call            */
+    uint64_t totalDeadPaths;  /* paths completed since start            */
+    uint64_t extensionRays;   /* closest-hit rays cast in this call      */
+    uint64_t shadowRays;      /* visibility rays cast in this call       */
+    uint32_t iterations;      /* bounces run in this call                */
+    uint32_t finished;        /* all totalSPP samples are in the film    */
+} MRayRenderStats;
+
+/* ---------------------------------------------------------------------- */
+/* Library                                                                 */
+/* ---------------------------------------------------------------------- */
+const char* mray_last_error(void);
+const char* mray_version(void);
+int         mray_device_count(int* hCount);
+
+/* ---------------------------------------------------------------------- */
+/* LBVH build stages (device pointers).  K1/K2/K4/K5/K6/K7 of the reference */
+/* ---------------------------------------------------------------------- */
+/* KCGenPrimCenters + KCGeneratePrimAABBs  (AcceleratorWork.kt.h:5-123).
+ * dCenters [n][3], dAabbs [n][6] (min xyz, max xyz).                      */
+int mray_prim_centers_aabbs(const float* dPositions, const uint32_t* dIndices,
+                            uint32_t triangleCount, float* dCenters, float* dAabbs,
+                            void* stream);
+/* Segmented transform-reduce of the AABBs into one scene box
+ * (AcceleratorLBVH.hpp:764-769).  dOut [6].                               */
+int mray_reduce_aabb(const float* dAabbs, uint32_t count, float* dOut, void* stream);
+/* KCGenMortonCode (AcceleratorLBVH.cu:96-168): 63-bit codes, bit-exact.   */
+int mray_morton63(const float* dCenters, uint32_t count, const float* dSceneAabb,
+                  uint64_t* dCodes, void* stream);
+/* 30-bit codes, Compose3D<uint32_t> (GraphicsFunctions.h:585-603).         */
+int mray_morton30(const float* dCenters, uint32_t count, const float* dSceneAabb,
+                  uint32_t* dCodes, void* stream);
+/* Stable ascending sort of (code, index) pairs over all 64 key bits
+ * (AlgRadixSortCUDA.h, used at AcceleratorLBVH.hpp:828-838).  dTemp needs
+ * mray_radix_sort_temp_bytes(count) bytes.                                 */
+size_t mray_radix_sort_temp_bytes(uint32_t count);
+int mray_radix_sort_u64_u32(uint64_t* dKeys, uint32_t* dValues, uint32_t count,
+                            void* dTemp, size_t tempBytes, void* stream);
+/* KCConstructLBVHInternalNodes + KCUnionLBVHBoundingBoxes
+ * (AcceleratorLBVH.cu:170-435).  Sorted codes and leaf order in; dNodes
+ * [count-1][3] {left, right, parent} with bit 31 = leaf, dNodeAabbs
+ * [count-1][6].  A single leaf produces one node {leaf 0, invalid, invalid}. */
+int mray_lbvh_build(const uint64_t* dSortedCodes, const uint32_t* dLeafOrder,
+                    const float* dLeafAabbs, uint32_t count, uint32_t* dNodes,
+                    float* dNodeAabbs, void* dTemp, size_t tempBytes, void* stream);
+size_t mray_lbvh_temp_bytes(uint32_t count);
+
+/* ---------------------------------------------------------------------- */
+/* Accelerator: scene upload, LBVH build, wide-BVH collapse, casting.       */
+/* ---------------------------------------------------------------------- */
+typedef struct MRayAccel MRayAccel;
+
+int mray_accel_create(const MRaySceneDesc* hScene, MRayAccel** out);
+int mray_accel_destroy(MRayAccel* accel);
+/* Sizes of the built structure: binary node count, wide node count, leaf
+ * triangle count.                                                         */
+int mray_accel_info(const MRayAccel* accel, uint32_t* hBinaryNodes,
+                    uint32_t* hWideNodes, uint32_t* hLeafTriangles);
+/* Copy out the reference-layout products of the build (host buffers):
+ * Morton codes in triangle order [n], leaf order [n], binary nodes
+ * [n-1][3] and node boxes [n-1][6].  Any pointer may be NULL.             */
+int mray_accel_read_lbvh(const MRayAccel* accel, uint64_t* hCodes, uint32_t* hLeafOrder,
+                         uint32_t* hNodes, float* hNodeAabbs);
+/* Closest hit (BaseAcceleratorI::CastRays).  dRays [n][8] = RayGMem
+ * {pos xyz, tMin, dir xyz, tMax}.  dPrim [n] triangle index or 0xFFFFFFFF;
+ * dT [n]; dBary [n][2] = MetaHit (a, b).  Ties in t resolve to the lower
+ * triangle index.                                                          */
+int mray_accel_cast_closest(const MRayAccel* accel, const float* dRays, uint32_t count,
+                            uint32_t* dPrim, float* dT, float* dBary, void* stream);
+/* Same cast, also adding {nodes visited, triangles tested} to dCounters[2]
+ * (u64); the counts give the kernel's algorithmic bytes per ray.  No
+ * reference counterpart: it instruments CastRays for the roofline.         */
+int mray_accel_cast_closest_counted(const MRayAccel* accel, const float* dRays, uint32_t count,
+                                    uint32_t* dPrim, float* dT, float* dBary,
+                                    unsigned long long* dCounters, void* stream);
+/* Any hit (CastVisibilityRays): dVisible [n] = 1 when nothing is hit.       */
+int mray_accel_cast_visibility(const MRayAccel* accel, const float* dRays, uint32_t count,
+                               uint32_t* dVisible, void* stream);
+
+/* ---------------------------------------------------------------------- */
+/* Path tracer renderer ("PathTracerRGB" / "PathTracerSpectral").           */
+/* ---------------------------------------------------------------------- */
+typedef struct MRayRenderer MRayRenderer;
+
+int mray_renderer_create(const MRayAccel* accel, const MRaySceneDesc* hScene,
+                         const MRayRenderParams* hParams, MRayRenderer** out);
+int mray_renderer_destroy(MRayRenderer* r);
+/* One DoRenderWork(): `iterations` wavefront bounces (refill, cast, shade,
+ * shadow cast, film).  No host synchronisation inside.                     */
+int mray_renderer_iterate(MRayRenderer* r, uint32_t iterations, void* stream);
+/* Run bounces until every totalSPP sample is in the film.                  */
+int mray_renderer_render_all(MRayRenderer* r, MRayRenderStats* hStats);
+/* Synchronises, then fills the stats of everything run since the last call. */
+int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* hStats);
+/* Film (RenderImage SoA): hOut [4][height][width] = R, G, B, weight sums;
+ * row 0 is the bottom row of the camera plane.                             */
+int mray_renderer_read_film(MRayRenderer* r, float* hOut);
+/* Reset the film and all path slots (StartRender again, same seed).        */
+int mray_renderer_reset(MRayRenderer* r);
+/* Device pointer of the film [4][height][width] (stays owned by r).         */
+int mray_renderer_film_device(MRayRenderer* r, float** dFilm);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MRAY_AMD_H */

@@ -1,0 +1,129 @@
+"""ctypes mirror of include/mray_amd.h (struct layouts and constants).
+
+The C structs are the drop-in boundary's plain-data types; this module only
+describes their layout so Python callers (tests, bench, the oracle wrapper)
+can build them.  It loads nothing.
+"""
+import ctypes as C
+
+MRAY_OK = 0
+
+MAT_LAMBERT = 0
+MAT_REFLECT = 1
+MAT_REFRACT = 2
+MAT_UNREAL = 3
+
+BOUNDARY_NULL = 0
+BOUNDARY_SKYSPHERE_COOCTA = 1
+
+LIGHT_FLAG = 0x80000000
+
+SAMPLE_PURE = 0
+SAMPLE_NEE = 1
+SAMPLE_NEE_MIS = 2
+
+INVALID_PRIM = 0xFFFFFFFF
+
+
+class MRayMaterial(C.Structure):
+    _fields_ = [
+        ("type", C.c_uint32),
+        ("albedo", C.c_float * 3),
+        ("roughness", C.c_float),
+        ("metallic", C.c_float),
+        ("specular", C.c_float),
+        ("cauchyFront", C.c_float * 3),
+        ("cauchyBack", C.c_float * 3),
+    ]
+
+
+class MRayCamera(C.Structure):
+    _fields_ = [
+        ("position", C.c_float * 3),
+        ("gaze", C.c_float * 3),
+        ("up", C.c_float * 3),
+        ("fov", C.c_float * 2),
+        ("nearFar", C.c_float * 2),
+    ]
+
+
+class MRaySceneDesc(C.Structure):
+    _fields_ = [
+        ("vertexCount", C.c_uint32),
+        ("triangleCount", C.c_uint32),
+        ("hPositions", C.POINTER(C.c_float)),
+        ("hTbnQuats", C.POINTER(C.c_float)),
+        ("hIndices", C.POINTER(C.c_uint32)),
+        ("hSurfaceKeys", C.POINTER(C.c_uint32)),
+        ("hCullBackFace", C.POINTER(C.c_uint8)),
+        ("materialCount", C.c_uint32),
+        ("hMaterials", C.POINTER(MRayMaterial)),
+        ("lightCount", C.c_uint32),
+        ("hLightTriangles", C.POINTER(C.c_uint32)),
+        ("hLightRadiance", C.POINTER(C.c_float)),
+        ("hLightTwoSided", C.POINTER(C.c_uint8)),
+        ("boundaryType", C.c_uint32),
+        ("boundaryRadiance", C.c_float * 3),
+        ("sceneDiameter", C.c_float),
+        ("camera", MRayCamera),
+    ]
+
+
+class MRayRenderParams(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("totalSPP", C.c_uint32),
+        ("rrRange", C.c_uint32 * 2),
+        ("sampleMode", C.c_uint32),
+        ("spectral", C.c_uint32),
+        ("maxRayCount", C.c_uint32),
+        ("seed", C.c_uint64),
+        ("filterRadius", C.c_float),
+        ("reserved", C.c_uint32),
+    ]
+
+
+class MRayRenderStats(C.Structure):
+    _fields_ = [
+        ("deadPaths", C.c_uint64),
+        ("totalDeadPaths", C.c_uint64),
+        ("extensionRays", C.c_uint64),
+        ("shadowRays", C.c_uint64),
+        ("iterations", C.c_uint32),
+        ("finished", C.c_uint32),
+    ]
+
+
+def render_params(width, height, spp, rr=(8, 8), mode=SAMPLE_NEE_MIS, spectral=False,
+                  seed=0, filter_radius=1.0, max_ray_count=0):
+    """MRayRenderParams with the reference's renderer-attribute meaning
+    (PathTracerRenderer.cu:127-142)."""
+    p = MRayRenderParams()
+    p.width = width
+    p.height = height
+    p.totalSPP = spp
+    p.rrRange[0] = rr[0]
+    p.rrRange[1] = rr[1]
+    p.sampleMode = mode
+    p.spectral = 1 if spectral else 0
+    p.maxRayCount = max_ray_count
+    p.seed = seed
+    p.filterRadius = filter_radius
+    return p
+
+
+def fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def u32ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+def u64ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint64))
+
+
+def u8ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))

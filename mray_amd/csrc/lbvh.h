@@ -1,0 +1,78 @@
+// LBVH build launchers (lbvh.hip) and the 4-wide BVH built from it.
+#pragma once
+
+#include "common.h"
+
+#include <vector>
+
+namespace mray
+{
+
+int    LaunchIota(uint32_t* dOut, uint32_t n, hipStream_t s);
+int    LaunchPrimCentersAabbs(const float* dPos, const uint32_t* dIdx, uint32_t n, float* dCenters,
+                              float* dAabbs, hipStream_t s);
+int    LaunchReduceAabb(const float* dAabbs, uint32_t n, float* dOut, float* dPartial,
+                        hipStream_t s);
+size_t ReduceAabbTempBytes();
+int    LaunchMorton63(const float* dCenters, uint32_t n, const float* dBox, uint64_t* dCodes,
+                      hipStream_t s);
+int    LaunchMorton30(const float* dCenters, uint32_t n, const float* dBox, uint32_t* dCodes,
+                      hipStream_t s);
+size_t RadixSortTempBytes(uint32_t n);
+int    LaunchRadixSort(uint64_t* dKeys, uint32_t* dVals, uint32_t n, void* dTemp,
+                       size_t tempBytes, hipStream_t s);
+size_t LbvhTempBytes(uint32_t n);
+int    LaunchLbvhBuild(const uint64_t* dSortedCodes, const uint32_t* dLeafOrder,
+                       const float* dLeafAabbs, uint32_t n, uint32_t* dNodes, float* dBoxes,
+                       uint32_t* dNodeRange, void* dTemp, size_t tempBytes, hipStream_t s,
+                       uint32_t strictTie = 0);
+// True when every internal node and leaf is reached exactly once from the
+// root and each node's two children split its leaf range at one point.
+bool ValidateBinaryTree(const std::vector<uint32_t>& nodes, const std::vector<uint32_t>& range,
+                        uint32_t n);
+
+// ---------------------------------------------------------------------------
+// 4-wide BVH node: one 128-byte line, child boxes as SoA lanes so a node is
+// eight dwordx4 loads.  child[k]: internal node index, or kLeafFlag |
+// (count-1) << 26 | first, indexing the triangle soup (leaf order).  Unused
+// slots carry an empty box (lo = +inf, hi = -inf).
+// ---------------------------------------------------------------------------
+struct alignas(128) BNode4
+{
+    float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(BNode4) == 128, "BNode4 must be one 128-byte line");
+
+static constexpr uint32_t kLeafCountShift = 26;
+static constexpr uint32_t kLeafStartMask = (1u << kLeafCountShift) - 1u;
+static constexpr uint32_t kLeafMax = 4;   // triangles per leaf slot
+
+// Soup triangle: v0.xyz + triangle id bits, v1.xyz + cull flag, v2.xyz + 0.
+struct alignas(16) SoupTri
+{
+    float v0[3];
+    uint32_t prim;
+    float v1[3];
+    uint32_t cull;
+    float v2[3];
+    uint32_t pad;
+};
+static_assert(sizeof(SoupTri) == 48, "SoupTri layout");
+
+struct WideBvh
+{
+    std::vector<BNode4> nodes;
+    uint32_t maxDepth = 0;     // root = depth 1
+    uint32_t leafSlots = 0;
+};
+
+// Collapse the binary LBVH (reference layout) into BNode4s, top-down,
+// expanding the child with the largest surface area until four children;
+// subtrees of <= kLeafMax triangles become leaves over their leaf-order range.
+WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
+                       const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
+                       const std::vector<uint32_t>& leafOrder);
+
+} // namespace mray

@@ -1,0 +1,983 @@
+// Wavefront path tracer on gfx950 ("PathTracerRGB" / "PathTracerSpectral").
+//
+// One iteration = one DoRenderWork() bounce of every path slot
+// (TracerDLL/PathTracerRenderer.cu:930-1074), as six kernels that read every
+// count from device memory, so iterations queue back-to-back with no host
+// synchronisation:
+//   KRefillCount / KRefillScan / KRefillApply  ReloadPaths: dead slots, in
+//        ascending slot order, take consecutive pixel indices from the global
+//        path counter (PathTracerRendererBase.cu:69-203 with the stable
+//        partition of Device/CPU/AlgBinaryPartitionCPU.h:31-48); live slots are
+//        appended to the active list with wave-aggregated atomics.
+//   KTrace    closest hit of every active slot (CastRays)
+//   KShade    boundary/light work or NEE (work 1) + BxDF scatter (work 0)
+//             (PathTracerRendererShaders.h:201-520)
+//   KShadow   any-hit visibility + KCAccumulateShadowRaysPT, then finished paths
+//             go to the film with float atomics (TextureFilter.cu:672-697).
+// Each slot consumes PCG32 numbers in the reference's per-slot order, so with
+// the same seed every path sample equals the reference schedule's.
+#include "common.h"
+#include "scene.h"
+#include "shading.h"
+#include "spectral.h"
+#include "traverse.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <vector>
+
+namespace mray
+{
+
+int DevAlloc(DeviceArena& arena, void** p, size_t bytes);
+
+enum : uint32_t { ST_ALIVE = 0, ST_DEAD = 1, ST_INVALID = 2 };
+enum : uint32_t { RT_CAMERA = 0, RT_PATH = 1, RT_SPECULAR = 2, RT_SHADOW = 3 };
+
+__device__ __forceinline__ uint32_t MDepth(uint32_t m) { return m & 0xFFFFu; }
+__device__ __forceinline__ uint32_t MStatus(uint32_t m) { return (m >> 16) & 3u; }
+__device__ __forceinline__ uint32_t MType(uint32_t m) { return (m >> 18) & 3u; }
+__device__ __forceinline__ bool MDispersed(uint32_t m) { return (m >> 20) & 1u; }
+__device__ __forceinline__ bool MPending(uint32_t m) { return (m >> 21) & 1u; }
+__device__ __forceinline__ uint32_t MPack(uint32_t depth, uint32_t status, uint32_t type,
+                                          bool disp, bool pending)
+{
+    return (depth & 0xFFFFu) | (status << 16) | (type << 18) | (uint32_t(disp) << 20) |
+           (uint32_t(pending) << 21);
+}
+
+struct SlotSoA
+{
+    float4* rayO;      // pos, tMin
+    float4* rayD;      // dir, tMax
+    uint4* hit;        // prim, t, a, b
+    float4* thr;
+    float4* rad;
+    uint32_t* meta;
+    float* prevPdf;
+    uint32_t* pixel;
+    float* filterW;
+    uint32_t* rng;
+    float4* sO;        // shadow ray pos, tMin
+    float4* sD;        // shadow ray dir, tMax
+    float4* sRad;
+    float4* waves;     // spectral only
+    float4* wpdf;
+};
+
+struct Sched
+{
+    unsigned long long counter;   // next global path index
+    unsigned long long limit;     // totalSPP * pixels
+    unsigned long long base;      // counter at this iteration's refill
+    unsigned long long deadTotal;
+    unsigned long long extRays;
+    unsigned long long shadowRays;
+    uint32_t fill;
+    uint32_t activeCount;
+    uint32_t iterations;
+    uint32_t pad;
+};
+
+struct RenderConst
+{
+    SceneView scene;
+    CameraGPU cam;
+    SpectralView spec;
+    uint32_t N;
+    uint32_t P;
+    uint32_t W;
+    uint32_t rr0, rr1;
+    uint32_t mode;
+    uint32_t spectral;
+    float* film;
+};
+
+static constexpr uint32_t kSlotTPB = 256;
+
+// ---------------------------------------------------------------------------
+// Refill
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kSlotTPB)
+KRefillCount(const uint32_t* __restrict__ meta, uint32_t N, uint32_t* __restrict__ blockCnt,
+             Sched* __restrict__ sched)
+{
+    __shared__ uint32_t wsum[kSlotTPB / 64];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool dead = (i < N) && (MStatus(meta[i]) != ST_ALIVE);
+    uint64_t b = __ballot(dead);
+    if((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = uint32_t(__popcll(b));
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        uint32_t s = 0;
+        for(uint32_t w = 0; w < kSlotTPB / 64; w++) s += wsum[w];
+        blockCnt[blockIdx.x] = s;
+    }
+    if(blockIdx.x == 0 && threadIdx.x == 0) sched->activeCount = 0;
+}
+
+__global__ void __launch_bounds__(1024)
+KRefillScan(uint32_t* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sched)
+{
+    __shared__ uint32_t s[1024];
+    __shared__ uint32_t carry;
+    if(threadIdx.x == 0) carry = 0;
+    __syncthreads();
+    for(uint32_t base = 0; base < count; base += 1024)
+    {
+        uint32_t i = base + threadIdx.x;
+        uint32_t v = (i < count) ? blockCnt[i] : 0u;
+        s[threadIdx.x] = v;
+        __syncthreads();
+        for(uint32_t off = 1; off < 1024; off <<= 1)
+        {
+            uint32_t add = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0u;
+            __syncthreads();
+            s[threadIdx.x] += add;
+            __syncthreads();
+        }
+        uint32_t incl = s[threadIdx.x];
+        if(i < count) blockCnt[i] = carry + incl - v;
+        __syncthreads();
+        if(threadIdx.x == 1023) carry += incl;
+        __syncthreads();
+    }
+    if(threadIdx.x == 0)
+    {
+        unsigned long long dead = carry;
+        unsigned long long remaining = sched->limit - sched->counter;
+        unsigned long long fill = (dead < remaining) ? dead : remaining;
+        sched->base = sched->counter;
+        sched->fill = uint32_t(fill);
+        sched->counter += fill;
+        sched->iterations += 1;
+    }
+}
+
+template<bool SPECTRAL>
+__global__ void __launch_bounds__(kSlotTPB)
+KRefillApply(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ blockOff,
+             Sched* __restrict__ sched, uint32_t* __restrict__ active)
+{
+    __shared__ uint32_t wsum[kSlotTPB / 64];
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t m = (i < rc.N) ? S.meta[i] : MPack(0, ST_ALIVE, 0, false, false);
+    bool dead = (i < rc.N) && (MStatus(m) != ST_ALIVE);
+    uint64_t b = __ballot(dead);
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    if(lane == 0) wsum[wave] = uint32_t(__popcll(b));
+    __syncthreads();
+    uint32_t waveOff = 0;
+    for(uint32_t w = 0; w < wave; w++) waveOff += wsum[w];
+    bool alive = (i < rc.N) && !dead;
+    if(dead)
+    {
+        uint32_t rank = blockOff[blockIdx.x] + waveOff + uint32_t(__popcll(b & lt));
+        if(rank < sched->fill)
+        {
+            // KCGenerateCamRaysStochastic (RayGenKernels.kt.h:155-226) +
+            // KCInitPathStateIndirect (PathTracerRendererBase.cu:35-57)
+            unsigned long long pix = (sched->base + rank) % rc.P;
+            uint32_t px = uint32_t(pix % rc.W), py = uint32_t(pix / rc.W);
+            uint32_t st = S.rng[i];
+            float xi0 = NextRN(st), xi1 = NextRN(st);
+            const CameraGPU& c = rc.cam;
+            float o0 = SampleGaussian(xi0, c.sigma);
+            float o1 = SampleGaussian(xi1, c.sigma);
+            float offPdf = Gaussian(o0, c.sigma) * Gaussian(o1, c.sigma);
+            float weight = Gaussian(o0, c.sigma) * Gaussian(o1, c.sigma);
+            float dx = c.planeSize[0] / float(c.width), dy = c.planeSize[1] / float(c.height);
+            float sdx = (float(px) + (o0 + 0.5f)) * dx;
+            float sdy = (float(py) + (o1 + 0.5f)) * dy;
+            V3 sp = {c.bottomLeft[0], c.bottomLeft[1], c.bottomLeft[2]};
+            sp = sp + sdx * V3{c.right[0], c.right[1], c.right[2]};
+            sp = sp + sdy * V3{c.up[0], c.up[1], c.up[2]};
+            V3 cp = {c.position[0], c.position[1], c.position[2]};
+            V3 dir = Normalize(sp - cp);
+            S.rayO[i] = make_float4(cp.x, cp.y, cp.z, c.nearFar[0]);
+            S.rayD[i] = make_float4(dir.x, dir.y, dir.z, c.nearFar[1]);
+            S.filterW[i] = weight / offPdf;
+            S.pixel[i] = uint32_t(pix);
+            S.rad[i] = make_float4(0, 0, 0, 0);
+            S.thr[i] = make_float4(1, 1, 1, 1);
+            S.prevPdf[i] = 0.0f;
+            if constexpr(SPECTRAL)
+            {
+                float xw = NextRN(st);
+                V4 w, p;
+                SampleWavelengths(xw, w, p);
+                S.waves[i] = make_float4(w.x, w.y, w.z, w.w);
+                S.wpdf[i] = make_float4(p.x, p.y, p.z, p.w);
+            }
+            S.rng[i] = st;
+            S.meta[i] = MPack(0, ST_ALIVE, RT_CAMERA, false, false);
+            alive = true;
+        }
+        else if(MStatus(m) != ST_INVALID)
+        {
+            S.meta[i] = MPack(MDepth(m), ST_INVALID, MType(m), MDispersed(m), false);
+        }
+    }
+    // append live slots to the active list, one atomic per wave
+    uint64_t am = __ballot(alive);
+    uint32_t waveBase = 0;
+    if(lane == 0 && am) waveBase = atomicAdd(&sched->activeCount, uint32_t(__popcll(am)));
+    waveBase = __shfl(waveBase, 0);
+    if(alive) active[waveBase + uint32_t(__popcll(am & lt))] = i;
+}
+
+// ---------------------------------------------------------------------------
+// Closest hit
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kTraceTPB)
+KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __restrict__ sched)
+{
+    __shared__ uint32_t lstack[kLdsStack * kTraceTPB];
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = sched->activeCount;
+    if(gtid == 0) sched->extRays += n;
+    for(uint32_t k = gtid; k < n; k += ctx.gridThreads)
+    {
+        uint32_t i = active[k];
+        float4 o = S.rayO[i], d = S.rayD[i];
+        HitRec h = Trace<false, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                       o.w, d.w);
+        S.hit[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.a),
+                              __float_as_uint(h.b));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Lights (LightsDefault.hpp, LightSampler.hpp, MetaLight.hpp:470-520)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ V3 CoOctaToDir(float s, float t)
+{
+    const float piOvr4 = kPi * 0.25f;
+    float u = s * 2.0f - 1.0f, v = t * 2.0f - 1.0f;
+    float ua = fabsf(u), va = fabsf(v);
+    float d = 1.0f - (ua + va);
+    float radius = 1.0f - fabsf(d);
+    float phiPrime = 0.0f;
+    if(radius != 0.0f) phiPrime = ((va - ua) / radius + 1) * piOvr4;
+    float sinP = sinf(phiPrime), cosP = cosf(phiPrime);
+    float cosPhi = (signbit(u) ? -1.0f : 1.0f) * cosP;
+    float sinPhi = (signbit(v) ? -1.0f : 1.0f) * sinP;
+    float z = (signbit(d) ? -1.0f : 1.0f) * (1.0f - radius * radius);
+    float xyFactor = radius * sqrtf(2.0f - radius * radius);
+    return {cosPhi * xyFactor, sinPhi * xyFactor, z};
+}
+
+template<bool SPECTRAL>
+__device__ __forceinline__ V4 ConvRadiance(const RenderConst& rc, V3 rgb, V4 waves)
+{
+    if constexpr(SPECTRAL) return SpectralRadiance(rc.spec, rgb, waves);
+    else return V4{rgb.x, rgb.y, rgb.z, 0.0f};
+}
+template<bool SPECTRAL>
+__device__ __forceinline__ V4 ConvAlbedo(const RenderConst& rc, V3 rgb, V4 waves)
+{
+    if constexpr(SPECTRAL) return SpectralAlbedo(rc.spec, rgb, waves);
+    else return V4{rgb.x, rgb.y, rgb.z, 0.0f};
+}
+
+template<bool SPECTRAL>
+__device__ __forceinline__ V4 TriLightEmitViaPoint(const RenderConst& rc, uint32_t li, V3 wO,
+                                                   V3 point, V4 waves)
+{
+    const SceneView& s = rc.scene;
+    V3 p0, p1, p2;
+    LoadTri(s, s.lightTri[li], p0, p1, p2);
+    float ha, hb;
+    ProjectedHit(p0, p1, p2, point, ha, hb);
+    if(!BaryInRange(ha, hb)) return Splat4(0.0f);
+    V3 n = TriNormal(p0, p1, p2);
+    float NdL = Dot(n, wO);
+    float4 lr = s.lightRad[li];
+    bool twoSided = lr.w != 0.0f;
+    if(!twoSided && NdL <= 0.0f) return Splat4(0.0f);
+    return ConvRadiance<SPECTRAL>(rc, V3{lr.x, lr.y, lr.z}, waves);
+}
+
+struct LightSample { V3 position; V4 emission; float pdf; };
+
+// DirectLightSamplerUniform::SampleLight (LightSampler.hpp:30-74)
+template<bool SPECTRAL>
+__device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const float* rn,
+                                                   V3 distantPoint, V4 waves)
+{
+    const SceneView& s = rc.scene;
+    uint32_t lightCount = s.lightCount + 1u;
+    float lightCountF = float(lightCount);
+    uint32_t li = uint32_t(rn[2] * lightCountF);
+    li = min(li, lightCount - 1u);
+    V3 point;
+    float pointPdf;
+    if(li < lightCount - 1u)
+    {
+        V3 p0, p1, p2;
+        LoadTri(s, s.lightTri[li], p0, p1, p2);
+        float r1 = sqrtf(rn[0]);
+        float r2 = rn[1];
+        float a = 1 - r1;
+        float b = (1 - r2) * r1;
+        float c = r1 * r2;
+        float area = TriArea(p0, p1, p2);
+        float sPdf = 1.0f / area;
+        V3 position = (p0 * a + p1 * b) + p2 * c;
+        V3 normal = TriNormal(p0, p1, p2);
+        V3 sd = distantPoint - position;
+        float distSqr = LengthSqr(sd);
+        sd = Normalize(sd);
+        float NdL = Dot(normal, sd);
+        bool twoSided = s.lightRad[li].w != 0.0f;
+        NdL = twoSided ? fabsf(NdL) : MaxF(0.0f, NdL);
+        float pdf = (NdL == 0.0f) ? 0.0f : sPdf / NdL;
+        pdf *= distSqr;
+        point = position;
+        pointPdf = pdf;
+    }
+    else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+    {
+        V3 dz = CoOctaToDir(rn[0], rn[1]);
+        V3 dirYUp = {dz.y, dz.z, dz.x};
+        point = distantPoint + dirYUp * s.sceneDiameter;
+        pointPdf = 1.0f * 0.25f * kInvPi;
+    }
+    else
+    {
+        point = {1e10f, 1e10f, 1e10f};
+        pointPdf = 0.0f;
+    }
+    V3 wO = Normalize(distantPoint - point);
+    V4 emission;
+    if(li < lightCount - 1u) emission = TriLightEmitViaPoint<SPECTRAL>(rc, li, wO, point, waves);
+    else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+        emission = ConvRadiance<SPECTRAL>(rc, V3{s.boundaryRad[0], s.boundaryRad[1],
+                                                 s.boundaryRad[2]}, waves);
+    else emission = Splat4(0.0f);
+    float selectionPdf = 1.0f / lightCountF;
+    return {point, emission, selectionPdf * pointPdf};
+}
+
+__device__ __forceinline__ float PdfLightTri(const SceneView& s, uint32_t li, float ha, float hb,
+                                             V3 rayPos, V3 rayDir)
+{
+    float selectionPDF = 1.0f / float(s.lightCount + 1u);
+    V3 p0, p1, p2;
+    LoadTri(s, s.lightTri[li], p0, p1, p2);
+    float pdf;
+    if(!BaryInRange(ha, hb)) pdf = 0.0f;
+    else
+    {
+        float hc = 1.0f - ha - hb;
+        V3 position = (p0 * ha + p1 * hb) + p2 * hc;
+        V3 normal = TriNormal(p0, p1, p2);
+        pdf = 1.0f / TriArea(p0, p1, p2);
+        float NdL = Dot(normal, -rayDir);
+        bool twoSided = s.lightRad[li].w != 0.0f;
+        NdL = twoSided ? fabsf(NdL) : MaxF(0.0f, NdL);
+        pdf = (NdL == 0.0f) ? 0.0f : pdf / NdL;
+        pdf *= LengthSqr(rayPos - position);
+    }
+    return pdf * selectionPDF;
+}
+__device__ __forceinline__ float PdfLightBoundary(const SceneView& s)
+{
+    float selectionPDF = 1.0f / float(s.lightCount + 1u);
+    float pdf = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA) ? 1.0f * 0.25f * kInvPi : 0.0f;
+    return pdf * selectionPDF;
+}
+
+// ---------------------------------------------------------------------------
+// Shade: boundary / light work, NEE (work 1), scatter (work 0)
+// ---------------------------------------------------------------------------
+template<bool SPECTRAL>
+__global__ void __launch_bounds__(kSlotTPB)
+KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
+       const Sched* __restrict__ sched, uint32_t gridThreads)
+{
+    const SceneView& s = rc.scene;
+    uint32_t n = sched->activeCount;
+    for(uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridThreads)
+    {
+        uint32_t i = active[k];
+        uint4 hh = S.hit[i];
+        uint32_t prim = hh.x;
+        float ha = __uint_as_float(hh.z), hb = __uint_as_float(hh.w);
+        uint32_t m = S.meta[i];
+        uint32_t depth = MDepth(m), type = MType(m);
+        bool dispersed = MDispersed(m);
+        float4 ro = S.rayO[i], rd = S.rayD[i];
+        V3 rpos = {ro.x, ro.y, ro.z}, rdir = {rd.x, rd.y, rd.z};
+        float4 t4 = S.thr[i], r4 = S.rad[i];
+        V4 thr = {t4.x, t4.y, t4.z, t4.w};
+        V4 rad = {r4.x, r4.y, r4.z, r4.w};
+        V4 waves = Splat4(0.0f);
+        if constexpr(SPECTRAL)
+        {
+            float4 w4 = S.waves[i];
+            waves = {w4.x, w4.y, w4.z, w4.w};
+        }
+        uint32_t li = kInvalid;
+        if(prim != kInvalid)
+        {
+            uint32_t key = s.surfKey[prim];
+            if(key & MRAY_LIGHT_FLAG) li = key & ~MRAY_LIGHT_FLAG;
+        }
+        bool isBoundary = (prim == kInvalid);
+        if(isBoundary || li != kInvalid)
+        {
+            V3 wO = -rdir;
+            V4 emission;
+            if(isBoundary)
+            {
+                emission = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+                    ? ConvRadiance<SPECTRAL>(rc, V3{s.boundaryRad[0], s.boundaryRad[1],
+                                                    s.boundaryRad[2]}, waves)
+                    : Splat4(0.0f);
+            }
+            else if(!BaryInRange(ha, hb)) emission = Splat4(0.0f);
+            else
+            {
+                V3 p0, p1, p2;
+                LoadTri(s, prim, p0, p1, p2);
+                V3 nrm = TriNormal(p0, p1, p2);
+                float NdL = Dot(nrm, wO);
+                float4 lr = s.lightRad[li];
+                emission = (lr.w == 0.0f && NdL <= 0.0f)
+                    ? Splat4(0.0f) : ConvRadiance<SPECTRAL>(rc, V3{lr.x, lr.y, lr.z}, waves);
+            }
+            if(rc.mode == MRAY_SAMPLE_PURE)
+            {
+                depth += 1u;
+                if(depth <= rc.rr1) rad = emission * thr;
+            }
+            else if(!(rc.mode == MRAY_SAMPLE_NEE && type != RT_CAMERA && type != RT_SPECULAR))
+            {
+                bool switchToMIS = (rc.mode == MRAY_SAMPLE_NEE_MIS && type == RT_PATH);
+                V4 t2 = thr;
+                if(switchToMIS)
+                {
+                    float p0 = S.prevPdf[i];
+                    float p1 = isBoundary ? PdfLightBoundary(s)
+                                          : PdfLightTri(s, li, ha, hb, rpos, rdir);
+                    float misPdf = Balance2(p0, 1.0f, p1, 1.0f);
+                    t2 = t2 * p0;
+                    t2 = DivideByPDF(t2, misPdf);
+                }
+                if((depth + 1u) <= rc.rr1) rad = rad + emission * t2;
+            }
+            S.rad[i] = make_float4(rad.x, rad.y, rad.z, rad.w);
+            S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
+            continue;
+        }
+
+        // Surface hit
+        uint32_t st = S.rng[i];
+        const MRayMaterial& M = s.mats[s.surfKey[prim]];
+        Surface sf = GenSurface(s, prim, ha, hb, rdir);
+        MatInst mat;
+        mat.type = M.type;
+        mat.albedo = ConvAlbedo<SPECTRAL>(rc, V3{M.albedo[0], M.albedo[1], M.albedo[2]}, waves);
+        mat.roughness = M.roughness;
+        mat.metallic = M.metallic;
+        mat.specular = M.specular;
+        if(M.type == MRAY_MAT_REFRACT)
+        {
+            if constexpr(SPECTRAL)
+            {
+                mat.frontIoR = CauchyIoR(waves.x, M.cauchyFront);
+                mat.backIoR = CauchyIoR(waves.x, M.cauchyBack);
+            }
+            else
+            {
+                mat.frontIoR = M.cauchyFront[0];
+                mat.backIoR = M.cauchyBack[0];
+            }
+        }
+        else mat.frontIoR = mat.backIoR = 1.0f;
+        V3 wO = Normalize(-rdir);
+        bool specular = IsSpecular(Specularity(mat));
+        bool pending = false;
+        if(rc.mode != MRAY_SAMPLE_PURE)
+        {
+            float rn[3];
+            rn[0] = NextRN(st);
+            rn[1] = NextRN(st);
+            rn[2] = NextRN(st);
+            V3 worldPos = sf.position;
+            LightSample ls = SampleLight<SPECTRAL>(rc, rn, worldPos, waves);
+            V3 sdir = Normalize(ls.position - worldPos);
+            V3 npos = Nudge(ls.position, -sdir);
+            float length = Length(npos - worldPos);
+            bool pt, dsp;
+            V4 refl = EvaluateBxDF(mat, sf, sdir, wO, pt, dsp);
+            V4 t2 = thr * refl;
+            float pdf;
+            if(rc.mode == MRAY_SAMPLE_NEE_MIS)
+                pdf = Balance2(PdfBxDF(mat, sf, sdir, wO), 1.0f, ls.pdf, 1.0f);
+            else pdf = ls.pdf;
+            V4 srad = DivideByPDF(t2 * ls.emission, pdf);
+            if(SPECTRAL && dsp)
+            {
+                float first = srad.x;
+                srad = Splat4(0.0f);
+                srad.x = first;
+            }
+            if(specular) type = RT_SPECULAR;
+            else
+            {
+                V3 nn = pt ? -sf.geoNormal : sf.geoNormal;
+                V3 spos = Nudge(worldPos, nn);
+                type = RT_SHADOW;
+                if((depth + 2u) <= rc.rr1)
+                {
+                    pending = true;
+                    S.sO[i] = make_float4(spos.x, spos.y, spos.z, kEps);
+                    S.sD[i] = make_float4(sdir.x, sdir.y, sdir.z, length * (1.0f - kLargeEps));
+                    S.sRad[i] = make_float4(srad.x, srad.y, srad.z, srad.w);
+                }
+            }
+        }
+        // WorkFunction (PathTracerRendererShaders.h:201-301)
+        uint32_t matRN = MatRNCount(mat.type);
+        float rn[4];
+        for(uint32_t k2 = 0; k2 < matRN + 1u; k2++) rn[k2] = NextRN(st);
+        BxDFSample bs = SampleBxDF(mat, sf, wO, rn);
+        bs.wIDir = Normalize(bs.wIDir);
+        V4 t3 = thr * bs.refl;
+        if(SPECTRAL && bs.dispersed) dispersed = true;
+        depth += 1u;
+        bool dead = (depth >= rc.rr1);
+        if(!dead && depth >= rc.rr0 && !specular)
+        {
+            const float chInv = SPECTRAL ? 0.25f : 0.33333333f;
+            float rrXi = rn[matRN];
+            float prob = ClampF(Sum4(t3) * chInv, 0.1f, 1.0f);
+            if(rrXi >= prob) dead = true;
+            else t3 = t3 * (1.0f / prob);
+        }
+        type = specular ? RT_SPECULAR : RT_PATH;
+        if(!dead)
+        {
+            V4 nt = DivideByPDF(t3, bs.pdf);
+            S.thr[i] = make_float4(nt.x, nt.y, nt.z, nt.w);
+            if(rc.mode == MRAY_SAMPLE_NEE_MIS) S.prevPdf[i] = bs.pdf;
+            V3 nn = bs.passedThrough ? -sf.geoNormal : sf.geoNormal;
+            V3 np = Nudge(bs.wIPos, nn);
+            S.rayO[i] = make_float4(np.x, np.y, np.z, kLargeEps);
+            S.rayD[i] = make_float4(bs.wIDir.x, bs.wIDir.y, bs.wIDir.z, 3.402823466e+38f);
+        }
+        S.rng[i] = st;
+        S.meta[i] = MPack(depth, dead ? ST_DEAD : ST_ALIVE, type, dispersed, pending);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Shadow rays + accumulation + film
+// ---------------------------------------------------------------------------
+template<bool SPECTRAL>
+__global__ void __launch_bounds__(kTraceTPB)
+KShadow(TraceCtx ctx, SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
+        Sched* __restrict__ sched)
+{
+    __shared__ uint32_t lstack[kLdsStack * kTraceTPB];
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t n = sched->activeCount;
+    const uint32_t iters = (n + ctx.gridThreads - 1) / ctx.gridThreads;
+    for(uint32_t it = 0; it < iters; it++)
+    {
+        uint32_t k = gtid + it * ctx.gridThreads;
+        bool valid = k < n;
+        uint32_t i = valid ? active[k] : 0u;
+        uint32_t m = valid ? S.meta[i] : 0u;
+        bool pending = valid && MPending(m);
+        bool died = valid && (MStatus(m) == ST_DEAD);
+        if(pending)
+        {
+            float4 o = S.sO[i], d = S.sD[i];
+            HitRec h = Trace<true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z},
+                                          V3{d.x, d.y, d.z}, o.w, d.w);
+            if(h.prim == kInvalid)
+            {
+                float4 a = S.rad[i], b = S.sRad[i];
+                S.rad[i] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+            }
+        }
+        if(died)
+        {
+            float4 r4 = S.rad[i];
+            V4 rad = {r4.x, r4.y, r4.z, r4.w};
+            if constexpr(SPECTRAL)
+            {
+                float4 w4 = S.waves[i], p4 = S.wpdf[i];
+                rad = SpectrumToRGB(rc.spec, rad, V4{w4.x, w4.y, w4.z, w4.w},
+                                    V4{p4.x, p4.y, p4.z, p4.w}, MDispersed(m));
+            }
+            float w = S.filterW[i];
+            float r = rad.x, g = rad.y, b = rad.z;
+            if(!(isfinite(r) && isfinite(g) && isfinite(b) && isfinite(rad.w)))
+            {
+                r = 1e7f; g = 0.0f; b = 1e7f;
+                w = w * 128.0f;
+            }
+            uint32_t px = S.pixel[i];
+            size_t P = rc.P;
+            atomicAdd(rc.film + px, r);
+            atomicAdd(rc.film + P + px, g);
+            atomicAdd(rc.film + 2 * P + px, b);
+            atomicAdd(rc.film + 3 * P + px, w);
+            S.meta[i] = MPack(MDepth(m), ST_INVALID, MType(m), MDispersed(m), false);
+        }
+        else if(pending)
+            S.meta[i] = m & ~(1u << 21);
+        uint64_t pm = __ballot(pending), dm = __ballot(died);
+        if(lane == 0)
+        {
+            if(pm) atomicAdd(&sched->shadowRays, (unsigned long long)__popcll(pm));
+            if(dm) atomicAdd(&sched->deadTotal, (unsigned long long)__popcll(dm));
+        }
+    }
+}
+
+__global__ void KZeroFloats(float* __restrict__ p, size_t n)
+{
+    size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for(; i < n; i += size_t(gridDim.x) * blockDim.x) p[i] = 0.0f;
+}
+
+__global__ void KResetSlots(uint32_t* __restrict__ meta, uint32_t* __restrict__ rng,
+                            const uint32_t* __restrict__ seeds, uint32_t N,
+                            Sched* __restrict__ sched, unsigned long long limit)
+{
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < N)
+    {
+        meta[i] = MPack(0, ST_INVALID, 0, false, false);
+        rng[i] = seeds[i];
+    }
+    if(i == 0)
+    {
+        sched->counter = 0;
+        sched->limit = limit;
+        sched->base = 0;
+        sched->deadTotal = 0;
+        sched->extRays = 0;
+        sched->shadowRays = 0;
+        sched->fill = 0;
+        sched->activeCount = 0;
+        sched->iterations = 0;
+    }
+}
+
+} // namespace mray
+
+// ===========================================================================
+// Renderer object + C-ABI
+// ===========================================================================
+using namespace mray;
+
+struct MRayRenderer
+{
+    const MRayAccel* accel = nullptr;
+    MRayRenderParams params{};
+    DeviceArena arena;
+    SlotSoA S{};
+    RenderConst rc{};
+    Sched* dSched = nullptr;
+    Sched* hSched = nullptr;      // pinned mirror
+    uint32_t* dBlockCnt = nullptr;
+    uint32_t* dActive = nullptr;
+    uint32_t* dSeeds = nullptr;
+    uint32_t nBlocks = 0;
+    uint32_t shadeGrid = 0;
+    unsigned long long limit = 0;
+    Sched last{};                 // stats at the previous mray_renderer_stats()
+    hipStream_t stream = nullptr;
+    SpectralTables spectral;
+};
+
+static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
+{
+    const uint32_t N = r->rc.N;
+    const TraceCtx& ctx = r->accel->ctx;
+    KRefillCount<<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, N, r->dBlockCnt, r->dSched);
+    KRefillScan<<<1, 1024, 0, s>>>(r->dBlockCnt, r->nBlocks, r->dSched);
+    uint32_t traceBlocks = ctx.gridThreads / kTraceTPB;
+    if(r->rc.spectral)
+    {
+        KRefillApply<true><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
+                                                           r->dActive);
+        KTrace<<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+        KShade<true><<<r->shadeGrid / kSlotTPB, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive,
+                                                                  r->dSched, r->shadeGrid);
+        KShadow<true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->rc, r->dActive, r->dSched);
+    }
+    else
+    {
+        KRefillApply<false><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
+                                                            r->dActive);
+        KTrace<<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+        KShade<false><<<r->shadeGrid / kSlotTPB, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive,
+                                                                   r->dSched, r->shadeGrid);
+        KShadow<false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->rc, r->dActive, r->dSched);
+    }
+    MRAY_HIP_CHECK(hipGetLastError());
+    return MRAY_OK;
+}
+
+static int ResetRenderer(MRayRenderer* r)
+{
+    hipStream_t s = r->stream;
+    size_t filmN = size_t(r->rc.P) * 4;
+    KZeroFloats<<<1024, 256, 0, s>>>(r->rc.film, filmN);
+    KResetSlots<<<DivUp(r->rc.N, 256), 256, 0, s>>>(r->S.meta, r->S.rng, r->dSeeds, r->rc.N,
+                                                     r->dSched, r->limit);
+    MRAY_HIP_CHECK(hipGetLastError());
+    MRAY_HIP_CHECK(hipStreamSynchronize(s));
+    std::memset(&r->last, 0, sizeof(Sched));
+    return MRAY_OK;
+}
+
+static void BuildCamera(const MRayCamera& c, uint32_t W, uint32_t H, float filterRadius,
+                        CameraGPU& out)
+{
+    // CameraPinhole ctor (CamerasDefault.hpp:8-36), same float operation order
+    // as the CPU restatement (glibc tanf on both sides).
+    V3 pos = {c.position[0], c.position[1], c.position[2]};
+    V3 up = {c.up[0], c.up[1], c.up[2]};
+    V3 gaze = {c.gaze[0], c.gaze[1], c.gaze[2]};
+    float widthHalf = std::tan(c.fov[0] * 0.5f) * c.nearFar[0];
+    float heightHalf = std::tan(c.fov[1] * 0.5f) * c.nearFar[0];
+    V3 gazeDir = gaze - pos;
+    V3 right = Normalize(Cross(gazeDir, up));
+    up = Normalize(Cross(right, gazeDir));
+    gazeDir = Normalize(Cross(up, right));
+    V3 bl = ((pos - right * widthHalf) - up * heightHalf) + gazeDir * c.nearFar[0];
+    out.position[0] = pos.x; out.position[1] = pos.y; out.position[2] = pos.z;
+    out.right[0] = right.x; out.right[1] = right.y; out.right[2] = right.z;
+    out.up[0] = up.x; out.up[1] = up.y; out.up[2] = up.z;
+    out.bottomLeft[0] = bl.x; out.bottomLeft[1] = bl.y; out.bottomLeft[2] = bl.z;
+    out.planeSize[0] = widthHalf * 2.0f;
+    out.planeSize[1] = heightHalf * 2.0f;
+    out.nearFar[0] = c.nearFar[0];
+    out.nearFar[1] = c.nearFar[1];
+    out.sigma = filterRadius * 0.285714f;
+    out.width = W;
+    out.height = H;
+}
+
+#define MRAY_TRY(expr)                  \
+    do {                                \
+        int r_ = (expr);                \
+        if(r_ != MRAY_OK) return r_;    \
+    } while(0)
+
+static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDesc* scene,
+                          const MRayRenderParams* p)
+{
+    r->accel = a;
+    r->params = *p;
+    const uint64_t P = uint64_t(p->width) * p->height;
+    uint32_t N = p->maxRayCount ? p->maxRayCount : uint32_t(P);
+    if(N > P) N = uint32_t(P);
+    r->rc.N = N;
+    r->rc.P = uint32_t(P);
+    r->rc.W = p->width;
+    r->rc.rr0 = p->rrRange[0];
+    r->rc.rr1 = p->rrRange[1];
+    r->rc.mode = p->sampleMode;
+    r->rc.spectral = p->spectral;
+    r->rc.scene = a->view;
+    float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
+    BuildCamera(scene ? scene->camera : a->camera, p->width, p->height, fr, r->rc.cam);
+    r->limit = (unsigned long long)p->totalSPP * P;
+    if(p->spectral)
+    {
+        MRAY_TRY(GetSpectralTables(r->spectral));
+        r->rc.spec = r->spectral.view;
+    }
+    DeviceArena& ar = r->arena;
+    auto A = [&](void** ptr, size_t bytes) { return DevAlloc(ar, ptr, bytes); };
+    MRAY_TRY(A((void**)&r->S.rayO, size_t(N) * 16));
+    MRAY_TRY(A((void**)&r->S.rayD, size_t(N) * 16));
+    MRAY_TRY(A((void**)&r->S.hit, size_t(N) * 16));
+    MRAY_TRY(A((void**)&r->S.thr, size_t(N) * 16));
+    MRAY_TRY(A((void**)&r->S.rad, size_t(N) * 16));
+    MRAY_TRY(A((void**)&r->S.meta, size_t(N) * 4));
+    MRAY_TRY(A((void**)&r->S.prevPdf, size_t(N) * 4));
+    MRAY_TRY(A((void**)&r->S.pixel, size_t(N) * 4));
+    MRAY_TRY(A((void**)&r->S.filterW, size_t(N) * 4));
+    MRAY_TRY(A((void**)&r->S.rng, size_t(N) * 4));
+    MRAY_TRY(A((void**)&r->S.sO, size_t(N) * 16));
+    MRAY_TRY(A((void**)&r->S.sD, size_t(N) * 16));
+    MRAY_TRY(A((void**)&r->S.sRad, size_t(N) * 16));
+    if(p->spectral)
+    {
+        MRAY_TRY(A((void**)&r->S.waves, size_t(N) * 16));
+        MRAY_TRY(A((void**)&r->S.wpdf, size_t(N) * 16));
+    }
+    MRAY_TRY(A((void**)&r->rc.film, size_t(P) * 4 * sizeof(float)));
+    r->nBlocks = DivUp(N, kSlotTPB);
+    MRAY_TRY(A((void**)&r->dBlockCnt, size_t(r->nBlocks) * 4));
+    MRAY_TRY(A((void**)&r->dActive, size_t(N) * 4));
+    MRAY_TRY(A((void**)&r->dSched, sizeof(Sched)));
+    MRAY_TRY(A((void**)&r->dSeeds, size_t(N) * 4));
+    MRAY_HIP_CHECK(hipHostMalloc((void**)&r->hSched, sizeof(Sched), hipHostMallocDefault));
+    // RNGGroupIndependent (Random.cu:661-703): seed32 = hi ^ lo of the 64-bit
+    // seed, one std::mt19937 draw per pixel through GenerateState.
+    {
+        std::vector<uint32_t> seeds(N);
+        uint32_t seed32 = uint32_t((p->seed >> 32) ^ (p->seed & 0xFFFFFFFFull));
+        std::mt19937 mt(seed32);
+        for(uint32_t i = 0; i < N; i++)
+        {
+            uint32_t x = uint32_t(mt());
+            uint32_t st = 0u * 747796405u + 2891336453u;
+            st += x;
+            st = st * 747796405u + 2891336453u;
+            seeds[i] = st;
+        }
+        MRAY_HIP_CHECK(hipMemcpy(r->dSeeds, seeds.data(), size_t(N) * 4, hipMemcpyHostToDevice));
+    }
+    int dev = 0;
+    MRAY_HIP_CHECK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    MRAY_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
+    r->shadeGrid = uint32_t(std::max(1, prop.multiProcessorCount)) * 8u * kSlotTPB;
+    return ResetRenderer(r);
+}
+
+extern "C" {
+
+int mray_renderer_create(const MRayAccel* accel, const MRaySceneDesc* hScene,
+                         const MRayRenderParams* hParams, MRayRenderer** out)
+{
+    if(!accel || !hParams || !out)
+    {
+        SetError("renderer_create: NULL argument");
+        return MRAY_ERR_INVALID;
+    }
+    *out = nullptr;
+    const MRayRenderParams& p = *hParams;
+    if(p.width == 0 || p.height == 0 || p.totalSPP == 0)
+    {
+        SetError("renderer_create: width, height and totalSPP must be non-zero");
+        return MRAY_ERR_INVALID;
+    }
+    if(uint64_t(p.width) * p.height > (1ull << 31))
+    {
+        SetError("renderer_create: image too large");
+        return MRAY_ERR_INVALID;
+    }
+    if(p.sampleMode > MRAY_SAMPLE_NEE_MIS)
+    {
+        // PathTracerRendererT::PushAttribute throws on unknown values
+        SetError("renderer_create: unknown sampleMode");
+        return MRAY_ERR_INVALID;
+    }
+    if(p.rrRange[1] == 0 || p.rrRange[0] > p.rrRange[1])
+    {
+        SetError("renderer_create: rrRange must satisfy 0 < max and start <= max");
+        return MRAY_ERR_INVALID;
+    }
+    MRayRenderer* r = new MRayRenderer();
+    int rv = CreateRenderer(r, accel, hScene, hParams);
+    if(rv != MRAY_OK)
+    {
+        if(r->hSched) (void)hipHostFree(r->hSched);
+        r->arena.Release();
+        delete r;
+        return rv;
+    }
+    *out = r;
+    return MRAY_OK;
+}
+
+int mray_renderer_destroy(MRayRenderer* r)
+{
+    if(!r) return MRAY_OK;
+    (void)hipDeviceSynchronize();
+    if(r->hSched) (void)hipHostFree(r->hSched);
+    r->arena.Release();
+    delete r;
+    return MRAY_OK;
+}
+
+int mray_renderer_reset(MRayRenderer* r)
+{
+    if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }
+    return ResetRenderer(r);
+}
+
+int mray_renderer_iterate(MRayRenderer* r, uint32_t iterations, void* stream)
+{
+    if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }
+    hipStream_t s = stream ? static_cast<hipStream_t>(stream) : r->stream;
+    for(uint32_t k = 0; k < iterations; k++) MRAY_TRY(EnqueueIteration(r, s));
+    return MRAY_OK;
+}
+
+int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* out)
+{
+    if(!r || !out) { SetError("renderer_stats: NULL argument"); return MRAY_ERR_INVALID; }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    MRAY_HIP_CHECK(hipMemcpy(r->hSched, r->dSched, sizeof(Sched), hipMemcpyDeviceToHost));
+    const Sched& c = *r->hSched;
+    out->deadPaths = c.deadTotal - r->last.deadTotal;
+    out->totalDeadPaths = c.deadTotal;
+    out->extensionRays = c.extRays - r->last.extRays;
+    out->shadowRays = c.shadowRays - r->last.shadowRays;
+    out->iterations = c.iterations - r->last.iterations;
+    out->finished = (c.deadTotal >= r->limit) ? 1u : 0u;
+    r->last = c;
+    return MRAY_OK;
+}
+
+int mray_renderer_render_all(MRayRenderer* r, MRayRenderStats* hStats)
+{
+    if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }
+    MRayRenderStats total{};
+    for(;;)
+    {
+        MRAY_TRY(mray_renderer_iterate(r, 8, nullptr));
+        MRayRenderStats st{};
+        MRAY_TRY(mray_renderer_stats(r, &st));
+        total.deadPaths += st.deadPaths;
+        total.extensionRays += st.extensionRays;
+        total.shadowRays += st.shadowRays;
+        total.iterations += st.iterations;
+        total.totalDeadPaths = st.totalDeadPaths;
+        if(st.finished) { total.finished = 1; break; }
+        if(st.deadPaths == 0 && st.extensionRays == 0)
+        {
+            SetError("render_all: no progress");
+            return MRAY_ERR_INVALID;
+        }
+    }
+    if(hStats) *hStats = total;
+    return MRAY_OK;
+}
+
+int mray_renderer_read_film(MRayRenderer* r, float* hOut)
+{
+    if(!r || !hOut) { SetError("read_film: NULL argument"); return MRAY_ERR_INVALID; }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    MRAY_HIP_CHECK(hipMemcpy(hOut, r->rc.film, size_t(r->rc.P) * 4 * sizeof(float),
+                             hipMemcpyDeviceToHost));
+    return MRAY_OK;
+}
+
+int mray_renderer_film_device(MRayRenderer* r, float** dFilm)
+{
+    if(!r || !dFilm) { SetError("film_device: NULL argument"); return MRAY_ERR_INVALID; }
+    *dFilm = r->rc.film;
+    return MRAY_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,73 @@
+// Device-resident scene: geometry + LBVH/4-wide BVH + shading tables.
+#pragma once
+
+#include "common.h"
+#include "lbvh.h"
+#include "traverse.h"
+
+#include <vector>
+
+namespace mray
+{
+
+struct DeviceArena
+{
+    std::vector<void*> ptrs;
+    void Release();
+};
+
+int DevAlloc(DeviceArena& arena, void** p, size_t bytes);
+
+// Shading-side view of the scene, passed by value to kernels.
+struct SceneView
+{
+    const float* __restrict__ pos;        // [V][3]
+    const float4* __restrict__ tbn;       // [V] (w,x,y,z)
+    const uint32_t* __restrict__ idx;     // [T][3]
+    const uint32_t* __restrict__ surfKey; // [T]
+    const MRayMaterial* __restrict__ mats;
+    const uint32_t* __restrict__ lightTri;
+    const float4* __restrict__ lightRad;  // rgb, w = two-sided flag
+    uint32_t lightCount;                  // emissive triangles (boundary excluded)
+    uint32_t boundaryType;
+    float boundaryRad[3];
+    float sceneDiameter;
+};
+
+} // namespace mray
+
+struct MRayAccel
+{
+    mray::DeviceArena arena;
+    uint32_t nTri = 0, nVert = 0;
+    uint32_t wideNodes = 0, leafSlots = 0, maxDepth = 0;
+    uint32_t strictTie = 0;       // 1: rebuilt with the (code, index) tie-break
+    float* dPos = nullptr;
+    uint32_t* dIdx = nullptr;
+    float* dCenters = nullptr;
+    float* dAabbs = nullptr;
+    float* dSceneBox = nullptr;
+    uint64_t* dCodes = nullptr;
+    uint64_t* dSortedCodes = nullptr;
+    uint32_t* dLeafOrder = nullptr;
+    uint32_t* dBinNodes = nullptr;
+    float* dBinBoxes = nullptr;
+    uint32_t* dBinRange = nullptr;
+    mray::BNode4* dNodes = nullptr;
+    mray::SoupTri* dSoup = nullptr;
+    mray::TraceCtx ctx{};
+    // shading tables
+    float4* dTbn = nullptr;
+    uint32_t* dSurfKey = nullptr;
+    MRayMaterial* dMats = nullptr;
+    uint32_t* dLightTri = nullptr;
+    float4* dLightRad = nullptr;
+    mray::SceneView view{};
+    MRayCamera camera{};
+    std::vector<MRayMaterial> hMats;
+};
+
+namespace mray
+{
+int UploadSceneData(MRayAccel* a, const MRaySceneDesc* s);
+}

@@ -1,0 +1,274 @@
+"""ctypes binding of libmray_amd.so (the gfx950 hot path, include/mray_amd.h).
+
+Device buffers are torch tensors on `cuda:0` (PyTorch is only the allocator
+and stream plumbing here); every compute call goes through the C-ABI.  There is
+no CPU fallback: a missing library or GPU raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmray_amd.so")
+_lib = None
+
+
+class MRayError(RuntimeError):
+    """Raised when a C-ABI call fails (the reference throws MRayError)."""
+
+
+# Every symbol include/mray_amd.h declares (checked by tests/test_abi.py).
+EXPORTS = [
+    "mray_last_error", "mray_version", "mray_device_count",
+    "mray_prim_centers_aabbs", "mray_reduce_aabb", "mray_morton63", "mray_morton30",
+    "mray_radix_sort_temp_bytes", "mray_radix_sort_u64_u32", "mray_lbvh_build",
+    "mray_lbvh_temp_bytes",
+    "mray_accel_create", "mray_accel_destroy", "mray_accel_info", "mray_accel_read_lbvh",
+    "mray_accel_cast_closest", "mray_accel_cast_closest_counted", "mray_accel_cast_visibility",
+    "mray_renderer_create", "mray_renderer_destroy", "mray_renderer_iterate",
+    "mray_renderer_render_all", "mray_renderer_stats", "mray_renderer_read_film",
+    "mray_renderer_reset", "mray_renderer_film_device",
+]
+
+
+def lib():
+    """Load libmray_amd.so (built by __graft_entry__.build()); raise if absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MRayError(f"{LIB_PATH} is not built; run __graft_entry__.build() first")
+    L = C.CDLL(LIB_PATH)
+    P = C.c_void_p
+    u32 = C.c_uint32
+    L.mray_last_error.restype = C.c_char_p
+    L.mray_version.restype = C.c_char_p
+    L.mray_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.mray_prim_centers_aabbs.argtypes = [P, P, u32, P, P, P]
+    L.mray_reduce_aabb.argtypes = [P, u32, P, P]
+    L.mray_morton63.argtypes = [P, u32, P, P, P]
+    L.mray_morton30.argtypes = [P, u32, P, P, P]
+    L.mray_radix_sort_temp_bytes.restype = C.c_size_t
+    L.mray_radix_sort_temp_bytes.argtypes = [u32]
+    L.mray_radix_sort_u64_u32.argtypes = [P, P, u32, P, C.c_size_t, P]
+    L.mray_lbvh_temp_bytes.restype = C.c_size_t
+    L.mray_lbvh_temp_bytes.argtypes = [u32]
+    L.mray_lbvh_build.argtypes = [P, P, P, u32, P, P, P, C.c_size_t, P]
+    L.mray_accel_create.argtypes = [C.POINTER(abi.MRaySceneDesc), C.POINTER(P)]
+    L.mray_accel_destroy.argtypes = [P]
+    L.mray_accel_info.argtypes = [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
+    L.mray_accel_read_lbvh.argtypes = [P, P, P, P, P]
+    L.mray_accel_cast_closest.argtypes = [P, P, u32, P, P, P, P]
+    L.mray_accel_cast_closest_counted.argtypes = [P, P, u32, P, P, P, P, P]
+    L.mray_accel_cast_visibility.argtypes = [P, P, u32, P, P]
+    L.mray_renderer_create.argtypes = [P, C.POINTER(abi.MRaySceneDesc),
+                                       C.POINTER(abi.MRayRenderParams), C.POINTER(P)]
+    L.mray_renderer_destroy.argtypes = [P]
+    L.mray_renderer_iterate.argtypes = [P, u32, P]
+    L.mray_renderer_render_all.argtypes = [P, C.POINTER(abi.MRayRenderStats)]
+    L.mray_renderer_stats.argtypes = [P, C.POINTER(abi.MRayRenderStats)]
+    L.mray_renderer_read_film.argtypes = [P, P]
+    L.mray_renderer_reset.argtypes = [P]
+    L.mray_renderer_film_device.argtypes = [P, C.POINTER(P)]
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != abi.MRAY_OK:
+        msg = lib().mray_last_error().decode(errors="replace")
+        raise MRayError(f"mray call failed ({rc}): {msg}")
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def require_gpu():
+    import torch
+    if not torch.cuda.is_available():
+        raise MRayError("no GPU visible: the gfx950 path has no CPU fallback")
+    return torch
+
+
+# ---------------------------------------------------------------------------
+# Stage-level entry points (device tensors in, device tensors out)
+# ---------------------------------------------------------------------------
+def prim_centers_aabbs(d_pos, d_idx):
+    torch = require_gpu()
+    n = d_idx.shape[0]
+    centers = torch.empty((n, 3), dtype=torch.float32, device=d_pos.device)
+    aabbs = torch.empty((n, 6), dtype=torch.float32, device=d_pos.device)
+    check(lib().mray_prim_centers_aabbs(_ptr(d_pos), _ptr(d_idx), n, _ptr(centers), _ptr(aabbs),
+                                        _stream()))
+    return centers, aabbs
+
+
+def reduce_aabb(d_aabbs):
+    torch = require_gpu()
+    out = torch.empty(6, dtype=torch.float32, device=d_aabbs.device)
+    check(lib().mray_reduce_aabb(_ptr(d_aabbs), d_aabbs.shape[0], _ptr(out), _stream()))
+    return out
+
+
+def morton63(d_centers, d_box):
+    torch = require_gpu()
+    n = d_centers.shape[0]
+    out = torch.empty(n, dtype=torch.int64, device=d_centers.device)
+    check(lib().mray_morton63(_ptr(d_centers), n, _ptr(d_box), _ptr(out), _stream()))
+    return out
+
+
+def morton30(d_centers, d_box):
+    torch = require_gpu()
+    n = d_centers.shape[0]
+    out = torch.empty(n, dtype=torch.int32, device=d_centers.device)
+    check(lib().mray_morton30(_ptr(d_centers), n, _ptr(d_box), _ptr(out), _stream()))
+    return out
+
+
+def radix_sort_u64_u32(d_keys, d_vals):
+    """In-place stable ascending sort; keys int64 tensor (bit pattern u64)."""
+    torch = require_gpu()
+    n = d_keys.shape[0]
+    nb = lib().mray_radix_sort_temp_bytes(n)
+    tmp = torch.empty(max(1, nb), dtype=torch.uint8, device=d_keys.device)
+    check(lib().mray_radix_sort_u64_u32(_ptr(d_keys), _ptr(d_vals), n, _ptr(tmp), nb, _stream()))
+
+
+def lbvh_build(d_sorted_codes, d_leaf_order, d_leaf_aabbs):
+    torch = require_gpu()
+    n = d_sorted_codes.shape[0]
+    m = max(n - 1, 1)
+    nodes = torch.empty((m, 3), dtype=torch.int32, device=d_sorted_codes.device)
+    boxes = torch.empty((m, 6), dtype=torch.float32, device=d_sorted_codes.device)
+    nb = lib().mray_lbvh_temp_bytes(n)
+    tmp = torch.empty(max(1, nb), dtype=torch.uint8, device=d_sorted_codes.device)
+    check(lib().mray_lbvh_build(_ptr(d_sorted_codes), _ptr(d_leaf_order), _ptr(d_leaf_aabbs), n,
+                                _ptr(nodes), _ptr(boxes), _ptr(tmp), nb, _stream()))
+    return nodes, boxes
+
+
+# ---------------------------------------------------------------------------
+# Accelerator and renderer objects
+# ---------------------------------------------------------------------------
+class Accel:
+    """Device scene + LBVH + 4-wide BVH (MRayAccel)."""
+
+    def __init__(self, scene):
+        require_gpu()
+        self.scene = scene
+        self.desc = scene.desc()
+        h = C.c_void_p()
+        check(lib().mray_accel_create(C.byref(self.desc), C.byref(h)))
+        self.h = h
+        self.n = scene.triangle_count
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mray_accel_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        check(lib().mray_accel_info(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"binary_nodes": a.value, "wide_nodes": b.value, "triangles": c.value}
+
+    def read_lbvh(self):
+        n = self.n
+        m = max(n - 1, 1)
+        codes = np.zeros(n, np.uint64)
+        order = np.zeros(n, np.uint32)
+        nodes = np.zeros((m, 3), np.uint32)
+        boxes = np.zeros((m, 6), np.float32)
+        check(lib().mray_accel_read_lbvh(self.h, codes.ctypes.data, order.ctypes.data,
+                                         nodes.ctypes.data, boxes.ctypes.data))
+        return codes, order, nodes, boxes
+
+    def cast_closest(self, d_rays, count_visits=False):
+        torch = require_gpu()
+        n = d_rays.shape[0]
+        dev = d_rays.device
+        prim = torch.empty(n, dtype=torch.int32, device=dev)
+        t = torch.empty(n, dtype=torch.float32, device=dev)
+        bary = torch.empty((n, 2), dtype=torch.float32, device=dev)
+        if count_visits:
+            ctr = torch.zeros(2, dtype=torch.int64, device=dev)
+            check(lib().mray_accel_cast_closest_counted(self.h, _ptr(d_rays), n, _ptr(prim),
+                                                        _ptr(t), _ptr(bary), _ptr(ctr),
+                                                        _stream()))
+            return prim, t, bary, ctr
+        check(lib().mray_accel_cast_closest(self.h, _ptr(d_rays), n, _ptr(prim), _ptr(t),
+                                            _ptr(bary), _stream()))
+        return prim, t, bary
+
+    def cast_visibility(self, d_rays):
+        torch = require_gpu()
+        n = d_rays.shape[0]
+        vis = torch.empty(n, dtype=torch.int32, device=d_rays.device)
+        check(lib().mray_accel_cast_visibility(self.h, _ptr(d_rays), n, _ptr(vis), _stream()))
+        return vis
+
+
+class Renderer:
+    """Wavefront path tracer (MRayRenderer)."""
+
+    def __init__(self, accel, params, scene=None):
+        self.accel = accel
+        self.params = params
+        self.width, self.height = params.width, params.height
+        h = C.c_void_p()
+        desc = accel.desc if scene is None else scene.desc()
+        check(lib().mray_renderer_create(accel.h, C.byref(desc), C.byref(params), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mray_renderer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def iterate(self, iterations=1, stream=None):
+        s = C.c_void_p(stream) if stream is not None else None
+        check(lib().mray_renderer_iterate(self.h, iterations, s))
+
+    def stats(self):
+        st = abi.MRayRenderStats()
+        check(lib().mray_renderer_stats(self.h, C.byref(st)))
+        return {"dead_paths": st.deadPaths, "total_dead_paths": st.totalDeadPaths,
+                "ext_rays": st.extensionRays, "shadow_rays": st.shadowRays,
+                "iterations": st.iterations, "finished": bool(st.finished)}
+
+    def render_all(self):
+        st = abi.MRayRenderStats()
+        check(lib().mray_renderer_render_all(self.h, C.byref(st)))
+        return {"dead_paths": st.deadPaths, "ext_rays": st.extensionRays,
+                "shadow_rays": st.shadowRays, "iterations": st.iterations,
+                "finished": bool(st.finished)}
+
+    def reset(self):
+        check(lib().mray_renderer_reset(self.h))
+
+    def film(self):
+        out = np.zeros((4, self.height, self.width), np.float32)
+        check(lib().mray_renderer_read_film(self.h, out.ctypes.data))
+        return out

@@ -1,0 +1,471 @@
+"""Seeded procedural scenes for the path-tracing workloads.
+
+Scenes are plain numpy arrays in the layout of MRaySceneDesc
+(include/mray_amd.h): indexed triangles, per-vertex world->tangent
+quaternions (the reference stores vertex normals as TBN quaternions,
+PrimitiveDefaultTriangle.cu:179), a surface key per triangle (material index
+or MRAY_LIGHT_FLAG | light index), back-face culling flags (the scene loader's
+default is `cullBackFace: true`, SceneLoaderMRay/JsonNode.hpp:32; light
+surfaces never cull, Tracer/AcceleratorCommon.cu:274), a material table,
+emissive triangles and a boundary light.  World is Y-up, right-handed.
+
+Random draws come from a bit-exact std::mt19937 (MT19937 below); a uniform
+real in [0, 1) is u32 / 2^32.
+"""
+import ctypes as C
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import abi
+
+
+# ---------------------------------------------------------------------------
+# std::mt19937 (ISO C++ [rand.eng.mers]); checked against the standard's
+# 10000th-output value in tests.
+# ---------------------------------------------------------------------------
+class MT19937:
+    def __init__(self, seed):
+        self.mt = [0] * 624
+        self.mt[0] = seed & 0xFFFFFFFF
+        for i in range(1, 624):
+            prev = self.mt[i - 1]
+            self.mt[i] = (1812433253 * (prev ^ (prev >> 30)) + i) & 0xFFFFFFFF
+        self.index = 624
+
+    def _twist(self):
+        mt = self.mt
+        for i in range(624):
+            y = (mt[i] & 0x80000000) | (mt[(i + 1) % 624] & 0x7FFFFFFF)
+            v = mt[(i + 397) % 624] ^ (y >> 1)
+            if y & 1:
+                v ^= 0x9908B0DF
+            mt[i] = v
+        self.index = 0
+
+    def __call__(self):
+        if self.index >= 624:
+            self._twist()
+        y = self.mt[self.index]
+        self.index += 1
+        y ^= (y >> 11)
+        y ^= (y << 7) & 0x9D2C5680
+        y ^= (y << 15) & 0xEFC60000
+        y ^= (y >> 18)
+        return y & 0xFFFFFFFF
+
+    def uniform(self, lo=0.0, hi=1.0):
+        return lo + (hi - lo) * (self() / 4294967296.0)
+
+
+# ---------------------------------------------------------------------------
+# Geometry helpers
+# ---------------------------------------------------------------------------
+def quat_world_to_tangent(n):
+    """Unit quaternions (w,x,y,z) q with R(q) n = +Z, for unit normals n [k,3].
+
+    q = normalize(1 + n.z, n x z); a normal facing -Z uses a half turn about X.
+    """
+    n = np.asarray(n, dtype=np.float64)
+    c = n[:, 2]
+    axis = np.stack([n[:, 1], -n[:, 0], np.zeros_like(c)], axis=1)   # n x (0,0,1)
+    q = np.concatenate([(1.0 + c)[:, None], axis], axis=1)
+    flip = c < -1.0 + 1e-7
+    q[flip] = np.array([0.0, 1.0, 0.0, 0.0])
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return q.astype(np.float32)
+
+
+def face_normal(p0, p1, p2):
+    n = np.cross(p1 - p0, p2 - p0)
+    return n / np.linalg.norm(n, axis=-1, keepdims=True)
+
+
+@dataclass
+class Scene:
+    positions: np.ndarray                  # [V,3] f32
+    tbn: np.ndarray                        # [V,4] f32
+    indices: np.ndarray                    # [T,3] u32
+    surface_keys: np.ndarray               # [T] u32
+    cull: np.ndarray                       # [T] u8
+    materials: list                        # list of dicts
+    light_tris: np.ndarray                 # [L] u32
+    light_radiance: np.ndarray             # [L,3] f32
+    light_two_sided: np.ndarray            # [L] u8
+    boundary_type: int = abi.BOUNDARY_NULL
+    boundary_radiance: tuple = (0.0, 0.0, 0.0)
+    scene_diameter: float = 0.0
+    camera: dict = field(default_factory=dict)
+    name: str = ""
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def triangle_count(self):
+        return int(self.indices.shape[0])
+
+    def desc(self):
+        """MRaySceneDesc pointing at this scene's arrays (kept alive by self)."""
+        arrs = {
+            "pos": np.ascontiguousarray(self.positions, dtype=np.float32),
+            "tbn": np.ascontiguousarray(self.tbn, dtype=np.float32),
+            "idx": np.ascontiguousarray(self.indices, dtype=np.uint32),
+            "key": np.ascontiguousarray(self.surface_keys, dtype=np.uint32),
+            "cull": np.ascontiguousarray(self.cull, dtype=np.uint8),
+            "ltri": np.ascontiguousarray(self.light_tris, dtype=np.uint32),
+            "lrad": np.ascontiguousarray(self.light_radiance, dtype=np.float32),
+            "l2s": np.ascontiguousarray(self.light_two_sided, dtype=np.uint8),
+        }
+        mats = (abi.MRayMaterial * max(1, len(self.materials)))()
+        for i, m in enumerate(self.materials):
+            mats[i].type = m["type"]
+            for k in range(3):
+                mats[i].albedo[k] = m.get("albedo", (0.0, 0.0, 0.0))[k]
+                mats[i].cauchyFront[k] = m.get("cauchyFront", (1.0, 0.0, 0.0))[k]
+                mats[i].cauchyBack[k] = m.get("cauchyBack", (1.0, 0.0, 0.0))[k]
+            mats[i].roughness = m.get("roughness", 0.0)
+            mats[i].metallic = m.get("metallic", 0.0)
+            mats[i].specular = m.get("specular", 0.0)
+        d = abi.MRaySceneDesc()
+        d.vertexCount = arrs["pos"].shape[0]
+        d.triangleCount = arrs["idx"].shape[0]
+        d.hPositions = abi.fptr(arrs["pos"])
+        d.hTbnQuats = abi.fptr(arrs["tbn"])
+        d.hIndices = abi.u32ptr(arrs["idx"])
+        d.hSurfaceKeys = abi.u32ptr(arrs["key"])
+        d.hCullBackFace = abi.u8ptr(arrs["cull"])
+        d.materialCount = len(self.materials)
+        d.hMaterials = C.cast(mats, C.POINTER(abi.MRayMaterial))
+        d.lightCount = arrs["ltri"].shape[0]
+        d.hLightTriangles = abi.u32ptr(arrs["ltri"])
+        d.hLightRadiance = abi.fptr(arrs["lrad"])
+        d.hLightTwoSided = abi.u8ptr(arrs["l2s"])
+        d.boundaryType = self.boundary_type
+        for k in range(3):
+            d.boundaryRadiance[k] = self.boundary_radiance[k]
+        d.sceneDiameter = self.scene_diameter
+        cam = self.camera
+        for k in range(3):
+            d.camera.position[k] = cam["position"][k]
+            d.camera.gaze[k] = cam["gaze"][k]
+            d.camera.up[k] = cam["up"][k]
+        d.camera.fov[0] = cam["fov"][0]
+        d.camera.fov[1] = cam["fov"][1]
+        d.camera.nearFar[0] = cam["nearFar"][0]
+        d.camera.nearFar[1] = cam["nearFar"][1]
+        self._keep = [arrs, mats]
+        return d
+
+
+class _Builder:
+    """Accumulates triangles with their own vertices (flat or smooth)."""
+
+    def __init__(self):
+        self.pos = []
+        self.nrm = []
+        self.idx = []
+        self.key = []
+        self.cull = []
+        self.nv = 0
+
+    def add_mesh(self, positions, indices, key, cull, normals=None):
+        positions = np.asarray(positions, dtype=np.float64)
+        indices = np.asarray(indices, dtype=np.int64)
+        if normals is None:
+            # flat: one vertex set per triangle
+            p = positions[indices.reshape(-1)].reshape(-1, 3, 3)
+            n = face_normal(p[:, 0], p[:, 1], p[:, 2])
+            positions = p.reshape(-1, 3)
+            normals = np.repeat(n, 3, axis=0)
+            indices = np.arange(positions.shape[0]).reshape(-1, 3)
+        self.pos.append(positions)
+        self.nrm.append(np.asarray(normals, dtype=np.float64))
+        self.idx.append(indices + self.nv)
+        t = indices.shape[0]
+        keys = np.broadcast_to(np.asarray(key, dtype=np.uint64), (t,)).astype(np.uint32)
+        self.key.append(keys)
+        self.cull.append(np.broadcast_to(np.asarray(cull, dtype=np.uint8), (t,)).copy())
+        self.nv += positions.shape[0]
+        return sum(a.shape[0] for a in self.idx) - t   # first triangle index
+
+    def finish(self):
+        pos = np.concatenate(self.pos).astype(np.float32)
+        nrm = np.concatenate(self.nrm)
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+        return (pos, quat_world_to_tangent(nrm),
+                np.concatenate(self.idx).astype(np.uint32),
+                np.concatenate(self.key).astype(np.uint32),
+                np.concatenate(self.cull).astype(np.uint8))
+
+
+def _quad(c, u, v):
+    """Two CCW triangles of the quad c +- u +- v; normal = u x v."""
+    c, u, v = (np.asarray(x, dtype=np.float64) for x in (c, u, v))
+    p = np.stack([c - u - v, c + u - v, c + u + v, c - u + v])
+    return p, np.array([[0, 1, 2], [0, 2, 3]])
+
+
+def _box(center, half, angle_y):
+    """12 outward-facing triangles of a box rotated about +Y."""
+    cx, cy, cz = center
+    hx, hy, hz = half
+    ca, sa = math.cos(angle_y), math.sin(angle_y)
+    rot = np.array([[ca, 0.0, sa], [0.0, 1.0, 0.0], [-sa, 0.0, ca]])
+    faces = [((1, 0, 0), (0, 0, -1), (0, 1, 0)),
+             ((-1, 0, 0), (0, 0, 1), (0, 1, 0)),
+             ((0, 1, 0), (1, 0, 0), (0, 0, -1)),
+             ((0, -1, 0), (1, 0, 0), (0, 0, 1)),
+             ((0, 0, 1), (1, 0, 0), (0, 1, 0)),
+             ((0, 0, -1), (-1, 0, 0), (0, 1, 0))]
+    pts, tris = [], []
+    h = np.array([hx, hy, hz])
+    for n, u, v in faces:
+        n, u, v = np.array(n, float), np.array(u, float), np.array(v, float)
+        p, t = _quad(n * h, u * h, v * h)
+        tris.append(t + len(pts) * 4)
+        pts.append(p)
+    p = np.concatenate(pts) @ rot.T + np.array([cx, cy, cz])
+    return p, np.concatenate(tris)
+
+
+def fov_y_from_x(fov_x, aspect):
+    """SceneLoaderMRay.cpp:595-662: fovY = 2 atan(tan(fovX / 2) / aspect)."""
+    return 2.0 * math.atan(math.tan(fov_x * 0.5) / aspect)
+
+
+# ---------------------------------------------------------------------------
+# Cornell-like box: 5 wall quads (10 tris), 2 boxes (24 tris) rotated about Y
+# by mt19937(1) angles in [0, 30 deg], one 0.25 x 0.25 ceiling light (2 tris,
+# radiance (17, 12, 4)); Lambert red / green / white 0.73.
+# ---------------------------------------------------------------------------
+def cornell(aspect=1.0, fov_deg=40.0):
+    rng = MT19937(1)
+    a0 = math.radians(rng.uniform(0.0, 30.0))
+    a1 = math.radians(rng.uniform(0.0, 30.0))
+    mats = [
+        {"type": abi.MAT_LAMBERT, "albedo": (0.73, 0.73, 0.73)},   # 0 white
+        {"type": abi.MAT_LAMBERT, "albedo": (0.63, 0.065, 0.05)},  # 1 red
+        {"type": abi.MAT_LAMBERT, "albedo": (0.14, 0.45, 0.09)},   # 2 green
+    ]
+    b = _Builder()
+    # walls, normals facing into the box
+    b.add_mesh(*_quad((0, -1, 0), (1, 0, 0), (0, 0, -1)), key=0, cull=1)   # floor, +Y
+    b.add_mesh(*_quad((0, 1, 0), (1, 0, 0), (0, 0, 1)), key=0, cull=1)     # ceiling, -Y
+    b.add_mesh(*_quad((0, 0, -1), (1, 0, 0), (0, 1, 0)), key=0, cull=1)    # back, +Z
+    b.add_mesh(*_quad((-1, 0, 0), (0, 0, -1), (0, 1, 0)), key=1, cull=1)   # left red, +X
+    b.add_mesh(*_quad((1, 0, 0), (0, 0, 1), (0, 1, 0)), key=2, cull=1)     # right green, -X
+    b.add_mesh(*_box((0.33, -0.7, 0.3), (0.3, 0.3, 0.3), a0), key=0, cull=1)
+    b.add_mesh(*_box((-0.33, -0.4, -0.3), (0.3, 0.6, 0.3), a1), key=0, cull=1)
+    # ceiling light, facing -Y, just below the ceiling
+    first = b.add_mesh(*_quad((0, 0.999, 0), (0.125, 0, 0), (0, 0, 0.125)),
+                       key=[abi.LIGHT_FLAG | 0, abi.LIGHT_FLAG | 1], cull=0)
+    pos, tbn, idx, key, cull = b.finish()
+    fov = math.radians(fov_deg)
+    cam = {"position": (0.0, 0.0, 3.5), "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
+           "fov": (fov, fov_y_from_x(fov, aspect)), "nearFar": (0.1, 1000.0)}
+    return Scene(positions=pos, tbn=tbn, indices=idx, surface_keys=key, cull=cull,
+                 materials=mats,
+                 light_tris=np.array([first, first + 1], dtype=np.uint32),
+                 light_radiance=np.array([[17, 12, 4], [17, 12, 4]], dtype=np.float32),
+                 light_two_sided=np.zeros(2, dtype=np.uint8),
+                 boundary_type=abi.BOUNDARY_NULL, scene_diameter=2.0 * math.sqrt(3.0),
+                 camera=cam, name="cornell")
+
+
+# ---------------------------------------------------------------------------
+# Heightfield + icospheres (1M-triangle class workload)
+# ---------------------------------------------------------------------------
+def _perlin_table(rng):
+    perm = list(range(256))
+    for i in range(255, 0, -1):               # Fisher-Yates with mt19937 draws
+        j = rng() % (i + 1)
+        perm[i], perm[j] = perm[j], perm[i]
+    ang = np.array([rng.uniform(0.0, 2.0 * math.pi) for _ in range(256)])
+    grads = np.stack([np.cos(ang), np.sin(ang)], axis=1)
+    return np.array(perm + perm, dtype=np.int64), grads
+
+
+def _perlin2(x, y, perm, grads):
+    xi = np.floor(x).astype(np.int64)
+    yi = np.floor(y).astype(np.int64)
+    xf = x - xi
+    yf = y - yi
+    xi &= 255
+    yi &= 255
+
+    def g(ix, iy, dx, dy):
+        h = perm[perm[ix] + iy] & 255
+        return grads[h, 0] * dx + grads[h, 1] * dy
+
+    def fade(t):
+        return t * t * t * (t * (t * 6 - 15) + 10)
+
+    u, v = fade(xf), fade(yf)
+    n00 = g(xi, yi, xf, yf)
+    n10 = g(xi + 1, yi, xf - 1, yf)
+    n01 = g(xi, yi + 1, xf, yf - 1)
+    n11 = g(xi + 1, yi + 1, xf - 1, yf - 1)
+    nx0 = n00 + u * (n10 - n00)
+    nx1 = n01 + u * (n11 - n01)
+    return nx0 + v * (nx1 - nx0)
+
+
+def _fbm(x, y, perm, grads, octaves=6):
+    total = np.zeros_like(x)
+    amp, freq, norm = 1.0, 2.0, 0.0
+    for _ in range(octaves):
+        total += amp * _perlin2(x * freq, y * freq, perm, grads)
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return total / norm
+
+
+def _icosphere1():
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0],
+                  [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], dtype=np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = [[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11],
+         [1, 5, 9], [5, 11, 4], [11, 10, 2], [10, 7, 6], [7, 1, 8],
+         [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9],
+         [4, 9, 5], [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]]
+    verts = list(map(tuple, v))
+    cache = {}
+
+    def mid(a, b):
+        k = (min(a, b), max(a, b))
+        if k not in cache:
+            m = (np.array(verts[a]) + np.array(verts[b])) * 0.5
+            m /= np.linalg.norm(m)
+            verts.append(tuple(m))
+            cache[k] = len(verts) - 1
+        return cache[k]
+
+    faces = []
+    for a, b, c in f:
+        ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+        faces += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+    return np.array(verts), np.array(faces, dtype=np.int64)
+
+
+def terrain_spheres(grid=1024, spheres=2000, width=1920, height=1080, fov_x_deg=60.0,
+                    light_grid=4):
+    """1024 x 1024 vertex heightfield (0.2 x 6-octave fBm, mt19937(2)) plus
+    `spheres` icospheres (80 tris, radius log-U[0.01, 0.1], mt19937(3)).
+    Materials U{0..3} (mt19937(4)) per terrain patch / sphere: Lambert
+    (albedo U[0.1,0.9]^3), Unreal metallic=1 roughness U[0.05,0.5], Refract
+    cauchyBack [1.5,0,0], Unreal metallic=0 specular 0.5 roughness 0.3.
+    Constant Skysphere (0.5) boundary and light_grid^2 downward quad lights
+    (radiance 10) at height 1.5."""
+    rng_terrain = MT19937(2)
+    perm, grads = _perlin_table(rng_terrain)
+    xs = np.linspace(-1.0, 1.0, grid)
+    X, Z = np.meshgrid(xs, xs, indexing="xy")
+    Y = 0.2 * _fbm(X, Z, perm, grads)
+    # smooth normals by central differences
+    dx = np.gradient(Y, xs, axis=1)
+    dz = np.gradient(Y, xs, axis=0)
+    nrm = np.stack([-dx, np.ones_like(Y), -dz], axis=-1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    tpos = np.stack([X, Y, Z], axis=-1).reshape(-1, 3)
+    tnrm = nrm.reshape(-1, 3)
+    ii, jj = np.meshgrid(np.arange(grid - 1), np.arange(grid - 1), indexing="xy")
+    v00 = (jj * grid + ii).reshape(-1)
+    v10 = v00 + 1
+    v01 = v00 + grid
+    v11 = v01 + 1
+    # CCW seen from +Y: (v00, v01, v11), (v00, v11, v10)
+    tris = np.concatenate([np.stack([v00, v01, v11], 1), np.stack([v00, v11, v10], 1)], 0)
+    cell = np.concatenate([np.stack([ii.reshape(-1), jj.reshape(-1)], 1)] * 2, 0)
+
+    rng_mat = MT19937(4)
+    mats = []
+
+    def new_material():
+        kind = rng_mat() % 4
+        if kind == 0:
+            m = {"type": abi.MAT_LAMBERT,
+                 "albedo": tuple(rng_mat.uniform(0.1, 0.9) for _ in range(3))}
+        elif kind == 1:
+            alb = tuple(rng_mat.uniform(0.1, 0.9) for _ in range(3))
+            m = {"type": abi.MAT_UNREAL, "albedo": alb, "metallic": 1.0,
+                 "roughness": rng_mat.uniform(0.05, 0.5), "specular": 0.5}
+        elif kind == 2:
+            m = {"type": abi.MAT_REFRACT, "cauchyFront": (1.0, 0.0, 0.0),
+                 "cauchyBack": (1.5, 0.0, 0.0)}
+        else:
+            alb = tuple(rng_mat.uniform(0.1, 0.9) for _ in range(3))
+            m = {"type": abi.MAT_UNREAL, "albedo": alb, "metallic": 0.0,
+                 "specular": 0.5, "roughness": 0.3}
+        mats.append(m)
+        return len(mats) - 1
+
+    patch = 64
+    npatch = (grid - 1 + patch - 1) // patch
+    patch_mat = np.array([new_material() for _ in range(npatch * npatch)], dtype=np.uint32)
+    tkey = patch_mat[(cell[:, 1] // patch) * npatch + (cell[:, 0] // patch)]
+    tcull = np.array([0 if mats[k]["type"] == abi.MAT_REFRACT else 1 for k in tkey],
+                     dtype=np.uint8)
+
+    b = _Builder()
+    b.add_mesh(tpos, tris, key=tkey, cull=tcull, normals=tnrm)
+
+    ico_v, ico_f = _icosphere1()
+    rng_sph = MT19937(3)
+    hf = Y.reshape(-1)
+    for _ in range(spheres):
+        px = rng_sph.uniform(-0.95, 0.95)
+        pz = rng_sph.uniform(-0.95, 0.95)
+        r = math.exp(rng_sph.uniform(math.log(0.01), math.log(0.1)))
+        gi = int(round((px + 1.0) * 0.5 * (grid - 1)))
+        gj = int(round((pz + 1.0) * 0.5 * (grid - 1)))
+        py = float(hf[gj * grid + gi]) + r * 0.8
+        k = new_material()
+        cull = 0 if mats[k]["type"] == abi.MAT_REFRACT else 1
+        b.add_mesh(ico_v * r + np.array([px, py, pz]), ico_f, key=k, cull=cull, normals=ico_v)
+
+    light_tris = []
+    lsize = 0.06
+    for j in range(light_grid):
+        for i in range(light_grid):
+            cx = -0.75 + 1.5 * i / max(1, light_grid - 1)
+            cz = -0.75 + 1.5 * j / max(1, light_grid - 1)
+            li = len(light_tris)
+            first = b.add_mesh(*_quad((cx, 1.5, cz), (lsize, 0, 0), (0, 0, lsize)),
+                               key=[abi.LIGHT_FLAG | li, abi.LIGHT_FLAG | (li + 1)], cull=0)
+            light_tris += [first, first + 1]
+    pos, tbn, idx, key, cull = b.finish()
+    nl = len(light_tris)
+    aspect = width / height
+    fov = math.radians(fov_x_deg)
+    cam = {"position": (0.0, 1.1, 2.1), "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
+           "fov": (fov, fov_y_from_x(fov, aspect)), "nearFar": (0.01, 1000.0)}
+    return Scene(positions=pos, tbn=tbn, indices=idx, surface_keys=key, cull=cull,
+                 materials=mats, light_tris=np.array(light_tris, dtype=np.uint32),
+                 light_radiance=np.full((nl, 3), 10.0, dtype=np.float32),
+                 light_two_sided=np.zeros(nl, dtype=np.uint8),
+                 boundary_type=abi.BOUNDARY_SKYSPHERE_COOCTA,
+                 boundary_radiance=(0.5, 0.5, 0.5), scene_diameter=2.0 * math.sqrt(3.0) * 2.0,
+                 camera=cam, name=f"terrain{grid}_spheres{spheres}")
+
+
+def random_triangles(n, seed=7, scale=1.0):
+    """Triangle soup for traversal tests: n random small triangles in a cube."""
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(-scale, scale, size=(n, 1, 3))
+    p = c + rng.normal(scale=0.05 * scale, size=(n, 3, 3))
+    pos = p.reshape(-1, 3).astype(np.float32)
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    nrm = face_normal(p[:, 0], p[:, 1], p[:, 2])
+    tbn = quat_world_to_tangent(np.repeat(nrm, 3, axis=0))
+    cull = (rng.uniform(size=n) < 0.5).astype(np.uint8)
+    cam = {"position": (0.0, 0.0, 3.0), "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
+           "fov": (math.radians(60), math.radians(60)), "nearFar": (0.01, 100.0)}
+    return Scene(positions=pos, tbn=tbn, indices=idx,
+                 surface_keys=np.zeros(n, dtype=np.uint32), cull=cull,
+                 materials=[{"type": abi.MAT_LAMBERT, "albedo": (0.5, 0.5, 0.5)}],
+                 light_tris=np.zeros(0, dtype=np.uint32),
+                 light_radiance=np.zeros((0, 3), dtype=np.float32),
+                 light_two_sided=np.zeros(0, dtype=np.uint8), camera=cam, name="soup")

@@ -1,0 +1,110 @@
+"""Closest-hit and visibility casting on the GPU vs the CPU oracle.
+
+Hit triangle ids, t and barycentrics must be bit-identical for identical rays
+(Moller-Trumbore as Ray.hpp:121-167; closest-t, then smaller triangle index).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mray_amd import abi, scenes
+from oracle import refcpu
+
+pytestmark = pytest.mark.gpu
+
+
+def _random_rays(n, seed, origin_scale=2.0):
+    rng = np.random.default_rng(seed)
+    o = rng.uniform(-origin_scale, origin_scale, size=(n, 3))
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3] = o
+    rays[:, 3] = 1e-4
+    rays[:, 4:7] = d
+    rays[:, 7] = 3.4e38
+    # a few short rays and axis-aligned directions
+    rays[: n // 10, 7] = rng.uniform(0.1, 2.0, size=n // 10)
+    rays[n // 10: n // 10 + 6, 4:7] = np.array([[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0],
+                                                [0, 0, 1], [0, 0, -1]], np.float32)
+    return rays
+
+
+def _compare(acc, oscene, rays, brute):
+    import torch
+    d = torch.from_numpy(rays).cuda()
+    prim, t, bary = acc.cast_closest(d)
+    prim = prim.cpu().numpy().view(np.uint32)
+    t = t.cpu().numpy()
+    bary = bary.cpu().numpy()
+    rp, rt, rb = oscene.cast_closest(rays, brute=brute)
+    hit = rp != abi.INVALID_PRIM
+    print(f"rays={len(rays)} hits={int(hit.sum())}", flush=True)
+    assert np.array_equal(prim, rp)
+    assert np.array_equal(t.view(np.uint32), rt.view(np.uint32))
+    assert np.array_equal(bary[hit].view(np.uint32), rb[hit].view(np.uint32))
+    vis = acc.cast_visibility(d).cpu().numpy()
+    rvis = oscene.cast_visibility(rays)
+    assert np.array_equal(vis.astype(np.uint32), rvis)
+    # any-hit agrees with closest hit
+    assert np.array_equal(vis == 1, ~hit)
+
+
+def test_closest_soup_vs_bruteforce(torch_gpu):
+    from mray_amd import native
+    s = scenes.random_triangles(3000, seed=9)
+    acc = native.Accel(s)
+    _compare(acc, refcpu.OracleScene(s), _random_rays(20000, 1), brute=True)
+
+
+def test_closest_soup_large(torch_gpu):
+    from mray_amd import native
+    s = scenes.random_triangles(200000, seed=4)
+    acc = native.Accel(s)
+    _compare(acc, refcpu.OracleScene(s), _random_rays(100000, 2), brute=False)
+
+
+def test_primary_rays_cornell(torch_gpu):
+    """Primary camera rays of the Cornell workload (identical rays, from the oracle)."""
+    from mray_amd import native
+    s = scenes.cornell()
+    acc = native.Accel(s)
+    o = refcpu.OracleScene(s)
+    params = abi.render_params(256, 256, 1)
+    rays, _ = o.camera_rays(params, 256 * 256)
+    _compare(acc, o, rays, brute=True)
+
+
+def test_primary_rays_terrain(torch_gpu):
+    from mray_amd import native
+    s = scenes.terrain_spheres(grid=256, spheres=300, width=320, height=180)
+    acc = native.Accel(s)
+    o = refcpu.OracleScene(s)
+    params = abi.render_params(320, 180, 1)
+    rays, _ = o.camera_rays(params, 320 * 180)
+    _compare(acc, o, rays, brute=False)
+
+
+def test_single_triangle_and_edges(torch_gpu):
+    """One-triangle scene (single-leaf LBVH case) and rays through edges/vertices."""
+    from mray_amd import native
+    pos = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    s = scenes.Scene(positions=pos, tbn=scenes.quat_world_to_tangent(np.tile([[0, 0, 1.0]], (3, 1))),
+                     indices=np.array([[0, 1, 2]], np.uint32),
+                     surface_keys=np.zeros(1, np.uint32), cull=np.zeros(1, np.uint8),
+                     materials=[{"type": abi.MAT_LAMBERT, "albedo": (0.5, 0.5, 0.5)}],
+                     light_tris=np.zeros(0, np.uint32), light_radiance=np.zeros((0, 3), np.float32),
+                     light_two_sided=np.zeros(0, np.uint8),
+                     camera={"position": (0, 0, 2), "gaze": (0, 0, 0), "up": (0, 1, 0),
+                             "fov": (math.radians(40),) * 2, "nearFar": (0.01, 100.0)})
+    acc = native.Accel(s)
+    xs = np.linspace(-0.25, 1.25, 61, dtype=np.float32)
+    g = np.array([[x, y] for x in xs for y in xs], np.float32)
+    rays = np.zeros((len(g), 8), np.float32)
+    rays[:, 0:2] = g
+    rays[:, 2] = 1.0
+    rays[:, 3] = 0.0
+    rays[:, 6] = -1.0
+    rays[:, 7] = 10.0
+    _compare(acc, refcpu.OracleScene(s), rays, brute=True)
