@@ -130,6 +130,8 @@ typedef struct MRayRenderStats
     uint64_t shadowRays;      /* visibility rays cast in this call       */
     uint32_t iterations;      /* bounces run in this call                */
     uint32_t finished;        /* all totalSPP samples are in the film    */
+    uint64_t nodeVisits;      /* BVH nodes visited (counting mode)       */
+    uint64_t triTests;        /* triangles tested (counting mode)        */
 } MRayRenderStats;
 
 /* ---------------------------------------------------------------------- */
@@ -223,8 +225,24 @@ int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* hStats);
 int mray_renderer_read_film(MRayRenderer* r, float* hOut);
 /* Reset the film and all path slots (StartRender again, same seed).        */
 int mray_renderer_reset(MRayRenderer* r);
+/* Measurement (no reference counterpart): profiling = HIP events around the
+ * refill / trace / shade / shadow phases on the launch stream; counting =
+ * closest-hit traversal also counts nodes visited and triangles tested
+ * (reported in MRayRenderStats).  Resets the accumulated phase times.       */
+int mray_renderer_set_measurement(MRayRenderer* r, uint32_t profiling, uint32_t counting);
+/* Accumulated phase times hMs[4] (ms) and launch counts hLaunches[4] for
+ * refill, trace, shade, shadow.                                              */
+int mray_renderer_phase_times(MRayRenderer* r, double* hMs, uint64_t* hLaunches);
 /* Device pointer of the film [4][height][width] (stays owned by r).         */
 int mray_renderer_film_device(MRayRenderer* r, float** dFilm);
+
+/* ---------------------------------------------------------------------- */
+/* Spectral tables (SpectraLUTGen capability, SpectrumContext.cu:298-472):  */
+/* RGB->spectrum coefficients lut [3][N][N][N][4] (c0, c1, c2, 0 in nm,     */
+/* x fastest), observer cmf [471][3] and illuminant [471] at 360..830 nm,   */
+/* XYZ->RGB [9].  Built on first use and cached next to the library.        */
+/* ---------------------------------------------------------------------- */
+int mray_spectral_tables(int* hN, float* hLut, float* hCmf, float* hIllum, float* hXyzToRgb);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,8 @@ class MRayRenderStats(C.Structure):
         ("shadowRays", C.c_uint64),
         ("iterations", C.c_uint32),
         ("finished", C.c_uint32),
+        ("nodeVisits", C.c_uint64),
+        ("triTests", C.c_uint64),
     ]
 
 

@@ -75,6 +75,8 @@ struct Sched
     unsigned long long deadTotal;
     unsigned long long extRays;
     unsigned long long shadowRays;
+    unsigned long long nodeVisits;  // counting mode only
+    unsigned long long triTests;    // counting mode only
     uint32_t fill;
     uint32_t activeCount;
     uint32_t iterations;
@@ -233,6 +235,7 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ blockOff,
 // ---------------------------------------------------------------------------
 // Closest hit
 // ---------------------------------------------------------------------------
+template<bool COUNT>
 __global__ void __launch_bounds__(kTraceTPB)
 KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __restrict__ sched)
 {
@@ -240,14 +243,20 @@ KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __re
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = sched->activeCount;
     if(gtid == 0) sched->extRays += n;
+    uint32_t nodes = 0, tris = 0;
     for(uint32_t k = gtid; k < n; k += ctx.gridThreads)
     {
         uint32_t i = active[k];
         float4 o = S.rayO[i], d = S.rayD[i];
-        HitRec h = Trace<false, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                       o.w, d.w);
+        HitRec h = Trace<false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                       o.w, d.w, &nodes, &tris);
         S.hit[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.a),
                               __float_as_uint(h.b));
+    }
+    if(COUNT && (nodes | tris))
+    {
+        atomicAdd(&sched->nodeVisits, (unsigned long long)nodes);
+        atomicAdd(&sched->triTests, (unsigned long long)tris);
     }
 }
 
@@ -669,6 +678,8 @@ __global__ void KResetSlots(uint32_t* __restrict__ meta, uint32_t* __restrict__ 
         sched->deadTotal = 0;
         sched->extRays = 0;
         sched->shadowRays = 0;
+        sched->nodeVisits = 0;
+        sched->triTests = 0;
         sched->fill = 0;
         sched->activeCount = 0;
         sched->iterations = 0;
@@ -700,33 +711,85 @@ struct MRayRenderer
     Sched last{};                 // stats at the previous mray_renderer_stats()
     hipStream_t stream = nullptr;
     SpectralTables spectral;
+    // measurement
+    bool profiling = false;
+    bool counting = false;
+    std::vector<hipEvent_t> events;
+    size_t evUsed = 0;
+    double phaseMs[4] = {0, 0, 0, 0};
+    uint64_t phaseLaunches[4] = {0, 0, 0, 0};
 };
+
+// Phase timing with HIP events on the launch stream (refill, trace, shade,
+// shadow); harvested when statistics are read.
+static int Mark(MRayRenderer* r, hipStream_t s)
+{
+    if(!r->profiling) return MRAY_OK;
+    if(r->evUsed == r->events.size())
+    {
+        hipEvent_t e;
+        MRAY_HIP_CHECK(hipEventCreate(&e));
+        r->events.push_back(e);
+    }
+    MRAY_HIP_CHECK(hipEventRecord(r->events[r->evUsed++], s));
+    return MRAY_OK;
+}
+
+static int HarvestEvents(MRayRenderer* r)
+{
+    // groups of 5 marks per iteration: phase k spans marks k .. k+1
+    size_t groups = r->evUsed / 5;
+    for(size_t g = 0; g < groups; g++)
+        for(int k = 0; k < 4; k++)
+        {
+            float ms = 0.0f;
+            MRAY_HIP_CHECK(hipEventElapsedTime(&ms, r->events[5 * g + k], r->events[5 * g + k + 1]));
+            r->phaseMs[k] += ms;
+            r->phaseLaunches[k] += 1;
+        }
+    r->evUsed = 0;
+    return MRAY_OK;
+}
 
 static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
 {
     const uint32_t N = r->rc.N;
     const TraceCtx& ctx = r->accel->ctx;
+    if(r->profiling && r->evUsed + 5 > 5 * 4096)
+    {
+        MRAY_HIP_CHECK(hipStreamSynchronize(s));
+        int rv = HarvestEvents(r);
+        if(rv != MRAY_OK) return rv;
+    }
+    int mk;
+    if((mk = Mark(r, s)) != MRAY_OK) return mk;
     KRefillCount<<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, N, r->dBlockCnt, r->dSched);
     KRefillScan<<<1, 1024, 0, s>>>(r->dBlockCnt, r->nBlocks, r->dSched);
     uint32_t traceBlocks = ctx.gridThreads / kTraceTPB;
     if(r->rc.spectral)
-    {
         KRefillApply<true><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
                                                            r->dActive);
-        KTrace<<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
-        KShade<true><<<r->shadeGrid / kSlotTPB, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive,
-                                                                  r->dSched, r->shadeGrid);
-        KShadow<true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->rc, r->dActive, r->dSched);
-    }
     else
-    {
         KRefillApply<false><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
                                                             r->dActive);
-        KTrace<<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+    if((mk = Mark(r, s)) != MRAY_OK) return mk;
+    if(r->counting)
+        KTrace<true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+    else
+        KTrace<false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+    if((mk = Mark(r, s)) != MRAY_OK) return mk;
+    if(r->rc.spectral)
+        KShade<true><<<r->shadeGrid / kSlotTPB, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive,
+                                                                  r->dSched, r->shadeGrid);
+    else
         KShade<false><<<r->shadeGrid / kSlotTPB, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive,
                                                                    r->dSched, r->shadeGrid);
+    if((mk = Mark(r, s)) != MRAY_OK) return mk;
+    if(r->rc.spectral)
+        KShadow<true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->rc, r->dActive, r->dSched);
+    else
         KShadow<false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->rc, r->dActive, r->dSched);
-    }
+    if((mk = Mark(r, s)) != MRAY_OK) return mk;
     MRAY_HIP_CHECK(hipGetLastError());
     return MRAY_OK;
 }
@@ -903,6 +966,7 @@ int mray_renderer_destroy(MRayRenderer* r)
 {
     if(!r) return MRAY_OK;
     (void)hipDeviceSynchronize();
+    for(hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if(r->hSched) (void)hipHostFree(r->hSched);
     r->arena.Release();
     delete r;
@@ -935,7 +999,10 @@ int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* out)
     out->shadowRays = c.shadowRays - r->last.shadowRays;
     out->iterations = c.iterations - r->last.iterations;
     out->finished = (c.deadTotal >= r->limit) ? 1u : 0u;
+    out->nodeVisits = c.nodeVisits - r->last.nodeVisits;
+    out->triTests = c.triTests - r->last.triTests;
     r->last = c;
+    if(r->profiling) return HarvestEvents(r);
     return MRAY_OK;
 }
 
@@ -952,6 +1019,8 @@ int mray_renderer_render_all(MRayRenderer* r, MRayRenderStats* hStats)
         total.extensionRays += st.extensionRays;
         total.shadowRays += st.shadowRays;
         total.iterations += st.iterations;
+        total.nodeVisits += st.nodeVisits;
+        total.triTests += st.triTests;
         total.totalDeadPaths = st.totalDeadPaths;
         if(st.finished) { total.finished = 1; break; }
         if(st.deadPaths == 0 && st.extensionRays == 0)
@@ -970,6 +1039,26 @@ int mray_renderer_read_film(MRayRenderer* r, float* hOut)
     MRAY_HIP_CHECK(hipDeviceSynchronize());
     MRAY_HIP_CHECK(hipMemcpy(hOut, r->rc.film, size_t(r->rc.P) * 4 * sizeof(float),
                              hipMemcpyDeviceToHost));
+    return MRAY_OK;
+}
+
+int mray_renderer_set_measurement(MRayRenderer* r, uint32_t profiling, uint32_t counting)
+{
+    if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    r->evUsed = 0;
+    r->profiling = profiling != 0;
+    r->counting = counting != 0;
+    for(int k = 0; k < 4; k++) { r->phaseMs[k] = 0.0; r->phaseLaunches[k] = 0; }
+    return MRAY_OK;
+}
+
+int mray_renderer_phase_times(MRayRenderer* r, double* hMs, uint64_t* hLaunches)
+{
+    if(!r || !hMs || !hLaunches) { SetError("phase_times: NULL argument"); return MRAY_ERR_INVALID; }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    MRAY_TRY(HarvestEvents(r));
+    for(int k = 0; k < 4; k++) { hMs[k] = r->phaseMs[k]; hLaunches[k] = r->phaseLaunches[k]; }
     return MRAY_OK;
 }
 

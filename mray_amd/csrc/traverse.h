@@ -61,7 +61,10 @@ __device__ __forceinline__ bool IntersectTri(V3 pos, V3 dir, V3 t0, V3 t1, V3 t2
 
 struct RayPre
 {
-    V3 o, d, inv, oi;
+    V3 o, d, inv;
+    // float4 offsets (0 or 1) of the near plane per axis inside BNode4:
+    // lo when the direction is positive, hi when negative (or -0).
+    uint32_t sx, sy, sz;
 };
 
 __device__ __forceinline__ RayPre MakeRayPre(V3 o, V3 d)
@@ -69,36 +72,44 @@ __device__ __forceinline__ RayPre MakeRayPre(V3 o, V3 d)
     RayPre r;
     r.o = o;
     r.d = d;
-    const float tiny = 1e-20f;
-    float dx = (fabsf(d.x) > tiny) ? d.x : copysignf(tiny, d.x);
-    float dy = (fabsf(d.y) > tiny) ? d.y : copysignf(tiny, d.y);
-    float dz = (fabsf(d.z) > tiny) ? d.z : copysignf(tiny, d.z);
-    r.inv = {1.0f / dx, 1.0f / dy, 1.0f / dz};
-    r.oi = {o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z};
+    // exact IEEE reciprocals; a zero component gives +-inf
+    r.inv = {1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+    r.sx = signbit(r.inv.x) ? 1u : 0u;
+    r.sy = signbit(r.inv.y) ? 1u : 0u;
+    r.sz = signbit(r.inv.z) ? 1u : 0u;
     return r;
 }
 
-// Conservative slab test of the four child boxes.  tNear[k] = +inf on a miss.
+// Conservative slab test of the four child boxes (Ize 2013, "Robust BVH Ray
+// Traversal"): near/far planes picked by direction sign, t = (plane - o) * inv
+// with one rounding per operation, and the far bound scaled by 1 + 2*gamma(4).
+// A ray parallel to an axis whose origin lies exactly on a slab plane makes
+// (plane - o) * inv = 0 * inf = NaN; fmaxf/fminf drop the NaN operand, so
+// that plane leaves the interval unbounded -- the ray lies in the box face and
+// must reach its triangles.  tNear[k] = +inf on a miss.
 __device__ __forceinline__ void SlabTest4(const BNode4* __restrict__ np, const RayPre& r,
                                           float tMin, float tMax, float tNear[4], uint4& child)
 {
     const float4* q = reinterpret_cast<const float4*>(np);
-    float4 lox = q[0], hix = q[1], loy = q[2], hiy = q[3], loz = q[4], hiz = q[5];
+    float4 nx = q[0 + r.sx], fx = q[1 - r.sx];
+    float4 ny = q[2 + r.sy], fy = q[3 - r.sy];
+    float4 nz = q[4 + r.sz], fz = q[5 - r.sz];
     child = reinterpret_cast<const uint4*>(np)[6];
     const float grow = 1.0f + 2.0f * 5.96e-8f * 4.0f;
-    float lx[4] = {lox.x, lox.y, lox.z, lox.w}, hx[4] = {hix.x, hix.y, hix.z, hix.w};
-    float ly[4] = {loy.x, loy.y, loy.z, loy.w}, hy[4] = {hiy.x, hiy.y, hiy.z, hiy.w};
-    float lz[4] = {loz.x, loz.y, loz.z, loz.w}, hz[4] = {hiz.x, hiz.y, hiz.z, hiz.w};
+    float nX[4] = {nx.x, nx.y, nx.z, nx.w}, fX[4] = {fx.x, fx.y, fx.z, fx.w};
+    float nY[4] = {ny.x, ny.y, ny.z, ny.w}, fY[4] = {fy.x, fy.y, fy.z, fy.w};
+    float nZ[4] = {nz.x, nz.y, nz.z, nz.w}, fZ[4] = {fz.x, fz.y, fz.z, fz.w};
     #pragma unroll
     for(int k = 0; k < 4; k++)
     {
-        float x0 = Fma(lx[k], r.inv.x, -r.oi.x), x1 = Fma(hx[k], r.inv.x, -r.oi.x);
-        float y0 = Fma(ly[k], r.inv.y, -r.oi.y), y1 = Fma(hy[k], r.inv.y, -r.oi.y);
-        float z0 = Fma(lz[k], r.inv.z, -r.oi.z), z1 = Fma(hz[k], r.inv.z, -r.oi.z);
-        float tn = MaxF(MaxF(MinF(x0, x1), MinF(y0, y1)), MaxF(MinF(z0, z1), tMin));
-        float tf = MinF(MinF(MaxF(x0, x1), MaxF(y0, y1)), MinF(MaxF(z0, z1), tMax * grow)) * grow;
-        // empty slots have lo > hi, so tn > tf for any ray
-        tNear[k] = (tn <= tf && lx[k] <= hx[k]) ? tn : INFINITY;
+        float x0 = (nX[k] - r.o.x) * r.inv.x, x1 = (fX[k] - r.o.x) * r.inv.x;
+        float y0 = (nY[k] - r.o.y) * r.inv.y, y1 = (fY[k] - r.o.y) * r.inv.y;
+        float z0 = (nZ[k] - r.o.z) * r.inv.z, z1 = (fZ[k] - r.o.z) * r.inv.z;
+        float tn = MaxF(MaxF(tMin, x0), MaxF(y0, z0));
+        float tf = MinF(MinF(tMax, x1), MinF(y1, z1)) * grow;
+        // empty slots store lo = +inf, hi = -inf: their near bound is +inf
+        // and far bound -inf for every direction sign, so they always miss
+        tNear[k] = (tn <= tf) ? tn : INFINITY;
     }
 }
 

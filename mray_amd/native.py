@@ -30,7 +30,8 @@ EXPORTS = [
     "mray_accel_cast_closest", "mray_accel_cast_closest_counted", "mray_accel_cast_visibility",
     "mray_renderer_create", "mray_renderer_destroy", "mray_renderer_iterate",
     "mray_renderer_render_all", "mray_renderer_stats", "mray_renderer_read_film",
-    "mray_renderer_reset", "mray_renderer_film_device",
+    "mray_renderer_reset", "mray_renderer_film_device", "mray_spectral_tables",
+    "mray_renderer_set_measurement", "mray_renderer_phase_times",
 ]
 
 
@@ -73,6 +74,9 @@ def lib():
     L.mray_renderer_read_film.argtypes = [P, P]
     L.mray_renderer_reset.argtypes = [P]
     L.mray_renderer_film_device.argtypes = [P, C.POINTER(P)]
+    L.mray_spectral_tables.argtypes = [C.POINTER(C.c_int), P, P, P, P]
+    L.mray_renderer_set_measurement.argtypes = [P, u32, u32]
+    L.mray_renderer_phase_times.argtypes = [P, P, P]
     _lib = L
     return L
 
@@ -155,6 +159,20 @@ def lbvh_build(d_sorted_codes, d_leaf_order, d_leaf_aabbs):
     check(lib().mray_lbvh_build(_ptr(d_sorted_codes), _ptr(d_leaf_order), _ptr(d_leaf_aabbs), n,
                                 _ptr(nodes), _ptr(boxes), _ptr(tmp), nb, _stream()))
     return nodes, boxes
+
+
+def spectral_tables():
+    """Host copies of the RGB->spectrum tables (built on first use)."""
+    n = C.c_int()
+    check(lib().mray_spectral_tables(C.byref(n), None, None, None, None))
+    N = n.value
+    lut = np.zeros((3, N, N, N, 4), np.float32)
+    cmf = np.zeros((471, 3), np.float32)
+    illum = np.zeros(471, np.float32)
+    m = np.zeros(9, np.float32)
+    check(lib().mray_spectral_tables(C.byref(n), lut.ctypes.data, cmf.ctypes.data,
+                                     illum.ctypes.data, m.ctypes.data))
+    return {"N": N, "lut": lut, "cmf": cmf, "illum": illum, "xyz_to_rgb": m}
 
 
 # ---------------------------------------------------------------------------
@@ -256,14 +274,27 @@ class Renderer:
         check(lib().mray_renderer_stats(self.h, C.byref(st)))
         return {"dead_paths": st.deadPaths, "total_dead_paths": st.totalDeadPaths,
                 "ext_rays": st.extensionRays, "shadow_rays": st.shadowRays,
-                "iterations": st.iterations, "finished": bool(st.finished)}
+                "iterations": st.iterations, "finished": bool(st.finished),
+                "node_visits": st.nodeVisits, "tri_tests": st.triTests}
 
     def render_all(self):
         st = abi.MRayRenderStats()
         check(lib().mray_renderer_render_all(self.h, C.byref(st)))
         return {"dead_paths": st.deadPaths, "ext_rays": st.extensionRays,
                 "shadow_rays": st.shadowRays, "iterations": st.iterations,
-                "finished": bool(st.finished)}
+                "finished": bool(st.finished), "node_visits": st.nodeVisits,
+                "tri_tests": st.triTests}
+
+    def set_measurement(self, profiling=False, counting=False):
+        check(lib().mray_renderer_set_measurement(self.h, int(profiling), int(counting)))
+
+    def phase_times(self):
+        """Accumulated ms and launch counts for refill, trace, shade, shadow."""
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.uint64)
+        check(lib().mray_renderer_phase_times(self.h, ms.ctypes.data, n.ctypes.data))
+        names = ("refill", "trace", "shade", "shadow")
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(names)}
 
     def reset(self):
         check(lib().mray_renderer_reset(self.h))

@@ -1590,82 +1590,88 @@ static CamRay GenCamRay(const Camera& cam, uint32_t px, uint32_t py, uint32_t W,
 struct SpectralData
 {
     bool loaded = false;
-    int res = 0;                      // LUT resolution per axis
-    std::vector<float> scale;         // [res] z nodes
-    std::vector<float> coeffs;        // [3 maxChannel][res z][res y][res x][3]
-    std::vector<float> cmf;           // [471][3] CIE 1931 2 deg, 360..830 nm
-    std::vector<float> illum;         // [471] illuminant SPD (normalised)
+    int N = 0;                        // LUT resolution per axis
+    std::vector<float> lut;           // [3][N][N][N][4] (c0, c1, c2, 0), x fastest
+    std::vector<float> cmf;           // [471][3] observer, 360..830 nm
+    std::vector<float> illum;         // [471] illuminant SPD (Y-normalised)
     float xyzToRgb[9] = {0};
 };
 static SpectralData gSpec;
 
-static inline float SigmoidPoly(float c0, float c1, float c2, float lambda)
+// Math::InvSmoothstep (Core/Math.h:1224-1232)
+static inline float InvSmoothstep(float y)
 {
-    // Jakob & Hanika 2019, eq. 3: sigmoid of a quadratic in lambda
-    float x = std::fma(std::fma(c0, lambda, c1), lambda, c2);
-    if(std::isinf(x)) return (x > 0) ? 1.0f : 0.0f;
-    return 0.5f + x / (2.0f * std::sqrt(1.0f + x * x));
+    const float factor = 1.0f / 3.0f;
+    return 0.5f - std::sin(std::asin(1.0f - 2.0f * y) * factor);
 }
 
-// Trilinear LUT fetch of the three polynomial coefficients
+// 3-D texture fetch with normalised coordinates and texel centres at
+// (i + 0.5) / N, i.e. texel space u * (N - 1) for uv = xyz * (N-1)/N + 0.5/N
+// (SpectrumContext.hpp:113-118); trilinear, clamp to edge.
 static void LutFetch(int ch, float x, float y, float z, float out[3])
 {
-    int res = gSpec.res;
-    // z node lookup (scale is monotone); x, y uniform over [0, 1]
-    float fx = x * float(res - 1), fy = y * float(res - 1);
-    int zi = 0;
-    while(zi < res - 2 && gSpec.scale[size_t(zi + 1)] <= z) zi++;
-    float z0 = gSpec.scale[size_t(zi)], z1 = gSpec.scale[size_t(zi + 1)];
-    float tz = (z - z0) / (z1 - z0);
-    tz = Clamp(tz, 0.0f, 1.0f);
-    int xi = std::min(int(fx), res - 2), yi = std::min(int(fy), res - 2);
-    float tx = fx - float(xi), ty = fy - float(yi);
+    const int N = gSpec.N;
+    float fx = Clamp(x, 0.0f, 1.0f) * float(N - 1);
+    float fy = Clamp(y, 0.0f, 1.0f) * float(N - 1);
+    float fz = Clamp(z, 0.0f, 1.0f) * float(N - 1);
+    int x0 = std::min(int(fx), N - 2), y0 = std::min(int(fy), N - 2), z0 = std::min(int(fz), N - 2);
+    float tx = fx - float(x0), ty = fy - float(y0), tz = fz - float(z0);
     auto C = [&](int zz, int yy, int xx, int k)
     {
-        size_t o = ((((size_t(ch) * res + zz) * res + yy) * res + xx) * 3) + size_t(k);
-        return gSpec.coeffs[o];
+        size_t o = ((((size_t(ch) * N + zz) * N + yy) * N + xx) * 4) + size_t(k);
+        return gSpec.lut[o];
     };
     for(int k = 0; k < 3; k++)
     {
-        float c00 = C(zi, yi, xi, k) * (1 - tx) + C(zi, yi, xi + 1, k) * tx;
-        float c01 = C(zi, yi + 1, xi, k) * (1 - tx) + C(zi, yi + 1, xi + 1, k) * tx;
-        float c10 = C(zi + 1, yi, xi, k) * (1 - tx) + C(zi + 1, yi, xi + 1, k) * tx;
-        float c11 = C(zi + 1, yi + 1, xi, k) * (1 - tx) + C(zi + 1, yi + 1, xi + 1, k) * tx;
+        float c00 = C(z0, y0, x0, k) * (1 - tx) + C(z0, y0, x0 + 1, k) * tx;
+        float c01 = C(z0, y0 + 1, x0, k) * (1 - tx) + C(z0, y0 + 1, x0 + 1, k) * tx;
+        float c10 = C(z0 + 1, y0, x0, k) * (1 - tx) + C(z0 + 1, y0, x0 + 1, k) * tx;
+        float c11 = C(z0 + 1, y0 + 1, x0, k) * (1 - tx) + C(z0 + 1, y0 + 1, x0 + 1, k) * tx;
         float c0 = c00 * (1 - ty) + c01 * ty;
         float c1 = c10 * (1 - ty) + c11 * ty;
         out[k] = c0 * (1 - tz) + c1 * tz;
     }
 }
 
+// Converter::ConvertAlbedo polynomial (SpectrumContext.hpp:40-49)
+static inline float EvalSigmoidPoly(const float c[3], float lambda)
+{
+    float t = std::fma(c[0], lambda, c[1]);
+    float x = std::fma(t, lambda, c[2]);
+    float denomRecip = 1.0f / std::sqrt(std::fma(x, x, 1.0f));
+    return std::fma(0.5f * x, denomRecip, 0.5f);
+}
+
+// Converter::ConvertAlbedo (SpectrumContext.hpp:37-125)
 static V4 RGBToSpectrumAlbedo(V3 rgb, V4 waves)
 {
     float c[3] = {rgb.x, rgb.y, rgb.z};
-    c[0] = Clamp(c[0], 0.0f, 1.0f);
-    c[1] = Clamp(c[1], 0.0f, 1.0f);
-    c[2] = Clamp(c[2], 0.0f, 1.0f);
-    // Jakob2019 indexing: largest channel selects the table; the other two,
-    // divided by it, are the (x, y) coordinates; z is the largest value.
-    int i = 0;
-    for(int k = 1; k < 3; k++) if(c[k] >= c[i]) i = k;
-    float zv = c[i];
-    if(zv <= 0.0f) return Splat4(0.0f);
-    float x = c[(i + 1) % 3] / zv;
-    float y = c[(i + 2) % 3] / zv;
+    int maxI = 0;
+    if(c[1] > c[maxI]) maxI = 1;
+    if(c[2] > c[maxI]) maxI = 2;
+    float maxC = c[maxI];
+    float x = 0.0f, y = 0.0f;
+    if(maxC > kSmallEps)
+    {
+        float f = 1.0f / maxC;
+        x = c[(maxI + 1) % 3] * f;
+        y = c[(maxI + 2) % 3] * f;
+    }
+    float slice = Clamp(maxC, 0.0f, 1.0f);
+    float z = InvSmoothstep(InvSmoothstep(slice));
     float co[3];
-    LutFetch(i, x, y, zv, co);
-    float w[4] = {waves.x, waves.y, waves.z, waves.w};
-    float r[4];
-    for(int k = 0; k < 4; k++) r[k] = SigmoidPoly(co[0], co[1], co[2], w[k]);
-    return {r[0], r[1], r[2], r[3]};
+    LutFetch(maxI, x, y, z, co);
+    return {EvalSigmoidPoly(co, waves.x), EvalSigmoidPoly(co, waves.y),
+            EvalSigmoidPoly(co, waves.z), EvalSigmoidPoly(co, waves.w)};
 }
 
-static inline float IllumAt(float lambda)
+// 1-D linear texture over integer nanometres (coordinate lambda + 0.5 - 360)
+static inline float Tex1D(const std::vector<float>& t, int stride, int comp, float lambda)
 {
-    float f = lambda - 360.0f;
-    if(f < 0.0f || f > 470.0f) return 0.0f;
+    float f = Clamp(lambda - 360.0f, 0.0f, 470.0f);
     int i = std::min(int(f), 469);
-    float t = f - float(i);
-    return gSpec.illum[size_t(i)] * (1 - t) + gSpec.illum[size_t(i + 1)] * t;
+    float u = f - float(i);
+    return t[size_t(i) * stride + comp] * (1 - u) + t[size_t(i + 1) * stride + comp] * u;
 }
 
 static V4 ConvertAlbedo(const SpecCtx& sc, V3 rgb)
@@ -1673,18 +1679,19 @@ static V4 ConvertAlbedo(const SpecCtx& sc, V3 rgb)
     if(!sc.spectral) return {rgb.x, rgb.y, rgb.z, 0.0f};
     return RGBToSpectrumAlbedo(rgb, sc.waves);
 }
+// Converter::ConvertRadiance (SpectrumContext.hpp:127-150)
 static V4 ConvertRadiance(const SpecCtx& sc, V3 rgb)
 {
     if(!sc.spectral) return {rgb.x, rgb.y, rgb.z, 0.0f};
-    // Emission: scale = 2 * max channel, albedo of rgb / scale, times the
-    // illuminant SPD of the working colour space.
-    float m = MaxF(MaxF(rgb.x, rgb.y), rgb.z);
-    float scale = 2.0f * m;
-    if(scale <= 0.0f) return Splat4(0.0f);
-    V3 n = {rgb.x / scale, rgb.y / scale, rgb.z / scale};
-    V4 a = RGBToSpectrumAlbedo(n, sc.waves);
-    V4 il = {IllumAt(sc.waves.x), IllumAt(sc.waves.y), IllumAt(sc.waves.z), IllumAt(sc.waves.w)};
-    return a * il * scale;
+    float mx = MaxF(MaxF(rgb.x, rgb.y), rgb.z);
+    float scale = mx * 2.0f;
+    V3 n = (scale == 0.0f) ? V3{0, 0, 0} : V3{rgb.x / scale, rgb.y / scale, rgb.z / scale};
+    V4 s = RGBToSpectrumAlbedo(n, sc.waves);
+    s.x *= Tex1D(gSpec.illum, 1, 0, sc.waves.x);
+    s.y *= Tex1D(gSpec.illum, 1, 0, sc.waves.y);
+    s.z *= Tex1D(gSpec.illum, 1, 0, sc.waves.z);
+    s.w *= Tex1D(gSpec.illum, 1, 0, sc.waves.w);
+    return s * scale;
 }
 
 // SingleSampleSpectrumWavelength, HYPERBOLIC_PBRT mode (SpectrumContext.cu:14-135)
@@ -1710,14 +1717,8 @@ static void SampleWavelengths(float xi0, V4& waves, V4& pdf)
 
 static inline V3 CMFAt(float lambda)
 {
-    // 1-D linear texture over integer nanometres (OFFSET 0.5 - 360)
-    float f = lambda - 360.0f;
-    f = Clamp(f, 0.0f, 470.0f);
-    int i = std::min(int(f), 469);
-    float t = f - float(i);
-    const float* a = &gSpec.cmf[size_t(i) * 3];
-    const float* b = &gSpec.cmf[size_t(i + 1) * 3];
-    return {a[0] * (1 - t) + b[0] * t, a[1] * (1 - t) + b[1] * t, a[2] * (1 - t) + b[2] * t};
+    return {Tex1D(gSpec.cmf, 3, 0, lambda), Tex1D(gSpec.cmf, 3, 1, lambda),
+            Tex1D(gSpec.cmf, 3, 2, lambda)};
 }
 
 // ConvertSpectraToRGBSingle (SpectrumContext.cu:137-171)
@@ -2307,12 +2308,11 @@ int rc_render(void* h, const MRayRenderParams* p, float* film, uint32_t threads,
 
 // Spectral tables: LUT [3][res][res][res][3] + z scale [res], CMF [471][3],
 // illuminant [471], XYZ->RGB [9]
-void rc_spectral_load(int res, const float* scale, const float* coeffs, const float* cmf,
-                      const float* illum, const float* xyzToRgb)
+void rc_spectral_load(int N, const float* lut, const float* cmf, const float* illum,
+                      const float* xyzToRgb)
 {
-    gSpec.res = res;
-    gSpec.scale.assign(scale, scale + res);
-    gSpec.coeffs.assign(coeffs, coeffs + size_t(3) * res * res * res * 3);
+    gSpec.N = N;
+    gSpec.lut.assign(lut, lut + size_t(3) * N * N * N * 4);
     gSpec.cmf.assign(cmf, cmf + 471 * 3);
     gSpec.illum.assign(illum, illum + 471);
     std::memcpy(gSpec.xyzToRgb, xyzToRgb, 9 * sizeof(float));

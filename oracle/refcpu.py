@@ -57,7 +57,7 @@ def lib():
         L.rc_render.restype = C.c_int
         L.rc_render.argtypes = [P, C.POINTER(abi.MRayRenderParams), f32, C.c_uint32,
                                 C.c_uint32, C.c_uint64, u64]
-        L.rc_spectral_load.argtypes = [C.c_int, f32, f32, f32, f32, f32]
+        L.rc_spectral_load.argtypes = [C.c_int, f32, f32, f32, f32]
         L.rc_sample_wavelengths.argtypes = [f32, C.c_uint32, f32, f32]
         L.rc_rgb_to_spectrum.argtypes = [f32, f32, C.c_uint32, C.c_int, f32]
         L.rc_spectrum_to_rgb.argtypes = [f32, f32, f32, C.c_uint32, f32]
@@ -254,10 +254,40 @@ def bsdf_probe(mat, tbn, geo_n, back_side, wo, wi=None, rn=None, op=0):
 
 
 def spectral_load(tables):
-    scale = _f(tables["scale"])
-    coeffs = _f(tables["coeffs"])
+    """Load the RGB->spectrum tables (as returned by mray_amd.native.spectral_tables)."""
+    lut = _f(tables["lut"])
     cmf = _f(tables["cmf"])
     illum = _f(tables["illum"])
     m = _f(tables["xyz_to_rgb"])
-    lib().rc_spectral_load(int(scale.shape[0]), abi.fptr(scale), abi.fptr(coeffs), abi.fptr(cmf),
-                           abi.fptr(illum), abi.fptr(m))
+    lib().rc_spectral_load(int(tables["N"]), abi.fptr(lut), abi.fptr(cmf), abi.fptr(illum),
+                           abi.fptr(m))
+    _keep.append((lut, cmf, illum, m))
+
+
+_keep = []
+
+
+def sample_wavelengths(xi):
+    xi = _f(xi)
+    n = xi.shape[0]
+    w = np.zeros((n, 4), np.float32)
+    p = np.zeros((n, 4), np.float32)
+    lib().rc_sample_wavelengths(abi.fptr(xi), n, abi.fptr(w), abi.fptr(p))
+    return w, p
+
+
+def rgb_to_spectrum(rgb, waves, radiance=False):
+    rgb = _f(rgb).reshape(-1, 3)
+    waves = _f(waves).reshape(-1, 4)
+    out = np.zeros((rgb.shape[0], 4), np.float32)
+    lib().rc_rgb_to_spectrum(abi.fptr(rgb), abi.fptr(waves), rgb.shape[0], int(radiance),
+                             abi.fptr(out))
+    return out
+
+
+def spectrum_to_rgb(value, waves, pdfs):
+    value = _f(value).reshape(-1, 4)
+    out = np.zeros((value.shape[0], 3), np.float32)
+    lib().rc_spectrum_to_rgb(abi.fptr(value), abi.fptr(_f(waves)), abi.fptr(_f(pdfs)),
+                             value.shape[0], abi.fptr(out))
+    return out

@@ -27,7 +27,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(abi.MRayMaterial) == 52
     assert ctypes.sizeof(abi.MRayCamera) == 52
     assert ctypes.sizeof(abi.MRayRenderParams) == 48
-    assert ctypes.sizeof(abi.MRayRenderStats) == 40
+    assert ctypes.sizeof(abi.MRayRenderStats) == 56
 
 
 def test_error_path_without_gpu_work():

@@ -124,8 +124,8 @@ def test_stage_lbvh_build_duplicate_runs(torch_gpu):
 def test_stage_lbvh_build_degenerate_codes(torch_gpu):
     """Codes that differ only in low bits next to equal-code runs: the
     reference's Delta yields a malformed tree.  The kernels stay in bounds and
-    produce the same (malformed) topology as the oracle; box values of a
-    malformed tree are undefined and not compared."""
+    produce the same (malformed) child links as the oracle; parent links and
+    box values of a malformed tree are order-dependent and not compared."""
     rng = np.random.default_rng(2)
     n = 3000
     codes = np.sort(rng.integers(0, 64, size=n).astype(np.uint64))
@@ -134,4 +134,6 @@ def test_stage_lbvh_build_degenerate_codes(torch_gpu):
     assert refcpu.lbvh_tree_valid(codes, order, strict=True)
     nodes, _ = _gpu_lbvh(codes, order, aabbs)
     rnodes, _ = refcpu.lbvh_from_sorted(codes, order, aabbs)
-    assert np.array_equal(nodes, rnodes)
+    # children are written by their own node's thread (deterministic); in a
+    # malformed tree several nodes claim one child, so parent words race
+    assert np.array_equal(nodes[:, :2], rnodes[:, :2])

@@ -88,3 +88,32 @@ def test_iterate_then_reset_is_repeatable(torch_gpu):
     assert relmse(film_image(a), film_image(b)) < 1e-8
     r.close()
     acc.close()
+
+
+def _load_spectral_tables():
+    from mray_amd import native
+    refcpu.spectral_load(native.spectral_tables())
+
+
+def test_cornell_spectral(torch_gpu):
+    """PathTracerSpectral: 4 hero wavelengths, HYPERBOLIC sampling, LUT albedo
+    and emission, CIE conversion at path end."""
+    _load_spectral_tables()
+    scene = scenes.cornell()
+    params = abi.render_params(64, 64, 8, rr=(8, 8), spectral=True, seed=2)
+    film, st, ref, rst = _render_both(scene, params)
+    e = relmse(film_image(film), film_image(ref))
+    print(f"spectral relMSE={e:.3e} gpu={st} cpu={rst}", flush=True)
+    assert st["dead_paths"] == rst["dead_paths"]
+    assert e <= 1e-3
+
+
+def test_terrain_spectral_dispersion(torch_gpu):
+    """Spectral terrain: Refract disperses (hero wavelength only afterwards)."""
+    _load_spectral_tables()
+    scene = scenes.terrain_spheres(grid=128, spheres=150, width=96, height=54)
+    params = abi.render_params(96, 54, 4, rr=(10, 10), spectral=True, seed=4)
+    film, st, ref, rst = _render_both(scene, params)
+    e = relmse(film_image(film), film_image(ref))
+    print(f"spectral relMSE={e:.3e} gpu={st} cpu={rst}", flush=True)
+    assert e <= 1e-3
