@@ -140,6 +140,9 @@ typedef struct MRayRenderStats
 const char* mray_last_error(void);
 const char* mray_version(void);
 int         mray_device_count(int* hCount);
+/* Device memory of the current GPU (TracerI::TotalDeviceMemory /
+ * UsedDeviceMemory, Core/TracerI.h:370-371).                              */
+int         mray_device_memory(size_t* hTotal, size_t* hUsed);
 
 /* ---------------------------------------------------------------------- */
 /* LBVH build stages (device pointers).  K1/K2/K4/K5/K6/K7 of the reference */

@@ -232,6 +232,16 @@ int mray_device_count(int* hCount)
     return MRAY_OK;
 }
 
+int mray_device_memory(size_t* hTotal, size_t* hUsed)
+{
+    if(!hTotal || !hUsed) return MRAY_ERR_INVALID;
+    size_t freeB = 0, total = 0;
+    MRAY_HIP_CHECK(hipMemGetInfo(&freeB, &total));
+    *hTotal = total;
+    *hUsed = total - freeB;
+    return MRAY_OK;
+}
+
 int mray_prim_centers_aabbs(const float* dPositions, const uint32_t* dIndices,
                             uint32_t triangleCount, float* dCenters, float* dAabbs, void* stream)
 {

@@ -22,7 +22,7 @@ class MRayError(RuntimeError):
 
 # Every symbol include/mray_amd.h declares (checked by tests/test_abi.py).
 EXPORTS = [
-    "mray_last_error", "mray_version", "mray_device_count",
+    "mray_last_error", "mray_version", "mray_device_count", "mray_device_memory",
     "mray_prim_centers_aabbs", "mray_reduce_aabb", "mray_morton63", "mray_morton30",
     "mray_radix_sort_temp_bytes", "mray_radix_sort_u64_u32", "mray_lbvh_build",
     "mray_lbvh_temp_bytes",
@@ -48,6 +48,7 @@ def lib():
     L.mray_last_error.restype = C.c_char_p
     L.mray_version.restype = C.c_char_p
     L.mray_device_count.argtypes = [C.POINTER(C.c_int)]
+    L.mray_device_memory.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
     L.mray_prim_centers_aabbs.argtypes = [P, P, u32, P, P, P]
     L.mray_reduce_aabb.argtypes = [P, u32, P, P]
     L.mray_morton63.argtypes = [P, u32, P, P, P]

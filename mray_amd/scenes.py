@@ -95,7 +95,7 @@ class Scene:
     light_two_sided: np.ndarray            # [L] u8
     boundary_type: int = abi.BOUNDARY_NULL
     boundary_radiance: tuple = (0.0, 0.0, 0.0)
-    scene_diameter: float = 0.0
+    scene_diameter: float = None           # None: from the AABB, as the tracer does
     camera: dict = field(default_factory=dict)
     name: str = ""
     _keep: list = field(default_factory=list, repr=False)
@@ -103,6 +103,16 @@ class Scene:
     @property
     def triangle_count(self):
         return int(self.indices.shape[0])
+
+    @property
+    def diameter(self):
+        """Boundary-light scene diameter: length of the XZ span of the scene
+        AABB (TracerBase.cpp:1651-1664)."""
+        if self.scene_diameter is not None:
+            return float(self.scene_diameter)
+        p = np.asarray(self.positions, np.float32)[np.asarray(self.indices).reshape(-1)]
+        span = p.max(axis=0) - p.min(axis=0)
+        return float(np.float32(np.sqrt(np.float32(span[0] * span[0] + span[2] * span[2]))))
 
     def desc(self):
         """MRaySceneDesc pointing at this scene's arrays (kept alive by self)."""
@@ -143,7 +153,7 @@ class Scene:
         d.boundaryType = self.boundary_type
         for k in range(3):
             d.boundaryRadiance[k] = self.boundary_radiance[k]
-        d.sceneDiameter = self.scene_diameter
+        d.sceneDiameter = self.diameter
         cam = self.camera
         for k in range(3):
             d.camera.position[k] = cam["position"][k]
@@ -268,7 +278,7 @@ def cornell(aspect=1.0, fov_deg=40.0):
                  light_tris=np.array([first, first + 1], dtype=np.uint32),
                  light_radiance=np.array([[17, 12, 4], [17, 12, 4]], dtype=np.float32),
                  light_two_sided=np.zeros(2, dtype=np.uint8),
-                 boundary_type=abi.BOUNDARY_NULL, scene_diameter=2.0 * math.sqrt(3.0),
+                 boundary_type=abi.BOUNDARY_NULL,
                  camera=cam, name="cornell")
 
 
@@ -447,7 +457,7 @@ def terrain_spheres(grid=1024, spheres=2000, width=1920, height=1080, fov_x_deg=
                  light_radiance=np.full((nl, 3), 10.0, dtype=np.float32),
                  light_two_sided=np.zeros(nl, dtype=np.uint8),
                  boundary_type=abi.BOUNDARY_SKYSPHERE_COOCTA,
-                 boundary_radiance=(0.5, 0.5, 0.5), scene_diameter=2.0 * math.sqrt(3.0) * 2.0,
+                 boundary_radiance=(0.5, 0.5, 0.5),
                  camera=cam, name=f"terrain{grid}_spheres{spheres}")
 
 

@@ -523,23 +523,29 @@ static inline TriHit IntersectTriangle(V3 pos, V3 dir, V3 t0, V3 t1, V3 t2, bool
     return r;
 }
 
+// Ray::IntersectsAABB (Ray.hpp:192-219): slab test with invD = 1/dir and a
+// swap on negative directions.  Restated with two changes that only affect
+// pruning, never which triangle is hit: the near/far plane is chosen by the
+// sign bit of invD before the subtraction, so a ray lying exactly in a slab
+// plane (0 * inf = NaN) leaves that slab unbounded (std::fmax / std::fmin drop
+// NaN) instead of being rejected; and the far bound is widened by 1 + 2*gamma(4)
+// (Ize 2013) so rounding never prunes a box the ray touches.  Closest hits
+// therefore equal the brute-force ones.
 static inline bool IntersectAABB(V3 pos, V3 dir, const float* box, float tMin, float tMax)
 {
-    V3 invD = {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
-    float bmin[3] = {box[0], box[1], box[2]};
-    float bmax[3] = {box[3], box[4], box[5]};
     float o[3] = {pos.x, pos.y, pos.z};
-    float id[3] = {invD.x, invD.y, invD.z};
+    float id[3] = {1.0f / dir.x, 1.0f / dir.y, 1.0f / dir.z};
     float t0v = tMin, t1v = tMax;
     for(int i = 0; i < 3; i++)
     {
-        float a = (bmin[i] - o[i]) * id[i];
-        float b = (bmax[i] - o[i]) * id[i];
-        if(id[i] < 0) std::swap(a, b);
-        t0v = MaxF(t0v, MinF(a, b));
-        t1v = MinF(t1v, MaxF(a, b));
+        bool neg = std::signbit(id[i]);
+        float nearP = neg ? box[3 + i] : box[i];
+        float farP = neg ? box[i] : box[3 + i];
+        t0v = MaxF(t0v, (nearP - o[i]) * id[i]);
+        t1v = MinF(t1v, (farP - o[i]) * id[i]);
     }
-    return t1v >= t0v;
+    const float grow = 1.0f + 2.0f * 5.96e-8f * 4.0f;
+    return t1v * grow >= t0v;
 }
 
 // Ray::Nudge (Ray.hpp:258-300), Ray Tracing Gems I ch. 6

@@ -108,3 +108,40 @@ def test_single_triangle_and_edges(torch_gpu):
     rays[:, 6] = -1.0
     rays[:, 7] = 10.0
     _compare(acc, refcpu.OracleScene(s), rays, brute=True)
+
+
+def test_rays_in_box_planes(torch_gpu):
+    """Axis-parallel rays (+0 and -0 direction components) whose origins lie
+    on the BVH box planes of a grid of axis-aligned quads."""
+    from mray_amd import native
+    pos, idx = [], []
+    for i in range(4):
+        for j in range(4):
+            b = len(pos)
+            x0, y0 = i / 4, j / 4
+            pos += [[x0, y0, 0], [x0 + 0.25, y0, 0], [x0 + 0.25, y0 + 0.25, 0], [x0, y0 + 0.25, 0]]
+            idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+    pos = np.array(pos, np.float32)
+    idx = np.array(idx, np.uint32)
+    n = len(idx)
+    s = scenes.Scene(positions=pos, tbn=np.tile(np.array([[1, 0, 0, 0]], np.float32), (len(pos), 1)),
+                     indices=idx, surface_keys=np.zeros(n, np.uint32), cull=np.zeros(n, np.uint8),
+                     materials=[{"type": abi.MAT_LAMBERT, "albedo": (1, 1, 1)}],
+                     light_tris=np.zeros(0, np.uint32), light_radiance=np.zeros((0, 3), np.float32),
+                     light_two_sided=np.zeros(0, np.uint8),
+                     camera={"position": (0, 0, 1), "gaze": (0, 0, 0), "up": (0, 1, 0),
+                             "fov": (1.0, 1.0), "nearFar": (0.1, 10.0)})
+    acc = native.Accel(s)
+    o = refcpu.OracleScene(s)
+    g = np.arange(0, 17, dtype=np.float32) / 16
+    xy = np.array([[x, y] for x in g for y in g], np.float32)
+    rays = np.zeros((len(xy), 8), np.float32)
+    rays[:, 0:2] = xy
+    rays[:, 2] = 1.0
+    rays[:, 6] = -1.0
+    rays[:, 7] = 10.0
+    neg = rays.copy()
+    neg[:, 4] = -0.0
+    neg[:, 5] = -0.0
+    for r in (rays, neg):
+        _compare(acc, o, r, brute=True)

@@ -163,3 +163,43 @@ def test_moller_trumbore_closed_form():
         assert bary[0][0] == np.float32(0.25) and bary[0][1] == np.float32(0.25)
         assert prim[1] == (0xFFFFFFFF if cull else 0)
         assert prim[2] == 0xFFFFFFFF and prim[3] == 0xFFFFFFFF
+
+
+def test_traversal_rays_in_box_planes():
+    """Axis-parallel rays whose origins lie exactly on box face planes (0 * inf
+    in the slab test) still reach their triangles: traversal == brute force."""
+    # axis-aligned unit quads on a 4 x 4 grid at z = 0, every vertex on x/y = k/4
+    pos, idx = [], []
+    for i in range(4):
+        for j in range(4):
+            b = len(pos)
+            x0, y0 = i / 4, j / 4
+            pos += [[x0, y0, 0], [x0 + 0.25, y0, 0], [x0 + 0.25, y0 + 0.25, 0], [x0, y0 + 0.25, 0]]
+            idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+    pos = np.array(pos, np.float32)
+    idx = np.array(idx, np.uint32)
+    n = len(idx)
+    s = scenes.Scene(positions=pos, tbn=np.tile(np.array([[1, 0, 0, 0]], np.float32), (len(pos), 1)),
+                     indices=idx, surface_keys=np.zeros(n, np.uint32), cull=np.zeros(n, np.uint8),
+                     materials=[{"type": 0, "albedo": (1, 1, 1)}],
+                     light_tris=np.zeros(0, np.uint32), light_radiance=np.zeros((0, 3), np.float32),
+                     light_two_sided=np.zeros(0, np.uint8),
+                     camera={"position": (0, 0, 1), "gaze": (0, 0, 0), "up": (0, 1, 0),
+                             "fov": (1.0, 1.0), "nearFar": (0.1, 10.0)})
+    o = refcpu.OracleScene(s)
+    g = np.arange(0, 17, dtype=np.float32) / 16          # includes every box plane
+    xy = np.array([[x, y] for x in g for y in g], np.float32)
+    rays = np.zeros((len(xy), 8), np.float32)
+    rays[:, 0:2] = xy
+    rays[:, 2] = 1.0
+    rays[:, 6] = -1.0          # direction (+0, +0, -1)
+    rays[:, 7] = 10.0
+    neg = rays.copy()
+    neg[:, 4] = -0.0           # direction (-0, -0, -1): the other slab sign
+    neg[:, 5] = -0.0
+    for r in (rays, neg):
+        p0, t0, _ = o.cast_closest(r, brute=False)
+        p1, t1, _ = o.cast_closest(r, brute=True)
+        assert (p1 != 0xFFFFFFFF).all()
+        assert np.array_equal(p0, p1) and np.array_equal(t0, t1)
+        assert (o.cast_visibility(r) == 0).all()

@@ -1,0 +1,121 @@
+"""TracerI host layer (include/mray_tracer.hpp, mray_amd/host/tracer.cpp).
+
+A C++ front-end (tests/cpp/tracer_driver) loads libmray_tracer.so with dlopen,
+constructs the tracer through ConstructTracer and feeds it a scene with the
+calls the reference's scene loader makes (groups, reservations, commits,
+TransientData attribute pushes, surfaces, light surfaces, boundary light,
+camera surface).  Without a GPU the driver stops after CommitSurfaces' host
+half and writes the flattened scene, which must reproduce the input scene
+triangle for triangle.  The error checks mirror the reference's MRayError
+behaviour (TracerBase.cpp:1713-1717, PathTracerRenderer.cu:139-140).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from mray_amd import abi, scenes
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "tests", "cpp", "tracer_driver")
+TRACER_LIB = os.path.join(ROOT, "mray_amd", "libmray_tracer.so")
+
+
+def _built():
+    if not (os.path.exists(DRIVER) and os.path.exists(TRACER_LIB)):
+        pytest.fail("libmray_tracer.so / tests/cpp/tracer_driver are not built; "
+                    "run __graft_entry__.build()")
+
+
+def write_scene_file(scene, path):
+    """The driver's input: the MRaySceneDesc arrays plus the camera."""
+    d = scene.desc()
+    mats = np.frombuffer(bytes(memoryview(scene._keep[1]))[:52 * len(scene.materials)], np.uint8)
+    cam = scene.camera
+    with open(path, "wb") as f:
+        f.write(np.array([len(scene.positions), scene.triangle_count, len(scene.materials),
+                          len(scene.light_tris), scene.boundary_type], np.uint32).tobytes())
+        f.write(np.array(list(scene.boundary_radiance) + [d.sceneDiameter], np.float32).tobytes())
+        f.write(np.ascontiguousarray(scene.positions, np.float32).tobytes())
+        f.write(np.ascontiguousarray(scene.tbn, np.float32).tobytes())
+        f.write(np.ascontiguousarray(scene.indices, np.uint32).tobytes())
+        f.write(np.ascontiguousarray(scene.surface_keys, np.uint32).tobytes())
+        f.write(np.ascontiguousarray(scene.cull, np.uint8).tobytes())
+        f.write(mats.tobytes())
+        f.write(np.ascontiguousarray(scene.light_tris, np.uint32).tobytes())
+        f.write(np.ascontiguousarray(scene.light_radiance, np.float32).tobytes())
+        f.write(np.ascontiguousarray(scene.light_two_sided, np.uint8).tobytes())
+        f.write(np.array(list(cam["position"]) + list(cam["gaze"]) + list(cam["up"]) +
+                         list(cam["fov"]) + list(cam["nearFar"]), np.float32).tobytes())
+
+
+def read_flat_file(path):
+    b = open(path, "rb").read()
+    o = 0
+
+    def take(dtype, n):
+        nonlocal o
+        a = np.frombuffer(b, dtype, count=n, offset=o)
+        o += a.nbytes
+        return a
+    V, T, M, L, btype = take(np.uint32, 5)
+    out = {"btype": int(btype), "brad": take(np.float32, 3), "diam": take(np.float32, 1)[0]}
+    out["pos"] = take(np.float32, V * 3).reshape(-1, 3)
+    out["tbn"] = take(np.float32, V * 4).reshape(-1, 4)
+    out["idx"] = take(np.uint32, T * 3).reshape(-1, 3)
+    out["keys"] = take(np.uint32, T)
+    out["cull"] = take(np.uint8, T)
+    out["mats"] = take(np.uint8, M * 52)
+    out["ltri"] = take(np.uint32, L)
+    out["lrad"] = take(np.float32, L * 3).reshape(-1, 3)
+    out["l2s"] = take(np.uint8, L)
+    assert o == len(b)
+    return out
+
+
+def _flatten(scene, tmp_path):
+    _built()
+    src = str(tmp_path / "scene.bin")
+    dst = str(tmp_path / "flat.bin")
+    write_scene_file(scene, src)
+    r = subprocess.run([DRIVER, TRACER_LIB, src, "flatten", dst], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    return read_flat_file(dst)
+
+
+def _check_same(scene, f):
+    idx = np.asarray(scene.indices)
+    assert np.array_equal(f["pos"][f["idx"]], np.asarray(scene.positions, np.float32)[idx])
+    assert np.array_equal(f["tbn"][f["idx"]], np.asarray(scene.tbn, np.float32)[idx])
+    assert np.array_equal(f["keys"], scene.surface_keys)
+    assert np.array_equal(f["cull"], scene.cull)
+    mats = np.frombuffer(bytes(memoryview(scene._keep[1]))[:52 * len(scene.materials)], np.uint8)
+    assert np.array_equal(f["mats"], mats)
+    assert np.array_equal(f["ltri"], scene.light_tris)
+    assert np.array_equal(f["lrad"], np.asarray(scene.light_radiance, np.float32))
+    assert np.array_equal(f["l2s"], scene.light_two_sided)
+    assert f["btype"] == scene.boundary_type
+    assert np.array_equal(f["brad"], np.asarray(scene.boundary_radiance, np.float32))
+    # the boundary light's diameter follows TracerBase.cpp:1651-1664
+    assert f["diam"] == np.float32(scene.diameter)
+
+
+def test_flatten_cornell(tmp_path):
+    s = scenes.cornell()
+    _check_same(s, _flatten(s, tmp_path))
+
+
+def test_flatten_terrain_with_skysphere(tmp_path):
+    s = scenes.terrain_spheres(grid=48, spheres=40, width=64, height=36)
+    assert s.boundary_type == abi.BOUNDARY_SKYSPHERE_COOCTA
+    _check_same(s, _flatten(s, tmp_path))
+
+
+def test_error_behaviour():
+    _built()
+    r = subprocess.run([DRIVER, TRACER_LIB, "errors"], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "FAIL" not in r.stdout
