@@ -44,7 +44,36 @@ def parse():
     ap.add_argument("--rgb", action="store_true", help="PathTracerRGB instead of Spectral")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--probe-copy-mib", type=int, default=0,
+                    help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
+
+
+def probe_copy(torch, mib):
+    """One plain copy kernel of known size (torch.neg: reads and writes
+    mib MiB once each), the reference point that scales rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    log(f"probe copy: {mib} MiB read + {mib} MiB written")
+    del src, dst
+
+
+def traffic_from_profiles(rays_per_launch):
+    """HBM bytes per KTrace launch from the committed rocprofv3 PMC summary
+    (DRAM bytes per traced ray, FETCH_SIZE + WRITE_SIZE scaled by the copy
+    probe), or None when no summary is present."""
+    path = os.path.join(ROOT, "profiles", "ktrace_hbm_traffic_1080p.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("dram_bytes_per_ray:"):
+            return rays_per_launch * float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(scene, params, budget_s):
@@ -52,18 +81,21 @@ def cpu_baseline(scene, params, budget_s):
     box's host cores, over a bounded number of paths of the same workload."""
     import numpy as np  # noqa: F401
     from oracle import refcpu
+    if params.spectral:
+        from mray_amd import native
+        refcpu.spectral_load(native.spectral_tables())
     threads = os.cpu_count() or 1
     o = refcpu.OracleScene(scene)
-    # calibrate on a small sample, then size the measured one to the budget
-    probe = 20000
-    t0 = time.perf_counter()
-    _, st = o.render(params, threads=threads, path_limit=probe)
-    dt = time.perf_counter() - t0
-    rate = st["dead_paths"] / max(dt, 1e-6)
-    sample = int(min(max(probe, rate * budget_s), params.width * params.height * params.totalSPP))
-    t0 = time.perf_counter()
-    _, st = o.render(params, threads=threads, path_limit=sample)
-    dt = time.perf_counter() - t0
+    # grow the sample until it takes about the budget (first paths of the frame)
+    total = params.width * params.height * params.totalSPP
+    sample = 20000
+    while True:
+        t0 = time.perf_counter()
+        _, st = o.render(params, threads=threads, path_limit=sample)
+        dt = time.perf_counter() - t0
+        if dt >= 0.5 * budget_s or sample >= total:
+            break
+        sample = int(min(total, max(2 * sample, sample * budget_s / max(dt, 1e-3))))
     cpu = platform.processor() or platform.machine()
     return {"value": st["dead_paths"] / dt / 1e6, "unit": "Mpaths/s", "cores": threads,
             "kind": "port",
@@ -80,6 +112,8 @@ def main():
     import numpy as np
     from mray_amd import abi, native, scenes
     torch = native.require_gpu()
+    if args.probe_copy_mib:
+        probe_copy(torch, args.probe_copy_mib)
     spectral = not args.rgb
     log("generating scene")
     t0 = time.perf_counter()
@@ -126,7 +160,8 @@ def main():
         for key in totals:
             totals[key] += st[key]
         log(f"step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms, {st['dead_paths']} paths, "
-            f"{st['iterations']} bounces")
+            f"{st['iterations']} bounces, {st['ext_rays']} extension rays, "
+            f"{st['shadow_rays']} shadow rays")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_all
     phases = r.phase_times()
@@ -141,6 +176,7 @@ def main():
     rays_per_launch = totals["ext_rays"] / max(1, trace_launches)
     bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
     achieved = rays_per_launch * bytes_per_ray / (trace_avg_ms * 1e-3) / 1e9
+    traffic = traffic_from_profiles(rays_per_launch)
     result = {
         "metric": METRIC,
         "value": round(mpaths, 3),
@@ -177,7 +213,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": None if traffic is None else round(traffic),
             "bytes_per_ray": round(bytes_per_ray, 1),
             "nodes_per_ray": round(nodes_per_ray, 2),
             "tris_per_ray": round(tris_per_ray, 2),

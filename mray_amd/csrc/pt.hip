@@ -4,16 +4,20 @@
 // (TracerDLL/PathTracerRenderer.cu:930-1074), as six kernels that read every
 // count from device memory, so iterations queue back-to-back with no host
 // synchronisation:
-//   KRefillCount / KRefillScan / KRefillApply  ReloadPaths: dead slots, in
-//        ascending slot order, take consecutive pixel indices from the global
-//        path counter (PathTracerRendererBase.cu:69-203 with the stable
-//        partition of Device/CPU/AlgBinaryPartitionCPU.h:31-48); live slots are
-//        appended to the active list with wave-aggregated atomics.
+//   KRefillCount / KRefillScan / KRefillApply  ReloadPaths: finished paths
+//        go to the film with float atomics (AddRadianceToRenderBufferThroughput,
+//        PathTracerRendererBase.cu:215-262); then free slots, in ascending slot
+//        order, take consecutive pixel indices from the global path counter
+//        (PathTracerRendererBase.cu:69-203 with the stable partition of
+//        Device/CPU/AlgBinaryPartitionCPU.h:31-48).  Live slots form the
+//        active list in ascending slot order; every position comes from the
+//        per-block scan, so no atomics and a deterministic order.
 //   KTrace    closest hit of every active slot (CastRays)
 //   KShade    boundary/light work or NEE (work 1) + BxDF scatter (work 0)
-//             (PathTracerRendererShaders.h:201-520)
-//   KShadow   any-hit visibility + KCAccumulateShadowRaysPT, then finished paths
-//             go to the film with float atomics (TextureFilter.cu:672-697).
+//             (PathTracerRendererShaders.h:201-520); slots with a shadow ray
+//             are appended to the shadow queue (one atomic per wave)
+//   KShadow   any-hit visibility over the shadow queue only, then
+//             KCAccumulateShadowRaysPT.
 // Each slot consumes PCG32 numbers in the reference's per-slot order, so with
 // the same seed every path sample equals the reference schedule's.
 #include "common.h"
@@ -40,7 +44,6 @@ __device__ __forceinline__ uint32_t MDepth(uint32_t m) { return m & 0xFFFFu; }
 __device__ __forceinline__ uint32_t MStatus(uint32_t m) { return (m >> 16) & 3u; }
 __device__ __forceinline__ uint32_t MType(uint32_t m) { return (m >> 18) & 3u; }
 __device__ __forceinline__ bool MDispersed(uint32_t m) { return (m >> 20) & 1u; }
-__device__ __forceinline__ bool MPending(uint32_t m) { return (m >> 21) & 1u; }
 __device__ __forceinline__ uint32_t MPack(uint32_t depth, uint32_t status, uint32_t type,
                                           bool disp, bool pending)
 {
@@ -80,8 +83,22 @@ struct Sched
     uint32_t fill;
     uint32_t activeCount;
     uint32_t iterations;
+    uint32_t shadowCount;
+    uint32_t traceNext;     // dynamic work counters (64 items per wave fetch)
+    uint32_t shadeNext;
+    uint32_t shadowNext;
     uint32_t pad;
 };
+
+// Persistent kernels take work 64 items at a time: one atomic per wave,
+// issued with every lane of the wave active (loop head), so waves that run
+// out of work early take more instead of idling behind a fixed share.
+__device__ __forceinline__ uint32_t WaveFetch(uint32_t* counter)
+{
+    uint32_t base = 0;
+    if((threadIdx.x & 63u) == 0) base = atomicAdd(counter, 64u);
+    return __shfl(base, 0);
+}
 
 struct RenderConst
 {
@@ -102,84 +119,152 @@ static constexpr uint32_t kSlotTPB = 256;
 // ---------------------------------------------------------------------------
 // Refill
 // ---------------------------------------------------------------------------
+// Per-block counts: x = free slots (dead or invalid), y = live slots,
+// z = finished paths waiting for the film.  KRefillScan turns x and y into
+// exclusive prefixes.
 __global__ void __launch_bounds__(kSlotTPB)
-KRefillCount(const uint32_t* __restrict__ meta, uint32_t N, uint32_t* __restrict__ blockCnt,
-             Sched* __restrict__ sched)
+KRefillCount(const uint32_t* __restrict__ meta, uint32_t N, uint4* __restrict__ blockCnt)
 {
-    __shared__ uint32_t wsum[kSlotTPB / 64];
+    __shared__ uint32_t wsum[3][kSlotTPB / 64];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool dead = (i < N) && (MStatus(meta[i]) != ST_ALIVE);
-    uint64_t b = __ballot(dead);
-    if((threadIdx.x & 63u) == 0) wsum[threadIdx.x >> 6] = uint32_t(__popcll(b));
+    uint32_t st = (i < N) ? MStatus(meta[i]) : ST_ALIVE;
+    uint64_t bf = __ballot((i < N) && st != ST_ALIVE);
+    uint64_t bl = __ballot((i < N) && st == ST_ALIVE);
+    uint64_t bd = __ballot((i < N) && st == ST_DEAD);
+    const uint32_t w = threadIdx.x >> 6;
+    if((threadIdx.x & 63u) == 0)
+    {
+        wsum[0][w] = uint32_t(__popcll(bf));
+        wsum[1][w] = uint32_t(__popcll(bl));
+        wsum[2][w] = uint32_t(__popcll(bd));
+    }
     __syncthreads();
     if(threadIdx.x == 0)
     {
-        uint32_t s = 0;
-        for(uint32_t w = 0; w < kSlotTPB / 64; w++) s += wsum[w];
-        blockCnt[blockIdx.x] = s;
+        uint4 c = make_uint4(0, 0, 0, 0);
+        for(uint32_t k = 0; k < kSlotTPB / 64; k++)
+        {
+            c.x += wsum[0][k];
+            c.y += wsum[1][k];
+            c.z += wsum[2][k];
+        }
+        blockCnt[blockIdx.x] = c;
     }
-    if(blockIdx.x == 0 && threadIdx.x == 0) sched->activeCount = 0;
 }
 
+// One block: exclusive scans of the free and live counts, then the refill
+// amount (ReloadPaths with TotalSampleLimit, PathTracerRendererBase.h:59-67).
 __global__ void __launch_bounds__(1024)
-KRefillScan(uint32_t* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sched)
+KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sched)
 {
-    __shared__ uint32_t s[1024];
-    __shared__ uint32_t carry;
-    if(threadIdx.x == 0) carry = 0;
+    __shared__ uint32_t sf[1024], sl[1024];
+    __shared__ uint32_t carryF, carryL;
+    __shared__ unsigned long long accum;
+    if(threadIdx.x == 0) { carryF = 0; carryL = 0; accum = 0; }
     __syncthreads();
+    unsigned long long myAccum = 0;
     for(uint32_t base = 0; base < count; base += 1024)
     {
         uint32_t i = base + threadIdx.x;
-        uint32_t v = (i < count) ? blockCnt[i] : 0u;
-        s[threadIdx.x] = v;
+        uint4 v = (i < count) ? blockCnt[i] : make_uint4(0, 0, 0, 0);
+        myAccum += v.z;
+        sf[threadIdx.x] = v.x;
+        sl[threadIdx.x] = v.y;
         __syncthreads();
         for(uint32_t off = 1; off < 1024; off <<= 1)
         {
-            uint32_t add = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0u;
+            uint32_t af = (threadIdx.x >= off) ? sf[threadIdx.x - off] : 0u;
+            uint32_t al = (threadIdx.x >= off) ? sl[threadIdx.x - off] : 0u;
             __syncthreads();
-            s[threadIdx.x] += add;
+            sf[threadIdx.x] += af;
+            sl[threadIdx.x] += al;
             __syncthreads();
         }
-        uint32_t incl = s[threadIdx.x];
-        if(i < count) blockCnt[i] = carry + incl - v;
+        uint32_t inF = sf[threadIdx.x], inL = sl[threadIdx.x];
+        if(i < count) blockCnt[i] = make_uint4(carryF + inF - v.x, carryL + inL - v.y, v.z, 0u);
         __syncthreads();
-        if(threadIdx.x == 1023) carry += incl;
+        if(threadIdx.x == 1023) { carryF += inF; carryL += inL; }
         __syncthreads();
     }
+    // number of finished paths: wave reduction, then one LDS atomic per wave
+    for(uint32_t o = 32; o > 0; o >>= 1) myAccum += __shfl_down(myAccum, o);
+    if((threadIdx.x & 63u) == 0) atomicAdd(&accum, myAccum);
+    __syncthreads();
     if(threadIdx.x == 0)
     {
-        unsigned long long dead = carry;
+        unsigned long long freeSlots = carryF;
         unsigned long long remaining = sched->limit - sched->counter;
-        unsigned long long fill = (dead < remaining) ? dead : remaining;
+        unsigned long long fill = (freeSlots < remaining) ? freeSlots : remaining;
         sched->base = sched->counter;
         sched->fill = uint32_t(fill);
         sched->counter += fill;
+        sched->activeCount = carryL + uint32_t(fill);
+        sched->deadTotal += accum;
+        sched->shadowCount = 0;
+        sched->traceNext = 0;
+        sched->shadeNext = 0;
+        sched->shadowNext = 0;
         sched->iterations += 1;
     }
 }
 
+// Finished path -> film (AddRadianceToRenderBufferThroughput,
+// PathTracerRendererBase.cu:215-262; accumulation as TextureFilter.cu:672-697):
+// spectral paths are converted to RGB first; a non-finite sample is written
+// as magenta with a large weight so it stays visible.
+template<bool SPECTRAL>
+__device__ __forceinline__ void SplatToFilm(const SlotSoA& S, const RenderConst& rc, uint32_t i,
+                                            uint32_t m)
+{
+    float4 r4 = S.rad[i];
+    V4 rad = {r4.x, r4.y, r4.z, r4.w};
+    if constexpr(SPECTRAL)
+    {
+        float4 w4 = S.waves[i], p4 = S.wpdf[i];
+        rad = SpectrumToRGB(rc.spec, rad, V4{w4.x, w4.y, w4.z, w4.w},
+                            V4{p4.x, p4.y, p4.z, p4.w}, MDispersed(m));
+    }
+    float w = S.filterW[i];
+    float r = rad.x, g = rad.y, b = rad.z;
+    if(!(isfinite(r) && isfinite(g) && isfinite(b) && isfinite(rad.w)))
+    {
+        r = 1e7f; g = 0.0f; b = 1e7f;
+        w = w * 128.0f;
+    }
+    uint32_t px = S.pixel[i];
+    size_t P = rc.P;
+    atomicAdd(rc.film + px, r);
+    atomicAdd(rc.film + P + px, g);
+    atomicAdd(rc.film + 2 * P + px, b);
+    atomicAdd(rc.film + 3 * P + px, w);
+}
+
 template<bool SPECTRAL>
 __global__ void __launch_bounds__(kSlotTPB)
-KRefillApply(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ blockOff,
-             Sched* __restrict__ sched, uint32_t* __restrict__ active)
+KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
+             const Sched* __restrict__ sched, uint32_t* __restrict__ active)
 {
-    __shared__ uint32_t wsum[kSlotTPB / 64];
+    __shared__ uint32_t wsum[kSlotTPB / 64], asum[kSlotTPB / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t m = (i < rc.N) ? S.meta[i] : MPack(0, ST_ALIVE, 0, false, false);
-    bool dead = (i < rc.N) && (MStatus(m) != ST_ALIVE);
-    uint64_t b = __ballot(dead);
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool inRange = i < rc.N;
+    uint32_t m = inRange ? S.meta[i] : MPack(0, ST_INVALID, 0, false, false);
+    const uint32_t status = MStatus(m);
+    const bool freeSlot = inRange && status != ST_ALIVE;
+    if(inRange && status == ST_DEAD) SplatToFilm<SPECTRAL>(S, rc, i, m);
+    uint64_t b = __ballot(freeSlot);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     if(lane == 0) wsum[wave] = uint32_t(__popcll(b));
     __syncthreads();
     uint32_t waveOff = 0;
     for(uint32_t w = 0; w < wave; w++) waveOff += wsum[w];
-    bool alive = (i < rc.N) && !dead;
-    if(dead)
+    const uint4 off = blockOff[blockIdx.x];
+    const uint32_t fill = sched->fill;
+    bool alive = inRange && !freeSlot;
+    if(freeSlot)
     {
-        uint32_t rank = blockOff[blockIdx.x] + waveOff + uint32_t(__popcll(b & lt));
-        if(rank < sched->fill)
+        uint32_t rank = off.x + waveOff + uint32_t(__popcll(b & lt));
+        if(rank < fill)
         {
             // KCGenerateCamRaysStochastic (RayGenKernels.kt.h:155-226) +
             // KCInitPathStateIndirect (PathTracerRendererBase.cu:35-57)
@@ -219,17 +304,22 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ blockOff,
             S.meta[i] = MPack(0, ST_ALIVE, RT_CAMERA, false, false);
             alive = true;
         }
-        else if(MStatus(m) != ST_INVALID)
+        else if(status != ST_INVALID)
         {
             S.meta[i] = MPack(MDepth(m), ST_INVALID, MType(m), MDispersed(m), false);
         }
     }
-    // append live slots to the active list, one atomic per wave
+    // active list position: live slots of earlier blocks, slots refilled in
+    // earlier blocks, then this block's earlier live slots (ascending order)
     uint64_t am = __ballot(alive);
-    uint32_t waveBase = 0;
-    if(lane == 0 && am) waveBase = atomicAdd(&sched->activeCount, uint32_t(__popcll(am)));
-    waveBase = __shfl(waveBase, 0);
-    if(alive) active[waveBase + uint32_t(__popcll(am & lt))] = i;
+    if(lane == 0) asum[wave] = uint32_t(__popcll(am));
+    __syncthreads();
+    if(alive)
+    {
+        uint32_t pos = off.y + min(off.x, fill);
+        for(uint32_t w = 0; w < wave; w++) pos += asum[w];
+        active[pos + uint32_t(__popcll(am & lt))] = i;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -244,8 +334,12 @@ KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __re
     uint32_t n = sched->activeCount;
     if(gtid == 0) sched->extRays += n;
     uint32_t nodes = 0, tris = 0;
-    for(uint32_t k = gtid; k < n; k += ctx.gridThreads)
+    for(;;)
     {
+        uint32_t k = WaveFetch(&sched->traceNext);
+        if(k >= n) break;
+        k += threadIdx.x & 63u;
+        if(k >= n) continue;
         uint32_t i = active[k];
         float4 o = S.rayO[i], d = S.rayD[i];
         HitRec h = Trace<false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
@@ -407,12 +501,16 @@ __device__ __forceinline__ float PdfLightBoundary(const SceneView& s)
 template<bool SPECTRAL>
 __global__ void __launch_bounds__(kSlotTPB)
 KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
-       const Sched* __restrict__ sched, uint32_t gridThreads)
+       Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
 {
     const SceneView& s = rc.scene;
     uint32_t n = sched->activeCount;
-    for(uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridThreads)
+    for(;;)
     {
+        uint32_t k = WaveFetch(&sched->shadeNext);
+        if(k >= n) break;
+        k += threadIdx.x & 63u;
+        if(k >= n) continue;
         uint32_t i = active[k];
         uint4 hh = S.hit[i];
         uint32_t prim = hh.x;
@@ -511,7 +609,6 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
         else mat.frontIoR = mat.backIoR = 1.0f;
         V3 wO = Normalize(-rdir);
         bool specular = IsSpecular(Specularity(mat));
-        bool pending = false;
         if(rc.mode != MRAY_SAMPLE_PURE)
         {
             float rn[3];
@@ -545,10 +642,19 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
                 type = RT_SHADOW;
                 if((depth + 2u) <= rc.rr1)
                 {
-                    pending = true;
                     S.sO[i] = make_float4(spos.x, spos.y, spos.z, kEps);
                     S.sD[i] = make_float4(sdir.x, sdir.y, sdir.z, length * (1.0f - kLargeEps));
                     S.sRad[i] = make_float4(srad.x, srad.y, srad.z, srad.w);
+                    // shadow queue append: one atomic per wave, from the
+                    // lowest active lane (only lanes with a shadow ray run this)
+                    const uint64_t pm = __ballot(1);
+                    const uint32_t lane = threadIdx.x & 63u;
+                    const uint32_t leader = uint32_t(__ffsll((unsigned long long)pm) - 1);
+                    uint32_t qBase = 0;
+                    if(lane == leader) qBase = atomicAdd(&sched->shadowCount, uint32_t(__popcll(pm)));
+                    qBase = __shfl(qBase, int(leader));
+                    const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+                    shadowQ[qBase + uint32_t(__popcll(pm & below))] = i;
                 }
             }
         }
@@ -582,74 +688,36 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
             S.rayD[i] = make_float4(bs.wIDir.x, bs.wIDir.y, bs.wIDir.z, 3.402823466e+38f);
         }
         S.rng[i] = st;
-        S.meta[i] = MPack(depth, dead ? ST_DEAD : ST_ALIVE, type, dispersed, pending);
+        S.meta[i] = MPack(depth, dead ? ST_DEAD : ST_ALIVE, type, dispersed, false);
     }
 }
 
 // ---------------------------------------------------------------------------
 // Shadow rays + accumulation + film
 // ---------------------------------------------------------------------------
-template<bool SPECTRAL>
+// KCAccumulateShadowRaysPT: a shadow ray that reaches its light point adds
+// its precomputed NEE radiance to the path.
 __global__ void __launch_bounds__(kTraceTPB)
-KShadow(TraceCtx ctx, SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
-        Sched* __restrict__ sched)
+KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __restrict__ sched)
 {
     __shared__ uint32_t lstack[kLdsStack * kTraceTPB];
-    const uint32_t lane = threadIdx.x & 63u;
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = sched->activeCount;
-    const uint32_t iters = (n + ctx.gridThreads - 1) / ctx.gridThreads;
-    for(uint32_t it = 0; it < iters; it++)
+    uint32_t n = sched->shadowCount;
+    if(gtid == 0) sched->shadowRays += n;
+    for(;;)
     {
-        uint32_t k = gtid + it * ctx.gridThreads;
-        bool valid = k < n;
-        uint32_t i = valid ? active[k] : 0u;
-        uint32_t m = valid ? S.meta[i] : 0u;
-        bool pending = valid && MPending(m);
-        bool died = valid && (MStatus(m) == ST_DEAD);
-        if(pending)
+        uint32_t k = WaveFetch(&sched->shadowNext);
+        if(k >= n) break;
+        k += threadIdx.x & 63u;
+        if(k >= n) continue;
+        uint32_t i = shadowQ[k];
+        float4 o = S.sO[i], d = S.sD[i];
+        HitRec h = Trace<true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                      o.w, d.w);
+        if(h.prim == kInvalid)
         {
-            float4 o = S.sO[i], d = S.sD[i];
-            HitRec h = Trace<true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z},
-                                          V3{d.x, d.y, d.z}, o.w, d.w);
-            if(h.prim == kInvalid)
-            {
-                float4 a = S.rad[i], b = S.sRad[i];
-                S.rad[i] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
-            }
-        }
-        if(died)
-        {
-            float4 r4 = S.rad[i];
-            V4 rad = {r4.x, r4.y, r4.z, r4.w};
-            if constexpr(SPECTRAL)
-            {
-                float4 w4 = S.waves[i], p4 = S.wpdf[i];
-                rad = SpectrumToRGB(rc.spec, rad, V4{w4.x, w4.y, w4.z, w4.w},
-                                    V4{p4.x, p4.y, p4.z, p4.w}, MDispersed(m));
-            }
-            float w = S.filterW[i];
-            float r = rad.x, g = rad.y, b = rad.z;
-            if(!(isfinite(r) && isfinite(g) && isfinite(b) && isfinite(rad.w)))
-            {
-                r = 1e7f; g = 0.0f; b = 1e7f;
-                w = w * 128.0f;
-            }
-            uint32_t px = S.pixel[i];
-            size_t P = rc.P;
-            atomicAdd(rc.film + px, r);
-            atomicAdd(rc.film + P + px, g);
-            atomicAdd(rc.film + 2 * P + px, b);
-            atomicAdd(rc.film + 3 * P + px, w);
-            S.meta[i] = MPack(MDepth(m), ST_INVALID, MType(m), MDispersed(m), false);
-        }
-        else if(pending)
-            S.meta[i] = m & ~(1u << 21);
-        uint64_t pm = __ballot(pending), dm = __ballot(died);
-        if(lane == 0)
-        {
-            if(pm) atomicAdd(&sched->shadowRays, (unsigned long long)__popcll(pm));
-            if(dm) atomicAdd(&sched->deadTotal, (unsigned long long)__popcll(dm));
+            float4 a = S.rad[i], b = S.sRad[i];
+            S.rad[i] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
         }
     }
 }
@@ -683,6 +751,10 @@ __global__ void KResetSlots(uint32_t* __restrict__ meta, uint32_t* __restrict__ 
         sched->fill = 0;
         sched->activeCount = 0;
         sched->iterations = 0;
+        sched->shadowCount = 0;
+        sched->traceNext = 0;
+        sched->shadeNext = 0;
+        sched->shadowNext = 0;
     }
 }
 
@@ -702,11 +774,14 @@ struct MRayRenderer
     RenderConst rc{};
     Sched* dSched = nullptr;
     Sched* hSched = nullptr;      // pinned mirror
-    uint32_t* dBlockCnt = nullptr;
+    uint4* dBlockCnt = nullptr;
+    uint32_t* dShadowQ = nullptr;
     uint32_t* dActive = nullptr;
     uint32_t* dSeeds = nullptr;
     uint32_t nBlocks = 0;
-    uint32_t shadeGrid = 0;
+    uint32_t traceBlocks = 0;     // resident blocks (occupancy) per launch
+    uint32_t shadeBlocks = 0;
+    uint32_t shadowBlocks = 0;
     unsigned long long limit = 0;
     Sched last{};                 // stats at the previous mray_renderer_stats()
     hipStream_t stream = nullptr;
@@ -763,9 +838,9 @@ static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
     }
     int mk;
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    KRefillCount<<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, N, r->dBlockCnt, r->dSched);
+    KRefillCount<<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, N, r->dBlockCnt);
     KRefillScan<<<1, 1024, 0, s>>>(r->dBlockCnt, r->nBlocks, r->dSched);
-    uint32_t traceBlocks = ctx.gridThreads / kTraceTPB;
+    const uint32_t traceBlocks = r->traceBlocks;
     if(r->rc.spectral)
         KRefillApply<true><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
                                                            r->dActive);
@@ -779,16 +854,13 @@ static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
         KTrace<false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     if(r->rc.spectral)
-        KShade<true><<<r->shadeGrid / kSlotTPB, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive,
-                                                                  r->dSched, r->shadeGrid);
+        KShade<true><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
+                                                         r->dShadowQ);
     else
-        KShade<false><<<r->shadeGrid / kSlotTPB, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive,
-                                                                   r->dSched, r->shadeGrid);
+        KShade<false><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
+                                                          r->dShadowQ);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    if(r->rc.spectral)
-        KShadow<true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->rc, r->dActive, r->dSched);
-    else
-        KShadow<false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->rc, r->dActive, r->dSched);
+    KShadow<<<r->shadowBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     MRAY_HIP_CHECK(hipGetLastError());
     return MRAY_OK;
@@ -887,7 +959,8 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
     }
     MRAY_TRY(A((void**)&r->rc.film, size_t(P) * 4 * sizeof(float)));
     r->nBlocks = DivUp(N, kSlotTPB);
-    MRAY_TRY(A((void**)&r->dBlockCnt, size_t(r->nBlocks) * 4));
+    MRAY_TRY(A((void**)&r->dBlockCnt, size_t(r->nBlocks) * sizeof(uint4)));
+    MRAY_TRY(A((void**)&r->dShadowQ, size_t(N) * 4));
     MRAY_TRY(A((void**)&r->dActive, size_t(N) * 4));
     MRAY_TRY(A((void**)&r->dSched, sizeof(Sched)));
     MRAY_TRY(A((void**)&r->dSeeds, size_t(N) * 4));
@@ -912,7 +985,23 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
     MRAY_HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     MRAY_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    r->shadeGrid = uint32_t(std::max(1, prop.multiProcessorCount)) * 8u * kSlotTPB;
+    // Persistent launches are sized to what is resident at once (occupancy x
+    // CUs): a block that is not resident would start only after a resident
+    // one finished, stretching the tail.  The traversal grid never exceeds
+    // the accelerator's spill-stack allocation (ctx.gridThreads).
+    const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
+    auto resident = [&](auto kernel, uint32_t tpb, uint32_t cap) -> int
+    {
+        int nb = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, int(tpb), 0);
+        if(e != hipSuccess || nb < 1) nb = 1;
+        return int(std::min<uint64_t>(uint64_t(cus) * uint32_t(nb), cap));
+    };
+    const uint32_t maxTraceBlocks = a->ctx.gridThreads / kTraceTPB;
+    r->traceBlocks = uint32_t(resident(KTrace<false>, kTraceTPB, maxTraceBlocks));
+    r->shadowBlocks = uint32_t(resident(KShadow, kTraceTPB, maxTraceBlocks));
+    r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true>, kSlotTPB, 1u << 20)
+                                          : resident(KShade<false>, kSlotTPB, 1u << 20));
     return ResetRenderer(r);
 }
 
