@@ -247,6 +247,58 @@ int mray_renderer_film_device(MRayRenderer* r, float** dFilm);
 /* ---------------------------------------------------------------------- */
 int mray_spectral_tables(int* hN, float* hLut, float* hCmf, float* hIllum, float* hXyzToRgb);
 
+
+/* ---------------------------------------------------------------------- */
+/* Neural radiance cache (no reference counterpart: the reference keeps    */
+/* only an irradiance hash grid, Tracer/HashGrid.h:17-476).  Multiresolution*/
+/* hash encoding + 64-wide MLP (6 layers, bf16 MFMA, fp32 accumulation),    */
+/* trained with Adam on relative L2 loss.                                    */
+/* Query record: float[11] = position in [0,1]^3, direction (theta, phi)    */
+/* and normal (theta, phi) in [0,1], roughness in [0,1], albedo RGB.         */
+/* ---------------------------------------------------------------------- */
+typedef struct MRayNrc MRayNrc;
+
+typedef struct MRayNrcConfig
+{
+    uint32_t levels;           /* hash levels (<= 16), 2 features each        */
+    uint32_t log2TableSize;    /* entries per level = 2^log2TableSize         */
+    uint32_t minResolution;    /* N_min                                       */
+    uint32_t maxResolution;    /* N_max                                       */
+    uint32_t maxBatch;         /* largest training batch (multiple of 32)     */
+    float    learningRate;
+    float    beta1, beta2, epsilon;
+    uint32_t reserved;
+    uint64_t seed;             /* parameter initialisation                    */
+} MRayNrcConfig;
+
+enum
+{
+    MRAY_NRC_WEIGHTS = 0,      /* fp32 masters [6][64 in][64 out]             */
+    MRAY_NRC_WEIGHT_GRADS = 1,
+    MRAY_NRC_GRID = 2,         /* fp32 masters [levels][2^log2T][2]           */
+    MRAY_NRC_GRID_GRADS = 3
+};
+
+int mray_nrc_create(const MRayNrcConfig* hCfg, MRayNrc** out);
+int mray_nrc_destroy(MRayNrc* nrc);
+/* Radiance of `count` records (or *dCount when dCount is not NULL, read on
+ * the device): dOut [n][3].                                                  */
+int mray_nrc_infer(MRayNrc* nrc, const float* dRecords, uint32_t count, const uint32_t* dCount,
+                   float* dOut, void* stream);
+/* Gradients of the relative L2 loss of one batch (count a multiple of 32),
+ * accumulated into the gradient buffers; mray_nrc_adam applies and clears them. */
+int mray_nrc_backward(MRayNrc* nrc, const float* dRecords, const float* dTargets, uint32_t count,
+                      void* stream);
+int mray_nrc_adam(MRayNrc* nrc, void* stream);
+/* backward + adam */
+int mray_nrc_train(MRayNrc* nrc, const float* dRecords, const float* dTargets, uint32_t count,
+                   void* stream);
+int mray_nrc_last_loss(MRayNrc* nrc, float* hLoss);
+int mray_nrc_param_counts(const MRayNrc* nrc, uint64_t* hMlp, uint64_t* hGrid);
+int mray_nrc_read(MRayNrc* nrc, uint32_t which, float* hOut);
+int mray_nrc_write(MRayNrc* nrc, uint32_t which, const float* hIn);
+int mray_nrc_zero_grads(MRayNrc* nrc, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,6 +97,28 @@ class MRayRenderStats(C.Structure):
     ]
 
 
+NRC_WEIGHTS = 0
+NRC_WEIGHT_GRADS = 1
+NRC_GRID = 2
+NRC_GRID_GRADS = 3
+
+
+class MRayNrcConfig(C.Structure):
+    _fields_ = [
+        ("levels", C.c_uint32),
+        ("log2TableSize", C.c_uint32),
+        ("minResolution", C.c_uint32),
+        ("maxResolution", C.c_uint32),
+        ("maxBatch", C.c_uint32),
+        ("learningRate", C.c_float),
+        ("beta1", C.c_float),
+        ("beta2", C.c_float),
+        ("epsilon", C.c_float),
+        ("reserved", C.c_uint32),
+        ("seed", C.c_uint64),
+    ]
+
+
 def render_params(width, height, spp, rr=(8, 8), mode=SAMPLE_NEE_MIS, spectral=False,
                   seed=0, filter_radius=1.0, max_ray_count=0):
     """MRayRenderParams with the reference's renderer-attribute meaning

@@ -32,6 +32,9 @@ EXPORTS = [
     "mray_renderer_render_all", "mray_renderer_stats", "mray_renderer_read_film",
     "mray_renderer_reset", "mray_renderer_film_device", "mray_spectral_tables",
     "mray_renderer_set_measurement", "mray_renderer_phase_times",
+    "mray_nrc_create", "mray_nrc_destroy", "mray_nrc_infer", "mray_nrc_backward", "mray_nrc_adam",
+    "mray_nrc_train", "mray_nrc_last_loss", "mray_nrc_param_counts", "mray_nrc_read",
+    "mray_nrc_write", "mray_nrc_zero_grads",
 ]
 
 
@@ -76,6 +79,17 @@ def lib():
     L.mray_renderer_reset.argtypes = [P]
     L.mray_renderer_film_device.argtypes = [P, C.POINTER(P)]
     L.mray_spectral_tables.argtypes = [C.POINTER(C.c_int), P, P, P, P]
+    L.mray_nrc_create.argtypes = [C.POINTER(abi.MRayNrcConfig), C.POINTER(P)]
+    L.mray_nrc_destroy.argtypes = [P]
+    L.mray_nrc_infer.argtypes = [P, P, u32, P, P, P]
+    L.mray_nrc_backward.argtypes = [P, P, P, u32, P]
+    L.mray_nrc_adam.argtypes = [P, P]
+    L.mray_nrc_train.argtypes = [P, P, P, u32, P]
+    L.mray_nrc_last_loss.argtypes = [P, C.POINTER(C.c_float)]
+    L.mray_nrc_param_counts.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
+    L.mray_nrc_read.argtypes = [P, u32, P]
+    L.mray_nrc_write.argtypes = [P, u32, P]
+    L.mray_nrc_zero_grads.argtypes = [P, P]
     L.mray_renderer_set_measurement.argtypes = [P, u32, u32]
     L.mray_renderer_phase_times.argtypes = [P, P, P]
     _lib = L
@@ -304,3 +318,74 @@ class Renderer:
         out = np.zeros((4, self.height, self.width), np.float32)
         check(lib().mray_renderer_read_film(self.h, out.ctypes.data))
         return out
+
+
+class Nrc:
+    """Neural radiance cache (mray_nrc_*): hash grid + 64-wide MLP on the GPU.
+    Records are float32 [n, 11] device tensors (see include/mray_amd.h)."""
+
+    def __init__(self, levels=16, log2_table=19, min_res=16, max_res=2048, max_batch=1 << 16,
+                 lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-15, seed=1):
+        require_gpu()
+        cfg = abi.MRayNrcConfig()
+        cfg.levels, cfg.log2TableSize = levels, log2_table
+        cfg.minResolution, cfg.maxResolution = min_res, max_res
+        cfg.maxBatch = max_batch
+        cfg.learningRate, cfg.beta1, cfg.beta2, cfg.epsilon = lr, beta1, beta2, eps
+        cfg.seed = seed
+        self.h = C.c_void_p()
+        check(lib().mray_nrc_create(C.byref(cfg), C.byref(self.h)))
+        m, g = C.c_uint64(), C.c_uint64()
+        check(lib().mray_nrc_param_counts(self.h, C.byref(m), C.byref(g)))
+        self.mlp_params, self.grid_params = m.value, g.value
+        self.levels, self.log2_table = levels, log2_table
+
+    def close(self):
+        if self.h:
+            lib().mray_nrc_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def infer(self, d_rec):
+        torch = require_gpu()
+        n = d_rec.shape[0]
+        out = torch.empty((n, 3), dtype=torch.float32, device=d_rec.device)
+        check(lib().mray_nrc_infer(self.h, _ptr(d_rec), n, None, _ptr(out), None))
+        torch.cuda.synchronize()
+        return out
+
+    def backward(self, d_rec, d_target):
+        check(lib().mray_nrc_backward(self.h, _ptr(d_rec), _ptr(d_target), d_rec.shape[0], None))
+
+    def adam(self):
+        check(lib().mray_nrc_adam(self.h, None))
+
+    def train(self, d_rec, d_target):
+        check(lib().mray_nrc_train(self.h, _ptr(d_rec), _ptr(d_target), d_rec.shape[0], None))
+
+    def loss(self):
+        v = C.c_float()
+        check(lib().mray_nrc_last_loss(self.h, C.byref(v)))
+        return v.value
+
+    def zero_grads(self):
+        check(lib().mray_nrc_zero_grads(self.h, None))
+
+    def read(self, which):
+        import numpy as np
+        n = self.mlp_params if which in (abi.NRC_WEIGHTS, abi.NRC_WEIGHT_GRADS) else self.grid_params
+        out = np.empty(n, np.float32)
+        check(lib().mray_nrc_read(self.h, which, abi.fptr(out)))
+        if which in (abi.NRC_WEIGHTS, abi.NRC_WEIGHT_GRADS):
+            return out.reshape(6, 64, 64)
+        return out.reshape(self.levels, 1 << self.log2_table, 2)
+
+    def write(self, which, arr):
+        import numpy as np
+        a = np.ascontiguousarray(arr, np.float32).reshape(-1)
+        check(lib().mray_nrc_write(self.h, which, abi.fptr(a)))

@@ -28,6 +28,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(abi.MRayCamera) == 52
     assert ctypes.sizeof(abi.MRayRenderParams) == 48
     assert ctypes.sizeof(abi.MRayRenderStats) == 56
+    assert ctypes.sizeof(abi.MRayNrcConfig) == 48
 
 
 def test_error_path_without_gpu_work():
