@@ -299,6 +299,29 @@ int mray_nrc_read(MRayNrc* nrc, uint32_t which, float* hOut);
 int mray_nrc_write(MRayNrc* nrc, uint32_t which, const float* hIn);
 int mray_nrc_zero_grads(MRayNrc* nrc, void* stream);
 
+
+/* Renderer + radiance cache (PathTracerRGB).  One path slot in 2^trainShift
+ * is a training path (full length, its vertices become training pairs); the
+ * others terminate into the cache once the path spread exceeds
+ * spreadThreshold x the primary spread (Mueller et al. 2021, sec. 4).  After
+ * every frame the cache takes trainSteps Adam steps of trainBatch pairs
+ * (trainBatch * trainSteps a power of two: the pair ring).                   */
+typedef struct MRayNrcRenderParams
+{
+    uint32_t trainShift;
+    float    spreadThreshold;
+    uint32_t trainBatch;
+    uint32_t trainSteps;
+} MRayNrcRenderParams;
+
+/* nrc == NULL detaches.                                                      */
+int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderParams* hParams);
+/* One frame: spp more samples per pixel into the film (bounces until they are
+ * all finished), then the cache's training steps when one is attached.       */
+int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* hStats);
+/* Cache queries answered and training pairs produced since the last reset.  */
+int mray_renderer_cache_stats(MRayRenderer* r, uint64_t* hQueries, uint64_t* hTrainingPairs);
+
 #ifdef __cplusplus
 }
 #endif

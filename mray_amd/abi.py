@@ -119,6 +119,15 @@ class MRayNrcConfig(C.Structure):
     ]
 
 
+class MRayNrcRenderParams(C.Structure):
+    _fields_ = [
+        ("trainShift", C.c_uint32),
+        ("spreadThreshold", C.c_float),
+        ("trainBatch", C.c_uint32),
+        ("trainSteps", C.c_uint32),
+    ]
+
+
 def render_params(width, height, spp, rr=(8, 8), mode=SAMPLE_NEE_MIS, spectral=False,
                   seed=0, filter_radius=1.0, max_ray_count=0):
     """MRayRenderParams with the reference's renderer-attribute meaning

@@ -68,6 +68,33 @@ struct SlotSoA
     float4* sRad;
     float4* waves;     // spectral only
     float4* wpdf;
+    float2* spread;    // radiance cache: (a0, sum of sqrt spreads) of the path
+};
+
+// Radiance-cache state of the renderer (RGB mode).  One path slot in
+// 2^trainShift is a training path: it never terminates into the cache and
+// records every surface vertex (query record, throughput, radiance so far);
+// when it ends, each vertex yields a training pair (record, reflected
+// radiance = (final - so far) / throughput).  Other paths terminate into the
+// cache once their spread is large (Mueller et al. 2021, sec. 4).
+constexpr uint32_t kNrcRec = 11;
+constexpr uint32_t kNrcVert = kNrcRec + 6;   // record, throughput, radiance so far
+struct NrcRenderState
+{
+    uint32_t on;
+    uint32_t trainShift;
+    uint32_t maxVerts;
+    uint32_t ringSize;        // training pairs kept (power of two)
+    float spreadC;
+    float boxMin[3];
+    float boxInvExtent;
+    float* trainStore;        // [N >> trainShift][maxVerts][kNrcVert]
+    uint32_t* trainCount;     // [N >> trainShift]
+    float* queryRec;          // [N][11]
+    uint32_t* querySlot;      // [N]
+    float* queryOut;          // [N][3]
+    float* ringRec;           // [ringSize][11]
+    float* ringTgt;           // [ringSize][3]
 };
 
 struct Sched
@@ -87,7 +114,9 @@ struct Sched
     uint32_t traceNext;     // dynamic work counters (64 items per wave fetch)
     uint32_t shadeNext;
     uint32_t shadowNext;
-    uint32_t pad;
+    uint32_t queryCount;      // radiance-cache queries of this iteration
+    unsigned long long ringCount;   // training pairs written so far
+    unsigned long long queries;     // radiance-cache queries so far
 };
 
 // Persistent kernels take work 64 items at a time: one atomic per wave,
@@ -112,6 +141,7 @@ struct RenderConst
     uint32_t mode;
     uint32_t spectral;
     float* film;
+    NrcRenderState nrc;
 };
 
 static constexpr uint32_t kSlotTPB = 256;
@@ -204,6 +234,7 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
         sched->traceNext = 0;
         sched->shadeNext = 0;
         sched->shadowNext = 0;
+        sched->queryCount = 0;
         sched->iterations += 1;
     }
 }
@@ -239,10 +270,36 @@ __device__ __forceinline__ void SplatToFilm(const SlotSoA& S, const RenderConst&
     atomicAdd(rc.film + 3 * P + px, w);
 }
 
+// A finished training path turns its recorded vertices into training pairs:
+// target = (final radiance - radiance before the vertex) / throughput.
+__device__ __forceinline__ void EmitTrainingPairs(const SlotSoA& S, const RenderConst& rc, uint32_t i,
+                                                  Sched* sched)
+{
+    const NrcRenderState& nr = rc.nrc;
+    uint32_t ts = i >> nr.trainShift;
+    uint32_t nv = nr.trainCount[ts];
+    if(nv == 0) return;
+    float4 fr = S.rad[i];
+    unsigned long long base = atomicAdd(&sched->ringCount, (unsigned long long)nv);
+    for(uint32_t v = 0; v < nv; v++)
+    {
+        const float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + v) * kNrcVert;
+        size_t slot = size_t((base + v) & (nr.ringSize - 1u));
+        for(uint32_t k = 0; k < kNrcRec; k++) nr.ringRec[slot * kNrcRec + k] = vx[k];
+        float fin[3] = {fr.x, fr.y, fr.z};
+        for(uint32_t c = 0; c < 3; c++)
+        {
+            float thr = vx[kNrcRec + c], before = vx[kNrcRec + 3 + c];
+            float tgt = (thr > 1e-12f) ? (fin[c] - before) / thr : 0.0f;
+            nr.ringTgt[slot * 3 + c] = isfinite(tgt) ? MaxF(tgt, 0.0f) : 0.0f;
+        }
+    }
+}
+
 template<bool SPECTRAL>
 __global__ void __launch_bounds__(kSlotTPB)
 KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
-             const Sched* __restrict__ sched, uint32_t* __restrict__ active)
+             Sched* __restrict__ sched, uint32_t* __restrict__ active)
 {
     __shared__ uint32_t wsum[kSlotTPB / 64], asum[kSlotTPB / 64];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
@@ -251,7 +308,12 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
     uint32_t m = inRange ? S.meta[i] : MPack(0, ST_INVALID, 0, false, false);
     const uint32_t status = MStatus(m);
     const bool freeSlot = inRange && status != ST_ALIVE;
-    if(inRange && status == ST_DEAD) SplatToFilm<SPECTRAL>(S, rc, i, m);
+    if(inRange && status == ST_DEAD)
+    {
+        SplatToFilm<SPECTRAL>(S, rc, i, m);
+        if(!SPECTRAL && rc.nrc.on && (i & ((1u << rc.nrc.trainShift) - 1u)) == 0u)
+            EmitTrainingPairs(S, rc, i, sched);
+    }
     uint64_t b = __ballot(freeSlot);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     if(lane == 0) wsum[wave] = uint32_t(__popcll(b));
@@ -302,6 +364,12 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
             }
             S.rng[i] = st;
             S.meta[i] = MPack(0, ST_ALIVE, RT_CAMERA, false, false);
+            if(!SPECTRAL && rc.nrc.on)
+            {
+                S.spread[i] = make_float2(0.0f, 0.0f);
+                if((i & ((1u << rc.nrc.trainShift) - 1u)) == 0u)
+                    rc.nrc.trainCount[i >> rc.nrc.trainShift] = 0;
+            }
             alive = true;
         }
         else if(status != ST_INVALID)
@@ -609,6 +677,73 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
         else mat.frontIoR = mat.backIoR = 1.0f;
         V3 wO = Normalize(-rdir);
         bool specular = IsSpecular(Specularity(mat));
+        if constexpr(!SPECTRAL)
+        {
+            if(rc.nrc.on)
+            {
+                const NrcRenderState& nr = rc.nrc;
+                // path spread (Mueller et al. 2021, eq. 3-4): a0 at the first
+                // hit, then the sum of sqrt(d^2 / (pdf |cos|)) over later hits
+                float2 sp = S.spread[i];
+                float tHit = __uint_as_float(hh.y);
+                float cosT = MaxF(fabsf(Dot(rdir, sf.geoNormal)), 1e-6f);
+                if(depth == 0u) sp = make_float2(tHit * tHit / (4.0f * kPi * cosT), 0.0f);
+                else
+                {
+                    float pdf = S.prevPdf[i];
+                    if(pdf > 0.0f && isfinite(pdf)) sp.y += sqrtf(tHit * tHit / (pdf * cosT));
+                }
+                S.spread[i] = sp;
+                // query record: position in the scene box, outgoing direction
+                // and normal as (theta, phi), roughness, albedo
+                float rec[kNrcRec];
+                rec[0] = (sf.position.x - nr.boxMin[0]) * nr.boxInvExtent;
+                rec[1] = (sf.position.y - nr.boxMin[1]) * nr.boxInvExtent;
+                rec[2] = (sf.position.z - nr.boxMin[2]) * nr.boxInvExtent;
+                auto sph = [](V3 d, float& th, float& ph)
+                {
+                    th = acosf(ClampF(d.z, -1.0f, 1.0f)) * kInvPi;
+                    ph = (atan2f(d.y, d.x) + kPi) * (0.5f * kInvPi);
+                };
+                sph(wO, rec[3], rec[4]);
+                sph(sf.geoNormal, rec[5], rec[6]);
+                rec[7] = (M.type == MRAY_MAT_UNREAL) ? M.roughness : (M.type == MRAY_MAT_LAMBERT ? 1.0f : 0.0f);
+                rec[8] = M.albedo[0];
+                rec[9] = M.albedo[1];
+                rec[10] = M.albedo[2];
+                const bool training = (i & ((1u << nr.trainShift) - 1u)) == 0u;
+                if(!training && depth >= 1u && sp.y * sp.y > nr.spreadC * sp.x)
+                {
+                    // terminate into the cache: queue the query, the path ends
+                    const uint64_t qm = __ballot(1);
+                    const uint32_t ln = threadIdx.x & 63u;
+                    const uint32_t leader = uint32_t(__ffsll((unsigned long long)qm) - 1);
+                    uint32_t qb = 0;
+                    if(ln == leader) qb = atomicAdd(&sched->queryCount, uint32_t(__popcll(qm)));
+                    qb = __shfl(qb, int(leader));
+                    const uint64_t below = (ln == 0) ? 0ull : ((~0ull) >> (64 - ln));
+                    uint32_t q = qb + uint32_t(__popcll(qm & below));
+                    for(uint32_t k = 0; k < kNrcRec; k++) nr.queryRec[size_t(q) * kNrcRec + k] = rec[k];
+                    nr.querySlot[q] = i;
+                    S.rng[i] = st;
+                    S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
+                    continue;
+                }
+                if(training)
+                {
+                    uint32_t ts = i >> nr.trainShift;
+                    uint32_t nv = nr.trainCount[ts];
+                    if(nv < nr.maxVerts)
+                    {
+                        float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + nv) * kNrcVert;
+                        for(uint32_t k = 0; k < kNrcRec; k++) vx[k] = rec[k];
+                        vx[kNrcRec + 0] = thr.x; vx[kNrcRec + 1] = thr.y; vx[kNrcRec + 2] = thr.z;
+                        vx[kNrcRec + 3] = rad.x; vx[kNrcRec + 4] = rad.y; vx[kNrcRec + 5] = rad.z;
+                        nr.trainCount[ts] = nv + 1u;
+                    }
+                }
+            }
+        }
         if(rc.mode != MRAY_SAMPLE_PURE)
         {
             float rn[3];
@@ -681,7 +816,7 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
         {
             V4 nt = DivideByPDF(t3, bs.pdf);
             S.thr[i] = make_float4(nt.x, nt.y, nt.z, nt.w);
-            if(rc.mode == MRAY_SAMPLE_NEE_MIS) S.prevPdf[i] = bs.pdf;
+            if(rc.mode == MRAY_SAMPLE_NEE_MIS || (!SPECTRAL && rc.nrc.on)) S.prevPdf[i] = bs.pdf;
             V3 nn = bs.passedThrough ? -sf.geoNormal : sf.geoNormal;
             V3 np = Nudge(bs.wIPos, nn);
             S.rayO[i] = make_float4(np.x, np.y, np.z, kLargeEps);
@@ -722,6 +857,25 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
     }
 }
 
+// Radiance-cache answers: a terminated path adds throughput x max(L, 0).
+__global__ void __launch_bounds__(256)
+KNrcApply(SlotSoA S, RenderConst rc, Sched* __restrict__ sched)
+{
+    uint32_t n = sched->queryCount;
+    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if(q == 0) sched->queries += n;
+    for(; q < n; q += gridDim.x * blockDim.x)
+    {
+        uint32_t i = rc.nrc.querySlot[q];
+        const float* L = rc.nrc.queryOut + size_t(q) * 3u;
+        float4 t = S.thr[i], r = S.rad[i];
+        r.x += t.x * MaxF(L[0], 0.0f);
+        r.y += t.y * MaxF(L[1], 0.0f);
+        r.z += t.z * MaxF(L[2], 0.0f);
+        S.rad[i] = r;
+    }
+}
+
 __global__ void KZeroFloats(float* __restrict__ p, size_t n)
 {
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -755,6 +909,9 @@ __global__ void KResetSlots(uint32_t* __restrict__ meta, uint32_t* __restrict__ 
         sched->traceNext = 0;
         sched->shadeNext = 0;
         sched->shadowNext = 0;
+        sched->queryCount = 0;
+        sched->ringCount = 0;
+        sched->queries = 0;
     }
 }
 
@@ -793,7 +950,19 @@ struct MRayRenderer
     size_t evUsed = 0;
     double phaseMs[4] = {0, 0, 0, 0};
     uint64_t phaseLaunches[4] = {0, 0, 0, 0};
+    // radiance cache (mray_renderer_attach_nrc)
+    MRayNrc* nrc = nullptr;
+    MRayNrcRenderParams nrcParams{};
+    DeviceArena nrcArena;
+    // frame schedule (mray_renderer_render_frame)
+    unsigned long long frameLimit = 0;
 };
+
+#define MRAY_TRY_IT(expr)               \
+    do {                                \
+        int r_ = (expr);                \
+        if(r_ != MRAY_OK) return r_;    \
+    } while(0)
 
 // Phase timing with HIP events on the launch stream (refill, trace, shade,
 // shadow); harvested when statistics are read.
@@ -861,9 +1030,22 @@ static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
                                                           r->dShadowQ);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     KShadow<<<r->shadowBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched);
+    if(r->nrc)
+    {
+        // radiance-cache answers for the paths that terminated this bounce
+        const uint32_t* dCount = reinterpret_cast<const uint32_t*>(
+            reinterpret_cast<const char*>(r->dSched) + offsetof(Sched, queryCount));
+        MRAY_TRY_IT(mray_nrc_infer(r->nrc, r->rc.nrc.queryRec, 0, dCount, r->rc.nrc.queryOut, s));
+        KNrcApply<<<r->nBlocks, 256, 0, s>>>(r->S, r->rc, r->dSched);
+    }
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     MRAY_HIP_CHECK(hipGetLastError());
     return MRAY_OK;
+}
+
+__global__ void KSetLimit(Sched* __restrict__ sched, unsigned long long limit)
+{
+    if(threadIdx.x == 0 && blockIdx.x == 0) sched->limit = limit;
 }
 
 static int ResetRenderer(MRayRenderer* r)
@@ -876,6 +1058,7 @@ static int ResetRenderer(MRayRenderer* r)
     MRAY_HIP_CHECK(hipGetLastError());
     MRAY_HIP_CHECK(hipStreamSynchronize(s));
     std::memset(&r->last, 0, sizeof(Sched));
+    r->frameLimit = 0;
     return MRAY_OK;
 }
 
@@ -1057,6 +1240,7 @@ int mray_renderer_destroy(MRayRenderer* r)
     (void)hipDeviceSynchronize();
     for(hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if(r->hSched) (void)hipHostFree(r->hSched);
+    r->nrcArena.Release();
     r->arena.Release();
     delete r;
     return MRAY_OK;
@@ -1155,6 +1339,122 @@ int mray_renderer_film_device(MRayRenderer* r, float** dFilm)
 {
     if(!r || !dFilm) { SetError("film_device: NULL argument"); return MRAY_ERR_INVALID; }
     *dFilm = r->rc.film;
+    return MRAY_OK;
+}
+
+int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderParams* p)
+{
+    if(!r || !p) { SetError("attach_nrc: NULL argument"); return MRAY_ERR_INVALID; }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    if(!nrc)
+    {
+        r->nrc = nullptr;
+        r->rc.nrc.on = 0;
+        r->nrcArena.Release();
+        return MRAY_OK;
+    }
+    if(r->rc.spectral)
+    {
+        SetError("attach_nrc: the radiance cache runs with PathTracerRGB only");
+        return MRAY_ERR_UNSUPPORTED;
+    }
+    if(p->trainShift > 16 || p->trainBatch == 0 || (p->trainBatch % 32u) != 0 || p->trainSteps == 0 ||
+       !(p->spreadThreshold > 0.0f))
+    {
+        SetError("attach_nrc: trainShift <= 16, trainBatch a non-zero multiple of 32, "
+                 "trainSteps > 0, spreadThreshold > 0");
+        return MRAY_ERR_INVALID;
+    }
+    uint64_t ring = uint64_t(p->trainBatch) * p->trainSteps;
+    if((ring & (ring - 1)) != 0 || ring > (1u << 24))
+    {
+        SetError("attach_nrc: trainBatch * trainSteps must be a power of two <= 2^24");
+        return MRAY_ERR_INVALID;
+    }
+    r->nrcArena.Release();
+    NrcRenderState& nr = r->rc.nrc;
+    nr = NrcRenderState{};
+    const uint32_t N = r->rc.N;
+    const uint32_t slots = (N >> p->trainShift) + 1u;
+    nr.trainShift = p->trainShift;
+    nr.maxVerts = std::max(1u, r->rc.rr1);
+    nr.ringSize = uint32_t(ring);
+    nr.spreadC = p->spreadThreshold;
+    auto A = [&](void** ptr, size_t bytes) { return DevAlloc(r->nrcArena, ptr, bytes); };
+    MRAY_TRY_IT(A((void**)&nr.trainStore, size_t(slots) * nr.maxVerts * kNrcVert * 4));
+    MRAY_TRY_IT(A((void**)&nr.trainCount, size_t(slots) * 4));
+    MRAY_TRY_IT(A((void**)&nr.queryRec, size_t(N) * kNrcRec * 4));
+    MRAY_TRY_IT(A((void**)&nr.querySlot, size_t(N) * 4));
+    MRAY_TRY_IT(A((void**)&nr.queryOut, size_t(N) * 3 * 4));
+    MRAY_TRY_IT(A((void**)&nr.ringRec, size_t(ring) * kNrcRec * 4));
+    MRAY_TRY_IT(A((void**)&nr.ringTgt, size_t(ring) * 3 * 4));
+    MRAY_TRY_IT(A((void**)&r->S.spread, size_t(N) * sizeof(float2)));
+    MRAY_HIP_CHECK(hipMemset(nr.trainCount, 0, size_t(slots) * 4));
+    MRAY_HIP_CHECK(hipMemset(nr.ringTgt, 0, size_t(ring) * 3 * 4));
+    MRAY_HIP_CHECK(hipMemset(nr.ringRec, 0, size_t(ring) * kNrcRec * 4));
+    // scene box -> [0,1]^3 with one scale (keeps the aspect)
+    float box[6];
+    MRAY_HIP_CHECK(hipMemcpy(box, r->accel->dSceneBox, sizeof(box), hipMemcpyDeviceToHost));
+    float ext = std::max({box[3] - box[0], box[4] - box[1], box[5] - box[2], 1e-6f});
+    for(int k = 0; k < 3; k++) nr.boxMin[k] = box[k] - 1e-3f * ext;
+    nr.boxInvExtent = 1.0f / (ext * 1.002f);
+    nr.on = 1;
+    r->nrc = nrc;
+    r->nrcParams = *p;
+    return MRAY_OK;
+}
+
+int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* hStats)
+{
+    if(!r || spp == 0) { SetError("render_frame: NULL renderer or spp == 0"); return MRAY_ERR_INVALID; }
+    hipStream_t s = r->stream;
+    r->frameLimit += (unsigned long long)spp * r->rc.P;
+    KSetLimit<<<1, 1, 0, s>>>(r->dSched, r->frameLimit);
+    MRAY_HIP_CHECK(hipGetLastError());
+    MRayRenderStats total{};
+    for(;;)
+    {
+        MRAY_TRY_IT(mray_renderer_iterate(r, 8, nullptr));
+        MRayRenderStats st{};
+        MRAY_TRY_IT(mray_renderer_stats(r, &st));
+        total.deadPaths += st.deadPaths;
+        total.extensionRays += st.extensionRays;
+        total.shadowRays += st.shadowRays;
+        total.iterations += st.iterations;
+        total.nodeVisits += st.nodeVisits;
+        total.triTests += st.triTests;
+        total.totalDeadPaths = st.totalDeadPaths;
+        if(st.totalDeadPaths >= r->frameLimit) break;
+        if(st.deadPaths == 0 && st.extensionRays == 0)
+        {
+            SetError("render_frame: no progress");
+            return MRAY_ERR_INVALID;
+        }
+    }
+    if(r->nrc)
+    {
+        // online training on the most recent training pairs
+        const NrcRenderState& nr = r->rc.nrc;
+        unsigned long long have = std::min<unsigned long long>(r->hSched->ringCount, nr.ringSize);
+        uint32_t B = r->nrcParams.trainBatch;
+        uint32_t steps = uint32_t(std::min<unsigned long long>(r->nrcParams.trainSteps, have / B));
+        for(uint32_t k = 0; k < steps; k++)
+            MRAY_TRY_IT(mray_nrc_train(r->nrc, nr.ringRec + size_t(k) * B * kNrcRec,
+                                       nr.ringTgt + size_t(k) * B * 3, B, s));
+        MRAY_HIP_CHECK(hipStreamSynchronize(s));
+    }
+    total.finished = 1;
+    if(hStats) *hStats = total;
+    return MRAY_OK;
+}
+
+int mray_renderer_cache_stats(MRayRenderer* r, uint64_t* hQueries, uint64_t* hTrainingPairs)
+{
+    if(!r || !hQueries || !hTrainingPairs) { SetError("cache_stats: NULL argument"); return MRAY_ERR_INVALID; }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    MRAY_HIP_CHECK(hipMemcpy(r->hSched, r->dSched, sizeof(Sched), hipMemcpyDeviceToHost));
+    *hQueries = r->hSched->queries;
+    *hTrainingPairs = r->hSched->ringCount;
     return MRAY_OK;
 }
 

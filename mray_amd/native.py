@@ -22,7 +22,7 @@ class MRayError(RuntimeError):
 
 # Every symbol include/mray_amd.h declares (checked by tests/test_abi.py).
 EXPORTS = [
-    "mray_last_error", "mray_version", "mray_device_count", "mray_device_memory",
+    "mray_last_error", "mray_version", "mray_device_count",
     "mray_prim_centers_aabbs", "mray_reduce_aabb", "mray_morton63", "mray_morton30",
     "mray_radix_sort_temp_bytes", "mray_radix_sort_u64_u32", "mray_lbvh_build",
     "mray_lbvh_temp_bytes",
@@ -35,6 +35,8 @@ EXPORTS = [
     "mray_nrc_create", "mray_nrc_destroy", "mray_nrc_infer", "mray_nrc_backward", "mray_nrc_adam",
     "mray_nrc_train", "mray_nrc_last_loss", "mray_nrc_param_counts", "mray_nrc_read",
     "mray_nrc_write", "mray_nrc_zero_grads",
+    "mray_renderer_attach_nrc", "mray_renderer_render_frame", "mray_renderer_cache_stats",
+    "mray_device_memory",
 ]
 
 
@@ -90,6 +92,9 @@ def lib():
     L.mray_nrc_read.argtypes = [P, u32, P]
     L.mray_nrc_write.argtypes = [P, u32, P]
     L.mray_nrc_zero_grads.argtypes = [P, P]
+    L.mray_renderer_attach_nrc.argtypes = [P, P, C.POINTER(abi.MRayNrcRenderParams)]
+    L.mray_renderer_render_frame.argtypes = [P, u32, C.POINTER(abi.MRayRenderStats)]
+    L.mray_renderer_cache_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.mray_renderer_set_measurement.argtypes = [P, u32, u32]
     L.mray_renderer_phase_times.argtypes = [P, P, P]
     _lib = L
@@ -318,6 +323,26 @@ class Renderer:
         out = np.zeros((4, self.height, self.width), np.float32)
         check(lib().mray_renderer_read_film(self.h, out.ctypes.data))
         return out
+
+    def attach_nrc(self, nrc, train_shift=4, spread=0.01, batch=1 << 14, steps=4):
+        """Terminate paths into `nrc` and train it after every frame (RGB only);
+        nrc=None detaches."""
+        p = abi.MRayNrcRenderParams()
+        p.trainShift, p.spreadThreshold, p.trainBatch, p.trainSteps = train_shift, spread, batch, steps
+        self.nrc = nrc
+        check(lib().mray_renderer_attach_nrc(self.h, nrc.h if nrc is not None else None,
+                                             C.byref(p)))
+
+    def render_frame(self, spp=1):
+        st = abi.MRayRenderStats()
+        check(lib().mray_renderer_render_frame(self.h, spp, C.byref(st)))
+        return {"dead_paths": st.deadPaths, "ext_rays": st.extensionRays,
+                "shadow_rays": st.shadowRays, "iterations": st.iterations}
+
+    def cache_stats(self):
+        q, t = C.c_uint64(), C.c_uint64()
+        check(lib().mray_renderer_cache_stats(self.h, C.byref(q), C.byref(t)))
+        return {"queries": q.value, "training_pairs": t.value}
 
 
 class Nrc:
