@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--grid", type=int, default=1024)
     ap.add_argument("--spheres", type=int, default=2000)
     ap.add_argument("--rgb", action="store_true", help="PathTracerRGB instead of Spectral")
+    ap.add_argument("--builder", choices=("sah", "lbvh"), default="sah",
+                    help="traversal tree: binned SAH (default) or the collapsed Karras LBVH")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--probe-copy-mib", type=int, default=0,
@@ -121,7 +123,7 @@ def main():
                                    height=args.height)
     log(f"scene: {scene.triangle_count} triangles in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    accel = native.Accel(scene)
+    accel = native.Accel(scene, builder=args.builder)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     info = accel.info()

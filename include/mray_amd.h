@@ -182,6 +182,20 @@ size_t mray_lbvh_temp_bytes(uint32_t count);
 typedef struct MRayAccel MRayAccel;
 
 int mray_accel_create(const MRaySceneDesc* hScene, MRayAccel** out);
+
+/* Traversal-tree builder.  Both build the reference's LBVH (K1-K7) on the
+ * GPU -- its products are what mray_accel_read_lbvh returns -- and differ in
+ * the binary tree collapsed into the 4-wide traversal BVH: SAH (default)
+ * uses a binned surface-area-heuristic tree over the same triangle boxes,
+ * LBVH collapses the Karras tree itself.  Hits are identical either way.   */
+enum { MRAY_BUILDER_SAH = 0, MRAY_BUILDER_LBVH = 1 };
+typedef struct MRayAccelOptions
+{
+    uint32_t builder;
+    uint32_t reserved[3];
+} MRayAccelOptions;
+int mray_accel_create_ex(const MRaySceneDesc* hScene, const MRayAccelOptions* hOptions,
+                         MRayAccel** out);
 int mray_accel_destroy(MRayAccel* accel);
 /* Sizes of the built structure: binary node count, wide node count, leaf
  * triangle count.                                                         */

@@ -24,6 +24,13 @@ SAMPLE_NEE_MIS = 2
 
 INVALID_PRIM = 0xFFFFFFFF
 
+BUILDER_SAH = 0
+BUILDER_LBVH = 1
+
+
+class MRayAccelOptions(C.Structure):
+    _fields_ = [("builder", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+
 
 class MRayMaterial(C.Structure):
     _fields_ = [

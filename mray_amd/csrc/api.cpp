@@ -112,7 +112,7 @@ static int ValidateScene(const MRaySceneDesc* s)
 // AcceleratorGroupLBVH::Construct + BaseAcceleratorLBVH::InternalConstruct
 // (AcceleratorLBVH.hpp:433-582, AcceleratorLBVH.cu:537-740): K1..K7 on the
 // GPU, then the 4-wide collapse and the triangle soup.
-static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s)
+static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder)
 {
     hipStream_t st = nullptr;
     const uint32_t n = s->triangleCount;
@@ -177,6 +177,23 @@ static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s)
             return MRAY_ERR_INVALID;
         }
     }
+    if(builder == MRAY_BUILDER_SAH && n > 1)
+    {
+        // traversal tree: binned SAH over the same triangle boxes (the LBVH
+        // above stays on the device for the reference-layout products)
+        std::vector<uint32_t> sNodes, sRange, sOrder;
+        std::vector<float> sBoxes;
+        if(!BuildSahBinary(aabbs, n, sNodes, sBoxes, sRange, sOrder))
+        {
+            SetError("SAH construction produced a malformed tree");
+            return MRAY_ERR_INVALID;
+        }
+        binNodes.swap(sNodes);
+        binBoxes.swap(sBoxes);
+        binRange.swap(sRange);
+        leafOrder.swap(sOrder);
+    }
+    a->builder = builder;
     WideBvh w = CollapseToBvh4(binNodes, binBoxes, binRange, aabbs, leafOrder);
     a->wideNodes = uint32_t(w.nodes.size());
     a->leafSlots = w.leafSlots;
@@ -293,13 +310,21 @@ int mray_lbvh_build(const uint64_t* dSortedCodes, const uint32_t* dLeafOrder,
 
 int mray_accel_create(const MRaySceneDesc* hScene, MRayAccel** out)
 {
-    if(!out) { SetError("out is NULL"); return MRAY_ERR_INVALID; }
+    MRayAccelOptions o{};
+    o.builder = MRAY_BUILDER_SAH;
+    return mray_accel_create_ex(hScene, &o, out);
+}
+
+int mray_accel_create_ex(const MRaySceneDesc* hScene, const MRayAccelOptions* hOpt, MRayAccel** out)
+{
+    if(!out || !hOpt) { SetError("out or options is NULL"); return MRAY_ERR_INVALID; }
+    if(hOpt->builder > MRAY_BUILDER_LBVH) { SetError("unknown builder"); return MRAY_ERR_INVALID; }
     *out = nullptr;
     MRAY_TRY(ValidateScene(hScene));
     MRayAccel* a = new MRayAccel();
     a->nTri = hScene->triangleCount;
     a->nVert = hScene->vertexCount;
-    int r = BuildAccel(a, hScene);
+    int r = BuildAccel(a, hScene, hOpt->builder);
     if(r == MRAY_OK) r = UploadSceneData(a, hScene);
     if(r != MRAY_OK)
     {

@@ -68,6 +68,13 @@ struct WideBvh
     uint32_t leafSlots = 0;
 };
 
+// Binned-SAH binary tree over triangle boxes aabbs [n][6], in the LBVH's
+// binary layout (nodes [n-1][3], boxes [n-1][6], leaf-order ranges [n-1][2])
+// plus its leaf order; false if the result fails ValidateBinaryTree.
+bool BuildSahBinary(const std::vector<float>& aabbs, uint32_t n, std::vector<uint32_t>& nodes,
+                    std::vector<float>& boxes, std::vector<uint32_t>& range,
+                    std::vector<uint32_t>& order);
+
 // Collapse the binary LBVH (reference layout) into BNode4s, top-down,
 // expanding the child with the largest surface area until four children;
 // subtrees of <= kLeafMax triangles become leaves over their leaf-order range.

@@ -42,6 +42,7 @@ struct MRayAccel
     uint32_t nTri = 0, nVert = 0;
     uint32_t wideNodes = 0, leafSlots = 0, maxDepth = 0;
     uint32_t strictTie = 0;       // 1: rebuilt with the (code, index) tie-break
+    uint32_t builder = 0;         // MRAY_BUILDER_* of the traversal tree
     float* dPos = nullptr;
     uint32_t* dIdx = nullptr;
     float* dCenters = nullptr;

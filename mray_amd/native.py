@@ -26,7 +26,7 @@ EXPORTS = [
     "mray_prim_centers_aabbs", "mray_reduce_aabb", "mray_morton63", "mray_morton30",
     "mray_radix_sort_temp_bytes", "mray_radix_sort_u64_u32", "mray_lbvh_build",
     "mray_lbvh_temp_bytes",
-    "mray_accel_create", "mray_accel_destroy", "mray_accel_info", "mray_accel_read_lbvh",
+    "mray_accel_create", "mray_accel_create_ex", "mray_accel_destroy", "mray_accel_info", "mray_accel_read_lbvh",
     "mray_accel_cast_closest", "mray_accel_cast_closest_counted", "mray_accel_cast_visibility",
     "mray_renderer_create", "mray_renderer_destroy", "mray_renderer_iterate",
     "mray_renderer_render_all", "mray_renderer_stats", "mray_renderer_read_film",
@@ -65,6 +65,8 @@ def lib():
     L.mray_lbvh_temp_bytes.argtypes = [u32]
     L.mray_lbvh_build.argtypes = [P, P, P, u32, P, P, P, C.c_size_t, P]
     L.mray_accel_create.argtypes = [C.POINTER(abi.MRaySceneDesc), C.POINTER(P)]
+    L.mray_accel_create_ex.argtypes = [C.POINTER(abi.MRaySceneDesc), C.POINTER(abi.MRayAccelOptions),
+                                       C.POINTER(P)]
     L.mray_accel_destroy.argtypes = [P]
     L.mray_accel_info.argtypes = [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32)]
     L.mray_accel_read_lbvh.argtypes = [P, P, P, P, P]
@@ -201,12 +203,14 @@ def spectral_tables():
 class Accel:
     """Device scene + LBVH + 4-wide BVH (MRayAccel)."""
 
-    def __init__(self, scene):
+    def __init__(self, scene, builder="sah"):
         require_gpu()
         self.scene = scene
         self.desc = scene.desc()
         h = C.c_void_p()
-        check(lib().mray_accel_create(C.byref(self.desc), C.byref(h)))
+        opt = abi.MRayAccelOptions()
+        opt.builder = {"sah": abi.BUILDER_SAH, "lbvh": abi.BUILDER_LBVH}[builder]
+        check(lib().mray_accel_create_ex(C.byref(self.desc), C.byref(opt), C.byref(h)))
         self.h = h
         self.n = scene.triangle_count
 

@@ -51,17 +51,19 @@ def _compare(acc, oscene, rays, brute):
     assert np.array_equal(vis == 1, ~hit)
 
 
-def test_closest_soup_vs_bruteforce(torch_gpu):
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
+def test_closest_soup_vs_bruteforce(torch_gpu, builder):
     from mray_amd import native
     s = scenes.random_triangles(3000, seed=9)
-    acc = native.Accel(s)
+    acc = native.Accel(s, builder=builder)
     _compare(acc, refcpu.OracleScene(s), _random_rays(20000, 1), brute=True)
 
 
-def test_closest_soup_large(torch_gpu):
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
+def test_closest_soup_large(torch_gpu, builder):
     from mray_amd import native
     s = scenes.random_triangles(200000, seed=4)
-    acc = native.Accel(s)
+    acc = native.Accel(s, builder=builder)
     _compare(acc, refcpu.OracleScene(s), _random_rays(100000, 2), brute=False)
 
 
@@ -76,10 +78,11 @@ def test_primary_rays_cornell(torch_gpu):
     _compare(acc, o, rays, brute=True)
 
 
-def test_primary_rays_terrain(torch_gpu):
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
+def test_primary_rays_terrain(torch_gpu, builder):
     from mray_amd import native
     s = scenes.terrain_spheres(grid=256, spheres=300, width=320, height=180)
-    acc = native.Accel(s)
+    acc = native.Accel(s, builder=builder)
     o = refcpu.OracleScene(s)
     params = abi.render_params(320, 180, 1)
     rays, _ = o.camera_rays(params, 320 * 180)
