@@ -22,7 +22,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mpaths/s at 1080p; equal-time relMSE: radiance-cache PT vs plain PT, 1 MI355X"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES = 128          # BNode4
+# bytes a node visit loads: BNode4 = six float4 boxes + uint4 children (112 B);
+# CNode8 = origin/exponent header + 48 B of 8-bit boxes + 32 B children (96 B)
+NODE_BYTES = {4: 112, 8: 96}
 TRI_BYTES = 48            # SoupTri
 RAY_BYTES = 32 + 28       # RayGMem in, hit record out (prim/t/bary + keys)
 
@@ -44,6 +46,8 @@ def parse():
     ap.add_argument("--rgb", action="store_true", help="PathTracerRGB instead of Spectral")
     ap.add_argument("--builder", choices=("sah", "lbvh"), default="sah",
                     help="traversal tree: binned SAH (default) or the collapsed Karras LBVH")
+    ap.add_argument("--width", type=int, choices=(4, 8), default=4,
+                    help="BVH node width: 4 (fp32 boxes) or 8 (compressed, 8-bit boxes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--nrc", type=int, default=1,
@@ -217,7 +221,7 @@ def main():
                                    height=args.height)
     log(f"scene: {scene.triangle_count} triangles in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    accel = native.Accel(scene, builder=args.builder)
+    accel = native.Accel(scene, builder=args.builder, width=args.width)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     info = accel.info()
@@ -270,7 +274,7 @@ def main():
     trace_ms, trace_launches = phases["trace"]
     trace_avg_ms = trace_ms / max(1, trace_launches)
     rays_per_launch = totals["ext_rays"] / max(1, trace_launches)
-    bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES
+    bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES[args.width] + tris_per_ray * TRI_BYTES
     achieved = rays_per_launch * bytes_per_ray / (trace_avg_ms * 1e-3) / 1e9
     traffic = traffic_from_profiles(rays_per_launch)
     result = {
@@ -304,7 +308,7 @@ def main():
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in phases.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": "KTrace (closest hit)",
+            "kernel": f"KTrace (closest hit, {args.width}-wide BVH)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

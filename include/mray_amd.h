@@ -192,7 +192,9 @@ enum { MRAY_BUILDER_SAH = 0, MRAY_BUILDER_LBVH = 1 };
 typedef struct MRayAccelOptions
 {
     uint32_t builder;
-    uint32_t reserved[3];
+    uint32_t width;            /* 4: fp32 4-wide nodes; 8: compressed 8-wide
+                                  nodes (8-bit quantised boxes); 0 = 4     */
+    uint32_t reserved[2];
 } MRayAccelOptions;
 int mray_accel_create_ex(const MRaySceneDesc* hScene, const MRayAccelOptions* hOptions,
                          MRayAccel** out);

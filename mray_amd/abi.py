@@ -29,7 +29,7 @@ BUILDER_LBVH = 1
 
 
 class MRayAccelOptions(C.Structure):
-    _fields_ = [("builder", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+    _fields_ = [("builder", C.c_uint32), ("width", C.c_uint32), ("reserved", C.c_uint32 * 2)]
 
 
 class MRayMaterial(C.Structure):

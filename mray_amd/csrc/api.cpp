@@ -112,7 +112,7 @@ static int ValidateScene(const MRaySceneDesc* s)
 // AcceleratorGroupLBVH::Construct + BaseAcceleratorLBVH::InternalConstruct
 // (AcceleratorLBVH.hpp:433-582, AcceleratorLBVH.cu:537-740): K1..K7 on the
 // GPU, then the 4-wide collapse and the triangle soup.
-static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder)
+static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder, uint32_t width)
 {
     hipStream_t st = nullptr;
     const uint32_t n = s->triangleCount;
@@ -194,11 +194,28 @@ static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder)
         leafOrder.swap(sOrder);
     }
     a->builder = builder;
-    WideBvh w = CollapseToBvh4(binNodes, binBoxes, binRange, aabbs, leafOrder);
-    a->wideNodes = uint32_t(w.nodes.size());
-    a->leafSlots = w.leafSlots;
-    a->maxDepth = w.maxDepth;
-    MRAY_TRY(Upload(ar, &a->dNodes, w.nodes.data(), w.nodes.size()));
+    a->width = width;
+    if(width == 8)
+    {
+        CBvh8 w;
+        if(!CollapseToCBvh8(binNodes, binBoxes, binRange, aabbs, leafOrder, w))
+        {
+            SetError("compressed 8-wide BVH: a child box has no conservative 8-bit encoding");
+            return MRAY_ERR_INVALID;
+        }
+        a->wideNodes = uint32_t(w.nodes.size());
+        a->leafSlots = w.leafSlots;
+        a->maxDepth = w.maxDepth;
+        MRAY_TRY(Upload(ar, &a->dCNodes, w.nodes.data(), w.nodes.size()));
+    }
+    else
+    {
+        WideBvh w = CollapseToBvh4(binNodes, binBoxes, binRange, aabbs, leafOrder);
+        a->wideNodes = uint32_t(w.nodes.size());
+        a->leafSlots = w.leafSlots;
+        a->maxDepth = w.maxDepth;
+        MRAY_TRY(Upload(ar, &a->dNodes, w.nodes.data(), w.nodes.size()));
+    }
     std::vector<SoupTri> soup(n);
     for(uint32_t p = 0; p < n; p++)
     {
@@ -216,20 +233,22 @@ static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder)
     }
     MRAY_TRY(Upload(ar, &a->dSoup, soup.data(), soup.size()));
 
-    // Traversal stack spill area: <= 3 entries per level below the root.
+    // Traversal stack spill area: <= width-1 entries per level below the root.
     int dev = 0;
     MRAY_HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     MRAY_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
     uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
     a->ctx.gridThreads = cus * 8u * kTraceTPB;
-    uint32_t need = 3u * a->maxDepth + 4u;
+    uint32_t need = (width - 1u) * a->maxDepth + 4u;
     a->ctx.gstackDepth = (need > kLdsStack) ? need - kLdsStack : 1u;
     MRAY_TRY(DevAlloc(ar, (void**)&a->ctx.gstack,
-                      size_t(a->ctx.gstackDepth) * a->ctx.gridThreads * sizeof(uint32_t)));
+                      size_t(a->ctx.gstackDepth) * a->ctx.gridThreads * sizeof(uint2)));
     a->ctx.maxVisits = a->wideNodes + 1u;
     a->ctx.nodes = a->dNodes;
+    a->ctx.cnodes = a->dCNodes;
     a->ctx.soup = a->dSoup;
+    a->ctx.width = width;
     return MRAY_OK;
 }
 
@@ -319,12 +338,14 @@ int mray_accel_create_ex(const MRaySceneDesc* hScene, const MRayAccelOptions* hO
 {
     if(!out || !hOpt) { SetError("out or options is NULL"); return MRAY_ERR_INVALID; }
     if(hOpt->builder > MRAY_BUILDER_LBVH) { SetError("unknown builder"); return MRAY_ERR_INVALID; }
+    const uint32_t width = hOpt->width ? hOpt->width : 4u;
+    if(width != 4u && width != 8u) { SetError("BVH width must be 4 or 8"); return MRAY_ERR_INVALID; }
     *out = nullptr;
     MRAY_TRY(ValidateScene(hScene));
     MRayAccel* a = new MRayAccel();
     a->nTri = hScene->triangleCount;
     a->nVert = hScene->vertexCount;
-    int r = BuildAccel(a, hScene, hOpt->builder);
+    int r = BuildAccel(a, hScene, hOpt->builder, width);
     if(r == MRAY_OK) r = UploadSceneData(a, hScene);
     if(r != MRAY_OK)
     {

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 namespace mray
 {
@@ -79,12 +80,30 @@ bool ValidateBinaryTree(const std::vector<uint32_t>& nodes, const std::vector<ui
     return visitedNodes == nodeCount && visitedLeaves == n;
 }
 
-WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
-                       const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
-                       const std::vector<uint32_t>& leafOrder)
+namespace
 {
-    WideBvh out;
+template<int W>
+struct WideF
+{
+    float box[W][6];
+    uint32_t child[W];
+    int count;
+};
+
+// Top-down collapse of a binary tree into W-wide nodes: expand the child with
+// the largest surface area until W children; subtrees of <= kLeafMax
+// triangles become leaves over their leaf-order range.  Nodes are numbered in
+// depth-first order (root 0).
+template<int W>
+void CollapseWide(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
+                  const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
+                  const std::vector<uint32_t>& leafOrder, std::vector<WideF<W>>& out,
+                  uint32_t& maxDepth, uint32_t& leafSlots)
+{
     const uint32_t n = uint32_t(leafOrder.size());
+    maxDepth = 0;
+    leafSlots = 0;
+    out.clear();
     auto childOf = [&](uint32_t parent, int side) -> BChild
     {
         BChild c;
@@ -107,31 +126,29 @@ WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<
         }
         return c;
     };
-    auto setEmpty = [](BNode4& nd, int k)
+    auto empty = []()
     {
-        nd.lox[k] = nd.loy[k] = nd.loz[k] = INFINITY;
-        nd.hix[k] = nd.hiy[k] = nd.hiz[k] = -INFINITY;
-        nd.child[k] = kInvalid;
+        WideF<W> nd;
+        for(int k = 0; k < W; k++)
+        {
+            nd.box[k][0] = nd.box[k][1] = nd.box[k][2] = INFINITY;
+            nd.box[k][3] = nd.box[k][4] = nd.box[k][5] = -INFINITY;
+            nd.child[k] = kInvalid;
+        }
+        nd.count = 0;
+        return nd;
     };
-    auto setBox = [](BNode4& nd, int k, const float* b)
-    {
-        nd.lox[k] = b[0]; nd.loy[k] = b[1]; nd.loz[k] = b[2];
-        nd.hix[k] = b[3]; nd.hiy[k] = b[4]; nd.hiz[k] = b[5];
-    };
-
-    out.nodes.reserve(std::max<size_t>(1, n / 2));
-    out.nodes.emplace_back();
-    BNode4& root = out.nodes[0];
-    for(int k = 0; k < 4; k++) setEmpty(root, k);
-    root.pad[0] = root.pad[1] = root.pad[2] = root.pad[3] = 0;
-    if(n == 0) return out;
+    out.reserve(std::max<size_t>(1, n / (W / 2)));
+    out.push_back(empty());
+    if(n == 0) return;
     if(n == 1)
     {
-        setBox(root, 0, &leafAabbs[size_t(leafOrder[0]) * 6]);
-        root.child[0] = kLeafFlag | 0u;
-        out.maxDepth = 1;
-        out.leafSlots = 1;
-        return out;
+        std::copy_n(&leafAabbs[size_t(leafOrder[0]) * 6], 6, out[0].box[0]);
+        out[0].child[0] = kLeafFlag | 0u;
+        out[0].count = 1;
+        maxDepth = 1;
+        leafSlots = 1;
+        return;
     }
     struct Work { uint32_t wide, bin, depth; };
     std::vector<Work> stack;
@@ -140,12 +157,12 @@ WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<
     {
         Work w = stack.back();
         stack.pop_back();
-        out.maxDepth = std::max(out.maxDepth, w.depth);
-        BChild kids[4];
+        maxDepth = std::max(maxDepth, w.depth);
+        BChild kids[W];
         int count = 2;
         kids[0] = childOf(w.bin, 0);
         kids[1] = childOf(w.bin, 1);
-        while(count < 4)
+        while(count < W)
         {
             int best = -1;
             float bestArea = -1.0f;
@@ -153,8 +170,8 @@ WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<
             {
                 uint32_t cnt = kids[k].hi - kids[k].lo + 1;
                 if(kids[k].leaf || cnt <= kLeafMax) continue;
-                float a = Area(kids[k].box);
-                if(a > bestArea) { bestArea = a; best = k; }
+                float ar = Area(kids[k].box);
+                if(ar > bestArea) { bestArea = ar; best = k; }
             }
             if(best < 0) break;
             BChild c = kids[best];
@@ -162,31 +179,126 @@ WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<
             kids[count++] = childOf(c.idx, 1);
         }
         // children ordered by leaf-order position (keeps memory order coherent)
-        std::sort(kids, kids + count, [](const BChild& a, const BChild& b) { return a.lo < b.lo; });
-        BNode4 nd;
-        for(int k = 0; k < 4; k++) setEmpty(nd, k);
-        nd.pad[0] = nd.pad[1] = nd.pad[2] = nd.pad[3] = 0;
+        std::sort(kids, kids + count, [](const BChild& x, const BChild& y) { return x.lo < y.lo; });
+        WideF<W> nd = empty();
+        nd.count = count;
         for(int k = 0; k < count; k++)
         {
             const BChild& c = kids[k];
-            setBox(nd, k, c.box);
+            std::copy_n(c.box, 6, nd.box[k]);
             uint32_t cnt = c.hi - c.lo + 1;
             if(cnt <= kLeafMax)
             {
                 nd.child[k] = kLeafFlag | ((cnt - 1) << kLeafCountShift) | c.lo;
-                out.leafSlots++;
+                leafSlots++;
             }
             else
             {
-                uint32_t wi = uint32_t(out.nodes.size());
-                out.nodes.emplace_back();
+                uint32_t wi = uint32_t(out.size());
+                out.push_back(empty());
                 nd.child[k] = wi;
                 stack.push_back({wi, c.idx, w.depth + 1});
             }
         }
-        out.nodes[w.wide] = nd;
+        out[w.wide] = nd;
+    }
+}
+
+// Smallest e with fmaf(255, 2^e, lo) >= hi (conservative decode of the top).
+int QuantExponent(float lo, float hi)
+{
+    float ext = hi - lo;
+    int e = -126;
+    if(ext > 0.0f)
+    {
+        e = int(std::ceil(std::log2(double(ext) / 255.0)));
+        e = std::max(e, -126);
+    }
+    while(e < 127 && std::fma(255.0f, std::ldexp(1.0f, e), lo) < hi) e++;
+    return e;
+}
+} // namespace
+
+WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
+                       const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
+                       const std::vector<uint32_t>& leafOrder)
+{
+    WideBvh out;
+    std::vector<WideF<4>> wide;
+    CollapseWide<4>(binNodes, binBoxes, binRange, leafAabbs, leafOrder, wide, out.maxDepth,
+                    out.leafSlots);
+    out.nodes.resize(wide.size());
+    for(size_t i = 0; i < wide.size(); i++)
+    {
+        BNode4& nd = out.nodes[i];
+        const WideF<4>& w = wide[i];
+        for(int k = 0; k < 4; k++)
+        {
+            nd.lox[k] = w.box[k][0]; nd.loy[k] = w.box[k][1]; nd.loz[k] = w.box[k][2];
+            nd.hix[k] = w.box[k][3]; nd.hiy[k] = w.box[k][4]; nd.hiz[k] = w.box[k][5];
+            nd.child[k] = w.child[k];
+            nd.pad[k] = 0;
+        }
     }
     return out;
+}
+
+bool QuantizeCNode8(const float box[8][6], const uint32_t child[8], int count, CNode8& out)
+{
+    std::memset(&out, 0, sizeof(out));
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for(int k = 0; k < count; k++)
+        for(int d = 0; d < 3; d++)
+        {
+            lo[d] = std::min(lo[d], box[k][d]);
+            hi[d] = std::max(hi[d], box[k][3 + d]);
+        }
+    uint8_t* q[6] = {out.qlox, out.qloy, out.qloz, out.qhix, out.qhiy, out.qhiz};
+    float org[3] = {lo[0], lo[1], lo[2]};
+    int ex[3] = {0, 0, 0};
+    if(count > 0)
+        for(int d = 0; d < 3; d++) ex[d] = QuantExponent(lo[d], hi[d]);
+    out.ox = org[0]; out.oy = org[1]; out.oz = org[2];
+    out.ex = int8_t(ex[0]); out.ey = int8_t(ex[1]); out.ez = int8_t(ex[2]);
+    out.nChild = uint8_t(count);
+    for(int k = 0; k < 8; k++)
+    {
+        out.child[k] = (k < count) ? child[k] : kInvalid;
+        if(k >= count)
+        {
+            // empty slot: lo = 255 > hi = 0 on every axis
+            for(int d = 0; d < 3; d++) { q[d][k] = 255; q[3 + d][k] = 0; }
+            continue;
+        }
+        for(int d = 0; d < 3; d++)
+        {
+            float s = std::ldexp(1.0f, ex[d]);
+            double fl = std::floor(double(box[k][d] - org[d]) / double(s));
+            int ql = int(std::max(0.0, std::min(255.0, fl)));
+            while(ql > 0 && std::fma(float(ql), s, org[d]) > box[k][d]) ql--;
+            double fh = std::ceil(double(box[k][3 + d] - org[d]) / double(s));
+            int qh = int(std::max(0.0, std::min(255.0, fh)));
+            while(qh < 255 && std::fma(float(qh), s, org[d]) < box[k][3 + d]) qh++;
+            if(std::fma(float(ql), s, org[d]) > box[k][d] || std::fma(float(qh), s, org[d]) < box[k][3 + d])
+                return false;
+            q[d][k] = uint8_t(ql);
+            q[3 + d][k] = uint8_t(qh);
+        }
+    }
+    return true;
+}
+
+bool CollapseToCBvh8(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
+                     const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
+                     const std::vector<uint32_t>& leafOrder, CBvh8& out)
+{
+    std::vector<WideF<8>> wide;
+    CollapseWide<8>(binNodes, binBoxes, binRange, leafAabbs, leafOrder, wide, out.maxDepth,
+                    out.leafSlots);
+    out.nodes.resize(wide.size());
+    for(size_t i = 0; i < wide.size(); i++)
+        if(!QuantizeCNode8(wide[i].box, wide[i].child, wide[i].count, out.nodes[i])) return false;
+    return true;
 }
 
 } // namespace mray

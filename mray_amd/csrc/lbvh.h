@@ -61,6 +61,30 @@ struct alignas(16) SoupTri
 };
 static_assert(sizeof(SoupTri) == 48, "SoupTri layout");
 
+// Compressed 8-wide node (one 128-byte line): child boxes quantised to 8
+// bits per plane on a per-node grid, decoded as fmaf(q, 2^e, origin) -- the
+// builder checks every decoded box contains the exact one, so traversal stays
+// conservative.  child[k] as in BNode4; empty slots have q_lo = 255 > q_hi = 0.
+struct alignas(128) CNode8
+{
+    float ox, oy, oz;
+    int8_t ex, ey, ez;
+    uint8_t nChild;
+    uint8_t qlox[8], qhix[8];
+    uint8_t qloy[8], qhiy[8];
+    uint8_t qloz[8], qhiz[8];
+    uint32_t child[8];
+    uint32_t pad[8];
+};
+static_assert(sizeof(CNode8) == 128, "CNode8 must be one 128-byte line");
+
+struct CBvh8
+{
+    std::vector<CNode8> nodes;
+    uint32_t maxDepth = 0;
+    uint32_t leafSlots = 0;
+};
+
 struct WideBvh
 {
     std::vector<BNode4> nodes;
@@ -81,5 +105,12 @@ bool BuildSahBinary(const std::vector<float>& aabbs, uint32_t n, std::vector<uin
 WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
                        const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
                        const std::vector<uint32_t>& leafOrder);
+
+// Same collapse with eight children per node, then quantised into CNode8s;
+// false if a box cannot be represented conservatively.
+bool CollapseToCBvh8(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
+                     const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
+                     const std::vector<uint32_t>& leafOrder, CBvh8& out);
+bool QuantizeCNode8(const float box[8][6], const uint32_t child[8], int count, CNode8& out);
 
 } // namespace mray

@@ -393,11 +393,11 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
 // ---------------------------------------------------------------------------
 // Closest hit
 // ---------------------------------------------------------------------------
-template<bool COUNT>
+template<int W, bool COUNT>
 __global__ void __launch_bounds__(kTraceTPB)
 KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __restrict__ sched)
 {
-    __shared__ uint32_t lstack[kLdsStack * kTraceTPB];
+    __shared__ uint2 lstack[kLdsStack * kTraceTPB];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = sched->activeCount;
     if(gtid == 0) sched->extRays += n;
@@ -410,7 +410,7 @@ KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __re
         if(k >= n) continue;
         uint32_t i = active[k];
         float4 o = S.rayO[i], d = S.rayD[i];
-        HitRec h = Trace<false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+        HitRec h = TraceW<W, false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
                                        o.w, d.w, &nodes, &tris);
         S.hit[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.a),
                               __float_as_uint(h.b));
@@ -832,10 +832,11 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
 // ---------------------------------------------------------------------------
 // KCAccumulateShadowRaysPT: a shadow ray that reaches its light point adds
 // its precomputed NEE radiance to the path.
+template<int W>
 __global__ void __launch_bounds__(kTraceTPB)
 KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __restrict__ sched)
 {
-    __shared__ uint32_t lstack[kLdsStack * kTraceTPB];
+    __shared__ uint2 lstack[kLdsStack * kTraceTPB];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = sched->shadowCount;
     if(gtid == 0) sched->shadowRays += n;
@@ -847,7 +848,7 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
         if(k >= n) continue;
         uint32_t i = shadowQ[k];
         float4 o = S.sO[i], d = S.sD[i];
-        HitRec h = Trace<true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+        HitRec h = TraceW<W, true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
                                       o.w, d.w);
         if(h.prim == kInvalid)
         {
@@ -1018,9 +1019,15 @@ static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
                                                             r->dActive);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     if(r->counting)
-        KTrace<true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+    {
+        if(ctx.width == 8) KTrace<8, true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+        else KTrace<4, true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+    }
     else
-        KTrace<false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+    {
+        if(ctx.width == 8) KTrace<8, false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+        else KTrace<4, false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
+    }
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     if(r->rc.spectral)
         KShade<true><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
@@ -1029,7 +1036,8 @@ static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
         KShade<false><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
                                                           r->dShadowQ);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    KShadow<<<r->shadowBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched);
+    if(ctx.width == 8) KShadow<8><<<r->shadowBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched);
+    else KShadow<4><<<r->shadowBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched);
     if(r->nrc)
     {
         // radiance-cache answers for the paths that terminated this bounce
@@ -1181,8 +1189,11 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
         return int(std::min<uint64_t>(uint64_t(cus) * uint32_t(nb), cap));
     };
     const uint32_t maxTraceBlocks = a->ctx.gridThreads / kTraceTPB;
-    r->traceBlocks = uint32_t(resident(KTrace<false>, kTraceTPB, maxTraceBlocks));
-    r->shadowBlocks = uint32_t(resident(KShadow, kTraceTPB, maxTraceBlocks));
+    const bool w8 = a->ctx.width == 8;
+    r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false>, kTraceTPB, maxTraceBlocks)
+                                 : resident(KTrace<4, false>, kTraceTPB, maxTraceBlocks));
+    r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8>, kTraceTPB, maxTraceBlocks)
+                                  : resident(KShadow<4>, kTraceTPB, maxTraceBlocks));
     r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true>, kSlotTPB, 1u << 20)
                                           : resident(KShade<false>, kSlotTPB, 1u << 20));
     return ResetRenderer(r);

@@ -43,6 +43,7 @@ struct MRayAccel
     uint32_t wideNodes = 0, leafSlots = 0, maxDepth = 0;
     uint32_t strictTie = 0;       // 1: rebuilt with the (code, index) tie-break
     uint32_t builder = 0;         // MRAY_BUILDER_* of the traversal tree
+    uint32_t width = 4;           // 4 (BNode4) or 8 (CNode8)
     float* dPos = nullptr;
     uint32_t* dIdx = nullptr;
     float* dCenters = nullptr;
@@ -55,6 +56,7 @@ struct MRayAccel
     float* dBinBoxes = nullptr;
     uint32_t* dBinRange = nullptr;
     mray::BNode4* dNodes = nullptr;
+    mray::CNode8* dCNodes = nullptr;
     mray::SoupTri* dSoup = nullptr;
     mray::TraceCtx ctx{};
     // shading tables

@@ -1,8 +1,11 @@
-// Closest-hit / any-hit traversal of the 4-wide BVH on gfx950.
+// Closest-hit / any-hit traversal of the wide BVH on gfx950: 4-wide nodes
+// with fp32 boxes (BNode4) or compressed 8-wide nodes (CNode8).
 //
-// One lane per ray.  The traversal stack keeps its first kLdsStack entries in
-// LDS (thread-strided, conflict-free) and spills deeper entries to a global
-// per-lane area, so any tree depth is handled without scratch.
+// One lane per ray.  The traversal stack holds (node, tNear) pairs: its first
+// kLdsStack entries live in LDS (thread-strided, conflict-free), deeper ones
+// spill to a global per-lane area, so any tree depth is handled without
+// scratch.  A popped entry whose tNear is beyond the closest hit found so far
+// is dropped without fetching the node.
 // Triangle test: Ray::IntersectsTriangle (Core/Ray.hpp:121-167), same
 // operation order and explicit FMAs as the reference.
 #pragma once
@@ -14,16 +17,18 @@ namespace mray
 {
 
 static constexpr uint32_t kTraceTPB = 256;
-static constexpr uint32_t kLdsStack = 16;
+static constexpr uint32_t kLdsStack = 12;
 
 struct TraceCtx
 {
-    const BNode4* __restrict__ nodes;
+    const BNode4* __restrict__ nodes;   // width 4
+    const CNode8* __restrict__ cnodes;  // width 8
     const SoupTri* __restrict__ soup;
-    uint32_t* __restrict__ gstack;   // [gstackDepth][gridThreads]
+    uint2* __restrict__ gstack;      // [gstackDepth][gridThreads] (node, tNear bits)
     uint32_t gstackDepth;            // entries beyond kLdsStack per lane
     uint32_t gridThreads;
     uint32_t maxVisits;              // node count: no correct traversal visits more
+    uint32_t width;                  // 4 or 8
 };
 
 struct HitRec
@@ -113,6 +118,51 @@ __device__ __forceinline__ void SlabTest4(const BNode4* __restrict__ np, const R
     }
 }
 
+// Compressed 8-wide node: decode each child box exactly as the builder
+// verified it (fmaf(q, 2^e, origin) contains the true box), then the same
+// robust slab test as SlabTest4.
+__device__ __forceinline__ float Pow2(int e) { return __uint_as_float(uint32_t(e + 127) << 23); }
+__device__ __forceinline__ float QByte(uint32_t lo4, uint32_t hi4, int k)
+{
+    uint32_t w = (k < 4) ? lo4 : hi4;
+    return float((w >> (8 * (k & 3))) & 0xFFu);
+}
+
+__device__ __forceinline__ void SlabTest8(const CNode8* __restrict__ np, const RayPre& r,
+                                          float tMin, float tMax, float tNear[8], uint32_t child[8])
+{
+    const uint4* q = reinterpret_cast<const uint4*>(np);
+    uint4 h = q[0], bx = q[1], by = q[2], bz = q[3], c0 = q[4], c1 = q[5];
+    const float ox = __uint_as_float(h.x), oy = __uint_as_float(h.y), oz = __uint_as_float(h.z);
+    const float sx = Pow2(int(int8_t(h.w & 0xFFu)));
+    const float sy = Pow2(int(int8_t((h.w >> 8) & 0xFFu)));
+    const float sz = Pow2(int(int8_t((h.w >> 16) & 0xFFu)));
+    child[0] = c0.x; child[1] = c0.y; child[2] = c0.z; child[3] = c0.w;
+    child[4] = c1.x; child[5] = c1.y; child[6] = c1.z; child[7] = c1.w;
+    const float grow = 1.0f + 2.0f * 5.96e-8f * 4.0f;
+    #pragma unroll
+    for(int k = 0; k < 8; k++)
+    {
+        float lx = __builtin_fmaf(QByte(bx.x, bx.y, k), sx, ox);
+        float hx = __builtin_fmaf(QByte(bx.z, bx.w, k), sx, ox);
+        float ly = __builtin_fmaf(QByte(by.x, by.y, k), sy, oy);
+        float hy = __builtin_fmaf(QByte(by.z, by.w, k), sy, oy);
+        float lz = __builtin_fmaf(QByte(bz.x, bz.y, k), sz, oz);
+        float hz = __builtin_fmaf(QByte(bz.z, bz.w, k), sz, oz);
+        float nx = r.sx ? hx : lx, fx = r.sx ? lx : hx;
+        float ny = r.sy ? hy : ly, fy = r.sy ? ly : hy;
+        float nz = r.sz ? hz : lz, fz = r.sz ? lz : hz;
+        float x0 = (nx - r.o.x) * r.inv.x, x1 = (fx - r.o.x) * r.inv.x;
+        float y0 = (ny - r.o.y) * r.inv.y, y1 = (fy - r.o.y) * r.inv.y;
+        float z0 = (nz - r.o.z) * r.inv.z, z1 = (fz - r.o.z) * r.inv.z;
+        float tn = MaxF(MaxF(tMin, x0), MaxF(y0, z0));
+        float tf = MinF(MinF(tMax, x1), MinF(y1, z1)) * grow;
+        // empty slots decode to lo > hi on every axis: near > far
+        bool valid = lx <= hx;
+        tNear[k] = (valid && tn <= tf) ? tn : INFINITY;
+    }
+}
+
 template<bool ANY_HIT>
 __device__ __forceinline__ bool TestLeafRange(const SoupTri* __restrict__ soup, uint32_t leaf,
                                               const RayPre& r, float tMin, float tMax,
@@ -147,11 +197,11 @@ __device__ __forceinline__ bool TestLeafRange(const SoupTri* __restrict__ soup, 
 
 // Closest hit; ties in t go to the smaller triangle index.
 // `lstack` = this block's LDS stack base; `gtid` = grid-wide lane id.
-template<bool ANY_HIT, bool COUNT>
-__device__ __forceinline__ HitRec Trace(const TraceCtx& ctx, uint32_t* lstack, uint32_t gtid,
-                                        V3 o, V3 d, float tMin, float tMax,
-                                        uint32_t* nodeVisits = nullptr,
-                                        uint32_t* triTests = nullptr)
+template<int W, bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uint32_t gtid,
+                                         V3 o, V3 d, float tMin, float tMax,
+                                         uint32_t* nodeVisits = nullptr,
+                                         uint32_t* triTests = nullptr)
 {
     HitRec best{kInvalid, INFINITY, 0.0f, 0.0f};
     RayPre r = MakeRayPre(o, d);
@@ -160,15 +210,20 @@ __device__ __forceinline__ HitRec Trace(const TraceCtx& ctx, uint32_t* lstack, u
     uint32_t sp = 0;
     uint32_t node = 0;
     const uint32_t spMax = kLdsStack + ctx.gstackDepth;
+    const float kGrow = 1.0000005f;
     for(uint32_t visits = 0; visits < ctx.maxVisits; visits++)
     {
         if(COUNT) (*nodeVisits)++;
         float curMax = (best.prim == kInvalid) ? tMax : best.t;
-        float tn[4];
-        uint4 ch;
-        SlabTest4(ctx.nodes + node, r, tMin, curMax, tn, ch);
-        uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
-        // sort (tNear, child) ascending: 5-comparator network
+        float tn[W];
+        uint32_t c[W];
+        if constexpr(W == 8) SlabTest8(ctx.cnodes + node, r, tMin, curMax, tn, c);
+        else
+        {
+            uint4 ch;
+            SlabTest4(ctx.nodes + node, r, tMin, curMax, tn, ch);
+            c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
+        }
         auto cs = [&](int i, int j)
         {
             if(tn[j] < tn[i])
@@ -179,19 +234,32 @@ __device__ __forceinline__ HitRec Trace(const TraceCtx& ctx, uint32_t* lstack, u
         };
         if(!ANY_HIT)
         {
-            cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+            // ascending tNear (optimal networks: 5 / 19 comparators)
+            if constexpr(W == 8)
+            {
+                cs(0, 2); cs(1, 3); cs(4, 6); cs(5, 7);
+                cs(0, 4); cs(1, 5); cs(2, 6); cs(3, 7);
+                cs(0, 1); cs(2, 3); cs(4, 5); cs(6, 7);
+                cs(2, 4); cs(3, 5);
+                cs(1, 4); cs(3, 6);
+                cs(1, 2); cs(3, 4); cs(5, 6);
+            }
+            else
+            {
+                cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+            }
         }
-        uint32_t next = kInvalid;
-        uint32_t inner[4];
+        uint32_t inner[W];
+        float innerT[W];
         int innerCount = 0;
         #pragma unroll
-        for(int k = 0; k < 4; k++)
+        for(int k = 0; k < W; k++)
         {
             if(tn[k] == INFINITY) continue;
             if(c[k] & kLeafFlag)
             {
                 float lim = (best.prim == kInvalid) ? tMax : best.t;
-                if(!ANY_HIT && tn[k] > lim * 1.0000005f) continue;
+                if(!ANY_HIT && tn[k] > lim * kGrow) continue;
                 bool h = TestLeafRange<ANY_HIT>(ctx.soup, c[k], r, tMin, tMax, best,
                                                 COUNT ? triTests : nullptr);
                 if(ANY_HIT && h)
@@ -200,31 +268,49 @@ __device__ __forceinline__ HitRec Trace(const TraceCtx& ctx, uint32_t* lstack, u
                     return best;
                 }
             }
-            else inner[innerCount++] = c[k];
+            else
+            {
+                inner[innerCount] = c[k];
+                innerT[innerCount] = tn[k];
+                innerCount++;
+            }
         }
+        uint32_t next = kInvalid;
         if(innerCount > 0)
         {
             next = inner[0];
             for(int k = innerCount - 1; k >= 1; k--)
             {
-                uint32_t v = inner[k];
-                if(sp >= spMax) break;   // unreachable: sized 3 per level
+                uint2 v = make_uint2(inner[k], __float_as_uint(innerT[k]));
+                if(sp >= spMax) break;   // unreachable: sized (W-1) per level
                 if(sp < kLdsStack) lstack[sp * B + tid] = v;
                 else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
                 sp++;
             }
         }
-        if(next == kInvalid)
+        while(next == kInvalid && sp > 0)
         {
-            if(sp == 0) break;
             sp--;
-            next = (sp < kLdsStack) ? lstack[sp * B + tid]
-                                    : ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
+            uint2 v = (sp < kLdsStack) ? lstack[sp * B + tid]
+                                       : ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
+            // drop subtrees that start beyond the closest hit so far
+            if(!ANY_HIT && best.prim != kInvalid && __uint_as_float(v.y) > best.t * kGrow) continue;
+            next = v.x;
         }
+        if(next == kInvalid) break;
         node = next;
     }
     if(ANY_HIT) best.prim = kInvalid;
     return best;
+}
+
+template<bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ HitRec Trace(const TraceCtx& ctx, uint2* lstack, uint32_t gtid,
+                                        V3 o, V3 d, float tMin, float tMax,
+                                        uint32_t* nodeVisits = nullptr,
+                                        uint32_t* triTests = nullptr)
+{
+    return TraceW<4, ANY_HIT, COUNT>(ctx, lstack, gtid, o, d, tMin, tMax, nodeVisits, triTests);
 }
 
 } // namespace mray

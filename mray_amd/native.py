@@ -203,13 +203,14 @@ def spectral_tables():
 class Accel:
     """Device scene + LBVH + 4-wide BVH (MRayAccel)."""
 
-    def __init__(self, scene, builder="sah"):
+    def __init__(self, scene, builder="sah", width=4):
         require_gpu()
         self.scene = scene
         self.desc = scene.desc()
         h = C.c_void_p()
         opt = abi.MRayAccelOptions()
         opt.builder = {"sah": abi.BUILDER_SAH, "lbvh": abi.BUILDER_LBVH}[builder]
+        opt.width = width
         check(lib().mray_accel_create_ex(C.byref(self.desc), C.byref(opt), C.byref(h)))
         self.h = h
         self.n = scene.triangle_count

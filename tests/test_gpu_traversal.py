@@ -51,19 +51,19 @@ def _compare(acc, oscene, rays, brute):
     assert np.array_equal(vis == 1, ~hit)
 
 
-@pytest.mark.parametrize("builder", ["sah", "lbvh"])
-def test_closest_soup_vs_bruteforce(torch_gpu, builder):
+@pytest.mark.parametrize("builder,width", [("sah", 4), ("lbvh", 4), ("sah", 8), ("lbvh", 8)])
+def test_closest_soup_vs_bruteforce(torch_gpu, builder, width):
     from mray_amd import native
     s = scenes.random_triangles(3000, seed=9)
-    acc = native.Accel(s, builder=builder)
+    acc = native.Accel(s, builder=builder, width=width)
     _compare(acc, refcpu.OracleScene(s), _random_rays(20000, 1), brute=True)
 
 
-@pytest.mark.parametrize("builder", ["sah", "lbvh"])
-def test_closest_soup_large(torch_gpu, builder):
+@pytest.mark.parametrize("builder,width", [("sah", 4), ("lbvh", 4), ("sah", 8)])
+def test_closest_soup_large(torch_gpu, builder, width):
     from mray_amd import native
     s = scenes.random_triangles(200000, seed=4)
-    acc = native.Accel(s, builder=builder)
+    acc = native.Accel(s, builder=builder, width=width)
     _compare(acc, refcpu.OracleScene(s), _random_rays(100000, 2), brute=False)
 
 
@@ -78,11 +78,11 @@ def test_primary_rays_cornell(torch_gpu):
     _compare(acc, o, rays, brute=True)
 
 
-@pytest.mark.parametrize("builder", ["sah", "lbvh"])
-def test_primary_rays_terrain(torch_gpu, builder):
+@pytest.mark.parametrize("builder,width", [("sah", 4), ("lbvh", 4), ("sah", 8)])
+def test_primary_rays_terrain(torch_gpu, builder, width):
     from mray_amd import native
     s = scenes.terrain_spheres(grid=256, spheres=300, width=320, height=180)
-    acc = native.Accel(s, builder=builder)
+    acc = native.Accel(s, builder=builder, width=width)
     o = refcpu.OracleScene(s)
     params = abi.render_params(320, 180, 1)
     rays, _ = o.camera_rays(params, 320 * 180)
@@ -113,7 +113,8 @@ def test_single_triangle_and_edges(torch_gpu):
     _compare(acc, refcpu.OracleScene(s), rays, brute=True)
 
 
-def test_rays_in_box_planes(torch_gpu):
+@pytest.mark.parametrize("width", [4, 8])
+def test_rays_in_box_planes(torch_gpu, width):
     """Axis-parallel rays (+0 and -0 direction components) whose origins lie
     on the BVH box planes of a grid of axis-aligned quads."""
     from mray_amd import native
@@ -134,7 +135,7 @@ def test_rays_in_box_planes(torch_gpu):
                      light_two_sided=np.zeros(0, np.uint8),
                      camera={"position": (0, 0, 1), "gaze": (0, 0, 0), "up": (0, 1, 0),
                              "fov": (1.0, 1.0), "nearFar": (0.1, 10.0)})
-    acc = native.Accel(s)
+    acc = native.Accel(s, width=width)
     o = refcpu.OracleScene(s)
     g = np.arange(0, 17, dtype=np.float32) / 16
     xy = np.array([[x, y] for x in g for y in g], np.float32)
