@@ -46,7 +46,9 @@ def parse():
     ap.add_argument("--rgb", action="store_true", help="PathTracerRGB instead of Spectral")
     ap.add_argument("--builder", choices=("sah", "lbvh"), default="sah",
                     help="traversal tree: binned SAH (default) or the collapsed Karras LBVH")
-    ap.add_argument("--width", type=int, choices=(4, 8), default=4,
+    ap.add_argument("--no-queues", action="store_true",
+                    help="shade every hit in one kernel instead of material-keyed queues")
+    ap.add_argument("--bvh-width", type=int, choices=(4, 8), default=4,
                     help="BVH node width: 4 (fp32 boxes) or 8 (compressed, 8-bit boxes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -221,13 +223,14 @@ def main():
                                    height=args.height)
     log(f"scene: {scene.triangle_count} triangles in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    accel = native.Accel(scene, builder=args.builder, width=args.width)
+    accel = native.Accel(scene, builder=args.builder, width=args.bvh_width)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
     info = accel.info()
     log(f"accel: {info} in {build_s:.2f} s")
     params = abi.render_params(args.width, args.height, args.spp, rr=(10, 10),
-                               mode=abi.SAMPLE_NEE_MIS, spectral=spectral, seed=0)
+                               mode=abi.SAMPLE_NEE_MIS, spectral=spectral, seed=0,
+                               flags=abi.RENDER_NO_MATERIAL_QUEUES if args.no_queues else 0)
     r = native.Renderer(accel, params)
     pixels = args.width * args.height
     paths_per_frame = pixels * args.spp
@@ -274,7 +277,7 @@ def main():
     trace_ms, trace_launches = phases["trace"]
     trace_avg_ms = trace_ms / max(1, trace_launches)
     rays_per_launch = totals["ext_rays"] / max(1, trace_launches)
-    bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES[args.width] + tris_per_ray * TRI_BYTES
+    bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES[args.bvh_width] + tris_per_ray * TRI_BYTES
     achieved = rays_per_launch * bytes_per_ray / (trace_avg_ms * 1e-3) / 1e9
     traffic = traffic_from_profiles(rays_per_launch)
     result = {
@@ -308,7 +311,7 @@ def main():
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in phases.items()},
         "roofline": {
             "bound": "hbm",
-            "kernel": f"KTrace (closest hit, {args.width}-wide BVH)",
+            "kernel": f"KTrace (closest hit, {args.bvh_width}-wide BVH)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

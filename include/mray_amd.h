@@ -118,8 +118,13 @@ typedef struct MRayRenderParams
     uint32_t maxRayCount;     /* path slots; 0 = width*height             */
     uint64_t seed;
     float    filterRadius;    /* Gaussian film filter radius (default 1)  */
-    uint32_t reserved;
+    uint32_t flags;           /* MRAY_RENDER_* schedule options           */
 } MRayRenderParams;
+
+/* Shade every hit in one kernel instead of material-keyed queues (one
+ * specialised shading launch per boundary/light and material type).  The
+ * image is the same either way.                                            */
+#define MRAY_RENDER_NO_MATERIAL_QUEUES 1u
 
 /* Per-iteration analytics (Common/AnalyticStructs.h:33-52). */
 typedef struct MRayRenderStats

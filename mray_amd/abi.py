@@ -87,7 +87,7 @@ class MRayRenderParams(C.Structure):
         ("maxRayCount", C.c_uint32),
         ("seed", C.c_uint64),
         ("filterRadius", C.c_float),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 
@@ -135,8 +135,11 @@ class MRayNrcRenderParams(C.Structure):
     ]
 
 
+RENDER_NO_MATERIAL_QUEUES = 1
+
+
 def render_params(width, height, spp, rr=(8, 8), mode=SAMPLE_NEE_MIS, spectral=False,
-                  seed=0, filter_radius=1.0, max_ray_count=0):
+                  seed=0, filter_radius=1.0, max_ray_count=0, flags=0):
     """MRayRenderParams with the reference's renderer-attribute meaning
     (PathTracerRenderer.cu:127-142)."""
     p = MRayRenderParams()
@@ -150,6 +153,7 @@ def render_params(width, height, spp, rr=(8, 8), mode=SAMPLE_NEE_MIS, spectral=F
     p.maxRayCount = max_ray_count
     p.seed = seed
     p.filterRadius = filter_radius
+    p.flags = flags
     return p
 
 

@@ -115,6 +115,8 @@ struct Sched
     uint32_t shadeNext;
     uint32_t shadowNext;
     uint32_t queryCount;      // radiance-cache queries of this iteration
+    uint32_t qCount[5];       // material-keyed queue sizes
+    uint32_t qNext[5];
     unsigned long long ringCount;   // training pairs written so far
     unsigned long long queries;     // radiance-cache queries so far
 };
@@ -145,6 +147,7 @@ struct RenderConst
 };
 
 static constexpr uint32_t kSlotTPB = 256;
+static constexpr uint32_t kMatQueues = 5;   // boundary/light + 4 material types
 
 // ---------------------------------------------------------------------------
 // Refill
@@ -235,6 +238,7 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
         sched->shadeNext = 0;
         sched->shadowNext = 0;
         sched->queryCount = 0;
+        for(int q = 0; q < 5; q++) { sched->qCount[q] = 0; sched->qNext[q] = 0; }
         sched->iterations += 1;
     }
 }
@@ -566,12 +570,267 @@ __device__ __forceinline__ float PdfLightBoundary(const SceneView& s)
 // ---------------------------------------------------------------------------
 // Shade: boundary / light work, NEE (work 1), scatter (work 0)
 // ---------------------------------------------------------------------------
+// Shading of one slot after its closest hit.  KEY < 0: any hit; KEY = 0:
+// boundary or light hits only; KEY = 1 + material type: surface hits of that
+// material only (the material switch folds away).
+template<bool SPECTRAL, int KEY>
+__device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& rc, uint32_t i,
+                                          Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
+{
+    const SceneView& s = rc.scene;
+    uint4 hh = S.hit[i];
+    uint32_t prim = hh.x;
+    float ha = __uint_as_float(hh.z), hb = __uint_as_float(hh.w);
+    uint32_t m = S.meta[i];
+    uint32_t depth = MDepth(m), type = MType(m);
+    bool dispersed = MDispersed(m);
+    float4 ro = S.rayO[i], rd = S.rayD[i];
+    V3 rpos = {ro.x, ro.y, ro.z}, rdir = {rd.x, rd.y, rd.z};
+    float4 t4 = S.thr[i], r4 = S.rad[i];
+    V4 thr = {t4.x, t4.y, t4.z, t4.w};
+    V4 rad = {r4.x, r4.y, r4.z, r4.w};
+    V4 waves = Splat4(0.0f);
+    if constexpr(SPECTRAL)
+    {
+        float4 w4 = S.waves[i];
+        waves = {w4.x, w4.y, w4.z, w4.w};
+    }
+    uint32_t li = kInvalid;
+    if(prim != kInvalid)
+    {
+        uint32_t key = s.surfKey[prim];
+        if(key & MRAY_LIGHT_FLAG) li = key & ~MRAY_LIGHT_FLAG;
+    }
+    bool isBoundary = (prim == kInvalid);
+    if(KEY <= 0 && (isBoundary || li != kInvalid))
+    {
+        V3 wO = -rdir;
+        V4 emission;
+        if(isBoundary)
+        {
+            emission = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+                ? ConvRadiance<SPECTRAL>(rc, V3{s.boundaryRad[0], s.boundaryRad[1],
+                                                s.boundaryRad[2]}, waves)
+                : Splat4(0.0f);
+        }
+        else if(!BaryInRange(ha, hb)) emission = Splat4(0.0f);
+        else
+        {
+            V3 p0, p1, p2;
+            LoadTri(s, prim, p0, p1, p2);
+            V3 nrm = TriNormal(p0, p1, p2);
+            float NdL = Dot(nrm, wO);
+            float4 lr = s.lightRad[li];
+            emission = (lr.w == 0.0f && NdL <= 0.0f)
+                ? Splat4(0.0f) : ConvRadiance<SPECTRAL>(rc, V3{lr.x, lr.y, lr.z}, waves);
+        }
+        if(rc.mode == MRAY_SAMPLE_PURE)
+        {
+            depth += 1u;
+            if(depth <= rc.rr1) rad = emission * thr;
+        }
+        else if(!(rc.mode == MRAY_SAMPLE_NEE && type != RT_CAMERA && type != RT_SPECULAR))
+        {
+            bool switchToMIS = (rc.mode == MRAY_SAMPLE_NEE_MIS && type == RT_PATH);
+            V4 t2 = thr;
+            if(switchToMIS)
+            {
+                float p0 = S.prevPdf[i];
+                float p1 = isBoundary ? PdfLightBoundary(s)
+                                      : PdfLightTri(s, li, ha, hb, rpos, rdir);
+                float misPdf = Balance2(p0, 1.0f, p1, 1.0f);
+                t2 = t2 * p0;
+                t2 = DivideByPDF(t2, misPdf);
+            }
+            if((depth + 1u) <= rc.rr1) rad = rad + emission * t2;
+        }
+        S.rad[i] = make_float4(rad.x, rad.y, rad.z, rad.w);
+        S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
+        return;
+    }
+
+    if constexpr(KEY == 0) return;
+    // Surface hit
+    uint32_t st = S.rng[i];
+    const MRayMaterial& M = s.mats[s.surfKey[prim]];
+    Surface sf = GenSurface(s, prim, ha, hb, rdir);
+    MatInst mat;
+    mat.type = (KEY > 0) ? uint32_t(KEY - 1) : M.type;
+    mat.albedo = ConvAlbedo<SPECTRAL>(rc, V3{M.albedo[0], M.albedo[1], M.albedo[2]}, waves);
+    mat.roughness = M.roughness;
+    mat.metallic = M.metallic;
+    mat.specular = M.specular;
+    if(mat.type == MRAY_MAT_REFRACT)
+    {
+        if constexpr(SPECTRAL)
+        {
+            mat.frontIoR = CauchyIoR(waves.x, M.cauchyFront);
+            mat.backIoR = CauchyIoR(waves.x, M.cauchyBack);
+        }
+        else
+        {
+            mat.frontIoR = M.cauchyFront[0];
+            mat.backIoR = M.cauchyBack[0];
+        }
+    }
+    else mat.frontIoR = mat.backIoR = 1.0f;
+    V3 wO = Normalize(-rdir);
+    bool specular = IsSpecular(Specularity(mat));
+    if constexpr(!SPECTRAL)
+    {
+        if(rc.nrc.on)
+        {
+            const NrcRenderState& nr = rc.nrc;
+            // path spread (Mueller et al. 2021, eq. 3-4): a0 at the first
+            // hit, then the sum of sqrt(d^2 / (pdf |cos|)) over later hits
+            float2 sp = S.spread[i];
+            float tHit = __uint_as_float(hh.y);
+            float cosT = MaxF(fabsf(Dot(rdir, sf.geoNormal)), 1e-6f);
+            if(depth == 0u) sp = make_float2(tHit * tHit / (4.0f * kPi * cosT), 0.0f);
+            else
+            {
+                float pdf = S.prevPdf[i];
+                if(pdf > 0.0f && isfinite(pdf)) sp.y += sqrtf(tHit * tHit / (pdf * cosT));
+            }
+            S.spread[i] = sp;
+            // query record: position in the scene box, outgoing direction
+            // and normal as (theta, phi), roughness, albedo
+            float rec[kNrcRec];
+            rec[0] = (sf.position.x - nr.boxMin[0]) * nr.boxInvExtent;
+            rec[1] = (sf.position.y - nr.boxMin[1]) * nr.boxInvExtent;
+            rec[2] = (sf.position.z - nr.boxMin[2]) * nr.boxInvExtent;
+            auto sph = [](V3 d, float& th, float& ph)
+            {
+                th = acosf(ClampF(d.z, -1.0f, 1.0f)) * kInvPi;
+                ph = (atan2f(d.y, d.x) + kPi) * (0.5f * kInvPi);
+            };
+            sph(wO, rec[3], rec[4]);
+            sph(sf.geoNormal, rec[5], rec[6]);
+            rec[7] = (mat.type == MRAY_MAT_UNREAL) ? M.roughness : (mat.type == MRAY_MAT_LAMBERT ? 1.0f : 0.0f);
+            rec[8] = M.albedo[0];
+            rec[9] = M.albedo[1];
+            rec[10] = M.albedo[2];
+            const bool training = (i & ((1u << nr.trainShift) - 1u)) == 0u;
+            if(!training && depth >= 1u && sp.y * sp.y > nr.spreadC * sp.x)
+            {
+                // terminate into the cache: queue the query, the path ends
+                const uint64_t qm = __ballot(1);
+                const uint32_t ln = threadIdx.x & 63u;
+                const uint32_t leader = uint32_t(__ffsll((unsigned long long)qm) - 1);
+                uint32_t qb = 0;
+                if(ln == leader) qb = atomicAdd(&sched->queryCount, uint32_t(__popcll(qm)));
+                qb = __shfl(qb, int(leader));
+                const uint64_t below = (ln == 0) ? 0ull : ((~0ull) >> (64 - ln));
+                uint32_t q = qb + uint32_t(__popcll(qm & below));
+                for(uint32_t k = 0; k < kNrcRec; k++) nr.queryRec[size_t(q) * kNrcRec + k] = rec[k];
+                nr.querySlot[q] = i;
+                S.rng[i] = st;
+                S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
+                return;
+            }
+            if(training)
+            {
+                uint32_t ts = i >> nr.trainShift;
+                uint32_t nv = nr.trainCount[ts];
+                if(nv < nr.maxVerts)
+                {
+                    float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + nv) * kNrcVert;
+                    for(uint32_t k = 0; k < kNrcRec; k++) vx[k] = rec[k];
+                    vx[kNrcRec + 0] = thr.x; vx[kNrcRec + 1] = thr.y; vx[kNrcRec + 2] = thr.z;
+                    vx[kNrcRec + 3] = rad.x; vx[kNrcRec + 4] = rad.y; vx[kNrcRec + 5] = rad.z;
+                    nr.trainCount[ts] = nv + 1u;
+                }
+            }
+        }
+    }
+    if(rc.mode != MRAY_SAMPLE_PURE)
+    {
+        float rn[3];
+        rn[0] = NextRN(st);
+        rn[1] = NextRN(st);
+        rn[2] = NextRN(st);
+        V3 worldPos = sf.position;
+        LightSample ls = SampleLight<SPECTRAL>(rc, rn, worldPos, waves);
+        V3 sdir = Normalize(ls.position - worldPos);
+        V3 npos = Nudge(ls.position, -sdir);
+        float length = Length(npos - worldPos);
+        bool pt, dsp;
+        V4 refl = EvaluateBxDF(mat, sf, sdir, wO, pt, dsp);
+        V4 t2 = thr * refl;
+        float pdf;
+        if(rc.mode == MRAY_SAMPLE_NEE_MIS)
+            pdf = Balance2(PdfBxDF(mat, sf, sdir, wO), 1.0f, ls.pdf, 1.0f);
+        else pdf = ls.pdf;
+        V4 srad = DivideByPDF(t2 * ls.emission, pdf);
+        if(SPECTRAL && dsp)
+        {
+            float first = srad.x;
+            srad = Splat4(0.0f);
+            srad.x = first;
+        }
+        if(specular) type = RT_SPECULAR;
+        else
+        {
+            V3 nn = pt ? -sf.geoNormal : sf.geoNormal;
+            V3 spos = Nudge(worldPos, nn);
+            type = RT_SHADOW;
+            if((depth + 2u) <= rc.rr1)
+            {
+                S.sO[i] = make_float4(spos.x, spos.y, spos.z, kEps);
+                S.sD[i] = make_float4(sdir.x, sdir.y, sdir.z, length * (1.0f - kLargeEps));
+                S.sRad[i] = make_float4(srad.x, srad.y, srad.z, srad.w);
+                // shadow queue append: one atomic per wave, from the
+                // lowest active lane (only lanes with a shadow ray run this)
+                const uint64_t pm = __ballot(1);
+                const uint32_t lane = threadIdx.x & 63u;
+                const uint32_t leader = uint32_t(__ffsll((unsigned long long)pm) - 1);
+                uint32_t qBase = 0;
+                if(lane == leader) qBase = atomicAdd(&sched->shadowCount, uint32_t(__popcll(pm)));
+                qBase = __shfl(qBase, int(leader));
+                const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+                shadowQ[qBase + uint32_t(__popcll(pm & below))] = i;
+            }
+        }
+    }
+    // WorkFunction (PathTracerRendererShaders.h:201-301)
+    uint32_t matRN = MatRNCount(mat.type);
+    float rn[4];
+    for(uint32_t k2 = 0; k2 < matRN + 1u; k2++) rn[k2] = NextRN(st);
+    BxDFSample bs = SampleBxDF(mat, sf, wO, rn);
+    bs.wIDir = Normalize(bs.wIDir);
+    V4 t3 = thr * bs.refl;
+    if(SPECTRAL && bs.dispersed) dispersed = true;
+    depth += 1u;
+    bool dead = (depth >= rc.rr1);
+    if(!dead && depth >= rc.rr0 && !specular)
+    {
+        const float chInv = SPECTRAL ? 0.25f : 0.33333333f;
+        float rrXi = rn[matRN];
+        float prob = ClampF(Sum4(t3) * chInv, 0.1f, 1.0f);
+        if(rrXi >= prob) dead = true;
+        else t3 = t3 * (1.0f / prob);
+    }
+    type = specular ? RT_SPECULAR : RT_PATH;
+    if(!dead)
+    {
+        V4 nt = DivideByPDF(t3, bs.pdf);
+        S.thr[i] = make_float4(nt.x, nt.y, nt.z, nt.w);
+        if(rc.mode == MRAY_SAMPLE_NEE_MIS || (!SPECTRAL && rc.nrc.on)) S.prevPdf[i] = bs.pdf;
+        V3 nn = bs.passedThrough ? -sf.geoNormal : sf.geoNormal;
+        V3 np = Nudge(bs.wIPos, nn);
+        S.rayO[i] = make_float4(np.x, np.y, np.z, kLargeEps);
+        S.rayD[i] = make_float4(bs.wIDir.x, bs.wIDir.y, bs.wIDir.z, 3.402823466e+38f);
+    }
+    S.rng[i] = st;
+    S.meta[i] = MPack(depth, dead ? ST_DEAD : ST_ALIVE, type, dispersed, false);
+}
+
+// Generic shading over the whole active list.
 template<bool SPECTRAL>
 __global__ void __launch_bounds__(kSlotTPB)
 KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
 {
-    const SceneView& s = rc.scene;
     uint32_t n = sched->activeCount;
     for(;;)
     {
@@ -579,251 +838,65 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
         if(k >= n) break;
         k += threadIdx.x & 63u;
         if(k >= n) continue;
-        uint32_t i = active[k];
-        uint4 hh = S.hit[i];
-        uint32_t prim = hh.x;
-        float ha = __uint_as_float(hh.z), hb = __uint_as_float(hh.w);
-        uint32_t m = S.meta[i];
-        uint32_t depth = MDepth(m), type = MType(m);
-        bool dispersed = MDispersed(m);
-        float4 ro = S.rayO[i], rd = S.rayD[i];
-        V3 rpos = {ro.x, ro.y, ro.z}, rdir = {rd.x, rd.y, rd.z};
-        float4 t4 = S.thr[i], r4 = S.rad[i];
-        V4 thr = {t4.x, t4.y, t4.z, t4.w};
-        V4 rad = {r4.x, r4.y, r4.z, r4.w};
-        V4 waves = Splat4(0.0f);
-        if constexpr(SPECTRAL)
-        {
-            float4 w4 = S.waves[i];
-            waves = {w4.x, w4.y, w4.z, w4.w};
-        }
-        uint32_t li = kInvalid;
-        if(prim != kInvalid)
-        {
-            uint32_t key = s.surfKey[prim];
-            if(key & MRAY_LIGHT_FLAG) li = key & ~MRAY_LIGHT_FLAG;
-        }
-        bool isBoundary = (prim == kInvalid);
-        if(isBoundary || li != kInvalid)
-        {
-            V3 wO = -rdir;
-            V4 emission;
-            if(isBoundary)
-            {
-                emission = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
-                    ? ConvRadiance<SPECTRAL>(rc, V3{s.boundaryRad[0], s.boundaryRad[1],
-                                                    s.boundaryRad[2]}, waves)
-                    : Splat4(0.0f);
-            }
-            else if(!BaryInRange(ha, hb)) emission = Splat4(0.0f);
-            else
-            {
-                V3 p0, p1, p2;
-                LoadTri(s, prim, p0, p1, p2);
-                V3 nrm = TriNormal(p0, p1, p2);
-                float NdL = Dot(nrm, wO);
-                float4 lr = s.lightRad[li];
-                emission = (lr.w == 0.0f && NdL <= 0.0f)
-                    ? Splat4(0.0f) : ConvRadiance<SPECTRAL>(rc, V3{lr.x, lr.y, lr.z}, waves);
-            }
-            if(rc.mode == MRAY_SAMPLE_PURE)
-            {
-                depth += 1u;
-                if(depth <= rc.rr1) rad = emission * thr;
-            }
-            else if(!(rc.mode == MRAY_SAMPLE_NEE && type != RT_CAMERA && type != RT_SPECULAR))
-            {
-                bool switchToMIS = (rc.mode == MRAY_SAMPLE_NEE_MIS && type == RT_PATH);
-                V4 t2 = thr;
-                if(switchToMIS)
-                {
-                    float p0 = S.prevPdf[i];
-                    float p1 = isBoundary ? PdfLightBoundary(s)
-                                          : PdfLightTri(s, li, ha, hb, rpos, rdir);
-                    float misPdf = Balance2(p0, 1.0f, p1, 1.0f);
-                    t2 = t2 * p0;
-                    t2 = DivideByPDF(t2, misPdf);
-                }
-                if((depth + 1u) <= rc.rr1) rad = rad + emission * t2;
-            }
-            S.rad[i] = make_float4(rad.x, rad.y, rad.z, rad.w);
-            S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
-            continue;
-        }
+        ShadeSlot<SPECTRAL, -1>(S, rc, active[k], sched, shadowQ);
+    }
+}
 
-        // Surface hit
-        uint32_t st = S.rng[i];
-        const MRayMaterial& M = s.mats[s.surfKey[prim]];
-        Surface sf = GenSurface(s, prim, ha, hb, rdir);
-        MatInst mat;
-        mat.type = M.type;
-        mat.albedo = ConvAlbedo<SPECTRAL>(rc, V3{M.albedo[0], M.albedo[1], M.albedo[2]}, waves);
-        mat.roughness = M.roughness;
-        mat.metallic = M.metallic;
-        mat.specular = M.specular;
-        if(M.type == MRAY_MAT_REFRACT)
+// Material-keyed queues (replacing the reference's work-key sort and
+// partition, RayPartitioner.cu:263-411): key 0 = boundary / light hits,
+// 1 + material type otherwise; one atomic per key per wave.
+__global__ void __launch_bounds__(kSlotTPB)
+KClassify(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active, Sched* __restrict__ sched,
+          uint32_t* __restrict__ queues, uint32_t N)
+{
+    const SceneView& s = rc.scene;
+    const uint32_t n = sched->activeCount;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for(uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n;
+        base += gridDim.x * blockDim.x)
+    {
+        uint32_t k = base + lane;
+        uint32_t key = kMatQueues;           // no work
+        uint32_t i = 0;
+        if(k < n)
         {
-            if constexpr(SPECTRAL)
-            {
-                mat.frontIoR = CauchyIoR(waves.x, M.cauchyFront);
-                mat.backIoR = CauchyIoR(waves.x, M.cauchyBack);
-            }
+            i = active[k];
+            uint32_t prim = S.hit[i].x;
+            if(prim == kInvalid) key = 0;
             else
             {
-                mat.frontIoR = M.cauchyFront[0];
-                mat.backIoR = M.cauchyBack[0];
+                uint32_t sk = s.surfKey[prim];
+                key = (sk & MRAY_LIGHT_FLAG) ? 0u : 1u + s.mats[sk].type;
             }
         }
-        else mat.frontIoR = mat.backIoR = 1.0f;
-        V3 wO = Normalize(-rdir);
-        bool specular = IsSpecular(Specularity(mat));
-        if constexpr(!SPECTRAL)
+        #pragma unroll
+        for(uint32_t q = 0; q < kMatQueues; q++)
         {
-            if(rc.nrc.on)
-            {
-                const NrcRenderState& nr = rc.nrc;
-                // path spread (Mueller et al. 2021, eq. 3-4): a0 at the first
-                // hit, then the sum of sqrt(d^2 / (pdf |cos|)) over later hits
-                float2 sp = S.spread[i];
-                float tHit = __uint_as_float(hh.y);
-                float cosT = MaxF(fabsf(Dot(rdir, sf.geoNormal)), 1e-6f);
-                if(depth == 0u) sp = make_float2(tHit * tHit / (4.0f * kPi * cosT), 0.0f);
-                else
-                {
-                    float pdf = S.prevPdf[i];
-                    if(pdf > 0.0f && isfinite(pdf)) sp.y += sqrtf(tHit * tHit / (pdf * cosT));
-                }
-                S.spread[i] = sp;
-                // query record: position in the scene box, outgoing direction
-                // and normal as (theta, phi), roughness, albedo
-                float rec[kNrcRec];
-                rec[0] = (sf.position.x - nr.boxMin[0]) * nr.boxInvExtent;
-                rec[1] = (sf.position.y - nr.boxMin[1]) * nr.boxInvExtent;
-                rec[2] = (sf.position.z - nr.boxMin[2]) * nr.boxInvExtent;
-                auto sph = [](V3 d, float& th, float& ph)
-                {
-                    th = acosf(ClampF(d.z, -1.0f, 1.0f)) * kInvPi;
-                    ph = (atan2f(d.y, d.x) + kPi) * (0.5f * kInvPi);
-                };
-                sph(wO, rec[3], rec[4]);
-                sph(sf.geoNormal, rec[5], rec[6]);
-                rec[7] = (M.type == MRAY_MAT_UNREAL) ? M.roughness : (M.type == MRAY_MAT_LAMBERT ? 1.0f : 0.0f);
-                rec[8] = M.albedo[0];
-                rec[9] = M.albedo[1];
-                rec[10] = M.albedo[2];
-                const bool training = (i & ((1u << nr.trainShift) - 1u)) == 0u;
-                if(!training && depth >= 1u && sp.y * sp.y > nr.spreadC * sp.x)
-                {
-                    // terminate into the cache: queue the query, the path ends
-                    const uint64_t qm = __ballot(1);
-                    const uint32_t ln = threadIdx.x & 63u;
-                    const uint32_t leader = uint32_t(__ffsll((unsigned long long)qm) - 1);
-                    uint32_t qb = 0;
-                    if(ln == leader) qb = atomicAdd(&sched->queryCount, uint32_t(__popcll(qm)));
-                    qb = __shfl(qb, int(leader));
-                    const uint64_t below = (ln == 0) ? 0ull : ((~0ull) >> (64 - ln));
-                    uint32_t q = qb + uint32_t(__popcll(qm & below));
-                    for(uint32_t k = 0; k < kNrcRec; k++) nr.queryRec[size_t(q) * kNrcRec + k] = rec[k];
-                    nr.querySlot[q] = i;
-                    S.rng[i] = st;
-                    S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
-                    continue;
-                }
-                if(training)
-                {
-                    uint32_t ts = i >> nr.trainShift;
-                    uint32_t nv = nr.trainCount[ts];
-                    if(nv < nr.maxVerts)
-                    {
-                        float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + nv) * kNrcVert;
-                        for(uint32_t k = 0; k < kNrcRec; k++) vx[k] = rec[k];
-                        vx[kNrcRec + 0] = thr.x; vx[kNrcRec + 1] = thr.y; vx[kNrcRec + 2] = thr.z;
-                        vx[kNrcRec + 3] = rad.x; vx[kNrcRec + 4] = rad.y; vx[kNrcRec + 5] = rad.z;
-                        nr.trainCount[ts] = nv + 1u;
-                    }
-                }
-            }
+            uint64_t m = __ballot(key == q);
+            if(!m) continue;
+            uint32_t qb = 0;
+            if(lane == 0) qb = atomicAdd(&sched->qCount[q], uint32_t(__popcll(m)));
+            qb = __shfl(qb, 0);
+            if(key == q) queues[size_t(q) * N + qb + uint32_t(__popcll(m & below))] = i;
         }
-        if(rc.mode != MRAY_SAMPLE_PURE)
-        {
-            float rn[3];
-            rn[0] = NextRN(st);
-            rn[1] = NextRN(st);
-            rn[2] = NextRN(st);
-            V3 worldPos = sf.position;
-            LightSample ls = SampleLight<SPECTRAL>(rc, rn, worldPos, waves);
-            V3 sdir = Normalize(ls.position - worldPos);
-            V3 npos = Nudge(ls.position, -sdir);
-            float length = Length(npos - worldPos);
-            bool pt, dsp;
-            V4 refl = EvaluateBxDF(mat, sf, sdir, wO, pt, dsp);
-            V4 t2 = thr * refl;
-            float pdf;
-            if(rc.mode == MRAY_SAMPLE_NEE_MIS)
-                pdf = Balance2(PdfBxDF(mat, sf, sdir, wO), 1.0f, ls.pdf, 1.0f);
-            else pdf = ls.pdf;
-            V4 srad = DivideByPDF(t2 * ls.emission, pdf);
-            if(SPECTRAL && dsp)
-            {
-                float first = srad.x;
-                srad = Splat4(0.0f);
-                srad.x = first;
-            }
-            if(specular) type = RT_SPECULAR;
-            else
-            {
-                V3 nn = pt ? -sf.geoNormal : sf.geoNormal;
-                V3 spos = Nudge(worldPos, nn);
-                type = RT_SHADOW;
-                if((depth + 2u) <= rc.rr1)
-                {
-                    S.sO[i] = make_float4(spos.x, spos.y, spos.z, kEps);
-                    S.sD[i] = make_float4(sdir.x, sdir.y, sdir.z, length * (1.0f - kLargeEps));
-                    S.sRad[i] = make_float4(srad.x, srad.y, srad.z, srad.w);
-                    // shadow queue append: one atomic per wave, from the
-                    // lowest active lane (only lanes with a shadow ray run this)
-                    const uint64_t pm = __ballot(1);
-                    const uint32_t lane = threadIdx.x & 63u;
-                    const uint32_t leader = uint32_t(__ffsll((unsigned long long)pm) - 1);
-                    uint32_t qBase = 0;
-                    if(lane == leader) qBase = atomicAdd(&sched->shadowCount, uint32_t(__popcll(pm)));
-                    qBase = __shfl(qBase, int(leader));
-                    const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-                    shadowQ[qBase + uint32_t(__popcll(pm & below))] = i;
-                }
-            }
-        }
-        // WorkFunction (PathTracerRendererShaders.h:201-301)
-        uint32_t matRN = MatRNCount(mat.type);
-        float rn[4];
-        for(uint32_t k2 = 0; k2 < matRN + 1u; k2++) rn[k2] = NextRN(st);
-        BxDFSample bs = SampleBxDF(mat, sf, wO, rn);
-        bs.wIDir = Normalize(bs.wIDir);
-        V4 t3 = thr * bs.refl;
-        if(SPECTRAL && bs.dispersed) dispersed = true;
-        depth += 1u;
-        bool dead = (depth >= rc.rr1);
-        if(!dead && depth >= rc.rr0 && !specular)
-        {
-            const float chInv = SPECTRAL ? 0.25f : 0.33333333f;
-            float rrXi = rn[matRN];
-            float prob = ClampF(Sum4(t3) * chInv, 0.1f, 1.0f);
-            if(rrXi >= prob) dead = true;
-            else t3 = t3 * (1.0f / prob);
-        }
-        type = specular ? RT_SPECULAR : RT_PATH;
-        if(!dead)
-        {
-            V4 nt = DivideByPDF(t3, bs.pdf);
-            S.thr[i] = make_float4(nt.x, nt.y, nt.z, nt.w);
-            if(rc.mode == MRAY_SAMPLE_NEE_MIS || (!SPECTRAL && rc.nrc.on)) S.prevPdf[i] = bs.pdf;
-            V3 nn = bs.passedThrough ? -sf.geoNormal : sf.geoNormal;
-            V3 np = Nudge(bs.wIPos, nn);
-            S.rayO[i] = make_float4(np.x, np.y, np.z, kLargeEps);
-            S.rayD[i] = make_float4(bs.wIDir.x, bs.wIDir.y, bs.wIDir.z, 3.402823466e+38f);
-        }
-        S.rng[i] = st;
-        S.meta[i] = MPack(depth, dead ? ST_DEAD : ST_ALIVE, type, dispersed, false);
+    }
+}
+
+template<bool SPECTRAL, int KEY>
+__global__ void __launch_bounds__(kSlotTPB)
+KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues, uint32_t N,
+        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
+{
+    const uint32_t n = sched->qCount[KEY];
+    const uint32_t* q = queues + size_t(KEY) * N;
+    for(;;)
+    {
+        uint32_t k = WaveFetch(&sched->qNext[KEY]);
+        if(k >= n) break;
+        k += threadIdx.x & 63u;
+        if(k >= n) continue;
+        ShadeSlot<SPECTRAL, KEY>(S, rc, q[k], sched, shadowQ);
     }
 }
 
@@ -939,6 +1012,9 @@ struct MRayRenderer
     uint32_t nBlocks = 0;
     uint32_t traceBlocks = 0;     // resident blocks (occupancy) per launch
     uint32_t shadeBlocks = 0;
+    uint32_t shadeQBlocks[5] = {0, 0, 0, 0, 0};
+    uint32_t classifyBlocks = 0;
+    uint32_t* dQueues = nullptr;  // [5][N] material-keyed queues (NULL: generic shading)
     uint32_t shadowBlocks = 0;
     unsigned long long limit = 0;
     Sched last{};                 // stats at the previous mray_renderer_stats()
@@ -996,6 +1072,27 @@ static int HarvestEvents(MRayRenderer* r)
     return MRAY_OK;
 }
 
+template<bool SPECTRAL>
+static void LaunchShadeQueues(MRayRenderer* r, hipStream_t s)
+{
+    const uint32_t N = r->rc.N;
+    KShadeQ<SPECTRAL, 0><<<r->shadeQBlocks[0], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 1><<<r->shadeQBlocks[1], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 2><<<r->shadeQBlocks[2], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 3><<<r->shadeQBlocks[3], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 4><<<r->shadeQBlocks[4], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+}
+
+template<bool SPECTRAL, typename F>
+static void ShadeQueueOccupancy(MRayRenderer* r, F&& resident)
+{
+    r->shadeQBlocks[0] = uint32_t(resident(KShadeQ<SPECTRAL, 0>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[1] = uint32_t(resident(KShadeQ<SPECTRAL, 1>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[2] = uint32_t(resident(KShadeQ<SPECTRAL, 2>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[3] = uint32_t(resident(KShadeQ<SPECTRAL, 3>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[4] = uint32_t(resident(KShadeQ<SPECTRAL, 4>, kSlotTPB, 1u << 20));
+}
+
 static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
 {
     const uint32_t N = r->rc.N;
@@ -1029,7 +1126,15 @@ static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
         else KTrace<4, false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
     }
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    if(r->rc.spectral)
+    if(r->dQueues)
+    {
+        // material-keyed queues, then one specialised shading launch per key
+        KClassify<<<r->classifyBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
+                                                         r->dQueues, N);
+        if(r->rc.spectral) LaunchShadeQueues<true>(r, s);
+        else LaunchShadeQueues<false>(r, s);
+    }
+    else if(r->rc.spectral)
         KShade<true><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
                                                          r->dShadowQ);
     else
@@ -1194,6 +1299,13 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
                                  : resident(KTrace<4, false>, kTraceTPB, maxTraceBlocks));
     r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8>, kTraceTPB, maxTraceBlocks)
                                   : resident(KShadow<4>, kTraceTPB, maxTraceBlocks));
+    if(!(p->flags & MRAY_RENDER_NO_MATERIAL_QUEUES))
+    {
+        MRAY_TRY(A((void**)&r->dQueues, size_t(kMatQueues) * N * 4));
+        r->classifyBlocks = uint32_t(resident(KClassify, kSlotTPB, 1u << 20));
+        if(p->spectral) ShadeQueueOccupancy<true>(r, resident);
+        else ShadeQueueOccupancy<false>(r, resident);
+    }
     r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true>, kSlotTPB, 1u << 20)
                                           : resident(KShade<false>, kSlotTPB, 1u << 20));
     return ResetRenderer(r);

@@ -117,3 +117,46 @@ def test_terrain_spectral_dispersion(torch_gpu):
     e = relmse(film_image(film), film_image(ref))
     print(f"spectral relMSE={e:.3e} gpu={st} cpu={rst}", flush=True)
     assert e <= 1e-3
+
+
+@pytest.mark.parametrize("spectral", [False, True])
+def test_material_queues_match_single_kernel(torch_gpu, spectral):
+    """Material-keyed queues (one specialised shading launch per key) and the
+    single shading kernel draw the same samples: only the order of the film's
+    float atomics differs."""
+    from mray_amd import native
+    if spectral:
+        _load_spectral_tables()
+    scene = scenes.terrain_spheres(grid=128, spheres=150, width=96, height=54)
+    films = []
+    for flags in (0, abi.RENDER_NO_MATERIAL_QUEUES):
+        acc = native.Accel(scene)
+        r = native.Renderer(acc, abi.render_params(96, 54, 4, rr=(10, 10), spectral=spectral,
+                                                   seed=9, flags=flags))
+        r.render_all()
+        films.append(r.film())
+        r.close()
+        acc.close()
+    e = relmse(film_image(films[0]), film_image(films[1]))
+    print(f"queues vs single kernel relMSE {e:.3e}", flush=True)
+    assert e < 1e-8
+
+
+@pytest.mark.parametrize("width", [8])
+def test_terrain_compressed_bvh_matches_oracle(torch_gpu, width):
+    """The compressed 8-wide BVH finds the same hits, so the image matches the
+    oracle like the 4-wide one."""
+    from mray_amd import native
+    scene = scenes.terrain_spheres(grid=128, spheres=150, width=96, height=54)
+    params = abi.render_params(96, 54, 4, rr=(10, 10), seed=3)
+    acc = native.Accel(scene, width=width)
+    r = native.Renderer(acc, params)
+    st = r.render_all()
+    film = r.film()
+    r.close()
+    acc.close()
+    ref, rst = refcpu.OracleScene(scene).render(params)
+    e = relmse(film_image(film), film_image(ref))
+    print(f"relMSE={e:.3e} gpu={st} cpu={rst}", flush=True)
+    assert st["dead_paths"] == rst["dead_paths"]
+    assert e <= 1e-3
