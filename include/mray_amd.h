@@ -125,6 +125,9 @@ typedef struct MRayRenderParams
  * specialised shading launch per boundary/light and material type).  The
  * image is the same either way.                                            */
 #define MRAY_RENDER_NO_MATERIAL_QUEUES 1u
+/* Launch every bounce directly instead of replaying a captured hipGraph of
+ * 8 bounces (mray_renderer_iterate on the renderer's own stream).          */
+#define MRAY_RENDER_NO_GRAPH 2u
 
 /* Per-iteration analytics (Common/AnalyticStructs.h:33-52). */
 typedef struct MRayRenderStats

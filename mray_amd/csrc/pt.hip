@@ -148,6 +148,7 @@ struct RenderConst
 
 static constexpr uint32_t kSlotTPB = 256;
 static constexpr uint32_t kMatQueues = 5;   // boundary/light + 4 material types
+static constexpr uint32_t kGraphIters = 8;  // bounces per captured hipGraph
 
 // ---------------------------------------------------------------------------
 // Refill
@@ -1018,7 +1019,9 @@ struct MRayRenderer
     uint32_t shadowBlocks = 0;
     unsigned long long limit = 0;
     Sched last{};                 // stats at the previous mray_renderer_stats()
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;     // the renderer's own (blocking) stream
+    hipGraphExec_t graph = nullptr;   // kGraphIters captured iterations
+    bool graphBroken = false;
     SpectralTables spectral;
     // measurement
     bool profiling = false;
@@ -1214,6 +1217,8 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
 {
     r->accel = a;
     r->params = *p;
+    // a blocking stream of default priority: orders with the null stream
+    MRAY_HIP_CHECK(hipStreamCreate(&r->stream));
     const uint64_t P = uint64_t(p->width) * p->height;
     uint32_t N = p->maxRayCount ? p->maxRayCount : uint32_t(P);
     if(N > P) N = uint32_t(P);
@@ -1349,6 +1354,7 @@ int mray_renderer_create(const MRayAccel* accel, const MRaySceneDesc* hScene,
     if(rv != MRAY_OK)
     {
         if(r->hSched) (void)hipHostFree(r->hSched);
+        if(r->stream) (void)hipStreamDestroy(r->stream);
         r->arena.Release();
         delete r;
         return rv;
@@ -1361,6 +1367,8 @@ int mray_renderer_destroy(MRayRenderer* r)
 {
     if(!r) return MRAY_OK;
     (void)hipDeviceSynchronize();
+    if(r->graph) (void)hipGraphExecDestroy(r->graph);
+    if(r->stream) (void)hipStreamDestroy(r->stream);
     for(hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if(r->hSched) (void)hipHostFree(r->hSched);
     r->nrcArena.Release();
@@ -1379,6 +1387,37 @@ int mray_renderer_iterate(MRayRenderer* r, uint32_t iterations, void* stream)
 {
     if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }
     hipStream_t s = stream ? static_cast<hipStream_t>(stream) : r->stream;
+    // Bursts of kGraphIters bounces replay one captured hipGraph on the
+    // renderer's stream (every kernel reads its counts from device memory, so
+    // the same graph serves every burst).  Measurement, a caller's stream or
+    // MRAY_RENDER_NO_GRAPH run the launches directly.
+    const bool useGraph = !stream && !r->profiling && !r->counting && !r->graphBroken &&
+                          !(r->params.flags & MRAY_RENDER_NO_GRAPH);
+    if(useGraph && iterations >= kGraphIters)
+    {
+        if(!r->graph)
+        {
+            hipGraph_t g = nullptr;
+            MRAY_HIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            int rv = MRAY_OK;
+            for(uint32_t k = 0; k < kGraphIters && rv == MRAY_OK; k++) rv = EnqueueIteration(r, s);
+            hipError_t ec = hipStreamEndCapture(s, &g);
+            if(rv != MRAY_OK || ec != hipSuccess || !g ||
+               hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0) != hipSuccess)
+            {
+                // fall back to direct launches for this renderer
+                (void)hipGetLastError();
+                r->graph = nullptr;
+                r->graphBroken = true;
+            }
+            if(g) (void)hipGraphDestroy(g);
+        }
+        while(r->graph && iterations >= kGraphIters)
+        {
+            MRAY_HIP_CHECK(hipGraphLaunch(r->graph, s));
+            iterations -= kGraphIters;
+        }
+    }
     for(uint32_t k = 0; k < iterations; k++) MRAY_TRY(EnqueueIteration(r, s));
     return MRAY_OK;
 }
@@ -1408,7 +1447,7 @@ int mray_renderer_render_all(MRayRenderer* r, MRayRenderStats* hStats)
     MRayRenderStats total{};
     for(;;)
     {
-        MRAY_TRY(mray_renderer_iterate(r, 8, nullptr));
+        MRAY_TRY(mray_renderer_iterate(r, kGraphIters, nullptr));
         MRayRenderStats st{};
         MRAY_TRY(mray_renderer_stats(r, &st));
         total.deadPaths += st.deadPaths;
@@ -1469,6 +1508,7 @@ int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderP
 {
     if(!r || !p) { SetError("attach_nrc: NULL argument"); return MRAY_ERR_INVALID; }
     MRAY_HIP_CHECK(hipDeviceSynchronize());
+    if(r->graph) { (void)hipGraphExecDestroy(r->graph); r->graph = nullptr; }
     if(!nrc)
     {
         r->nrc = nullptr;
@@ -1537,7 +1577,7 @@ int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* h
     MRayRenderStats total{};
     for(;;)
     {
-        MRAY_TRY_IT(mray_renderer_iterate(r, 8, nullptr));
+        MRAY_TRY_IT(mray_renderer_iterate(r, kGraphIters, nullptr));
         MRayRenderStats st{};
         MRAY_TRY_IT(mray_renderer_stats(r, &st));
         total.deadPaths += st.deadPaths;
