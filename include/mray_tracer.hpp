@@ -489,6 +489,21 @@ class TracerI
     virtual const TracerParameters& Parameters() const = 0;
 };
 
+// Ids of what a scene file defined (the reference's SceneLoaderI returns
+// the same kind of id pack, Core/SceneLoaderI.h:49-59).
+struct SceneIds
+{
+    std::vector<SurfaceId>      surfaces;
+    std::vector<LightSurfaceId> lightSurfaces;
+    std::vector<CamSurfaceId>   cameraSurfaces;
+    LightSurfaceId              boundary{0};
+};
+
+// Reads an MRay JSON scene file (Docs/markdown/scene/mrayScene.md) and issues
+// the TracerI calls that build it; CommitSurfaces is left to the caller.
+// Throws MRayError for malformed files and for types the path tracer lacks.
+SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path);
+
 }  // namespace mray_amd
 
 // Factory pair (TracerDLL/EntryPoint.h:14-18).  The caller owns the returned

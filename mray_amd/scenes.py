@@ -479,3 +479,96 @@ def random_triangles(n, seed=7, scale=1.0):
                  light_tris=np.zeros(0, dtype=np.uint32),
                  light_radiance=np.zeros((0, 3), dtype=np.float32),
                  light_two_sided=np.zeros(0, dtype=np.uint8), camera=cam, name="soup")
+
+
+def to_mray_json(scene, path):
+    """Write `scene` as an MRay JSON scene file (the reference's scene format,
+    Docs/markdown/scene/mrayScene.md): one inline nodeTriangleIndexed
+    primitive per run of triangles sharing a material (or consecutive lights)
+    and a cull flag, in triangle order, so the tracer's flattened scene keeps
+    the triangle order.  Normals are recovered from the TBN quaternions."""
+    import json
+    pos = np.asarray(scene.positions, np.float32)
+    idx = np.asarray(scene.indices, np.int64)
+    keys = np.asarray(scene.surface_keys, np.uint32)
+    cull = np.asarray(scene.cull, np.uint8)
+    q = np.asarray(scene.tbn, np.float64)
+    # n = R(q)^T z for the world-to-tangent rotation q
+    w, x, y, z = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    nrm = np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], 1)
+    lrad = np.asarray(scene.light_radiance, np.float32)
+    l2s = np.asarray(scene.light_two_sided, np.uint8)
+    runs = []
+    for t in range(len(keys)):
+        light = bool(keys[t] & abi.LIGHT_FLAG)
+        k = int(keys[t] & ~np.uint32(abi.LIGHT_FLAG)) if light else int(keys[t])
+        if runs:
+            r = runs[-1]
+            if r["light"] and light and r["cull"] == cull[t] and k == r["key"] + r["count"] and \
+                    np.array_equal(lrad[k], lrad[r["key"]]) and l2s[k] == l2s[r["key"]]:
+                r["count"] += 1
+                continue
+            if not r["light"] and not light and r["key"] == k and r["cull"] == cull[t]:
+                r["count"] += 1
+                continue
+        runs.append({"first": t, "count": 1, "light": light, "key": k, "cull": int(cull[t])})
+    prims, surfaces, lights, light_surfs = [], [], [], []
+    lights.append({"id": 0, "type": "Null"})
+    for pid, r in enumerate(runs):
+        tri = idx[r["first"]:r["first"] + r["count"]]
+        vs, inv = np.unique(tri.reshape(-1), return_inverse=True)
+        prims.append({"id": pid, "type": "Triangle", "tag": "nodeTriangleIndexed",
+                      "position": pos[vs].astype(float).tolist(),
+                      "normal": nrm[vs].astype(float).tolist(),
+                      "index": inv.reshape(-1, 3).astype(int).tolist()})
+        if r["light"]:
+            lid = len(lights)
+            lights.append({"id": lid, "type": "Primitive", "primitive": pid,
+                           "radiance": lrad[r["key"]].astype(float).tolist(),
+                           "isTwoSided": bool(l2s[r["key"]])})
+            light_surfs.append({"light": lid})
+        else:
+            surfaces.append({"transform": 0, "material": r["key"], "primitive": pid,
+                             "cullBackFace": bool(r["cull"])})
+    names = {abi.MAT_LAMBERT: "Lambert", abi.MAT_REFLECT: "Reflect", abi.MAT_REFRACT: "Refract",
+             abi.MAT_UNREAL: "Unreal"}
+    mats = []
+    for mid, m in enumerate(scene.materials):
+        e = {"id": mid, "type": names[m["type"]]}
+        if m["type"] in (abi.MAT_LAMBERT, abi.MAT_UNREAL):
+            e["albedo"] = list(m.get("albedo", (0.0, 0.0, 0.0)))
+        if m["type"] == abi.MAT_UNREAL:
+            e["roughness"] = m.get("roughness", 0.0)
+            e["specular"] = m.get("specular", 0.0)
+            e["metallic"] = m.get("metallic", 0.0)
+        if m["type"] == abi.MAT_REFRACT:
+            e["cauchyBack"] = list(m.get("cauchyBack", (1.0, 0.0, 0.0)))
+            e["cauchyFront"] = list(m.get("cauchyFront", (1.0, 0.0, 0.0)))
+        mats.append(e)
+    boundary_light = 0
+    if scene.boundary_type == abi.BOUNDARY_SKYSPHERE_COOCTA:
+        boundary_light = len(lights)
+        lights.append({"id": boundary_light, "type": "Skysphere_CoOcta",
+                       "radiance": list(scene.boundary_radiance)})
+    cam = scene.camera
+    fx, fy = cam["fov"]
+    doc = {
+        "Cameras": [{"id": 0, "type": "Pinhole", "isFovX": True, "fov": math.degrees(fx),
+                     "aspect": math.tan(fx * 0.5) / math.tan(fy * 0.5),
+                     "planes": list(cam["nearFar"]), "gaze": list(cam["gaze"]),
+                     "position": list(cam["position"]), "up": list(cam["up"])}],
+        "Lights": lights,
+        "Mediums": [{"id": 0, "type": "Vacuum"}],
+        "Transforms": [{"id": 0, "type": "Identity"}],
+        "Textures": [],
+        "Materials": mats,
+        "Primitives": prims,
+        "Surfaces": surfaces,
+        "LightSurfaces": light_surfs,
+        "CameraSurfaces": [{"camera": 0}],
+        "Boundary": {"light": boundary_light, "transform": 0, "medium": 0},
+    }
+    with open(path, "w") as f:
+        f.write("// written by mray_amd.scenes.to_mray_json\n")
+        json.dump(doc, f)
+

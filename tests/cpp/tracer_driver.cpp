@@ -15,6 +15,8 @@
 //   specular, cauchyFront[3], cauchyBack[3]); u32 lightTris[L];
 //   f32 lightRad[L*3]; u8 twoSided[L]; f32 camera[13] (position, gaze, up,
 //   fov[2], nearFar[2]).
+// A scene path ending in .json is read with LoadSceneMRay (MRay scene format)
+// instead; the first camera surface renders.
 // Triangles are grouped into maximal runs of one material (or of consecutive
 // lights) and one cull flag; each run becomes one primitive batch and one
 // surface (or light surface), in triangle order.
@@ -374,12 +376,25 @@ int Run(int argc, char** argv)
         if(slots) tp.parallelizationHint = slots;
         else tp.parallelizationHint = W * H;
     }
-    SceneFile s = Load(argv[2]);
+    const std::string scenePath = argv[2];
+    const bool isJson = scenePath.size() > 5 && scenePath.substr(scenePath.size() - 5) == ".json";
+    SceneFile s;
+    if(!isJson) s = Load(argv[2]);
     TracerI* t = ctor(tp);
     int rc = 0;
     try
     {
-        CamSurfaceId cs = Feed(*t, s);
+        CamSurfaceId cs{0};
+        if(isJson)
+        {
+            // MRay scene file through the library's loader
+            SceneIds ids = LoadSceneMRay(*t, scenePath);
+            if(ids.cameraSurfaces.empty()) throw MRayError("scene has no camera surface");
+            cs = ids.cameraSurfaces.front();
+            std::printf("loaded: %zu surfaces, %zu light surfaces, %zu camera surfaces\n",
+                        ids.surfaces.size(), ids.lightSurfaces.size(), ids.cameraSurfaces.size());
+        }
+        else cs = Feed(*t, s);
         if(mode == "flatten")
         {
             auto flat = reinterpret_cast<Flat>(dlsym(so, "mray_tracer_flatten_to_file"));

@@ -62,3 +62,26 @@ def test_tracer_api_matches_cabi_terrain(torch_gpu, tmp_path):
     e = relmse(film_image(a), film_image(b))
     print(f"relMSE TracerI vs C-ABI = {e:.3e}", flush=True)
     assert e < 1e-8
+
+
+def test_json_scene_renders_like_cabi(torch_gpu, tmp_path):
+    """Cornell box written as an MRay JSON scene file, loaded by LoadSceneMRay
+    and rendered through TracerI, against the same scene through the C-ABI
+    (TBN quaternions are rebuilt from normals, so samples agree up to float
+    rounding: the equal-spp image bar applies)."""
+    import subprocess
+    from tests.test_tracer_api import DRIVER, TRACER_LIB
+    scene = scenes.cornell()
+    src = str(tmp_path / "cornell.json")
+    scenes.to_mray_json(scene, src)
+    out = str(tmp_path / "film.bin")
+    cmd = [DRIVER, TRACER_LIB, src, "render", out, "64", "64", "4", "0", str(abi.SAMPLE_NEE_MIS),
+           "8", "8", "5", "0"]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    a = np.fromfile(out, np.float32).reshape(4, 64, 64)
+    b = _cabi_render(scene, abi.render_params(64, 64, 4, rr=(8, 8), mode=abi.SAMPLE_NEE_MIS, seed=5))
+    e = relmse(film_image(a), film_image(b))
+    print(f"relMSE JSON/TracerI vs C-ABI = {e:.3e}", flush=True)
+    assert e <= 1e-3

@@ -119,3 +119,95 @@ def test_error_behaviour():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "FAIL" not in r.stdout
+
+
+def _flatten_json(scene, tmp_path):
+    _built()
+    src = str(tmp_path / "scene.json")
+    dst = str(tmp_path / "flat.bin")
+    scenes.to_mray_json(scene, src)
+    r = subprocess.run([DRIVER, TRACER_LIB, src, "flatten", dst], capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stderr
+    return read_flat_file(dst)
+
+
+def _check_same_json(scene, f):
+    """As _check_same, with TBN quaternions rebuilt from the written normals
+    (so equal up to float rounding) instead of copied."""
+    scene.desc()
+    idx = np.asarray(scene.indices)
+    assert np.array_equal(f["pos"][f["idx"]], np.asarray(scene.positions, np.float32)[idx])
+    assert np.allclose(f["tbn"][f["idx"]], np.asarray(scene.tbn, np.float32)[idx], atol=2e-6)
+    assert np.array_equal(f["keys"], scene.surface_keys)
+    assert np.array_equal(f["cull"], scene.cull)
+    mats = np.frombuffer(bytes(memoryview(scene._keep[1]))[:52 * len(scene.materials)], np.uint8)
+    got = f["mats"].view(np.uint32).reshape(-1, 13)
+    want = mats.view(np.uint32).reshape(-1, 13)
+    assert np.array_equal(got[:, 0], want[:, 0])                       # types
+    assert np.allclose(got[:, 1:].view(np.float32), want[:, 1:].view(np.float32), rtol=1e-6)
+    assert np.array_equal(f["ltri"], scene.light_tris)
+    assert np.allclose(f["lrad"], np.asarray(scene.light_radiance, np.float32))
+    assert np.array_equal(f["l2s"], scene.light_two_sided)
+    assert f["btype"] == scene.boundary_type
+    assert np.allclose(f["brad"], np.asarray(scene.boundary_radiance, np.float32))
+
+
+def test_json_scene_cornell(tmp_path):
+    """MRay JSON scene file -> LoadSceneMRay -> TracerI -> the same scene."""
+    s = scenes.cornell()
+    _check_same_json(s, _flatten_json(s, tmp_path))
+
+
+def test_json_scene_terrain(tmp_path):
+    s = scenes.terrain_spheres(grid=40, spheres=30, width=64, height=36)
+    _check_same_json(s, _flatten_json(s, tmp_path))
+
+
+def test_json_scene_transforms_and_errors(tmp_path):
+    """A hand-written file: a unit quad placed by a 'trs' transform and by a
+    'matrix' transform, both of which the flattened positions must show; and
+    an unknown material type must throw."""
+    import json
+    quad = {"id": 0, "type": "Triangle", "tag": "nodeTriangleIndexed",
+            "position": [[-0.5, -0.5, 0], [0.5, -0.5, 0], [0.5, 0.5, 0], [-0.5, 0.5, 0]],
+            "index": [[0, 1, 2], [0, 2, 3]]}
+    doc = {
+        "Cameras": [{"id": 0, "type": "Pinhole", "isFovX": True, "fov": 40.0, "aspect": 1.0,
+                     "planes": [0.1, 100.0], "gaze": [0, 0, 0], "position": [0, 0, 4], "up": [0, 1, 0]}],
+        "Lights": [{"id": 0, "type": "Null"}],
+        "Mediums": [], "Textures": [],
+        "Transforms": [{"id": 0, "type": "Identity"},
+                       {"id": 1, "type": "Single", "layout": "trs", "translate": [1, 2, 3],
+                        "rotate": [0, 0, 90], "scale": [2, 2, 2]},
+                       {"id": 2, "type": "Single", "layout": "matrix",
+                        "matrix": [1, 0, 0, 5, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]}],
+        "Materials": [{"id": [0, 1], "type": "Lambert", "albedo": [[0.5, 0.5, 0.5], [0.2, 0.3, 0.4]]}],
+        "Primitives": [quad],
+        "Surfaces": [{"transform": 1, "material": 0, "primitive": 0},
+                     {"transform": 2, "material": 1, "primitive": 0, "cullBackFace": False}],
+        "LightSurfaces": [], "CameraSurfaces": [{"camera": 0}],
+        "Boundary": {"light": 0},
+    }
+    src = str(tmp_path / "t.json")
+    with open(src, "w") as fh:
+        json.dump(doc, fh)
+    dst = str(tmp_path / "flat.bin")
+    r = subprocess.run([DRIVER, TRACER_LIB, src, "flatten", dst], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    f = read_flat_file(dst)
+    p = f["pos"][f["idx"]]
+    # trs: scale 2, rotate 90 deg about Z, then translate (1,2,3): (-0.5,-0.5,0) -> (2,1,3)
+    assert np.allclose(p[0, 0], [2.0, 1.0, 3.0], atol=1e-5)
+    # matrix: translate x by 5
+    assert np.allclose(p[2, 0], [4.5, -0.5, 0.0], atol=1e-6)
+    assert list(f["cull"]) == [1, 1, 0, 0]
+    assert list(f["keys"]) == [0, 0, 1, 1]
+    doc["Materials"][0]["type"] = "Glass"
+    with open(src, "w") as fh:
+        json.dump(doc, fh)
+    r = subprocess.run([DRIVER, TRACER_LIB, src, "flatten", dst], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode != 0 and "Unknown material type" in r.stderr
