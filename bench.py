@@ -46,16 +46,17 @@ def parse():
     ap.add_argument("--rgb", action="store_true", help="PathTracerRGB instead of Spectral")
     ap.add_argument("--builder", choices=("sah", "lbvh"), default="sah",
                     help="traversal tree: binned SAH (default) or the collapsed Karras LBVH")
-    ap.add_argument("--no-queues", action="store_true",
-                    help="shade every hit in one kernel instead of material-keyed queues")
+    ap.add_argument("--queues", action="store_true",
+                    help="material-keyed shading queues instead of one shading kernel")
     ap.add_argument("--bvh-width", type=int, choices=(4, 8), default=4,
                     help="BVH node width: 4 (fp32 boxes) or 8 (compressed, 8-bit boxes)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--nrc", type=int, default=1,
                     help="run the radiance-cache equal-time leg (Cornell box, 1 spp per frame)")
-    ap.add_argument("--nrc-width", type=int, default=1920)
-    ap.add_argument("--nrc-height", type=int, default=1080)
+    ap.add_argument("--nrc-width", type=int, default=480)
+    ap.add_argument("--nrc-height", type=int, default=270)
+    ap.add_argument("--nrc-train-shift", type=int, default=3)
     ap.add_argument("--nrc-frames", type=int, default=256)
     ap.add_argument("--nrc-ref-spp", type=int, default=16384)
     ap.add_argument("--probe-copy-mib", type=int, default=0,
@@ -141,7 +142,12 @@ def nrc_equal_time(args):
     acc = native.Accel(scene)
     t0 = time.perf_counter()
     ref_r = native.Renderer(acc, abi.render_params(W, H, args.nrc_ref_spp, rr=(8, 8), seed=99))
-    ref_r.render_all()
+    done = 0
+    while done < args.nrc_ref_spp:
+        step = min(512, args.nrc_ref_spp - done)
+        ref_r.render_frame(step)
+        done += step
+        log(f"nrc leg: reference {done}/{args.nrc_ref_spp} spp")
     ref = film_image(ref_r.film())
     ref_s = time.perf_counter() - t0
     ref_r.close()
@@ -152,7 +158,7 @@ def nrc_equal_time(args):
         nrc = None
         if with_cache:
             nrc = native.Nrc(seed=2)
-            r.attach_nrc(nrc, train_shift=4, spread=0.01, batch=1 << 14, steps=4)
+            r.attach_nrc(nrc, train_shift=args.nrc_train_shift, spread=0.01, batch=1 << 14, steps=4)
         curve = []
         spent = 0.0
         nxt = 1
@@ -164,6 +170,8 @@ def nrc_equal_time(args):
             spent += time.perf_counter() - t
             if f == nxt or f == args.nrc_frames:
                 curve.append((f, spent, relmse(film_image(r.film()), ref)))
+                log(f"nrc leg: {'cached' if with_cache else 'plain'} frame {f}: "
+                    f"{spent:.3f} s, relMSE {curve[-1][2]:.4e}")
                 nxt *= 2
         stats = r.cache_stats() if with_cache else None
         r.close()
@@ -230,7 +238,7 @@ def main():
     log(f"accel: {info} in {build_s:.2f} s")
     params = abi.render_params(args.width, args.height, args.spp, rr=(10, 10),
                                mode=abi.SAMPLE_NEE_MIS, spectral=spectral, seed=0,
-                               flags=abi.RENDER_NO_MATERIAL_QUEUES if args.no_queues else 0)
+                               flags=abi.RENDER_MATERIAL_QUEUES if args.queues else 0)
     r = native.Renderer(accel, params)
     pixels = args.width * args.height
     paths_per_frame = pixels * args.spp

@@ -121,10 +121,12 @@ typedef struct MRayRenderParams
     uint32_t flags;           /* MRAY_RENDER_* schedule options           */
 } MRayRenderParams;
 
-/* Shade every hit in one kernel instead of material-keyed queues (one
- * specialised shading launch per boundary/light and material type).  The
- * image is the same either way.                                            */
-#define MRAY_RENDER_NO_MATERIAL_QUEUES 1u
+/* Material-keyed queues: classify hits (boundary/light + 4 material types)
+ * and shade each queue with a specialised kernel, instead of one shading
+ * kernel over all hits.  Same image either way; off by default because the
+ * per-queue atomics and launches cost more than the divergence they remove
+ * on the measured scenes.                                                  */
+#define MRAY_RENDER_MATERIAL_QUEUES 1u
 /* Launch every bounce directly instead of replaying a captured hipGraph of
  * 8 bounces (mray_renderer_iterate on the renderer's own stream).          */
 #define MRAY_RENDER_NO_GRAPH 2u

@@ -135,7 +135,7 @@ class MRayNrcRenderParams(C.Structure):
     ]
 
 
-RENDER_NO_MATERIAL_QUEUES = 1
+RENDER_MATERIAL_QUEUES = 1
 RENDER_NO_GRAPH = 2
 
 

@@ -832,15 +832,11 @@ __global__ void __launch_bounds__(kSlotTPB)
 KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
 {
+    // static grid-stride: shading is short per item, so a per-wave atomic
+    // fetch would cost as much as the work it hands out
     uint32_t n = sched->activeCount;
-    for(;;)
-    {
-        uint32_t k = WaveFetch(&sched->shadeNext);
-        if(k >= n) break;
-        k += threadIdx.x & 63u;
-        if(k >= n) continue;
+    for(uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
         ShadeSlot<SPECTRAL, -1>(S, rc, active[k], sched, shadowQ);
-    }
 }
 
 // Material-keyed queues (replacing the reference's work-key sort and
@@ -891,14 +887,8 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues, uint32_t
 {
     const uint32_t n = sched->qCount[KEY];
     const uint32_t* q = queues + size_t(KEY) * N;
-    for(;;)
-    {
-        uint32_t k = WaveFetch(&sched->qNext[KEY]);
-        if(k >= n) break;
-        k += threadIdx.x & 63u;
-        if(k >= n) continue;
+    for(uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
         ShadeSlot<SPECTRAL, KEY>(S, rc, q[k], sched, shadowQ);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1304,7 +1294,7 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
                                  : resident(KTrace<4, false>, kTraceTPB, maxTraceBlocks));
     r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8>, kTraceTPB, maxTraceBlocks)
                                   : resident(KShadow<4>, kTraceTPB, maxTraceBlocks));
-    if(!(p->flags & MRAY_RENDER_NO_MATERIAL_QUEUES))
+    if(p->flags & MRAY_RENDER_MATERIAL_QUEUES)
     {
         MRAY_TRY(A((void**)&r->dQueues, size_t(kMatQueues) * N * 4));
         r->classifyBlocks = uint32_t(resident(KClassify, kSlotTPB, 1u << 20));

@@ -17,7 +17,7 @@ import os
 import re
 import sys
 
-TRACE_KERNEL = "KTrace<false>"
+TRACE_KERNEL = "KTrace<4, false>"
 COPY_KERNEL = "neg"
 
 

@@ -129,7 +129,7 @@ def test_material_queues_match_single_kernel(torch_gpu, spectral):
         _load_spectral_tables()
     scene = scenes.terrain_spheres(grid=128, spheres=150, width=96, height=54)
     films = []
-    for flags in (0, abi.RENDER_NO_MATERIAL_QUEUES):
+    for flags in (abi.RENDER_MATERIAL_QUEUES, 0):
         acc = native.Accel(scene)
         r = native.Renderer(acc, abi.render_params(96, 54, 4, rr=(10, 10), spectral=spectral,
                                                    seed=9, flags=flags))
