@@ -266,6 +266,73 @@ int mray_renderer_phase_times(MRayRenderer* r, double* hMs, uint64_t* hLaunches)
 int mray_renderer_film_device(MRayRenderer* r, float** dFilm);
 
 /* ---------------------------------------------------------------------- */
+/* Batch: many scenes and viewports of uneven size in one renderer.  No    */
+/* reference counterpart: a TracerI renders one camera of one committed    */
+/* scene at a time (Core/TracerI.h:330-363, CommitSurfaces / StartRender / */
+/* DoRenderWork).  Every bounce of the batch is one launch per kernel over */
+/* one shared pool of path slots; paths take global indices view by view  */
+/* (view k owns [sum of earlier width*height*spp, + its own)), so each     */
+/* view's image is the one a single renderer with the same slot pool would */
+/* produce for that path range.  The returned MRayRenderer works with      */
+/* mray_renderer_iterate / _render_all / _stats / _reset / _destroy /      */
+/* _set_measurement / _phase_times; mray_renderer_read_film returns every  */
+/* view's film, concatenated in view order.                                */
+/* ---------------------------------------------------------------------- */
+typedef struct MRayBatchScene MRayBatchScene;
+
+/* Merge the scenes of `count` accelerators (same BVH width) into one set of
+ * device arrays: scene k keeps its triangles, tree, materials, lights,
+ * boundary light and camera.  The accelerators may be destroyed afterwards. */
+int mray_batch_scene_create(const MRayAccel* const* accels, uint32_t count, MRayBatchScene** out);
+int mray_batch_scene_destroy(MRayBatchScene* scene);
+int mray_batch_scene_info(const MRayBatchScene* scene, uint32_t* hScenes, uint32_t* hTriangles,
+                          uint32_t* hNodes, uint32_t* hWidth);
+
+typedef struct MRayBatchView
+{
+    uint32_t   scene;          /* index into the merged scenes               */
+    uint32_t   width, height;
+    uint32_t   spp;
+    uint32_t   useCamera;      /* 1: `camera` below; 0: the scene's camera   */
+    uint32_t   reserved;
+    MRayCamera camera;
+} MRayBatchView;
+
+/* hParams: sampleMode, rrRange, spectral, seed, filterRadius, flags as for
+ * one scene; maxRayCount = path slots of the pool (0: min(total pixels,
+ * 2^21)); width, height and totalSPP are ignored (per view).              */
+int mray_batch_create(const MRayBatchScene* scene, const MRayBatchView* hViews, uint32_t viewCount,
+                      const MRayRenderParams* hParams, MRayRenderer** out);
+/* Film of one view: hOut [4][height][width], as mray_renderer_read_film.   */
+int mray_batch_read_film(MRayRenderer* r, uint32_t view, float* hOut);
+/* Per-view latency: hMs[v] = time from the frame's first refill to the
+ * moment view v's last path reached its film (device wall clock), or -1
+ * while view v is unfinished.                                              */
+int mray_batch_view_times(MRayRenderer* r, double* hMs);
+
+/* Autotuner for a batch (no reference counterpart).  Renders the batch at
+ * 1 spp per view for each candidate setting and keeps the fastest: BVH
+ * width (one merged scene per width offered, NULL for a width not built),
+ * material-keyed shading queues on/off, path-slot pool size and the
+ * traversal workgroup size.  hTrials [maxTrials] (may be NULL) receives
+ * every measured candidate; *hBest the fastest.                            */
+typedef struct MRayBatchTune
+{
+    uint32_t width;            /* 4 or 8                                     */
+    uint32_t queues;           /* 1: MRAY_RENDER_MATERIAL_QUEUES             */
+    uint32_t slots;            /* maxRayCount                                */
+    uint32_t traceTPB;         /* traversal workgroup size (128 or 256)      */
+    double   mpathsPerS;       /* measured 1-spp throughput                  */
+} MRayBatchTune;
+int mray_batch_autotune(const MRayBatchScene* scene4, const MRayBatchScene* scene8,
+                        const MRayBatchView* hViews, uint32_t viewCount,
+                        const MRayRenderParams* hParams, MRayBatchTune* hBest,
+                        MRayBatchTune* hTrials, uint32_t maxTrials, uint32_t* hTrialCount);
+/* Use traversal workgroup size tpb (128 or 256) for a batch renderer's
+ * closest-hit and shadow launches (the autotuner's knob; default 256).     */
+int mray_batch_set_trace_tpb(MRayRenderer* r, uint32_t tpb);
+
+/* ---------------------------------------------------------------------- */
 /* Spectral tables (SpectraLUTGen capability, SpectrumContext.cu:298-472):  */
 /* RGB->spectrum coefficients lut [3][N][N][N][4] (c0, c1, c2, 0 in nm,     */
 /* x fastest), observer cmf [471][3] and illuminant [471] at 360..830 nm,   */

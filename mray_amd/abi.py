@@ -91,6 +91,45 @@ class MRayRenderParams(C.Structure):
     ]
 
 
+class MRayBatchView(C.Structure):
+    _fields_ = [
+        ("scene", C.c_uint32),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("spp", C.c_uint32),
+        ("useCamera", C.c_uint32),
+        ("reserved", C.c_uint32),
+        ("camera", MRayCamera),
+    ]
+
+
+class MRayBatchTune(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32),
+        ("queues", C.c_uint32),
+        ("slots", C.c_uint32),
+        ("traceTPB", C.c_uint32),
+        ("mpathsPerS", C.c_double),
+    ]
+
+
+def batch_views(specs):
+    """MRayBatchView array from dicts {scene, width, height, spp[, camera]};
+    a camera dict (position, gaze, up, fov, nearFar) overrides the scene's."""
+    arr = (MRayBatchView * len(specs))()
+    for k, d in enumerate(specs):
+        v = arr[k]
+        v.scene, v.width, v.height, v.spp = d["scene"], d["width"], d["height"], d["spp"]
+        cam = d.get("camera")
+        v.useCamera = 1 if cam is not None else 0
+        if cam is not None:
+            for name in ("position", "gaze", "up", "fov", "nearFar"):
+                vals = cam[name]
+                for i, x in enumerate(vals):
+                    getattr(v.camera, name)[i] = x
+    return arr
+
+
 class MRayRenderStats(C.Structure):
     _fields_ = [
         ("deadPaths", C.c_uint64),

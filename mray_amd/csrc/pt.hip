@@ -29,6 +29,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <chrono>
 #include <random>
 #include <vector>
 
@@ -69,6 +70,7 @@ struct SlotSoA
     float4* waves;     // spectral only
     float4* wpdf;
     float2* spread;    // radiance cache: (a0, sum of sqrt spreads) of the path
+    uint32_t* view;    // batch: viewport of the path
 };
 
 // Radiance-cache state of the renderer (RGB mode).  One path slot in
@@ -119,6 +121,7 @@ struct Sched
     uint32_t qNext[5];
     unsigned long long ringCount;   // training pairs written so far
     unsigned long long queries;     // radiance-cache queries so far
+    unsigned long long startClock;  // wall clock at the first refill
 };
 
 // Persistent kernels take work 64 items at a time: one atomic per wave,
@@ -130,6 +133,23 @@ __device__ __forceinline__ uint32_t WaveFetch(uint32_t* counter)
     if((threadIdx.x & 63u) == 0) base = atomicAdd(counter, 64u);
     return __shfl(base, 0);
 }
+
+// One viewport of a batch: its camera and film, its range of the global path
+// index space, and the scene it renders inside the merged scene arrays (tree
+// root, light range, boundary light).
+struct ViewGPU
+{
+    CameraGPU cam;
+    float* film;                      // [4][P]
+    unsigned long long pathBegin;     // global path indices [pathBegin, pathEnd)
+    unsigned long long pathEnd;
+    uint32_t P, W;
+    uint32_t root;
+    uint32_t lightOff, lightCount;
+    uint32_t boundaryType;
+    float boundaryRad[3];
+    float sceneDiameter;
+};
 
 struct RenderConst
 {
@@ -144,7 +164,33 @@ struct RenderConst
     uint32_t spectral;
     float* film;
     NrcRenderState nrc;
+    // batch kernels only
+    const ViewGPU* views;             // ordered by pathBegin
+    uint32_t viewCount;
+    unsigned long long* viewDone;     // finished paths per view
+    unsigned long long* viewClock;    // wall clock when a view's last path reached the film
 };
+
+// Scene of slot i: the whole scene, or in a batch the merged arrays with the
+// slot's view's lights and boundary light.
+template<bool BATCH>
+__device__ __forceinline__ SceneView SlotScene(const RenderConst& rc, const SlotSoA& S, uint32_t i)
+{
+    SceneView s = rc.scene;
+    if constexpr(BATCH)
+    {
+        const ViewGPU& v = rc.views[S.view[i]];
+        s.lightTri += v.lightOff;
+        s.lightRad += v.lightOff;
+        s.lightCount = v.lightCount;
+        s.boundaryType = v.boundaryType;
+        s.boundaryRad[0] = v.boundaryRad[0];
+        s.boundaryRad[1] = v.boundaryRad[1];
+        s.boundaryRad[2] = v.boundaryRad[2];
+        s.sceneDiameter = v.sceneDiameter;
+    }
+    return s;
+}
 
 static constexpr uint32_t kSlotTPB = 256;
 static constexpr uint32_t kMatQueues = 5;   // boundary/light + 4 material types
@@ -240,6 +286,7 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
         sched->shadowNext = 0;
         sched->queryCount = 0;
         for(int q = 0; q < 5; q++) { sched->qCount[q] = 0; sched->qNext[q] = 0; }
+        if(sched->iterations == 0) sched->startClock = wall_clock64();
         sched->iterations += 1;
     }
 }
@@ -248,7 +295,7 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
 // PathTracerRendererBase.cu:215-262; accumulation as TextureFilter.cu:672-697):
 // spectral paths are converted to RGB first; a non-finite sample is written
 // as magenta with a large weight so it stays visible.
-template<bool SPECTRAL>
+template<bool SPECTRAL, bool BATCH>
 __device__ __forceinline__ void SplatToFilm(const SlotSoA& S, const RenderConst& rc, uint32_t i,
                                             uint32_t m)
 {
@@ -268,11 +315,42 @@ __device__ __forceinline__ void SplatToFilm(const SlotSoA& S, const RenderConst&
         w = w * 128.0f;
     }
     uint32_t px = S.pixel[i];
+    float* film = rc.film;
     size_t P = rc.P;
-    atomicAdd(rc.film + px, r);
-    atomicAdd(rc.film + P + px, g);
-    atomicAdd(rc.film + 2 * P + px, b);
-    atomicAdd(rc.film + 3 * P + px, w);
+    if constexpr(BATCH)
+    {
+        const ViewGPU& v = rc.views[S.view[i]];
+        film = v.film;
+        P = v.P;
+    }
+    atomicAdd(film + px, r);
+    atomicAdd(film + P + px, g);
+    atomicAdd(film + 2 * P + px, b);
+    atomicAdd(film + 3 * P + px, w);
+}
+
+// Batch: finished paths per view, one atomic per view present in the wave;
+// the lane that completes a view stamps the wall clock.  Called by every lane.
+__device__ __forceinline__ void CountFinished(const SlotSoA& S, const RenderConst& rc, bool fin,
+                                              uint32_t i)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t v = fin ? S.view[i] : 0u;
+    uint64_t left = __ballot(fin);
+    while(left)
+    {
+        const uint32_t leader = uint32_t(__ffsll((unsigned long long)left) - 1);
+        const uint32_t lv = __shfl(v, int(leader));
+        const uint64_t same = __ballot(fin && v == lv);
+        if(lane == leader)
+        {
+            const unsigned long long c = (unsigned long long)__popcll(same);
+            const unsigned long long before = atomicAdd(rc.viewDone + lv, c);
+            const ViewGPU& V = rc.views[lv];
+            if(before + c == V.pathEnd - V.pathBegin) rc.viewClock[lv] = wall_clock64();
+        }
+        left &= ~same;
+    }
 }
 
 // A finished training path turns its recorded vertices into training pairs:
@@ -301,7 +379,48 @@ __device__ __forceinline__ void EmitTrainingPairs(const SlotSoA& S, const Render
     }
 }
 
+// Camera path of global pixel `pix` of a W-wide image seen through camera c
+// into slot i (KCGenerateCamRaysStochastic, RayGenKernels.kt.h:155-226, and
+// KCInitPathStateIndirect, PathTracerRendererBase.cu:35-57).
 template<bool SPECTRAL>
+__device__ __forceinline__ void StartCameraPath(const SlotSoA& S, const CameraGPU& c, uint32_t W,
+                                                uint32_t i, uint32_t pix)
+{
+    uint32_t px = pix % W, py = pix / W;
+    uint32_t st = S.rng[i];
+    float xi0 = NextRN(st), xi1 = NextRN(st);
+    float o0 = SampleGaussian(xi0, c.sigma);
+    float o1 = SampleGaussian(xi1, c.sigma);
+    float offPdf = Gaussian(o0, c.sigma) * Gaussian(o1, c.sigma);
+    float weight = Gaussian(o0, c.sigma) * Gaussian(o1, c.sigma);
+    float dx = c.planeSize[0] / float(c.width), dy = c.planeSize[1] / float(c.height);
+    float sdx = (float(px) + (o0 + 0.5f)) * dx;
+    float sdy = (float(py) + (o1 + 0.5f)) * dy;
+    V3 sp = {c.bottomLeft[0], c.bottomLeft[1], c.bottomLeft[2]};
+    sp = sp + sdx * V3{c.right[0], c.right[1], c.right[2]};
+    sp = sp + sdy * V3{c.up[0], c.up[1], c.up[2]};
+    V3 cp = {c.position[0], c.position[1], c.position[2]};
+    V3 dir = Normalize(sp - cp);
+    S.rayO[i] = make_float4(cp.x, cp.y, cp.z, c.nearFar[0]);
+    S.rayD[i] = make_float4(dir.x, dir.y, dir.z, c.nearFar[1]);
+    S.filterW[i] = weight / offPdf;
+    S.pixel[i] = pix;
+    S.rad[i] = make_float4(0, 0, 0, 0);
+    S.thr[i] = make_float4(1, 1, 1, 1);
+    S.prevPdf[i] = 0.0f;
+    if constexpr(SPECTRAL)
+    {
+        float xw = NextRN(st);
+        V4 w, p;
+        SampleWavelengths(xw, w, p);
+        S.waves[i] = make_float4(w.x, w.y, w.z, w.w);
+        S.wpdf[i] = make_float4(p.x, p.y, p.z, p.w);
+    }
+    S.rng[i] = st;
+    S.meta[i] = MPack(0, ST_ALIVE, RT_CAMERA, false, false);
+}
+
+template<bool SPECTRAL, bool BATCH>
 __global__ void __launch_bounds__(kSlotTPB)
 KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
              Sched* __restrict__ sched, uint32_t* __restrict__ active)
@@ -315,10 +434,11 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
     const bool freeSlot = inRange && status != ST_ALIVE;
     if(inRange && status == ST_DEAD)
     {
-        SplatToFilm<SPECTRAL>(S, rc, i, m);
+        SplatToFilm<SPECTRAL, BATCH>(S, rc, i, m);
         if(!SPECTRAL && rc.nrc.on && (i & ((1u << rc.nrc.trainShift) - 1u)) == 0u)
             EmitTrainingPairs(S, rc, i, sched);
     }
+    if constexpr(BATCH) CountFinished(S, rc, inRange && status == ST_DEAD, i);
     uint64_t b = __ballot(freeSlot);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     if(lane == 0) wsum[wave] = uint32_t(__popcll(b));
@@ -333,42 +453,22 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
         uint32_t rank = off.x + waveOff + uint32_t(__popcll(b & lt));
         if(rank < fill)
         {
-            // KCGenerateCamRaysStochastic (RayGenKernels.kt.h:155-226) +
-            // KCInitPathStateIndirect (PathTracerRendererBase.cu:35-57)
-            unsigned long long pix = (sched->base + rank) % rc.P;
-            uint32_t px = uint32_t(pix % rc.W), py = uint32_t(pix / rc.W);
-            uint32_t st = S.rng[i];
-            float xi0 = NextRN(st), xi1 = NextRN(st);
-            const CameraGPU& c = rc.cam;
-            float o0 = SampleGaussian(xi0, c.sigma);
-            float o1 = SampleGaussian(xi1, c.sigma);
-            float offPdf = Gaussian(o0, c.sigma) * Gaussian(o1, c.sigma);
-            float weight = Gaussian(o0, c.sigma) * Gaussian(o1, c.sigma);
-            float dx = c.planeSize[0] / float(c.width), dy = c.planeSize[1] / float(c.height);
-            float sdx = (float(px) + (o0 + 0.5f)) * dx;
-            float sdy = (float(py) + (o1 + 0.5f)) * dy;
-            V3 sp = {c.bottomLeft[0], c.bottomLeft[1], c.bottomLeft[2]};
-            sp = sp + sdx * V3{c.right[0], c.right[1], c.right[2]};
-            sp = sp + sdy * V3{c.up[0], c.up[1], c.up[2]};
-            V3 cp = {c.position[0], c.position[1], c.position[2]};
-            V3 dir = Normalize(sp - cp);
-            S.rayO[i] = make_float4(cp.x, cp.y, cp.z, c.nearFar[0]);
-            S.rayD[i] = make_float4(dir.x, dir.y, dir.z, c.nearFar[1]);
-            S.filterW[i] = weight / offPdf;
-            S.pixel[i] = uint32_t(pix);
-            S.rad[i] = make_float4(0, 0, 0, 0);
-            S.thr[i] = make_float4(1, 1, 1, 1);
-            S.prevPdf[i] = 0.0f;
-            if constexpr(SPECTRAL)
+            const unsigned long long g = sched->base + rank;
+            if constexpr(BATCH)
             {
-                float xw = NextRN(st);
-                V4 w, p;
-                SampleWavelengths(xw, w, p);
-                S.waves[i] = make_float4(w.x, w.y, w.z, w.w);
-                S.wpdf[i] = make_float4(p.x, p.y, p.z, p.w);
+                // the view whose path range holds g (views ordered by pathBegin)
+                uint32_t lo = 0, hi = rc.viewCount - 1u;
+                while(lo < hi)
+                {
+                    uint32_t mid = (lo + hi + 1u) >> 1;
+                    if(rc.views[mid].pathBegin <= g) lo = mid;
+                    else hi = mid - 1u;
+                }
+                const ViewGPU& v = rc.views[lo];
+                S.view[i] = lo;
+                StartCameraPath<SPECTRAL>(S, v.cam, v.W, i, uint32_t((g - v.pathBegin) % v.P));
             }
-            S.rng[i] = st;
-            S.meta[i] = MPack(0, ST_ALIVE, RT_CAMERA, false, false);
+            else StartCameraPath<SPECTRAL>(S, rc.cam, rc.W, i, uint32_t(g % rc.P));
             if(!SPECTRAL && rc.nrc.on)
             {
                 S.spread[i] = make_float2(0.0f, 0.0f);
@@ -398,11 +498,12 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
 // ---------------------------------------------------------------------------
 // Closest hit
 // ---------------------------------------------------------------------------
-template<int W, bool COUNT>
-__global__ void __launch_bounds__(kTraceTPB)
-KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __restrict__ sched)
+template<int W, bool COUNT, bool BATCH, int TPB = kTraceTPB>
+__global__ void __launch_bounds__(TPB)
+KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __restrict__ sched,
+       const ViewGPU* __restrict__ views)
 {
-    __shared__ uint2 lstack[kLdsStack * kTraceTPB];
+    __shared__ uint2 lstack[kLdsStack * TPB];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = sched->activeCount;
     if(gtid == 0) sched->extRays += n;
@@ -415,8 +516,9 @@ KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __re
         if(k >= n) continue;
         uint32_t i = active[k];
         float4 o = S.rayO[i], d = S.rayD[i];
+        const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
         HitRec h = TraceW<W, false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                       o.w, d.w, &nodes, &tris);
+                                       o.w, d.w, root, &nodes, &tris);
         S.hit[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.a),
                               __float_as_uint(h.b));
     }
@@ -461,10 +563,9 @@ __device__ __forceinline__ V4 ConvAlbedo(const RenderConst& rc, V3 rgb, V4 waves
 }
 
 template<bool SPECTRAL>
-__device__ __forceinline__ V4 TriLightEmitViaPoint(const RenderConst& rc, uint32_t li, V3 wO,
-                                                   V3 point, V4 waves)
+__device__ __forceinline__ V4 TriLightEmitViaPoint(const RenderConst& rc, const SceneView& s,
+                                                   uint32_t li, V3 wO, V3 point, V4 waves)
 {
-    const SceneView& s = rc.scene;
     V3 p0, p1, p2;
     LoadTri(s, s.lightTri[li], p0, p1, p2);
     float ha, hb;
@@ -482,10 +583,9 @@ struct LightSample { V3 position; V4 emission; float pdf; };
 
 // DirectLightSamplerUniform::SampleLight (LightSampler.hpp:30-74)
 template<bool SPECTRAL>
-__device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const float* rn,
-                                                   V3 distantPoint, V4 waves)
+__device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const SceneView& s,
+                                                   const float* rn, V3 distantPoint, V4 waves)
 {
-    const SceneView& s = rc.scene;
     uint32_t lightCount = s.lightCount + 1u;
     float lightCountF = float(lightCount);
     uint32_t li = uint32_t(rn[2] * lightCountF);
@@ -530,7 +630,7 @@ __device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const 
     }
     V3 wO = Normalize(distantPoint - point);
     V4 emission;
-    if(li < lightCount - 1u) emission = TriLightEmitViaPoint<SPECTRAL>(rc, li, wO, point, waves);
+    if(li < lightCount - 1u) emission = TriLightEmitViaPoint<SPECTRAL>(rc, s, li, wO, point, waves);
     else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
         emission = ConvRadiance<SPECTRAL>(rc, V3{s.boundaryRad[0], s.boundaryRad[1],
                                                  s.boundaryRad[2]}, waves);
@@ -574,11 +674,11 @@ __device__ __forceinline__ float PdfLightBoundary(const SceneView& s)
 // Shading of one slot after its closest hit.  KEY < 0: any hit; KEY = 0:
 // boundary or light hits only; KEY = 1 + material type: surface hits of that
 // material only (the material switch folds away).
-template<bool SPECTRAL, int KEY>
+template<bool SPECTRAL, int KEY, bool BATCH>
 __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& rc, uint32_t i,
                                           Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
 {
-    const SceneView& s = rc.scene;
+    const SceneView s = SlotScene<BATCH>(rc, S, i);
     uint4 hh = S.hit[i];
     uint32_t prim = hh.x;
     float ha = __uint_as_float(hh.z), hb = __uint_as_float(hh.w);
@@ -751,7 +851,7 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
         rn[1] = NextRN(st);
         rn[2] = NextRN(st);
         V3 worldPos = sf.position;
-        LightSample ls = SampleLight<SPECTRAL>(rc, rn, worldPos, waves);
+        LightSample ls = SampleLight<SPECTRAL>(rc, s, rn, worldPos, waves);
         V3 sdir = Normalize(ls.position - worldPos);
         V3 npos = Nudge(ls.position, -sdir);
         float length = Length(npos - worldPos);
@@ -827,7 +927,7 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
 }
 
 // Generic shading over the whole active list.
-template<bool SPECTRAL>
+template<bool SPECTRAL, bool BATCH>
 __global__ void __launch_bounds__(kSlotTPB)
 KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
@@ -836,7 +936,7 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
     // fetch would cost as much as the work it hands out
     uint32_t n = sched->activeCount;
     for(uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        ShadeSlot<SPECTRAL, -1>(S, rc, active[k], sched, shadowQ);
+        ShadeSlot<SPECTRAL, -1, BATCH>(S, rc, active[k], sched, shadowQ);
 }
 
 // Material-keyed queues (replacing the reference's work-key sort and
@@ -880,7 +980,7 @@ KClassify(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active, Sched*
     }
 }
 
-template<bool SPECTRAL, int KEY>
+template<bool SPECTRAL, int KEY, bool BATCH>
 __global__ void __launch_bounds__(kSlotTPB)
 KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues, uint32_t N,
         Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
@@ -888,7 +988,7 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues, uint32_t
     const uint32_t n = sched->qCount[KEY];
     const uint32_t* q = queues + size_t(KEY) * N;
     for(uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        ShadeSlot<SPECTRAL, KEY>(S, rc, q[k], sched, shadowQ);
+        ShadeSlot<SPECTRAL, KEY, BATCH>(S, rc, q[k], sched, shadowQ);
 }
 
 // ---------------------------------------------------------------------------
@@ -896,11 +996,12 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues, uint32_t
 // ---------------------------------------------------------------------------
 // KCAccumulateShadowRaysPT: a shadow ray that reaches its light point adds
 // its precomputed NEE radiance to the path.
-template<int W>
-__global__ void __launch_bounds__(kTraceTPB)
-KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __restrict__ sched)
+template<int W, bool BATCH, int TPB = kTraceTPB>
+__global__ void __launch_bounds__(TPB)
+KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __restrict__ sched,
+        const ViewGPU* __restrict__ views)
 {
-    __shared__ uint2 lstack[kLdsStack * kTraceTPB];
+    __shared__ uint2 lstack[kLdsStack * TPB];
     uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t n = sched->shadowCount;
     if(gtid == 0) sched->shadowRays += n;
@@ -912,8 +1013,9 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
         if(k >= n) continue;
         uint32_t i = shadowQ[k];
         float4 o = S.sO[i], d = S.sD[i];
+        const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
         HitRec h = TraceW<W, true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                      o.w, d.w);
+                                      o.w, d.w, root);
         if(h.prim == kInvalid)
         {
             float4 a = S.rad[i], b = S.sRad[i];
@@ -1026,6 +1128,15 @@ struct MRayRenderer
     DeviceArena nrcArena;
     // frame schedule (mray_renderer_render_frame)
     unsigned long long frameLimit = 0;
+    // traversal context: the accelerator's, or the merged batch scene's
+    TraceCtx ctx{};
+    // batch (mray_batch_create)
+    const MRayBatchScene* bscene = nullptr;
+    std::vector<ViewGPU> hViews;
+    ViewGPU* dViews = nullptr;
+    unsigned long long* dViewDone = nullptr;
+    unsigned long long* dViewClock = nullptr;
+    uint32_t traceTPB = kTraceTPB;
 };
 
 #define MRAY_TRY_IT(expr)               \
@@ -1065,31 +1176,68 @@ static int HarvestEvents(MRayRenderer* r)
     return MRAY_OK;
 }
 
-template<bool SPECTRAL>
+template<bool SPECTRAL, bool BATCH>
 static void LaunchShadeQueues(MRayRenderer* r, hipStream_t s)
 {
     const uint32_t N = r->rc.N;
-    KShadeQ<SPECTRAL, 0><<<r->shadeQBlocks[0], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
-    KShadeQ<SPECTRAL, 1><<<r->shadeQBlocks[1], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
-    KShadeQ<SPECTRAL, 2><<<r->shadeQBlocks[2], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
-    KShadeQ<SPECTRAL, 3><<<r->shadeQBlocks[3], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
-    KShadeQ<SPECTRAL, 4><<<r->shadeQBlocks[4], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 0, BATCH><<<r->shadeQBlocks[0], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 1, BATCH><<<r->shadeQBlocks[1], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 2, BATCH><<<r->shadeQBlocks[2], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 3, BATCH><<<r->shadeQBlocks[3], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 4, BATCH><<<r->shadeQBlocks[4], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
 }
 
-template<bool SPECTRAL, typename F>
+template<bool SPECTRAL, bool BATCH, typename F>
 static void ShadeQueueOccupancy(MRayRenderer* r, F&& resident)
 {
-    r->shadeQBlocks[0] = uint32_t(resident(KShadeQ<SPECTRAL, 0>, kSlotTPB, 1u << 20));
-    r->shadeQBlocks[1] = uint32_t(resident(KShadeQ<SPECTRAL, 1>, kSlotTPB, 1u << 20));
-    r->shadeQBlocks[2] = uint32_t(resident(KShadeQ<SPECTRAL, 2>, kSlotTPB, 1u << 20));
-    r->shadeQBlocks[3] = uint32_t(resident(KShadeQ<SPECTRAL, 3>, kSlotTPB, 1u << 20));
-    r->shadeQBlocks[4] = uint32_t(resident(KShadeQ<SPECTRAL, 4>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[0] = uint32_t(resident(KShadeQ<SPECTRAL, 0, BATCH>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[1] = uint32_t(resident(KShadeQ<SPECTRAL, 1, BATCH>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[2] = uint32_t(resident(KShadeQ<SPECTRAL, 2, BATCH>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[3] = uint32_t(resident(KShadeQ<SPECTRAL, 3, BATCH>, kSlotTPB, 1u << 20));
+    r->shadeQBlocks[4] = uint32_t(resident(KShadeQ<SPECTRAL, 4, BATCH>, kSlotTPB, 1u << 20));
 }
 
-static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
+// Closest-hit launch; a batch may use 128-thread traversal workgroups.
+template<bool COUNT, bool BATCH>
+static void LaunchTrace(MRayRenderer* r, hipStream_t s)
+{
+    const TraceCtx& ctx = r->ctx;
+    const uint32_t nb = r->traceBlocks;
+    if constexpr(BATCH)
+    {
+        if(r->traceTPB == 128)
+        {
+            if(ctx.width == 8) KTrace<8, COUNT, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews);
+            else KTrace<4, COUNT, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews);
+            return;
+        }
+    }
+    if(ctx.width == 8) KTrace<8, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews);
+    else KTrace<4, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews);
+}
+
+template<bool BATCH>
+static void LaunchShadow(MRayRenderer* r, hipStream_t s)
+{
+    const TraceCtx& ctx = r->ctx;
+    const uint32_t nb = r->shadowBlocks;
+    if constexpr(BATCH)
+    {
+        if(r->traceTPB == 128)
+        {
+            if(ctx.width == 8) KShadow<8, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews);
+            else KShadow<4, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews);
+            return;
+        }
+    }
+    if(ctx.width == 8) KShadow<8, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews);
+    else KShadow<4, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews);
+}
+
+template<bool BATCH>
+static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
 {
     const uint32_t N = r->rc.N;
-    const TraceCtx& ctx = r->accel->ctx;
     if(r->profiling && r->evUsed + 5 > 5 * 4096)
     {
         MRAY_HIP_CHECK(hipStreamSynchronize(s));
@@ -1100,43 +1248,33 @@ static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     KRefillCount<<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, N, r->dBlockCnt);
     KRefillScan<<<1, 1024, 0, s>>>(r->dBlockCnt, r->nBlocks, r->dSched);
-    const uint32_t traceBlocks = r->traceBlocks;
     if(r->rc.spectral)
-        KRefillApply<true><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
-                                                           r->dActive);
+        KRefillApply<true, BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
+                                                                  r->dActive);
     else
-        KRefillApply<false><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
-                                                            r->dActive);
+        KRefillApply<false, BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
+                                                                   r->dActive);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    if(r->counting)
-    {
-        if(ctx.width == 8) KTrace<8, true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
-        else KTrace<4, true><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
-    }
-    else
-    {
-        if(ctx.width == 8) KTrace<8, false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
-        else KTrace<4, false><<<traceBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched);
-    }
+    if(r->counting) LaunchTrace<true, BATCH>(r, s);
+    else LaunchTrace<false, BATCH>(r, s);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     if(r->dQueues)
     {
         // material-keyed queues, then one specialised shading launch per key
         KClassify<<<r->classifyBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
                                                          r->dQueues, N);
-        if(r->rc.spectral) LaunchShadeQueues<true>(r, s);
-        else LaunchShadeQueues<false>(r, s);
+        if(r->rc.spectral) LaunchShadeQueues<true, BATCH>(r, s);
+        else LaunchShadeQueues<false, BATCH>(r, s);
     }
     else if(r->rc.spectral)
-        KShade<true><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
-                                                         r->dShadowQ);
+        KShade<true, BATCH><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
+                                                                r->dShadowQ);
     else
-        KShade<false><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
-                                                          r->dShadowQ);
+        KShade<false, BATCH><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
+                                                                 r->dShadowQ);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    if(ctx.width == 8) KShadow<8><<<r->shadowBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched);
-    else KShadow<4><<<r->shadowBlocks, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched);
-    if(r->nrc)
+    LaunchShadow<BATCH>(r, s);
+    if(!BATCH && r->nrc)
     {
         // radiance-cache answers for the paths that terminated this bounce
         const uint32_t* dCount = reinterpret_cast<const uint32_t*>(
@@ -1147,6 +1285,11 @@ static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     MRAY_HIP_CHECK(hipGetLastError());
     return MRAY_OK;
+}
+
+static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
+{
+    return r->bscene ? EnqueueIterationT<true>(r, s) : EnqueueIterationT<false>(r, s);
 }
 
 __global__ void KSetLimit(Sched* __restrict__ sched, unsigned long long limit)
@@ -1161,6 +1304,12 @@ static int ResetRenderer(MRayRenderer* r)
     KZeroFloats<<<1024, 256, 0, s>>>(r->rc.film, filmN);
     KResetSlots<<<DivUp(r->rc.N, 256), 256, 0, s>>>(r->S.meta, r->S.rng, r->dSeeds, r->rc.N,
                                                      r->dSched, r->limit);
+    if(r->bscene)
+    {
+        const size_t V = r->hViews.size();
+        KZeroFloats<<<DivUp(2 * V, 256), 256, 0, s>>>(reinterpret_cast<float*>(r->dViewDone), 2 * V);
+        KZeroFloats<<<DivUp(2 * V, 256), 256, 0, s>>>(reinterpret_cast<float*>(r->dViewClock), 2 * V);
+    }
     MRAY_HIP_CHECK(hipGetLastError());
     MRAY_HIP_CHECK(hipStreamSynchronize(s));
     std::memset(&r->last, 0, sizeof(Sched));
@@ -1202,27 +1351,14 @@ static void BuildCamera(const MRayCamera& c, uint32_t W, uint32_t H, float filte
         if(r_ != MRAY_OK) return r_;    \
     } while(0)
 
-static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDesc* scene,
-                          const MRayRenderParams* p)
+static int SizeLaunches(MRayRenderer* r);
+
+// Path-slot state, queues, seeds and launch sizes, shared by the one-scene
+// and batch renderers (r->params, r->rc.N, r->ctx and r->bscene set).
+static int InitSlots(MRayRenderer* r, size_t filmPixels)
 {
-    r->accel = a;
-    r->params = *p;
-    // a blocking stream of default priority: orders with the null stream
-    MRAY_HIP_CHECK(hipStreamCreate(&r->stream));
-    const uint64_t P = uint64_t(p->width) * p->height;
-    uint32_t N = p->maxRayCount ? p->maxRayCount : uint32_t(P);
-    if(N > P) N = uint32_t(P);
-    r->rc.N = N;
-    r->rc.P = uint32_t(P);
-    r->rc.W = p->width;
-    r->rc.rr0 = p->rrRange[0];
-    r->rc.rr1 = p->rrRange[1];
-    r->rc.mode = p->sampleMode;
-    r->rc.spectral = p->spectral;
-    r->rc.scene = a->view;
-    float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
-    BuildCamera(scene ? scene->camera : a->camera, p->width, p->height, fr, r->rc.cam);
-    r->limit = (unsigned long long)p->totalSPP * P;
+    const MRayRenderParams* p = &r->params;
+    const uint32_t N = r->rc.N;
     if(p->spectral)
     {
         MRAY_TRY(GetSpectralTables(r->spectral));
@@ -1248,7 +1384,8 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
         MRAY_TRY(A((void**)&r->S.waves, size_t(N) * 16));
         MRAY_TRY(A((void**)&r->S.wpdf, size_t(N) * 16));
     }
-    MRAY_TRY(A((void**)&r->rc.film, size_t(P) * 4 * sizeof(float)));
+    if(r->bscene) MRAY_TRY(A((void**)&r->S.view, size_t(N) * 4));
+    MRAY_TRY(A((void**)&r->rc.film, filmPixels * 4 * sizeof(float)));
     r->nBlocks = DivUp(N, kSlotTPB);
     MRAY_TRY(A((void**)&r->dBlockCnt, size_t(r->nBlocks) * sizeof(uint4)));
     MRAY_TRY(A((void**)&r->dShadowQ, size_t(N) * 4));
@@ -1272,14 +1409,21 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
         }
         MRAY_HIP_CHECK(hipMemcpy(r->dSeeds, seeds.data(), size_t(N) * 4, hipMemcpyHostToDevice));
     }
+    return SizeLaunches(r);
+}
+
+// Persistent launches are sized to what is resident at once (occupancy x
+// CUs): a block that is not resident would start only after a resident one
+// finished, stretching the tail.  The traversal grid never exceeds the
+// spill-stack allocation (ctx.gridThreads).
+static int SizeLaunches(MRayRenderer* r)
+{
+    const MRayRenderParams* p = &r->params;
+    const uint32_t N = r->rc.N;
     int dev = 0;
     MRAY_HIP_CHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
     MRAY_HIP_CHECK(hipGetDeviceProperties(&prop, dev));
-    // Persistent launches are sized to what is resident at once (occupancy x
-    // CUs): a block that is not resident would start only after a resident
-    // one finished, stretching the tail.  The traversal grid never exceeds
-    // the accelerator's spill-stack allocation (ctx.gridThreads).
     const uint32_t cus = uint32_t(std::max(1, prop.multiProcessorCount));
     auto resident = [&](auto kernel, uint32_t tpb, uint32_t cap) -> int
     {
@@ -1288,22 +1432,160 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
         if(e != hipSuccess || nb < 1) nb = 1;
         return int(std::min<uint64_t>(uint64_t(cus) * uint32_t(nb), cap));
     };
-    const uint32_t maxTraceBlocks = a->ctx.gridThreads / kTraceTPB;
-    const bool w8 = a->ctx.width == 8;
-    r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false>, kTraceTPB, maxTraceBlocks)
-                                 : resident(KTrace<4, false>, kTraceTPB, maxTraceBlocks));
-    r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8>, kTraceTPB, maxTraceBlocks)
-                                  : resident(KShadow<4>, kTraceTPB, maxTraceBlocks));
+    const uint32_t tpb = r->traceTPB;
+    const uint32_t maxTraceBlocks = r->ctx.gridThreads / tpb;
+    const bool w8 = r->ctx.width == 8;
+    if(!r->bscene)
+    {
+        r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false, false>, tpb, maxTraceBlocks)
+                                     : resident(KTrace<4, false, false>, tpb, maxTraceBlocks));
+        r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8, false>, tpb, maxTraceBlocks)
+                                      : resident(KShadow<4, false>, tpb, maxTraceBlocks));
+    }
+    else if(tpb == 128)
+    {
+        r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false, true, 128>, tpb, maxTraceBlocks)
+                                     : resident(KTrace<4, false, true, 128>, tpb, maxTraceBlocks));
+        r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8, true, 128>, tpb, maxTraceBlocks)
+                                      : resident(KShadow<4, true, 128>, tpb, maxTraceBlocks));
+    }
+    else
+    {
+        r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false, true>, tpb, maxTraceBlocks)
+                                     : resident(KTrace<4, false, true>, tpb, maxTraceBlocks));
+        r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8, true>, tpb, maxTraceBlocks)
+                                      : resident(KShadow<4, true>, tpb, maxTraceBlocks));
+    }
     if(p->flags & MRAY_RENDER_MATERIAL_QUEUES)
     {
-        MRAY_TRY(A((void**)&r->dQueues, size_t(kMatQueues) * N * 4));
+        if(!r->dQueues) MRAY_TRY(DevAlloc(r->arena, (void**)&r->dQueues, size_t(kMatQueues) * N * 4));
         r->classifyBlocks = uint32_t(resident(KClassify, kSlotTPB, 1u << 20));
-        if(p->spectral) ShadeQueueOccupancy<true>(r, resident);
-        else ShadeQueueOccupancy<false>(r, resident);
+        if(r->bscene)
+        {
+            if(p->spectral) ShadeQueueOccupancy<true, true>(r, resident);
+            else ShadeQueueOccupancy<false, true>(r, resident);
+        }
+        else if(p->spectral) ShadeQueueOccupancy<true, false>(r, resident);
+        else ShadeQueueOccupancy<false, false>(r, resident);
     }
-    r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true>, kSlotTPB, 1u << 20)
-                                          : resident(KShade<false>, kSlotTPB, 1u << 20));
+    if(r->bscene)
+        r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, true>, kSlotTPB, 1u << 20)
+                                              : resident(KShade<false, true>, kSlotTPB, 1u << 20));
+    else
+        r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, false>, kSlotTPB, 1u << 20)
+                                              : resident(KShade<false, false>, kSlotTPB, 1u << 20));
+    return MRAY_OK;
+}
+
+static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDesc* scene,
+                          const MRayRenderParams* p)
+{
+    r->accel = a;
+    r->ctx = a->ctx;
+    r->params = *p;
+    // a blocking stream of default priority: orders with the null stream
+    MRAY_HIP_CHECK(hipStreamCreate(&r->stream));
+    const uint64_t P = uint64_t(p->width) * p->height;
+    uint32_t N = p->maxRayCount ? p->maxRayCount : uint32_t(P);
+    if(N > P) N = uint32_t(P);
+    r->rc.N = N;
+    r->rc.P = uint32_t(P);
+    r->rc.W = p->width;
+    r->rc.rr0 = p->rrRange[0];
+    r->rc.rr1 = p->rrRange[1];
+    r->rc.mode = p->sampleMode;
+    r->rc.spectral = p->spectral;
+    r->rc.scene = a->view;
+    float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
+    BuildCamera(scene ? scene->camera : a->camera, p->width, p->height, fr, r->rc.cam);
+    r->limit = (unsigned long long)p->totalSPP * P;
+    MRAY_TRY(InitSlots(r, size_t(P)));
     return ResetRenderer(r);
+}
+
+static int CheckSampling(const MRayRenderParams& p, const char* who)
+{
+    if(p.sampleMode > MRAY_SAMPLE_NEE_MIS)
+    {
+        // PathTracerRendererT::PushAttribute throws on unknown values
+        SetError(std::string(who) + ": unknown sampleMode");
+        return MRAY_ERR_INVALID;
+    }
+    if(p.rrRange[1] == 0 || p.rrRange[0] > p.rrRange[1])
+    {
+        SetError(std::string(who) + ": rrRange must satisfy 0 < max and start <= max");
+        return MRAY_ERR_INVALID;
+    }
+    return MRAY_OK;
+}
+
+static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayBatchView* views,
+                       uint32_t viewCount, const MRayRenderParams* p)
+{
+    r->bscene = b;
+    r->ctx = b->ctx;
+    r->params = *p;
+    MRAY_HIP_CHECK(hipStreamCreate(&r->stream));
+    r->rc.rr0 = p->rrRange[0];
+    r->rc.rr1 = p->rrRange[1];
+    r->rc.mode = p->sampleMode;
+    r->rc.spectral = p->spectral;
+    r->rc.scene = b->view;
+    const float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
+    uint64_t pixels = 0, paths = 0;
+    r->hViews.resize(viewCount);
+    for(uint32_t k = 0; k < viewCount; k++)
+    {
+        const MRayBatchView& v = views[k];
+        const MRayBatchScene::Part& part = b->parts[v.scene];
+        ViewGPU& g = r->hViews[k];
+        g = ViewGPU{};
+        BuildCamera(v.useCamera ? v.camera : part.camera, v.width, v.height, fr, g.cam);
+        g.P = v.width * v.height;
+        g.W = v.width;
+        g.film = reinterpret_cast<float*>(uintptr_t(pixels * 4 * sizeof(float)));  // offset, fixed below
+        g.pathBegin = paths;
+        g.pathEnd = paths + uint64_t(g.P) * v.spp;
+        g.root = part.root;
+        g.lightOff = part.lightOff;
+        g.lightCount = part.lightCount;
+        g.boundaryType = part.boundaryType;
+        for(int c = 0; c < 3; c++) g.boundaryRad[c] = part.boundaryRad[c];
+        g.sceneDiameter = part.sceneDiameter;
+        pixels += g.P;
+        paths = g.pathEnd;
+    }
+    uint32_t N = p->maxRayCount ? p->maxRayCount : uint32_t(std::min<uint64_t>(pixels, 1u << 21));
+    if(N > pixels) N = uint32_t(pixels);
+    r->rc.N = N;
+    r->rc.P = uint32_t(pixels);
+    r->rc.W = 0;
+    r->limit = paths;
+    MRAY_TRY(InitSlots(r, size_t(pixels)));
+    for(ViewGPU& g : r->hViews)
+        g.film = r->rc.film + (reinterpret_cast<uintptr_t>(g.film) / sizeof(float));
+    DeviceArena& ar = r->arena;
+    MRAY_TRY(DevAlloc(ar, (void**)&r->dViews, viewCount * sizeof(ViewGPU)));
+    MRAY_TRY(DevAlloc(ar, (void**)&r->dViewDone, viewCount * sizeof(unsigned long long)));
+    MRAY_TRY(DevAlloc(ar, (void**)&r->dViewClock, viewCount * sizeof(unsigned long long)));
+    MRAY_HIP_CHECK(hipMemcpy(r->dViews, r->hViews.data(), viewCount * sizeof(ViewGPU),
+                             hipMemcpyHostToDevice));
+    r->rc.views = r->dViews;
+    r->rc.viewCount = viewCount;
+    r->rc.viewDone = r->dViewDone;
+    r->rc.viewClock = r->dViewClock;
+    return ResetRenderer(r);
+}
+
+static void FreeRenderer(MRayRenderer* r)
+{
+    if(r->graph) (void)hipGraphExecDestroy(r->graph);
+    if(r->stream) (void)hipStreamDestroy(r->stream);
+    for(hipEvent_t e : r->events) (void)hipEventDestroy(e);
+    if(r->hSched) (void)hipHostFree(r->hSched);
+    r->nrcArena.Release();
+    r->arena.Release();
+    delete r;
 }
 
 extern "C" {
@@ -1328,25 +1610,13 @@ int mray_renderer_create(const MRayAccel* accel, const MRaySceneDesc* hScene,
         SetError("renderer_create: image too large");
         return MRAY_ERR_INVALID;
     }
-    if(p.sampleMode > MRAY_SAMPLE_NEE_MIS)
-    {
-        // PathTracerRendererT::PushAttribute throws on unknown values
-        SetError("renderer_create: unknown sampleMode");
-        return MRAY_ERR_INVALID;
-    }
-    if(p.rrRange[1] == 0 || p.rrRange[0] > p.rrRange[1])
-    {
-        SetError("renderer_create: rrRange must satisfy 0 < max and start <= max");
-        return MRAY_ERR_INVALID;
-    }
+    MRAY_TRY(CheckSampling(p, "renderer_create"));
     MRayRenderer* r = new MRayRenderer();
     int rv = CreateRenderer(r, accel, hScene, hParams);
     if(rv != MRAY_OK)
     {
-        if(r->hSched) (void)hipHostFree(r->hSched);
-        if(r->stream) (void)hipStreamDestroy(r->stream);
-        r->arena.Release();
-        delete r;
+        (void)hipDeviceSynchronize();
+        FreeRenderer(r);
         return rv;
     }
     *out = r;
@@ -1357,13 +1627,7 @@ int mray_renderer_destroy(MRayRenderer* r)
 {
     if(!r) return MRAY_OK;
     (void)hipDeviceSynchronize();
-    if(r->graph) (void)hipGraphExecDestroy(r->graph);
-    if(r->stream) (void)hipStreamDestroy(r->stream);
-    for(hipEvent_t e : r->events) (void)hipEventDestroy(e);
-    if(r->hSched) (void)hipHostFree(r->hSched);
-    r->nrcArena.Release();
-    r->arena.Release();
-    delete r;
+    FreeRenderer(r);
     return MRAY_OK;
 }
 
@@ -1506,6 +1770,11 @@ int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderP
         r->nrcArena.Release();
         return MRAY_OK;
     }
+    if(r->bscene)
+    {
+        SetError("attach_nrc: a batch renderer has no radiance cache");
+        return MRAY_ERR_UNSUPPORTED;
+    }
     if(r->rc.spectral)
     {
         SetError("attach_nrc: the radiance cache runs with PathTracerRGB only");
@@ -1560,6 +1829,7 @@ int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderP
 int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* hStats)
 {
     if(!r || spp == 0) { SetError("render_frame: NULL renderer or spp == 0"); return MRAY_ERR_INVALID; }
+    if(r->bscene) { SetError("render_frame: not for a batch renderer"); return MRAY_ERR_UNSUPPORTED; }
     hipStream_t s = r->stream;
     r->frameLimit += (unsigned long long)spp * r->rc.P;
     KSetLimit<<<1, 1, 0, s>>>(r->dSched, r->frameLimit);
@@ -1608,6 +1878,168 @@ int mray_renderer_cache_stats(MRayRenderer* r, uint64_t* hQueries, uint64_t* hTr
     MRAY_HIP_CHECK(hipMemcpy(r->hSched, r->dSched, sizeof(Sched), hipMemcpyDeviceToHost));
     *hQueries = r->hSched->queries;
     *hTrainingPairs = r->hSched->ringCount;
+    return MRAY_OK;
+}
+
+int mray_batch_create(const MRayBatchScene* scene, const MRayBatchView* hViews, uint32_t viewCount,
+                      const MRayRenderParams* hParams, MRayRenderer** out)
+{
+    if(!scene || !hViews || !hParams || !out || viewCount == 0)
+    {
+        SetError("batch_create: NULL argument or no views");
+        return MRAY_ERR_INVALID;
+    }
+    *out = nullptr;
+    MRAY_TRY(CheckSampling(*hParams, "batch_create"));
+    uint64_t pixels = 0;
+    for(uint32_t k = 0; k < viewCount; k++)
+    {
+        const MRayBatchView& v = hViews[k];
+        if(v.scene >= scene->parts.size())
+        {
+            SetError("batch_create: view " + std::to_string(k) + " names scene " +
+                     std::to_string(v.scene) + " of " + std::to_string(scene->parts.size()));
+            return MRAY_ERR_INVALID;
+        }
+        if(v.width == 0 || v.height == 0 || v.spp == 0)
+        {
+            SetError("batch_create: view width, height and spp must be non-zero");
+            return MRAY_ERR_INVALID;
+        }
+        pixels += uint64_t(v.width) * v.height;
+    }
+    if(pixels > (1ull << 31))
+    {
+        SetError("batch_create: the views hold more than 2^31 pixels");
+        return MRAY_ERR_INVALID;
+    }
+    MRayRenderer* r = new MRayRenderer();
+    int rv = CreateBatch(r, scene, hViews, viewCount, hParams);
+    if(rv != MRAY_OK)
+    {
+        (void)hipDeviceSynchronize();
+        FreeRenderer(r);
+        return rv;
+    }
+    *out = r;
+    return MRAY_OK;
+}
+
+int mray_batch_read_film(MRayRenderer* r, uint32_t view, float* hOut)
+{
+    if(!r || !hOut) { SetError("batch_read_film: NULL argument"); return MRAY_ERR_INVALID; }
+    if(!r->bscene || view >= r->hViews.size())
+    {
+        SetError("batch_read_film: not a batch renderer or view out of range");
+        return MRAY_ERR_INVALID;
+    }
+    const ViewGPU& v = r->hViews[view];
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    MRAY_HIP_CHECK(hipMemcpy(hOut, v.film, size_t(v.P) * 4 * sizeof(float), hipMemcpyDeviceToHost));
+    return MRAY_OK;
+}
+
+int mray_batch_view_times(MRayRenderer* r, double* hMs)
+{
+    if(!r || !hMs) { SetError("batch_view_times: NULL argument"); return MRAY_ERR_INVALID; }
+    if(!r->bscene) { SetError("batch_view_times: not a batch renderer"); return MRAY_ERR_INVALID; }
+    const size_t V = r->hViews.size();
+    std::vector<unsigned long long> clk(V);
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    MRAY_HIP_CHECK(hipMemcpy(clk.data(), r->dViewClock, V * sizeof(unsigned long long),
+                             hipMemcpyDeviceToHost));
+    MRAY_HIP_CHECK(hipMemcpy(r->hSched, r->dSched, sizeof(Sched), hipMemcpyDeviceToHost));
+    int dev = 0, khz = 0;
+    MRAY_HIP_CHECK(hipGetDevice(&dev));
+    MRAY_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    const unsigned long long t0 = r->hSched->startClock;
+    for(size_t k = 0; k < V; k++)
+        hMs[k] = (clk[k] == 0 || khz <= 0) ? -1.0 : double(clk[k] - t0) / double(khz);
+    return MRAY_OK;
+}
+
+int mray_batch_set_trace_tpb(MRayRenderer* r, uint32_t tpb)
+{
+    if(!r || !r->bscene || (tpb != 128 && tpb != 256))
+    {
+        SetError("batch_set_trace_tpb: a batch renderer and tpb 128 or 256");
+        return MRAY_ERR_INVALID;
+    }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    if(r->graph) { (void)hipGraphExecDestroy(r->graph); r->graph = nullptr; }
+    r->traceTPB = tpb;
+    return SizeLaunches(r);
+}
+
+int mray_batch_autotune(const MRayBatchScene* scene4, const MRayBatchScene* scene8,
+                        const MRayBatchView* hViews, uint32_t viewCount,
+                        const MRayRenderParams* hParams, MRayBatchTune* hBest,
+                        MRayBatchTune* hTrials, uint32_t maxTrials, uint32_t* hTrialCount)
+{
+    if((!scene4 && !scene8) || !hViews || !hParams || !hBest || viewCount == 0)
+    {
+        SetError("batch_autotune: NULL argument");
+        return MRAY_ERR_INVALID;
+    }
+    if((scene4 && scene4->width != 4) || (scene8 && scene8->width != 8))
+    {
+        SetError("batch_autotune: scene4 / scene8 must hold 4-wide / 8-wide trees");
+        return MRAY_ERR_INVALID;
+    }
+    // trials: every view at 1 spp, so each scene weighs in as in the real batch
+    std::vector<MRayBatchView> tv(hViews, hViews + viewCount);
+    uint64_t pixels = 0;
+    for(MRayBatchView& v : tv)
+    {
+        v.spp = 1;
+        pixels += uint64_t(v.width) * v.height;
+    }
+    std::vector<uint32_t> slotCands;
+    for(uint32_t c : {1u << 20, 1u << 21, 1u << 22})
+    {
+        uint32_t n = uint32_t(std::min<uint64_t>(c, pixels));
+        if(std::find(slotCands.begin(), slotCands.end(), n) == slotCands.end()) slotCands.push_back(n);
+    }
+    MRayBatchTune best{};
+    uint32_t nTrials = 0;
+    const MRayBatchScene* scenes[2] = {scene4, scene8};
+    for(const MRayBatchScene* sc : scenes)
+    {
+        if(!sc) continue;
+        for(uint32_t queues = 0; queues < 2; queues++)
+            for(uint32_t slots : slotCands)
+            {
+                MRayRenderParams p = *hParams;
+                p.maxRayCount = slots;
+                p.flags = (p.flags & ~MRAY_RENDER_MATERIAL_QUEUES) | (queues ? MRAY_RENDER_MATERIAL_QUEUES : 0u);
+                MRayRenderer* r = nullptr;
+                MRAY_TRY(mray_batch_create(sc, tv.data(), viewCount, &p, &r));
+                for(uint32_t tpb : {256u, 128u})
+                {
+                    int rv = mray_batch_set_trace_tpb(r, tpb);
+                    MRayRenderStats st{};
+                    // untimed pass (graph capture, caches), then the timed one
+                    if(rv == MRAY_OK) rv = mray_renderer_render_all(r, &st);
+                    if(rv == MRAY_OK) rv = mray_renderer_reset(r);
+                    auto t0 = std::chrono::steady_clock::now();
+                    if(rv == MRAY_OK) rv = mray_renderer_render_all(r, &st);
+                    auto t1 = std::chrono::steady_clock::now();
+                    if(rv != MRAY_OK)
+                    {
+                        mray_renderer_destroy(r);
+                        return rv;
+                    }
+                    double sec = std::chrono::duration<double>(t1 - t0).count();
+                    MRayBatchTune t{sc->width, queues, slots, tpb, double(st.totalDeadPaths) / sec * 1e-6};
+                    if(hTrials && nTrials < maxTrials) hTrials[nTrials] = t;
+                    nTrials++;
+                    if(t.mpathsPerS > best.mpathsPerS) best = t;
+                }
+                mray_renderer_destroy(r);
+            }
+    }
+    *hBest = best;
+    if(hTrialCount) *hTrialCount = nTrials;
     return MRAY_OK;
 }
 

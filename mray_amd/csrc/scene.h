@@ -70,6 +70,33 @@ struct MRayAccel
     std::vector<MRayMaterial> hMats;
 };
 
+// Several scenes merged for one batch of viewports: every scene's triangle
+// soup, traversal tree, vertices and shading tables concatenated on the device
+// with their indices relocated (tree children, leaf ranges, soup triangle
+// ids, vertex indices, material keys, light triangles), so one traversal
+// context and one SceneView serve all of them; a scene is entered at its
+// own root node.
+struct MRayBatchScene
+{
+    struct Part
+    {
+        uint32_t root = 0;          // first node of the scene's tree
+        uint32_t triOff = 0, nTri = 0;
+        uint32_t vertOff = 0, matOff = 0;
+        uint32_t lightOff = 0, lightCount = 0;
+        uint32_t boundaryType = 0;
+        float boundaryRad[3] = {0, 0, 0};
+        float sceneDiameter = 0;
+        MRayCamera camera{};
+    };
+    mray::DeviceArena arena;
+    std::vector<Part> parts;
+    uint32_t width = 4;
+    uint32_t nodes = 0, tris = 0, verts = 0, mats = 0, lights = 0;
+    mray::SceneView view{};
+    mray::TraceCtx ctx{};
+};
+
 namespace mray
 {
 int UploadSceneData(MRayAccel* a, const MRaySceneDesc* s);

@@ -196,10 +196,11 @@ __device__ __forceinline__ bool TestLeafRange(const SoupTri* __restrict__ soup, 
 }
 
 // Closest hit; ties in t go to the smaller triangle index.
-// `lstack` = this block's LDS stack base; `gtid` = grid-wide lane id.
+// `lstack` = this block's LDS stack base; `gtid` = grid-wide lane id;
+// `root` = the tree's root node (0, or a scene's first node in a batch).
 template<int W, bool ANY_HIT, bool COUNT>
 __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uint32_t gtid,
-                                         V3 o, V3 d, float tMin, float tMax,
+                                         V3 o, V3 d, float tMin, float tMax, uint32_t root,
                                          uint32_t* nodeVisits = nullptr,
                                          uint32_t* triTests = nullptr)
 {
@@ -208,7 +209,7 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
     const uint32_t tid = threadIdx.x;
     const uint32_t B = blockDim.x;
     uint32_t sp = 0;
-    uint32_t node = 0;
+    uint32_t node = root;
     const uint32_t spMax = kLdsStack + ctx.gstackDepth;
     const float kGrow = 1.0000005f;
     for(uint32_t visits = 0; visits < ctx.maxVisits; visits++)
@@ -310,7 +311,7 @@ __device__ __forceinline__ HitRec Trace(const TraceCtx& ctx, uint2* lstack, uint
                                         uint32_t* nodeVisits = nullptr,
                                         uint32_t* triTests = nullptr)
 {
-    return TraceW<4, ANY_HIT, COUNT>(ctx, lstack, gtid, o, d, tMin, tMax, nodeVisits, triTests);
+    return TraceW<4, ANY_HIT, COUNT>(ctx, lstack, gtid, o, d, tMin, tMax, 0u, nodeVisits, triTests);
 }
 
 } // namespace mray

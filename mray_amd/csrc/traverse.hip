@@ -19,7 +19,7 @@ KCCastClosest(TraceCtx ctx, const float4* __restrict__ rays, uint32_t n,
         float4 a = rays[2 * size_t(i)];
         float4 b = rays[2 * size_t(i) + 1];
         HitRec h = TraceW<W, false, COUNT>(ctx, lstack, gtid, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z},
-                                       a.w, b.w, &nodeVisits, &triTests);
+                                       a.w, b.w, 0u, &nodeVisits, &triTests);
         prim[i] = h.prim;
         if(tOut) tOut[i] = (h.prim == kInvalid) ? b.w : h.t;
         if(bary)
@@ -47,7 +47,7 @@ KCCastVisibility(TraceCtx ctx, const float4* __restrict__ rays, uint32_t n,
         float4 a = rays[2 * size_t(i)];
         float4 b = rays[2 * size_t(i) + 1];
         HitRec h = TraceW<W, true, false>(ctx, lstack, gtid, V3{a.x, a.y, a.z}, V3{b.x, b.y, b.z},
-                                      a.w, b.w);
+                                      a.w, b.w, 0u);
         visible[i] = (h.prim == kInvalid) ? 1u : 0u;
     }
 }

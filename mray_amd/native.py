@@ -37,6 +37,9 @@ EXPORTS = [
     "mray_nrc_write", "mray_nrc_zero_grads",
     "mray_renderer_attach_nrc", "mray_renderer_render_frame", "mray_renderer_cache_stats",
     "mray_device_memory",
+    "mray_batch_scene_create", "mray_batch_scene_destroy", "mray_batch_scene_info",
+    "mray_batch_create", "mray_batch_read_film", "mray_batch_view_times", "mray_batch_autotune",
+    "mray_batch_set_trace_tpb",
 ]
 
 
@@ -99,6 +102,16 @@ def lib():
     L.mray_renderer_cache_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.mray_renderer_set_measurement.argtypes = [P, u32, u32]
     L.mray_renderer_phase_times.argtypes = [P, P, P]
+    L.mray_batch_scene_create.argtypes = [P, u32, C.POINTER(P)]
+    L.mray_batch_scene_destroy.argtypes = [P]
+    L.mray_batch_scene_info.argtypes = [P, C.POINTER(u32), C.POINTER(u32), C.POINTER(u32),
+                                        C.POINTER(u32)]
+    L.mray_batch_create.argtypes = [P, P, u32, C.POINTER(abi.MRayRenderParams), C.POINTER(P)]
+    L.mray_batch_read_film.argtypes = [P, u32, P]
+    L.mray_batch_view_times.argtypes = [P, P]
+    L.mray_batch_set_trace_tpb.argtypes = [P, u32]
+    L.mray_batch_autotune.argtypes = [P, P, P, u32, C.POINTER(abi.MRayRenderParams),
+                                      C.POINTER(abi.MRayBatchTune), P, u32, C.POINTER(u32)]
     _lib = L
     return L
 
@@ -419,3 +432,79 @@ class Nrc:
         import numpy as np
         a = np.ascontiguousarray(arr, np.float32).reshape(-1)
         check(lib().mray_nrc_write(self.h, which, abi.fptr(a)))
+
+
+class BatchScene:
+    """Scenes of several accelerators merged for one batch (MRayBatchScene)."""
+
+    def __init__(self, accels):
+        require_gpu()
+        arr = (C.c_void_p * len(accels))(*[a.h.value for a in accels])
+        h = C.c_void_p()
+        check(lib().mray_batch_scene_create(arr, len(accels), C.byref(h)))
+        self.h = h
+        self.count = len(accels)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mray_batch_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        v = [C.c_uint32() for _ in range(4)]
+        check(lib().mray_batch_scene_info(self.h, *[C.byref(x) for x in v]))
+        return {"scenes": v[0].value, "triangles": v[1].value, "nodes": v[2].value,
+                "width": v[3].value}
+
+
+def _tune_dict(t):
+    return {"width": t.width, "queues": t.queues, "slots": t.slots, "trace_tpb": t.traceTPB,
+            "mpaths_per_s": t.mpathsPerS}
+
+
+def batch_autotune(scene4, scene8, views, params, max_trials=64):
+    """Autotuner: best setting and every trial (see mray_batch_autotune)."""
+    arr = abi.batch_views(views)
+    best = abi.MRayBatchTune()
+    trials = (abi.MRayBatchTune * max_trials)()
+    n = C.c_uint32()
+    check(lib().mray_batch_autotune(scene4.h if scene4 is not None else None,
+                                    scene8.h if scene8 is not None else None,
+                                    arr, len(views), C.byref(params), C.byref(best), trials,
+                                    max_trials, C.byref(n)))
+    return _tune_dict(best), [_tune_dict(trials[k]) for k in range(min(n.value, max_trials))]
+
+
+class BatchRenderer(Renderer):
+    """Batch of views over a BatchScene in one renderer (mray_batch_create)."""
+
+    def __init__(self, bscene, views, params):
+        self.bscene = bscene
+        self.params = params
+        self.views = [dict(v) for v in views]
+        arr = abi.batch_views(views)
+        h = C.c_void_p()
+        check(lib().mray_batch_create(bscene.h, arr, len(views), C.byref(params), C.byref(h)))
+        self.h = h
+
+    def film(self, view=None):
+        if view is None:
+            return [self.film(k) for k in range(len(self.views))]
+        v = self.views[view]
+        out = np.zeros((4, v["height"], v["width"]), np.float32)
+        check(lib().mray_batch_read_film(self.h, view, out.ctypes.data))
+        return out
+
+    def view_times(self):
+        ms = np.zeros(len(self.views), np.float64)
+        check(lib().mray_batch_view_times(self.h, ms.ctypes.data))
+        return ms
+
+    def set_trace_tpb(self, tpb):
+        check(lib().mray_batch_set_trace_tpb(self.h, tpb))

@@ -361,15 +361,16 @@ def _icosphere1():
 
 
 def terrain_spheres(grid=1024, spheres=2000, width=1920, height=1080, fov_x_deg=60.0,
-                    light_grid=4):
+                    light_grid=4, seed=0):
     """1024 x 1024 vertex heightfield (0.2 x 6-octave fBm, mt19937(2)) plus
     `spheres` icospheres (80 tris, radius log-U[0.01, 0.1], mt19937(3)).
     Materials U{0..3} (mt19937(4)) per terrain patch / sphere: Lambert
     (albedo U[0.1,0.9]^3), Unreal metallic=1 roughness U[0.05,0.5], Refract
     cauchyBack [1.5,0,0], Unreal metallic=0 specular 0.5 roughness 0.3.
     Constant Skysphere (0.5) boundary and light_grid^2 downward quad lights
-    (radiance 10) at height 1.5."""
-    rng_terrain = MT19937(2)
+    (radiance 10) at height 1.5.  seed s > 0 moves the three generator
+    seeds to 2 + 3s, 3 + 3s, 4 + 3s (an independent scene of the same kind)."""
+    rng_terrain = MT19937(2 + 3 * seed)
     perm, grads = _perlin_table(rng_terrain)
     xs = np.linspace(-1.0, 1.0, grid)
     X, Z = np.meshgrid(xs, xs, indexing="xy")
@@ -390,7 +391,7 @@ def terrain_spheres(grid=1024, spheres=2000, width=1920, height=1080, fov_x_deg=
     tris = np.concatenate([np.stack([v00, v01, v11], 1), np.stack([v00, v11, v10], 1)], 0)
     cell = np.concatenate([np.stack([ii.reshape(-1), jj.reshape(-1)], 1)] * 2, 0)
 
-    rng_mat = MT19937(4)
+    rng_mat = MT19937(4 + 3 * seed)
     mats = []
 
     def new_material():
@@ -423,7 +424,7 @@ def terrain_spheres(grid=1024, spheres=2000, width=1920, height=1080, fov_x_deg=
     b.add_mesh(tpos, tris, key=tkey, cull=tcull, normals=tnrm)
 
     ico_v, ico_f = _icosphere1()
-    rng_sph = MT19937(3)
+    rng_sph = MT19937(3 + 3 * seed)
     hf = Y.reshape(-1)
     for _ in range(spheres):
         px = rng_sph.uniform(-0.95, 0.95)
@@ -458,7 +459,32 @@ def terrain_spheres(grid=1024, spheres=2000, width=1920, height=1080, fov_x_deg=
                  light_two_sided=np.zeros(nl, dtype=np.uint8),
                  boundary_type=abi.BOUNDARY_SKYSPHERE_COOCTA,
                  boundary_radiance=(0.5, 0.5, 0.5),
-                 camera=cam, name=f"terrain{grid}_spheres{spheres}")
+                 camera=cam, name=f"terrain{grid}_spheres{spheres}" + (f"_s{seed}" if seed else ""))
+
+
+def terrain_spheres_sized(triangles, width, height, seed):
+    """terrain_spheres with about `triangles` triangles, terrain and spheres
+    in the 1024-grid scene's proportion (2000 spheres per 2.25 M)."""
+    spheres = max(1, int(round(2000 * triangles / 2253090)))
+    rest = max(2 * 16 * 16, triangles - 80 * spheres - 32)
+    grid = int(round(math.sqrt(rest / 2.0))) + 1
+    return terrain_spheres(grid=grid, spheres=spheres, width=width, height=height, seed=seed)
+
+
+def batch_workload(count=32, seed=1, min_tris=10_000, max_tris=2_000_000, spp=16):
+    """Independent terrain scenes with triangle counts log-uniform in
+    [min_tris, max_tris] and resolutions drawn from 256^2, 512^2, 1280x720
+    and 1920x1080 (mt19937(seed)).  Returns the scenes and the view dicts
+    for mray_batch_create (view k renders scene k with its own camera)."""
+    rng = MT19937(seed)
+    res = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    out, views = [], []
+    for k in range(count):
+        t = math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))
+        w, h = res[rng() % 4]
+        out.append(terrain_spheres_sized(int(t), w, h, seed=10 + k))
+        views.append({"scene": k, "width": w, "height": h, "spp": spp})
+    return out, views
 
 
 def random_triangles(n, seed=7, scale=1.0):

@@ -2079,6 +2079,110 @@ static RenderOut Render(const Scene& s, const MRayRenderParams& p, float* film, 
     return out;
 }
 
+// Batch of views over several scenes (mray_batch_create semantics; no
+// reference counterpart): one slot pool, global path indices taken view by
+// view (view k owns [sum of earlier W*H*spp, + its own)), each slot shaded
+// in its view's scene and splatted into its view's film.  The schedule is
+// Render's: ascending dead slots take the next consecutive indices.
+struct BatchViewRT
+{
+    const Scene* s;
+    Camera cam;
+    uint32_t W, H;
+    uint64_t P, begin, end;
+    float* film;
+};
+
+static RenderOut RenderBatch(const std::vector<BatchViewRT>& views, const MRayRenderParams& p,
+                             uint32_t threads)
+{
+    RenderOut out;
+    uint64_t pixels = 0;
+    for(const BatchViewRT& v : views) pixels += v.P;
+    const uint64_t limit = views.back().end;
+    uint32_t N = p.maxRayCount ? p.maxRayCount : uint32_t(std::min<uint64_t>(pixels, 1u << 21));
+    if(N > pixels) N = uint32_t(pixels);
+    const float filterRadius = (p.filterRadius > 0.0f) ? p.filterRadius : 1.0f;
+    std::vector<uint32_t> states(N);
+    SeedStates(p.seed, N, states.data());
+    std::vector<Slot> slots(N);
+    std::vector<uint32_t> slotView(N, 0);
+    for(uint32_t i = 0; i < N; i++)
+    {
+        std::memset(&slots[i], 0, sizeof(Slot));
+        slots[i].status = ST_INVALID;
+        slots[i].rng = states[i];
+    }
+    for(const BatchViewRT& v : views) std::memset(v.film, 0, sizeof(float) * 4 * v.P);
+    uint64_t counter = 0;
+    std::vector<RenderCounters> ctr(std::max(threads, 1u));
+    std::vector<uint32_t> alive;
+    alive.reserve(N);
+    for(;;)
+    {
+        uint64_t k = 0;
+        for(uint32_t i = 0; i < N; i++)
+        {
+            Slot& sl = slots[i];
+            if(sl.status == ST_ALIVE) continue;
+            if(counter + k < limit)
+            {
+                const uint64_t g = counter + k;
+                uint32_t vi = 0;
+                while(vi + 1 < views.size() && views[vi + 1].begin <= g) vi++;
+                const BatchViewRT& v = views[vi];
+                uint64_t pix = (g - v.begin) % v.P;
+                uint32_t px = uint32_t(pix % v.W), py = uint32_t(pix / v.W);
+                float xi0 = ToFloat01(PcgNext(sl.rng));
+                float xi1 = ToFloat01(PcgNext(sl.rng));
+                CamRay cr = GenCamRay(v.cam, px, py, v.W, v.H, xi0, xi1, filterRadius);
+                sl.pos = cr.pos; sl.dir = cr.dir; sl.tMin = cr.tMin; sl.tMax = cr.tMax;
+                sl.filterWeight = cr.weight;
+                sl.pixel = uint32_t(pix);
+                sl.radiance = Splat4(0.0f);
+                sl.throughput = Splat4(1.0f);
+                sl.depth = 0;
+                sl.status = ST_ALIVE;
+                sl.type = RT_CAMERA;
+                sl.dispersed = false;
+                sl.prevMatPdf = 0.0f;
+                if(p.spectral)
+                {
+                    float xw = ToFloat01(PcgNext(sl.rng));
+                    SampleWavelengths(xw, sl.waves, sl.wavePdf);
+                }
+                slotView[i] = vi;
+                k++;
+            }
+            else sl.status = ST_INVALID;
+        }
+        counter += k;
+        alive.clear();
+        for(uint32_t i = 0; i < N; i++) if(slots[i].status == ST_ALIVE) alive.push_back(i);
+        if(alive.empty()) break;
+        out.iterations++;
+        for(auto& c : ctr) c = RenderCounters{};
+        ParallelFor(uint32_t(alive.size()), threads, [&](uint32_t i, uint32_t t)
+        {
+            uint32_t si = alive[i];
+            BounceSlot(*views[slotView[si]].s, p, slots[si], ctr[t]);
+        });
+        for(auto& c : ctr) { out.extRays += c.extRays; out.shadowRays += c.shadowRays; }
+        for(uint32_t i : alive)
+        {
+            Slot& sl = slots[i];
+            if(sl.status != ST_DEAD) continue;
+            const BatchViewRT& v = views[slotView[i]];
+            V4 rad = sl.radiance;
+            if(p.spectral) rad = SpectrumToRGB(rad, sl.waves, sl.wavePdf, sl.dispersed);
+            FilmAdd(v.film, v.W, v.H, sl.pixel, rad, sl.filterWeight);
+            sl.status = ST_INVALID;
+            out.deadPaths++;
+        }
+    }
+    return out;
+}
+
 } // namespace rc
 
 // ==========================================================================
@@ -2302,6 +2406,44 @@ int rc_render(void* h, const MRayRenderParams* p, float* film, uint32_t threads,
     if(p->spectral && !gSpec.loaded) return 1;
     if(threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     RenderOut o = Render(s, *p, film, threads, maxIterations ? maxIterations : 0xFFFFFFFFu, pathLimit);
+    if(stats)
+    {
+        stats[0] = o.deadPaths;
+        stats[1] = o.extRays;
+        stats[2] = o.shadowRays;
+        stats[3] = o.iterations;
+    }
+    return 0;
+}
+
+// Batch render; views as MRayBatchView (scene index into `scenes`), films[v]
+// [4][H][W] per view.  stats[4] as rc_render.
+int rc_render_batch(void* const* scenes, uint32_t nScenes, const MRayBatchView* views,
+                    uint32_t nViews, const MRayRenderParams* p, float* const* films,
+                    uint32_t threads, uint64_t* stats)
+{
+    if(p->spectral && !gSpec.loaded) return 1;
+    if(nViews == 0) return 2;
+    if(threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::vector<BatchViewRT> vs(nViews);
+    uint64_t paths = 0;
+    for(uint32_t k = 0; k < nViews; k++)
+    {
+        const MRayBatchView& v = views[k];
+        if(v.scene >= nScenes) return 2;
+        const Scene* s = static_cast<const Scene*>(scenes[v.scene]);
+        BatchViewRT& o = vs[k];
+        o.s = s;
+        o.cam = MakeCamera(v.useCamera ? v.camera : s->cam);
+        o.W = v.width;
+        o.H = v.height;
+        o.P = uint64_t(v.width) * v.height;
+        o.begin = paths;
+        o.end = paths + o.P * v.spp;
+        o.film = films[k];
+        paths = o.end;
+    }
+    RenderOut o = RenderBatch(vs, *p, threads);
     if(stats)
     {
         stats[0] = o.deadPaths;

@@ -57,6 +57,9 @@ def lib():
         L.rc_render.restype = C.c_int
         L.rc_render.argtypes = [P, C.POINTER(abi.MRayRenderParams), f32, C.c_uint32,
                                 C.c_uint32, C.c_uint64, u64]
+        L.rc_render_batch.restype = C.c_int
+        L.rc_render_batch.argtypes = [P, C.c_uint32, P, C.c_uint32, C.POINTER(abi.MRayRenderParams),
+                                      P, C.c_uint32, u64]
         L.rc_spectral_load.argtypes = [C.c_int, f32, f32, f32, f32]
         L.rc_sample_wavelengths.argtypes = [f32, C.c_uint32, f32, f32]
         L.rc_rgb_to_spectrum.argtypes = [f32, f32, C.c_uint32, C.c_int, f32]
@@ -219,6 +222,22 @@ class OracleScene:
     def light_pdf(self, li, a, b, ray_pos, ray_dir):
         return float(lib().rc_light_pdf(self.h, li, a, b, abi.fptr(_f(ray_pos)),
                                         abi.fptr(_f(ray_dir))))
+
+
+def render_batch(scenes, views, params, threads=0):
+    """Batch render over OracleScenes (views as for mray_batch_create);
+    returns one [4][H][W] film per view and the stats."""
+    arr = abi.batch_views(views)
+    films = [np.zeros((4, v["height"], v["width"]), np.float32) for v in views]
+    fp = (C.c_void_p * len(films))(*[f.ctypes.data for f in films])
+    sp = (C.c_void_p * len(scenes))(*[s.h for s in scenes])
+    stats = np.zeros(4, np.uint64)
+    rc = lib().rc_render_batch(sp, len(scenes), arr, len(views), C.byref(params), fp, threads,
+                               abi.u64ptr(stats))
+    if rc != 0:
+        raise RuntimeError(f"rc_render_batch failed ({rc})")
+    return films, {"dead_paths": int(stats[0]), "ext_rays": int(stats[1]),
+                   "shadow_rays": int(stats[2]), "iterations": int(stats[3])}
 
 
 def seed_states(seed, count):

@@ -29,6 +29,8 @@ def test_struct_layouts():
     assert ctypes.sizeof(abi.MRayRenderParams) == 48
     assert ctypes.sizeof(abi.MRayRenderStats) == 56
     assert ctypes.sizeof(abi.MRayNrcConfig) == 48
+    assert ctypes.sizeof(abi.MRayBatchView) == 76
+    assert ctypes.sizeof(abi.MRayBatchTune) == 24
 
 
 def test_error_path_without_gpu_work():
