@@ -64,7 +64,12 @@ def parse():
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--batch-spp", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true",
-                    help="batch: skip the autotuner (4-wide, no queues, 2^21 slots, 256 threads)")
+                    help="batch: skip the autotuner and use --batch-slots / --batch-tpb, "
+                         "4-wide, no queues")
+    ap.add_argument("--batch-slots", type=int, default=1 << 22)
+    ap.add_argument("--batch-tpb", type=int, choices=(128, 256), default=128)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="batch: skip rendering the views one at a time for comparison")
     ap.add_argument("--probe-copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
@@ -234,20 +239,24 @@ def bench_batch(args):
     log(f"scenes: {tris} triangles in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     merged = {}
+    singles = []
     for width in ((4,) if args.no_autotune else (4, 8)):
         accs = []
         for s in scs:
             accs.append(native.Accel(s, width=width))
         merged[width] = native.BatchScene(accs)
-        for a in accs:
-            a.close()
+        if width == 4 and not args.no_sequential:
+            singles = accs          # kept for the one-view-at-a-time comparison
+        else:
+            for a in accs:
+                a.close()
         log(f"merged {width}-wide scene: {merged[width].info()}")
     build_s = time.perf_counter() - t0
     params = abi.render_params(1, 1, 1, rr=(8, 8), mode=abi.SAMPLE_NEE_MIS, spectral=spectral,
                                seed=0)
     trials = []
     if args.no_autotune:
-        best = {"width": 4, "queues": 0, "slots": 1 << 21, "trace_tpb": 256}
+        best = {"width": 4, "queues": 0, "slots": args.batch_slots, "trace_tpb": args.batch_tpb}
     else:
         t0 = time.perf_counter()
         best, trials = native.batch_autotune(merged[4], merged[8], views, params)
@@ -341,6 +350,26 @@ def bench_batch(args):
     }
     r.close()
     bs.close()
+    if singles:
+        # the same views rendered one at a time by one-scene renderers (a
+        # slot per pixel, the default pool), for comparison with the batch
+        seq_s = 0.0
+        for s, a, v in zip(scs, singles, views):
+            p1 = abi.render_params(v["width"], v["height"], v["spp"], rr=(8, 8),
+                                   mode=abi.SAMPLE_NEE_MIS, spectral=spectral, seed=0)
+            r1 = native.Renderer(a, p1)
+            r1.render_all()
+            r1.reset()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r1.render_all()
+            torch.cuda.synchronize()
+            seq_s += time.perf_counter() - t0
+            r1.close()
+            a.close()
+        result["sequential"] = {"ms_per_step": round(seq_s * 1e3, 2),
+                                "mpaths_per_s": round(paths / seq_s / 1e6, 3)}
+        log(f"one view at a time: {result['sequential']}")
     print(json.dumps(result), flush=True)
 
 

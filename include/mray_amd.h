@@ -305,9 +305,10 @@ int mray_batch_create(const MRayBatchScene* scene, const MRayBatchView* hViews, 
                       const MRayRenderParams* hParams, MRayRenderer** out);
 /* Film of one view: hOut [4][height][width], as mray_renderer_read_film.   */
 int mray_batch_read_film(MRayRenderer* r, uint32_t view, float* hOut);
-/* Per-view latency: hMs[v] = time from the frame's first refill to the
- * moment view v's last path reached its film (device wall clock), or -1
- * while view v is unfinished.                                              */
+/* Per-view latency: hMs[v] = device wall-clock time from the frame's first
+ * refill to the refill that splats view v's last paths (v and every view
+ * before it have then no path running or unissued), or -1 while v is
+ * unfinished.                                                              */
 int mray_batch_view_times(MRayRenderer* r, double* hMs);
 
 /* Autotuner for a batch (no reference counterpart).  Renders the batch at

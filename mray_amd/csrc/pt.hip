@@ -122,6 +122,8 @@ struct Sched
     unsigned long long ringCount;   // training pairs written so far
     unsigned long long queries;     // radiance-cache queries so far
     unsigned long long startClock;  // wall clock at the first refill
+    uint32_t viewsDone;             // batch: views [0, viewsDone) are complete
+    uint32_t viewBase;              // batch: view of the first path this refill issues
 };
 
 // Persistent kernels take work 64 items at a time: one atomic per wave,
@@ -166,10 +168,24 @@ struct RenderConst
     NrcRenderState nrc;
     // batch kernels only
     const ViewGPU* views;             // ordered by pathBegin
+    const unsigned long long* viewBegin;  // [viewCount + 1] path index ranges
     uint32_t viewCount;
-    unsigned long long* viewDone;     // finished paths per view
-    unsigned long long* viewClock;    // wall clock when a view's last path reached the film
+    unsigned long long* viewClock;    // wall clock when a view completed
 };
+
+// View whose path range holds global path index g: the last v in [lo, hi]
+// with begin[v] <= g (views ordered by their first path index).
+__device__ __forceinline__ uint32_t ViewOfPath(const unsigned long long* __restrict__ begin,
+                                               uint32_t lo, uint32_t hi, unsigned long long g)
+{
+    while(lo < hi)
+    {
+        uint32_t mid = (lo + hi + 1u) >> 1;
+        if(begin[mid] <= g) lo = mid;
+        else hi = mid - 1u;
+    }
+    return lo;
+}
 
 // Scene of slot i: the whole scene, or in a batch the merged arrays with the
 // slot's view's lights and boundary light.
@@ -202,12 +218,21 @@ static constexpr uint32_t kGraphIters = 8;  // bounces per captured hipGraph
 // Per-block counts: x = free slots (dead or invalid), y = live slots,
 // z = finished paths waiting for the film.  KRefillScan turns x and y into
 // exclusive prefixes.
+template<bool BATCH>
 __global__ void __launch_bounds__(kSlotTPB)
-KRefillCount(const uint32_t* __restrict__ meta, uint32_t N, uint4* __restrict__ blockCnt)
+KRefillCount(const uint32_t* __restrict__ meta, const uint32_t* __restrict__ view, uint32_t N,
+             uint4* __restrict__ blockCnt)
 {
-    __shared__ uint32_t wsum[3][kSlotTPB / 64];
+    __shared__ uint32_t wsum[4][kSlotTPB / 64];
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t st = (i < N) ? MStatus(meta[i]) : ST_ALIVE;
+    // batch: lowest view among the paths still running in this block
+    uint32_t vmin = 0xFFFFFFFFu;
+    if constexpr(BATCH)
+    {
+        if(i < N && st == ST_ALIVE) vmin = view[i];
+        for(int o = 32; o > 0; o >>= 1) vmin = min(vmin, uint32_t(__shfl_xor(int(vmin), o)));
+    }
     uint64_t bf = __ballot((i < N) && st != ST_ALIVE);
     uint64_t bl = __ballot((i < N) && st == ST_ALIVE);
     uint64_t bd = __ballot((i < N) && st == ST_DEAD);
@@ -217,16 +242,18 @@ KRefillCount(const uint32_t* __restrict__ meta, uint32_t N, uint4* __restrict__ 
         wsum[0][w] = uint32_t(__popcll(bf));
         wsum[1][w] = uint32_t(__popcll(bl));
         wsum[2][w] = uint32_t(__popcll(bd));
+        wsum[3][w] = vmin;
     }
     __syncthreads();
     if(threadIdx.x == 0)
     {
-        uint4 c = make_uint4(0, 0, 0, 0);
+        uint4 c = make_uint4(0, 0, 0, 0xFFFFFFFFu);
         for(uint32_t k = 0; k < kSlotTPB / 64; k++)
         {
             c.x += wsum[0][k];
             c.y += wsum[1][k];
             c.z += wsum[2][k];
+            c.w = min(c.w, wsum[3][k]);
         }
         blockCnt[blockIdx.x] = c;
     }
@@ -234,20 +261,28 @@ KRefillCount(const uint32_t* __restrict__ meta, uint32_t N, uint4* __restrict__ 
 
 // One block: exclusive scans of the free and live counts, then the refill
 // amount (ReloadPaths with TotalSampleLimit, PathTracerRendererBase.h:59-67).
+// Batch views complete when every path below both the lowest view still
+// running and the view of the next unissued path has reached the film:
+// those are stamped with the wall clock here (their last paths are splatted
+// by the refill that follows).
 __global__ void __launch_bounds__(1024)
-KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sched)
+KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sched,
+            const unsigned long long* __restrict__ viewBegin, uint32_t viewCount,
+            unsigned long long* __restrict__ viewClock)
 {
     __shared__ uint32_t sf[1024], sl[1024];
-    __shared__ uint32_t carryF, carryL;
+    __shared__ uint32_t carryF, carryL, vmin;
     __shared__ unsigned long long accum;
-    if(threadIdx.x == 0) { carryF = 0; carryL = 0; accum = 0; }
+    if(threadIdx.x == 0) { carryF = 0; carryL = 0; accum = 0; vmin = 0xFFFFFFFFu; }
     __syncthreads();
     unsigned long long myAccum = 0;
+    uint32_t myMin = 0xFFFFFFFFu;
     for(uint32_t base = 0; base < count; base += 1024)
     {
         uint32_t i = base + threadIdx.x;
-        uint4 v = (i < count) ? blockCnt[i] : make_uint4(0, 0, 0, 0);
+        uint4 v = (i < count) ? blockCnt[i] : make_uint4(0, 0, 0, 0xFFFFFFFFu);
         myAccum += v.z;
+        myMin = min(myMin, v.w);
         sf[threadIdx.x] = v.x;
         sl[threadIdx.x] = v.y;
         __syncthreads();
@@ -269,7 +304,24 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
     // number of finished paths: wave reduction, then one LDS atomic per wave
     for(uint32_t o = 32; o > 0; o >>= 1) myAccum += __shfl_down(myAccum, o);
     if((threadIdx.x & 63u) == 0) atomicAdd(&accum, myAccum);
+    if(viewClock)
+    {
+        for(int o = 32; o > 0; o >>= 1) myMin = min(myMin, uint32_t(__shfl_xor(int(myMin), o)));
+        if((threadIdx.x & 63u) == 0) atomicMin(&vmin, myMin);
+    }
     __syncthreads();
+    if(threadIdx.x == 0 && viewClock)
+    {
+        const unsigned long long next = sched->counter;
+        uint32_t vb = (next >= viewBegin[viewCount]) ? viewCount
+                                                     : ViewOfPath(viewBegin, 0u, viewCount - 1u, next);
+        sched->viewBase = min(vb, viewCount - 1u);
+        uint32_t bound = min(vb, vmin);
+        if(bound > viewCount) bound = viewCount;
+        const unsigned long long now = wall_clock64();
+        for(uint32_t v = sched->viewsDone; v < bound; v++) viewClock[v] = now;
+        if(bound > sched->viewsDone) sched->viewsDone = bound;
+    }
     if(threadIdx.x == 0)
     {
         unsigned long long freeSlots = carryF;
@@ -327,30 +379,6 @@ __device__ __forceinline__ void SplatToFilm(const SlotSoA& S, const RenderConst&
     atomicAdd(film + P + px, g);
     atomicAdd(film + 2 * P + px, b);
     atomicAdd(film + 3 * P + px, w);
-}
-
-// Batch: finished paths per view, one atomic per view present in the wave;
-// the lane that completes a view stamps the wall clock.  Called by every lane.
-__device__ __forceinline__ void CountFinished(const SlotSoA& S, const RenderConst& rc, bool fin,
-                                              uint32_t i)
-{
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t v = fin ? S.view[i] : 0u;
-    uint64_t left = __ballot(fin);
-    while(left)
-    {
-        const uint32_t leader = uint32_t(__ffsll((unsigned long long)left) - 1);
-        const uint32_t lv = __shfl(v, int(leader));
-        const uint64_t same = __ballot(fin && v == lv);
-        if(lane == leader)
-        {
-            const unsigned long long c = (unsigned long long)__popcll(same);
-            const unsigned long long before = atomicAdd(rc.viewDone + lv, c);
-            const ViewGPU& V = rc.views[lv];
-            if(before + c == V.pathEnd - V.pathBegin) rc.viewClock[lv] = wall_clock64();
-        }
-        left &= ~same;
-    }
 }
 
 // A finished training path turns its recorded vertices into training pairs:
@@ -438,7 +466,6 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
         if(!SPECTRAL && rc.nrc.on && (i & ((1u << rc.nrc.trainShift) - 1u)) == 0u)
             EmitTrainingPairs(S, rc, i, sched);
     }
-    if constexpr(BATCH) CountFinished(S, rc, inRange && status == ST_DEAD, i);
     uint64_t b = __ballot(freeSlot);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     if(lane == 0) wsum[wave] = uint32_t(__popcll(b));
@@ -456,14 +483,8 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
             const unsigned long long g = sched->base + rank;
             if constexpr(BATCH)
             {
-                // the view whose path range holds g (views ordered by pathBegin)
-                uint32_t lo = 0, hi = rc.viewCount - 1u;
-                while(lo < hi)
-                {
-                    uint32_t mid = (lo + hi + 1u) >> 1;
-                    if(rc.views[mid].pathBegin <= g) lo = mid;
-                    else hi = mid - 1u;
-                }
+                // the view whose path range holds g (at or after this refill's first)
+                const uint32_t lo = ViewOfPath(rc.viewBegin, sched->viewBase, rc.viewCount - 1u, g);
                 const ViewGPU& v = rc.views[lo];
                 S.view[i] = lo;
                 StartCameraPath<SPECTRAL>(S, v.cam, v.W, i, uint32_t((g - v.pathBegin) % v.P));
@@ -1079,6 +1100,9 @@ __global__ void KResetSlots(uint32_t* __restrict__ meta, uint32_t* __restrict__ 
         sched->queryCount = 0;
         sched->ringCount = 0;
         sched->queries = 0;
+        sched->startClock = 0;
+        sched->viewsDone = 0;
+        sched->viewBase = 0;
     }
 }
 
@@ -1134,7 +1158,7 @@ struct MRayRenderer
     const MRayBatchScene* bscene = nullptr;
     std::vector<ViewGPU> hViews;
     ViewGPU* dViews = nullptr;
-    unsigned long long* dViewDone = nullptr;
+    unsigned long long* dViewBegin = nullptr;
     unsigned long long* dViewClock = nullptr;
     uint32_t traceTPB = kTraceTPB;
 };
@@ -1246,8 +1270,9 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
     }
     int mk;
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    KRefillCount<<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, N, r->dBlockCnt);
-    KRefillScan<<<1, 1024, 0, s>>>(r->dBlockCnt, r->nBlocks, r->dSched);
+    KRefillCount<BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, r->S.view, N, r->dBlockCnt);
+    KRefillScan<<<1, 1024, 0, s>>>(r->dBlockCnt, r->nBlocks, r->dSched, r->rc.viewBegin,
+                                   r->rc.viewCount, BATCH ? r->dViewClock : nullptr);
     if(r->rc.spectral)
         KRefillApply<true, BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
                                                                   r->dActive);
@@ -1307,7 +1332,6 @@ static int ResetRenderer(MRayRenderer* r)
     if(r->bscene)
     {
         const size_t V = r->hViews.size();
-        KZeroFloats<<<DivUp(2 * V, 256), 256, 0, s>>>(reinterpret_cast<float*>(r->dViewDone), 2 * V);
         KZeroFloats<<<DivUp(2 * V, 256), 256, 0, s>>>(reinterpret_cast<float*>(r->dViewClock), 2 * V);
     }
     MRAY_HIP_CHECK(hipGetLastError());
@@ -1566,13 +1590,20 @@ static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayBatch
         g.film = r->rc.film + (reinterpret_cast<uintptr_t>(g.film) / sizeof(float));
     DeviceArena& ar = r->arena;
     MRAY_TRY(DevAlloc(ar, (void**)&r->dViews, viewCount * sizeof(ViewGPU)));
-    MRAY_TRY(DevAlloc(ar, (void**)&r->dViewDone, viewCount * sizeof(unsigned long long)));
+    MRAY_TRY(DevAlloc(ar, (void**)&r->dViewBegin, (viewCount + 1) * sizeof(unsigned long long)));
     MRAY_TRY(DevAlloc(ar, (void**)&r->dViewClock, viewCount * sizeof(unsigned long long)));
     MRAY_HIP_CHECK(hipMemcpy(r->dViews, r->hViews.data(), viewCount * sizeof(ViewGPU),
                              hipMemcpyHostToDevice));
     r->rc.views = r->dViews;
     r->rc.viewCount = viewCount;
-    r->rc.viewDone = r->dViewDone;
+    {
+        std::vector<unsigned long long> begin(viewCount + 1);
+        for(uint32_t k = 0; k < viewCount; k++) begin[k] = r->hViews[k].pathBegin;
+        begin[viewCount] = r->hViews[viewCount - 1].pathEnd;
+        MRAY_HIP_CHECK(hipMemcpy(r->dViewBegin, begin.data(), begin.size() * sizeof(unsigned long long),
+                                 hipMemcpyHostToDevice));
+    }
+    r->rc.viewBegin = r->dViewBegin;
     r->rc.viewClock = r->dViewClock;
     return ResetRenderer(r);
 }
