@@ -340,6 +340,20 @@ int mray_batch_set_trace_tpb(MRayRenderer* r, uint32_t tpb);
 /* XYZ->RGB [9].  Built on first use and cached next to the library.        */
 /* ---------------------------------------------------------------------- */
 int mray_spectral_tables(int* hN, float* hLut, float* hCmf, float* hIllum, float* hXyzToRgb);
+/* The RGB->spectrum coefficients as a reference .mrspectra file
+ * (SpectraLUTGen/main.cpp:590-609): "MR_SPECTRA", u32 N = 64, u32 mode = 1,
+ * then nine N^3 float volumes [largest channel][c0, c1, c2], x fastest.
+ * The library keeps its own tables in that format at
+ * <library dir>/SpectraLUT/REC_709.mrspectra, where the reference's loader
+ * looks (SpectrumContext.cu:298-316), and reads a file placed there.        */
+int mray_spectral_write_mrspectra(const char* path);
+/* Read and validate a .mrspectra file with the reference loader's checks
+ * (SpectrumContext.cu:317-351: character code, endianness, N, mode) into
+ * hLut [3][N][N][N][4] (c0, c1, c2, 0); either output may be NULL.         */
+int mray_spectral_read_mrspectra(const char* path, int* hN, float* hLut);
+/* Use a .mrspectra file's coefficients for every spectral renderer created
+ * afterwards; an error once a spectral renderer has uploaded the tables.   */
+int mray_spectral_use_mrspectra(const char* path);
 
 
 /* ---------------------------------------------------------------------- */

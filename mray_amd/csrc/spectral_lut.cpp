@@ -30,6 +30,7 @@
 #include <vector>
 
 #include <dlfcn.h>
+#include <sys/stat.h>
 
 namespace mray
 {
@@ -250,17 +251,81 @@ struct HostTables
     bool built = false;
 };
 
-std::string CachePath()
+// The table file lives where the reference looks for it, relative to the
+// binary: <library dir>/SpectraLUT/REC_709.mrspectra (LUT_FOLDER_NAME and
+// the colour-space name, Core/ColorFunctions.h:112-116, Definitions.h:615).
+std::string LibDir()
 {
     Dl_info info;
-    if(dladdr(reinterpret_cast<void*>(&CachePath), &info) && info.dli_fname)
+    if(dladdr(reinterpret_cast<void*>(&LibDir), &info) && info.dli_fname)
     {
         std::string p(info.dli_fname);
         size_t s = p.find_last_of('/');
-        std::string dir = (s == std::string::npos) ? std::string(".") : p.substr(0, s);
-        return dir + "/rgb2spec_rec709_planck6504_64_v3.bin";
+        return (s == std::string::npos) ? std::string(".") : p.substr(0, s);
     }
-    return "rgb2spec_rec709_planck6504_64.bin";
+    return ".";
+}
+std::string CachePath() { return LibDir() + "/SpectraLUT/REC_709.mrspectra"; }
+
+// .mrspectra (SpectraLUTGen/main.cpp:590-609, read back by
+// ReadMRSpectraFileHeader, SpectrumContext.cu:298-352): the ten characters
+// "MR_SPECTRA", u32 resolution (64), u32 mode (1 = float), then nine
+// N^3 float volumes, x fastest, ordered [largest channel][c0, c1, c2].
+constexpr char kCC[10] = {'M', 'R', '_', 'S', 'P', 'E', 'C', 'T', 'R', 'A'};
+
+bool WriteMrSpectra(const std::vector<float>& lut, const std::string& path, std::string& err)
+{
+    const size_t N3 = size_t(kLutN) * kLutN * kLutN;
+    std::vector<float> vol(N3);
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if(!f) { err = "cannot open " + path + " for writing"; return false; }
+    uint32_t n = kLutN, mode = 1;
+    bool ok = std::fwrite(kCC, 1, sizeof(kCC), f) == sizeof(kCC) &&
+              std::fwrite(&n, 4, 1, f) == 1 && std::fwrite(&mode, 4, 1, f) == 1;
+    for(int l = 0; l < 3 && ok; l++)
+        for(int c = 0; c < 3 && ok; c++)
+        {
+            for(size_t v = 0; v < N3; v++) vol[v] = lut[(size_t(l) * N3 + v) * 4 + c];
+            ok = std::fwrite(vol.data(), 4, N3, f) == N3;
+        }
+    ok = (std::fclose(f) == 0) && ok;
+    if(!ok) err = "short write to " + path;
+    return ok;
+}
+
+bool ReadMrSpectra(const std::string& path, std::vector<float>& lut, std::string& err)
+{
+    FILE* f = std::fopen(path.c_str(), "rb");
+    if(!f) { err = "Unable to open spectra lut file " + path; return false; }
+    char cc[10] = {};
+    uint32_t n = 0, mode = 0;
+    bool ok = std::fread(cc, 1, sizeof(cc), f) == sizeof(cc);
+    const char rev[4] = {kCC[3], kCC[2], kCC[1], kCC[0]};
+    if(ok && std::memcmp(cc, rev, 4) == 0)
+        err = "Wrong character code in .mrspectra file! Endianness issue maybe";
+    else if(!ok || std::memcmp(cc, kCC, sizeof(kCC)) != 0)
+        err = "Wrong character code in .mrspectra file!";
+    else if(std::fread(&n, 4, 1, f) != 1 || n != uint32_t(kLutN))
+        err = "Wrong size (" + std::to_string(n) + "), spectra lut size must be " +
+              std::to_string(kLutN) + "!";
+    else if(std::fread(&mode, 4, 1, f) != 1 || mode != 1)
+        err = "Wrong mode (" + std::to_string(mode) + "), spectra lut must have mode \"1\" (aka. Float)!";
+    if(!err.empty()) { std::fclose(f); return false; }
+    const size_t N3 = size_t(kLutN) * kLutN * kLutN;
+    std::vector<float> vol(N3);
+    lut.assign(3 * N3 * 4, 0.0f);
+    for(int l = 0; l < 3 && err.empty(); l++)
+        for(int c = 0; c < 3 && err.empty(); c++)
+        {
+            if(std::fread(vol.data(), 4, N3, f) != N3) { err = "Unable to read sprectum lut file!"; break; }
+            for(size_t v = 0; v < N3; v++)
+            {
+                if(!std::isfinite(vol[v])) { err = "non-finite coefficient in " + path; break; }
+                lut[(size_t(l) * N3 + v) * 4 + c] = vol[v];
+            }
+        }
+    std::fclose(f);
+    return err.empty();
 }
 
 void BuildLut(HostTables& h)
@@ -329,46 +394,31 @@ HostTables& Host()
     std::lock_guard<std::mutex> lk(m);
     if(h.built) return h;
     h.colour = MakeColour();
-    const size_t count = size_t(3) * kLutN * kLutN * kLutN * 4;
-    std::string path = CachePath();
-    bool loaded = false;
-    if(FILE* f = std::fopen(path.c_str(), "rb"))
-    {
-        char magic[8] = {};
-        uint32_t n = 0;
-        if(std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, "MRSPEC01", 8) == 0 &&
-           std::fread(&n, 4, 1, f) == 1 && n == uint32_t(kLutN))
-        {
-            h.lut.resize(count);
-            loaded = (std::fread(h.lut.data(), 4, count, f) == count);
-        }
-        std::fclose(f);
-    }
-    if(!loaded)
+    std::string err;
+    if(!ReadMrSpectra(CachePath(), h.lut, err))
     {
         BuildLut(h);
-        if(FILE* f = std::fopen(path.c_str(), "wb"))
-        {
-            uint32_t n = kLutN;
-            std::fwrite("MRSPEC01", 1, 8, f);
-            std::fwrite(&n, 4, 1, f);
-            std::fwrite(h.lut.data(), 4, count, f);
-            std::fclose(f);
-        }
+        (void)::mkdir((LibDir() + "/SpectraLUT").c_str(), 0755);
+        std::string werr;
+        (void)WriteMrSpectra(h.lut, CachePath(), werr);   // cache only: failure is harmless
     }
     h.built = true;
     return h;
 }
 
+// Tables read from a file replace the built ones, before first use only.
+std::mutex gDevMutex;
+bool gDevUploaded = false;
+
 } // namespace
 
 int GetSpectralTables(SpectralTables& out)
 {
-    static std::mutex m;
     static SpectralTables dev;
-    std::lock_guard<std::mutex> lk(m);
+    std::lock_guard<std::mutex> lk(gDevMutex);
     if(!dev.ready)
     {
+        gDevUploaded = true;
         HostTables& h = Host();
         float4* dLut = nullptr;
         float4* dCmf = nullptr;
@@ -413,6 +463,44 @@ int mray_spectral_tables(int* hN, float* hLut, float* hCmf, float* hIllum, float
     if(hIllum) std::memcpy(hIllum, h.colour.illum.data(), h.colour.illum.size() * sizeof(float));
     if(hXyzToRgb)
         for(int k = 0; k < 9; k++) hXyzToRgb[k] = float(h.colour.xyzToRgb[k]);
+    return MRAY_OK;
+}
+
+int mray_spectral_write_mrspectra(const char* path)
+{
+    using namespace mray;
+    if(!path) { SetError("spectral_write_mrspectra: NULL path"); return MRAY_ERR_INVALID; }
+    std::string err;
+    if(!WriteMrSpectra(Host().lut, path, err)) { SetError(err); return MRAY_ERR_INVALID; }
+    return MRAY_OK;
+}
+
+int mray_spectral_read_mrspectra(const char* path, int* hN, float* hLut)
+{
+    using namespace mray;
+    if(!path) { SetError("spectral_read_mrspectra: NULL path"); return MRAY_ERR_INVALID; }
+    std::vector<float> lut;
+    std::string err;
+    if(!ReadMrSpectra(path, lut, err)) { SetError(err); return MRAY_ERR_INVALID; }
+    if(hN) *hN = kLutN;
+    if(hLut) std::memcpy(hLut, lut.data(), lut.size() * sizeof(float));
+    return MRAY_OK;
+}
+
+int mray_spectral_use_mrspectra(const char* path)
+{
+    using namespace mray;
+    if(!path) { SetError("spectral_use_mrspectra: NULL path"); return MRAY_ERR_INVALID; }
+    std::vector<float> lut;
+    std::string err;
+    if(!ReadMrSpectra(path, lut, err)) { SetError(err); return MRAY_ERR_INVALID; }
+    std::lock_guard<std::mutex> lk(gDevMutex);
+    if(gDevUploaded)
+    {
+        SetError("spectral_use_mrspectra: the tables are already in use by a spectral renderer");
+        return MRAY_ERR_INVALID;
+    }
+    Host().lut.swap(lut);
     return MRAY_OK;
 }
 

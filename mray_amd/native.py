@@ -40,6 +40,7 @@ EXPORTS = [
     "mray_batch_scene_create", "mray_batch_scene_destroy", "mray_batch_scene_info",
     "mray_batch_create", "mray_batch_read_film", "mray_batch_view_times", "mray_batch_autotune",
     "mray_batch_set_trace_tpb",
+    "mray_spectral_write_mrspectra", "mray_spectral_read_mrspectra", "mray_spectral_use_mrspectra",
 ]
 
 
@@ -86,6 +87,9 @@ def lib():
     L.mray_renderer_reset.argtypes = [P]
     L.mray_renderer_film_device.argtypes = [P, C.POINTER(P)]
     L.mray_spectral_tables.argtypes = [C.POINTER(C.c_int), P, P, P, P]
+    L.mray_spectral_write_mrspectra.argtypes = [C.c_char_p]
+    L.mray_spectral_read_mrspectra.argtypes = [C.c_char_p, C.POINTER(C.c_int), P]
+    L.mray_spectral_use_mrspectra.argtypes = [C.c_char_p]
     L.mray_nrc_create.argtypes = [C.POINTER(abi.MRayNrcConfig), C.POINTER(P)]
     L.mray_nrc_destroy.argtypes = [P]
     L.mray_nrc_infer.argtypes = [P, P, u32, P, P, P]
@@ -194,6 +198,20 @@ def lbvh_build(d_sorted_codes, d_leaf_order, d_leaf_aabbs):
     check(lib().mray_lbvh_build(_ptr(d_sorted_codes), _ptr(d_leaf_order), _ptr(d_leaf_aabbs), n,
                                 _ptr(nodes), _ptr(boxes), _ptr(tmp), nb, _stream()))
     return nodes, boxes
+
+
+def write_mrspectra(path):
+    """The library's RGB->spectrum coefficients as a reference .mrspectra file."""
+    check(lib().mray_spectral_write_mrspectra(str(path).encode()))
+
+
+def read_mrspectra(path):
+    """Validated .mrspectra contents as lut [3][N][N][N][4] (c0, c1, c2, 0)."""
+    n = C.c_int()
+    check(lib().mray_spectral_read_mrspectra(str(path).encode(), C.byref(n), None))
+    lut = np.zeros((3, n.value, n.value, n.value, 4), np.float32)
+    check(lib().mray_spectral_read_mrspectra(str(path).encode(), None, lut.ctypes.data))
+    return lut
 
 
 def spectral_tables():
