@@ -252,6 +252,17 @@ int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* hStats);
 /* Film (RenderImage SoA): hOut [4][height][width] = R, G, B, weight sums;
  * row 0 is the bottom row of the camera plane.                             */
 int mray_renderer_read_film(MRayRenderer* r, float* hOut);
+/* Film taken for one image section: hOut as mray_renderer_read_film, then
+ * the device film is cleared, so consecutive takes hold disjoint sums of the
+ * paths splatted in between -- what RenderImage::TransferToHost hands the
+ * runner each iteration (Tracer/RenderImage.cpp:164-209), which then
+ * accumulates sum / weight (MRay/RunCommand.cpp:609-700).                    */
+int mray_renderer_take_film(MRayRenderer* r, float* hOut);
+/* Film [4][height][width] (R, G, B sums, weight sums; row 0 = bottom) as an
+ * OpenEXR file of per-pixel means, RGB 32-bit float, top row first: the
+ * image the reference's run command writes on triggerSave
+ * (MRay/RunCommand.cpp:873-930, ImageLoader.cpp:90-140).  No GPU work.     */
+int mray_film_write_exr(const float* hFilm, uint32_t width, uint32_t height, const char* path);
 /* Reset the film and all path slots (StartRender again, same seed).        */
 int mray_renderer_reset(MRayRenderer* r);
 /* Measurement (no reference counterpart): profiling = HIP events around the

@@ -24,9 +24,11 @@
 #pragma once
 
 #include <array>
+#include <condition_variable>
 #include <cstddef>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <optional>
 #include <span>
 #include <stdexcept>
@@ -336,7 +338,52 @@ struct RendererOutput
     bool                           triggerSave = false;
 };
 
-class TimelineSemaphore;   // opaque: images are complete when DoRenderWork returns
+// Host timeline semaphore shared with the image consumer (Visor, the run
+// command).  The consumer owns it and hands it over in SetupRenderEnv, so it
+// has the reference's member layout and protocol (Core/TimelineSemaphore.h:
+// 31-110): Acquire(v) passes at once for v == 0, otherwise waits until the
+// value equals v or the semaphore is invalidated (then returns false);
+// Release() adds one and wakes every waiter.
+class TimelineSemaphore
+{
+    uint64_t                value;
+    std::mutex              mutex;
+    std::condition_variable cVar;
+    bool                    invalid;
+
+    public:
+    explicit TimelineSemaphore(uint64_t initialValue) : value(initialValue), invalid(false) {}
+    [[nodiscard]] bool Acquire(uint64_t waitValue)
+    {
+        if(waitValue == 0) return true;
+        std::unique_lock<std::mutex> lock(mutex);
+        cVar.wait(lock, [&] { return value == waitValue || invalid; });
+        return !invalid;
+    }
+    void Release()
+    {
+        std::unique_lock<std::mutex> lock(mutex);
+        value++;
+        cVar.notify_all();
+    }
+    void Reset()
+    {
+        std::unique_lock<std::mutex> lock(mutex);
+        value = 0;
+        cVar.notify_all();
+    }
+    void Invalidate()
+    {
+        std::unique_lock<std::mutex> lock(mutex);
+        invalid = true;
+        cVar.notify_all();
+    }
+    [[nodiscard]] bool IsInvalidated()
+    {
+        std::unique_lock<std::mutex> lock(mutex);
+        return invalid;
+    }
+};
 class ThreadPool;          // opaque: the host work here is single-threaded
 using GPUThreadInitFunction = void (*)();
 

@@ -1762,6 +1762,19 @@ int mray_renderer_read_film(MRayRenderer* r, float* hOut)
     return MRAY_OK;
 }
 
+int mray_renderer_take_film(MRayRenderer* r, float* hOut)
+{
+    if(!r || !hOut) { SetError("take_film: NULL argument"); return MRAY_ERR_INVALID; }
+    const size_t n = size_t(r->rc.P) * 4;
+    MRAY_HIP_CHECK(hipStreamSynchronize(r->stream));
+    MRAY_HIP_CHECK(hipMemcpyAsync(hOut, r->rc.film, n * sizeof(float), hipMemcpyDeviceToHost,
+                                  r->stream));
+    KZeroFloats<<<DivUp(n, 256 * 8), 256, 0, r->stream>>>(r->rc.film, n);
+    MRAY_HIP_CHECK(hipGetLastError());
+    MRAY_HIP_CHECK(hipStreamSynchronize(r->stream));
+    return MRAY_OK;
+}
+
 int mray_renderer_set_measurement(MRayRenderer* r, uint32_t profiling, uint32_t counting)
 {
     if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }

@@ -14,6 +14,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <string>
@@ -320,7 +322,27 @@ class Tracer final : public TracerI
     MRayRenderer* renderer = nullptr;
     uint32_t activeRenderer = 0;
     RenderImageParams imgParams{};
-    std::vector<float> hostFilm;
+    // image hand-over (Tracer/RenderImage.cpp:150-209): a host staging buffer
+    // aligned for import by the consumer, filled with the sums splatted
+    // since the previous section, guarded by the consumer's semaphore
+    struct Staging
+    {
+        float* p = nullptr;
+        size_t bytes = 0;
+        ~Staging() { std::free(p); }
+        void Allocate(size_t need, size_t align)
+        {
+            std::free(p);
+            bytes = (need + align - 1) / align * align;
+            p = static_cast<float*>(std::aligned_alloc(align, bytes));
+            if(!p) throw MRayError("Unable to allocate the image staging buffer");
+            std::memset(p, 0, bytes);
+        }
+    } staging;
+    TimelineSemaphore* sem = nullptr;
+    uint32_t importAlignment = 64;
+    uint64_t acquireValue = 0;
+    bool saveImage = false;
     uint64_t sampleCounter = 0;
     double lastIterMs = 0;
 
@@ -1134,7 +1156,14 @@ class Tracer final : public TracerI
     }
 
     // =============================== Rendering ===============================
-    void SetupRenderEnv(TimelineSemaphore*, uint32_t, uint64_t) override {}
+    void SetupRenderEnv(TimelineSemaphore* semaphore, uint32_t alignment,
+                        uint64_t initialAcquireValue) override
+    {
+        // TracerBase::SetupRenderEnv -> RenderImage (TracerBase.cpp:1701-1707)
+        sem = semaphore;
+        importAlignment = std::max<uint32_t>(64u, alignment);
+        acquireValue = initialAcquireValue;
+    }
 
     RenderBufferInfo StartRender(RendererId rid, CamSurfaceId cs, RenderImageParams ip,
                                  Optional<uint32_t>, Optional<uint32_t>) override
@@ -1168,11 +1197,12 @@ class Tracer final : public TracerI
         activeRenderer = U(rid);
         imgParams = ip;
         size_t P = size_t(p.width) * p.height;
-        hostFilm.assign(4 * P, 0.0f);
+        staging.Allocate(4 * P * sizeof(float), importAlignment);
         sampleCounter = 0;
+        saveImage = true;
         RenderBufferInfo info{};
-        info.data = reinterpret_cast<const std::byte*>(hostFilm.data());
-        info.totalSize = hostFilm.size() * sizeof(float);
+        info.data = reinterpret_cast<const std::byte*>(staging.p);
+        info.totalSize = staging.bytes;
         info.renderColorSpace = MRayColorSpaceEnum::MR_REC_709;
         info.resolution = ip.resolution;
         return info;
@@ -1194,7 +1224,6 @@ class Tracer final : public TracerI
         CheckRc(mray_renderer_iterate(renderer, rr.burstSize, nullptr), "DoRenderWork");
         MRayRenderStats st{};
         CheckRc(mray_renderer_stats(renderer, &st), "DoRenderWork");
-        CheckRc(mray_renderer_read_film(renderer, hostFilm.data()), "DoRenderWork");
         double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         lastIterMs = ms;
         sampleCounter = st.totalDeadPaths;
@@ -1209,17 +1238,33 @@ class Tracer final : public TracerI
         a.renderResolution = imgParams.resolution;
         a.outputColorSpace = MRayColorSpaceEnum::MR_REC_709;
         a.usedGPUMemoryBytes = UsedDeviceMemory();
+        RendererOutput out;
+        out.analytics = a;
+        // RenderImage::TransferToHost (RenderImage.cpp:164-209): wait until
+        // the consumer has released the staging buffer, fill it, release,
+        // and tell the consumer which value to wait for.  A semaphore that
+        // cannot be acquired means the consumer is closing: no section.
+        if(sem && !sem->Acquire(acquireValue)) return out;
+        CheckRc(mray_renderer_take_film(renderer, staging.p), "DoRenderWork");
+        uint64_t waitCounter = 0;
+        if(sem)
+        {
+            sem->Release();
+            acquireValue += 2;
+            waitCounter = acquireValue - 1;
+        }
         RenderImageSection sec{};
         sec.pixelMin = Vector2ui{{0, 0}};
         sec.pixelMax = imgParams.resolution;
         sec.globalWeight = 0.0f;
-        sec.waitCounter = 0;
+        sec.waitCounter = waitCounter;
         sec.pixStartOffsets = {0, P * sizeof(float), 2 * P * sizeof(float)};
         sec.weightStartOffset = 3 * P * sizeof(float);
-        RendererOutput out;
-        out.analytics = a;
         out.imageOut = sec;
-        out.triggerSave = st.finished != 0;
+        // one save request when the sample limit is reached
+        // (PathTracerRenderer.cu:1063-1065)
+        out.triggerSave = saveImage && st.finished != 0;
+        if(out.triggerSave) saveImage = false;
         return out;
     }
 

@@ -41,6 +41,7 @@ EXPORTS = [
     "mray_batch_create", "mray_batch_read_film", "mray_batch_view_times", "mray_batch_autotune",
     "mray_batch_set_trace_tpb",
     "mray_spectral_write_mrspectra", "mray_spectral_read_mrspectra", "mray_spectral_use_mrspectra",
+    "mray_renderer_take_film", "mray_film_write_exr",
 ]
 
 
@@ -84,6 +85,8 @@ def lib():
     L.mray_renderer_render_all.argtypes = [P, C.POINTER(abi.MRayRenderStats)]
     L.mray_renderer_stats.argtypes = [P, C.POINTER(abi.MRayRenderStats)]
     L.mray_renderer_read_film.argtypes = [P, P]
+    L.mray_renderer_take_film.argtypes = [P, P]
+    L.mray_film_write_exr.argtypes = [P, u32, u32, C.c_char_p]
     L.mray_renderer_reset.argtypes = [P]
     L.mray_renderer_film_device.argtypes = [P, C.POINTER(P)]
     L.mray_spectral_tables.argtypes = [C.POINTER(C.c_int), P, P, P, P]
@@ -212,6 +215,13 @@ def read_mrspectra(path):
     lut = np.zeros((3, n.value, n.value, n.value, 4), np.float32)
     check(lib().mray_spectral_read_mrspectra(str(path).encode(), None, lut.ctypes.data))
     return lut
+
+
+def write_exr(film, path):
+    """Film [4][H][W] (sums + weights) as the OpenEXR image of per-pixel means."""
+    film = np.ascontiguousarray(film, np.float32)
+    _, h, w = film.shape
+    check(lib().mray_film_write_exr(film.ctypes.data, w, h, str(path).encode()))
 
 
 def spectral_tables():
@@ -358,6 +368,12 @@ class Renderer:
     def film(self):
         out = np.zeros((4, self.height, self.width), np.float32)
         check(lib().mray_renderer_read_film(self.h, out.ctypes.data))
+        return out
+
+    def take_film(self):
+        """Sums splatted since the last take; the device film is cleared."""
+        out = np.zeros((4, self.height, self.width), np.float32)
+        check(lib().mray_renderer_take_film(self.h, out.ctypes.data))
         return out
 
     def attach_nrc(self, nrc, train_shift=4, spread=0.01, batch=1 << 14, steps=4):

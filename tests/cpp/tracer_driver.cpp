@@ -4,9 +4,15 @@
 // a scene loader makes (SceneLoaderMRay.cpp: groups -> reservations ->
 // commits -> attribute pushes -> surfaces), then either
 //   flatten <out>   writes CommitSurfaces' host-side scene (no GPU), or
-//   render  <out> W H spp spectral mode rr0 rr1 seed slots
-//                   renders through CreateRenderer / StartRender /
-//                   DoRenderWork and writes the film (4 x H x W floats).
+//   render  <out> W H spp spectral mode rr0 rr1 seed slots [exr prefix]
+//                   renders through SetupRenderEnv / CreateRenderer /
+//                   StartRender / DoRenderWork with a consumer thread that
+//                   plays the reference's run command: it waits on the
+//                   shared TimelineSemaphore for each section's waitCounter,
+//                   accumulates the section into double mean RGB + weight
+//                   (RunCommand.cpp:609-700) and releases.  <out> receives
+//                   the means and weights (4 x H x W floats); on triggerSave
+//                   the image is also saved as <prefix>_<wpp>wpp.exr.
 //
 // Input scene file (little-endian):
 //   u32 V, T, M, L, boundaryType; f32 boundaryRadiance[3], sceneDiameter;
@@ -25,10 +31,15 @@
 #include <dlfcn.h>
 
 #include <algorithm>
+#include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
+#include <deque>
 #include <map>
+#include <mutex>
 #include <string>
+#include <thread>
 
 using namespace mray_amd;
 
@@ -414,24 +425,110 @@ int Run(int argc, char** argv)
             t->PushRendererAttribute(r, 3, Str(modes[sampleMode]));
             t->PushRendererAttribute(r, 4, Data(std::vector<Vector2ui>{Vector2ui{{rr0, rr1}}}));
             t->PushRendererAttribute(r, 5, Str("Uniform"));
-            t->SetupRenderEnv(nullptr, 4096, 0);
+            TimelineSemaphore sem(0);
+            t->SetupRenderEnv(&sem, 4096, 0);
             RenderImageParams ip{Vector2ui{{W, H}}, Vector2ui{{0, 0}}, Vector2ui{{W, H}}};
             RenderBufferInfo info = t->StartRender(r, cs, ip, std::nullopt, std::nullopt);
-            RendererOutput out;
-            uint32_t calls = 0;
-            do
+            if(reinterpret_cast<uintptr_t>(info.data) % 4096 != 0)
+                throw MRayError("staging buffer is not aligned to the import alignment");
+            // consumer: the run command's section loop
+            const size_t P = size_t(W) * H;
+            std::vector<double> acc(4 * P, 0.0);
+            std::deque<RenderImageSection> queue;
+            std::mutex qm;
+            std::condition_variable qc;
+            bool done = false;
+            uint32_t consumed = 0;
+            std::thread consumer([&]
             {
-                out = t->DoRenderWork();
-                calls++;
-                if(calls > 100000) throw MRayError("render did not finish");
-            } while(!out.triggerSave);
-            std::printf("rendered: %u DoRenderWork calls, %.3f spp, last %.3f %s\n", calls,
+                for(;;)
+                {
+                    RenderImageSection sec;
+                    {
+                        std::unique_lock<std::mutex> lk(qm);
+                        qc.wait(lk, [&] { return done || !queue.empty(); });
+                        if(queue.empty()) return;
+                        sec = queue.front();
+                        queue.pop_front();
+                    }
+                    if(!sem.Acquire(sec.waitCounter)) return;
+                    const float* rgb[3];
+                    for(int c = 0; c < 3; c++)
+                        rgb[c] = reinterpret_cast<const float*>(info.data + sec.pixStartOffsets[c]);
+                    const float* w = reinterpret_cast<const float*>(info.data + sec.weightStartOffset);
+                    for(size_t i = 0; i < P; i++)
+                    {
+                        double sOut = acc[3 * P + i], total = sOut + double(w[i]);
+                        for(int c = 0; c < 3; c++)
+                            acc[c * P + i] = (total == 0.0) ? 0.0
+                                : (acc[c * P + i] * sOut + double(rgb[c][i])) / total;
+                        acc[3 * P + i] = total;
+                    }
+                    consumed++;
+                    sem.Release();
+                }
+            });
+            RendererOutput out;
+            uint32_t calls = 0, saves = 0;
+            try
+            {
+                do
+                {
+                    out = t->DoRenderWork();
+                    calls++;
+                    if(out.triggerSave) saves++;
+                    if(out.imageOut)
+                    {
+                        std::lock_guard<std::mutex> lk(qm);
+                        queue.push_back(*out.imageOut);
+                        qc.notify_all();
+                    }
+                    if(calls > 100000) throw MRayError("render did not finish");
+                } while(!out.triggerSave);
+                // one more call after the save request must not ask again
+                RendererOutput extra = t->DoRenderWork();
+                if(extra.triggerSave) throw MRayError("triggerSave reported twice");
+                if(extra.imageOut)
+                {
+                    std::lock_guard<std::mutex> lk(qm);
+                    queue.push_back(*extra.imageOut);
+                    qc.notify_all();
+                }
+            }
+            catch(...)
+            {
+                sem.Invalidate();
+                { std::lock_guard<std::mutex> lk(qm); done = true; qc.notify_all(); }
+                consumer.join();
+                throw;
+            }
+            { std::lock_guard<std::mutex> lk(qm); done = true; qc.notify_all(); }
+            consumer.join();
+            std::printf("rendered: %u DoRenderWork calls, %u sections consumed, %u save request, "
+                        "%.3f spp, last %.3f %s\n", calls + 1, consumed, saves,
                         out.analytics->workPerPixel, out.analytics->throughput,
                         out.analytics->throughputSuffix.c_str());
+            std::vector<float> img(4 * P);
+            for(size_t i = 0; i < 4 * P; i++) img[i] = float(acc[i]);
             FILE* f = std::fopen(argv[4], "wb");
             if(!f) throw MRayError("cannot write film");
-            std::fwrite(info.data, 1, info.totalSize, f);
+            std::fwrite(img.data(), sizeof(float), img.size(), f);
             std::fclose(f);
+            if(argc > 14)
+            {
+                // RunCommand.cpp:873-930: sums back from mean x weight, then
+                // <prefix>_<wpp>wpp.exr
+                for(size_t i = 0; i < P; i++)
+                    for(int c = 0; c < 3; c++) img[c * P + i] = float(acc[c * P + i] * acc[3 * P + i]);
+                using Exr = int (*)(const float*, uint32_t, uint32_t, const char*);
+                auto exr = reinterpret_cast<Exr>(dlsym(so, "mray_film_write_exr"));
+                if(!exr) throw MRayError("mray_film_write_exr not found");
+                char name[64];
+                std::snprintf(name, sizeof(name), "_%08.0fwpp.exr", std::round(out.analytics->workPerPixel));
+                std::string path = std::string(argv[14]) + name;
+                if(exr(img.data(), W, H, path.c_str()) != 0) throw MRayError("EXR write failed");
+                std::printf("saved %s\n", path.c_str());
+            }
             t->StopRender();
             t->DestroyRenderer(r);
         }

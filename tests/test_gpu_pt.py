@@ -184,3 +184,33 @@ def test_graph_replays_match_direct_launches(torch_gpu):
         assert e < 1e-8
     graphed.close()
     acc.close()
+
+
+def test_take_film_sections_sum_to_the_film(torch_gpu):
+    """mray_renderer_take_film hands out disjoint per-section sums: their total
+    equals the film of an identical render read once (float addition order
+    aside), and the film is empty after the last take."""
+    from mray_amd import native
+    scene = scenes.cornell()
+    params = abi.render_params(64, 64, 4, rr=(8, 8), seed=4)
+    acc = native.Accel(scene)
+    r = native.Renderer(acc, params)
+    total = np.zeros((4, 64, 64), np.float64)
+    sections = 0
+    for _ in range(10000):
+        r.iterate(8)
+        st = r.stats()
+        total += r.take_film()
+        sections += 1
+        if st["finished"]:
+            break
+    assert st["finished"] and sections > 1
+    assert not r.take_film().any()
+    r2 = native.Renderer(acc, params)
+    r2.render_all()
+    full = r2.film()
+    r.close()
+    r2.close()
+    acc.close()
+    assert np.allclose(total[3], 4.0)
+    assert np.allclose(total, full, rtol=1e-5, atol=1e-6)

@@ -16,18 +16,25 @@ from tests.test_tracer_api import DRIVER, TRACER_LIB, _built, write_scene_file
 pytestmark = pytest.mark.gpu
 
 
-def _driver_render(scene, tmp_path, w, h, spp, spectral, mode, rr, seed):
+def _driver_render(scene, tmp_path, w, h, spp, spectral, mode, rr, seed, exr_prefix=None):
+    """The driver plays the run command: it accumulates the image sections
+    handed over through the TimelineSemaphore into means + weights.  Returned
+    as a film (sums = mean x weight) for comparison with the C-ABI's."""
     _built()
     src = str(tmp_path / "scene.bin")
     out = str(tmp_path / "film.bin")
     write_scene_file(scene, src)
     cmd = [DRIVER, TRACER_LIB, src, "render", out, str(w), str(h), str(spp), str(int(spectral)),
            str(mode), str(rr[0]), str(rr[1]), str(seed), "0"]
+    if exr_prefix:
+        cmd.append(exr_prefix)
     print("run:", " ".join(cmd[2:]), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
-    return np.fromfile(out, np.float32).reshape(4, h, w)
+    assert "1 save request" in r.stdout
+    a = np.fromfile(out, np.float32).reshape(4, h, w)
+    return np.concatenate([a[:3] * a[3:4], a[3:4]])
 
 
 def _cabi_render(scene, params):
@@ -80,8 +87,29 @@ def test_json_scene_renders_like_cabi(torch_gpu, tmp_path):
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
-    a = np.fromfile(out, np.float32).reshape(4, 64, 64)
+    a = np.fromfile(out, np.float32).reshape(4, 64, 64)      # consumer means + weights
+    a = np.concatenate([a[:3] * a[3:4], a[3:4]])
     b = _cabi_render(scene, abi.render_params(64, 64, 4, rr=(8, 8), mode=abi.SAMPLE_NEE_MIS, seed=5))
     e = relmse(film_image(a), film_image(b))
     print(f"relMSE JSON/TracerI vs C-ABI = {e:.3e}", flush=True)
     assert e <= 1e-3
+
+
+def test_tracer_saves_exr_like_the_run_command(torch_gpu, tmp_path):
+    """triggerSave once at the sample limit; the consumer's accumulated image
+    saved as <prefix>_<wpp>wpp.exr equals the C-ABI render's means, top row
+    first."""
+    from tests.exr_read import read_exr
+    scene = scenes.cornell()
+    prefix = str(tmp_path / "cornell")
+    a = _driver_render(scene, tmp_path, 48, 32, 4, False, abi.SAMPLE_NEE_MIS, (8, 8), 3,
+                       exr_prefix=prefix)
+    path = prefix + "_00000004wpp.exr"
+    assert os.path.exists(path), os.listdir(tmp_path)
+    _, ch = read_exr(path)
+    img = film_image(a)
+    for k, c in enumerate("RGB"):
+        assert np.allclose(ch[c], img[k][::-1], rtol=1e-5, atol=1e-7), c
+    b = _cabi_render(scene, abi.render_params(48, 32, 4, rr=(8, 8), mode=abi.SAMPLE_NEE_MIS,
+                                              seed=3))
+    assert relmse(img, film_image(b)) < 1e-6
