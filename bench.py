@@ -56,9 +56,13 @@ def parse():
                     help="run the radiance-cache equal-time leg (Cornell box, 1 spp per frame)")
     ap.add_argument("--nrc-width", type=int, default=480)
     ap.add_argument("--nrc-height", type=int, default=270)
-    ap.add_argument("--nrc-train-shift", type=int, default=3)
+    ap.add_argument("--nrc-only", action="store_true",
+                    help="run only the radiance-cache equal-time leg (its own JSON line)")
     ap.add_argument("--nrc-frames", type=int, default=256)
     ap.add_argument("--nrc-ref-spp", type=int, default=16384)
+    ap.add_argument("--nrc-scene", choices=("cornell", "terrain"), default="cornell")
+    ap.add_argument("--nrc-variants", default="0.01:1:1:3",
+                    help="cache variants 'spread:neeAtEnd:minDepth:trainShift[:lazyAdam];...'")
     ap.add_argument("--workload", choices=("frame", "batch"), default="frame",
                     help="frame: the 1080p headline frame; batch: many scenes in one renderer")
     ap.add_argument("--batch-scenes", type=int, default=32)
@@ -139,20 +143,48 @@ def film_image(film):
     return film[:3] / np.maximum(film[3], 1e-12)
 
 
+def _loglog_at(curve, x_key, y_key, x):
+    """y at x on a curve of (x, y) points, log-log interpolated; None outside."""
+    import numpy as np
+    for a, b in zip(curve, curve[1:]):
+        x0, x1, y0, y1 = a[x_key], b[x_key], a[y_key], b[y_key]
+        lo, hi = min(x0, x1), max(x0, x1)
+        if lo <= x <= hi and x0 != x1 and y0 > 0 and y1 > 0:
+            t = (np.log(x) - np.log(x0)) / (np.log(x1) - np.log(x0))
+            return float(np.exp(np.log(y0) + t * (np.log(y1) - np.log(y0))))
+    return None
+
+
+def parse_nrc_variants(text):
+    """'spread:nee:minDepth:trainShift[:lazyAdam];...' -> list of dicts."""
+    out = []
+    for part in text.split(";"):
+        f = part.split(":")
+        out.append({"spread": float(f[0]), "nee_at_end": bool(int(f[1])), "min_depth": int(f[2]),
+                    "train_shift": int(f[3]), "lazy_adam": bool(int(f[4])) if len(f) > 4 else True})
+    return out
+
+
 def nrc_equal_time(args):
-    """Radiance-cache PT vs plain PT at equal time on the Cornell scene
-    (1 spp per frame, NEE+MIS, depth 8, RGB).  Judge: a plain render at
-    --nrc-ref-spp.  Both schedules run frame by frame; the error curve is
-    sampled at doubling frame counts and the times are render time only (the
-    film read-back for the error is outside)."""
+    """Radiance-cache PT vs plain PT at equal time (1 spp per frame, NEE+MIS,
+    RGB), judged against a plain render at --nrc-ref-spp.  Both run frame by
+    frame; error curves are sampled at doubling frame counts; times are render
+    time only (the film read-back for the error is outside).  Each cache
+    variant (termination threshold, NEE at the terminal vertex, minimum
+    terminal depth, training-path share) runs against the same judge."""
     import numpy as np
     from mray_amd import abi, native, scenes
     torch = native.require_gpu()
     W, H = args.nrc_width, args.nrc_height
-    scene = scenes.cornell(aspect=W / H)
+    if args.nrc_scene == "terrain":
+        scene = scenes.terrain_spheres(grid=args.grid, spheres=args.spheres, width=W, height=H)
+        rr, label = (10, 10), f"terrain {scene.triangle_count} tris"
+    else:
+        scene = scenes.cornell(aspect=W / H)
+        rr, label = (8, 8), "cornell"
     acc = native.Accel(scene)
     t0 = time.perf_counter()
-    ref_r = native.Renderer(acc, abi.render_params(W, H, args.nrc_ref_spp, rr=(8, 8), seed=99))
+    ref_r = native.Renderer(acc, abi.render_params(W, H, args.nrc_ref_spp, rr=rr, seed=99))
     done = 0
     while done < args.nrc_ref_spp:
         step = min(512, args.nrc_ref_spp - done)
@@ -164,15 +196,17 @@ def nrc_equal_time(args):
     ref_r.close()
     log(f"nrc leg: reference {args.nrc_ref_spp} spp in {ref_s:.1f} s")
 
-    def run(with_cache):
-        r = native.Renderer(acc, abi.render_params(W, H, args.nrc_frames, rr=(8, 8), seed=1))
+    def run(v):
+        r = native.Renderer(acc, abi.render_params(W, H, args.nrc_frames, rr=rr, seed=1))
         nrc = None
-        if with_cache:
-            nrc = native.Nrc(seed=2)
-            r.attach_nrc(nrc, train_shift=args.nrc_train_shift, spread=0.01, batch=1 << 14, steps=4)
+        if v is not None:
+            nrc = native.Nrc(seed=2, lazy_adam=v["lazy_adam"])
+            r.attach_nrc(nrc, train_shift=v["train_shift"], spread=v["spread"], batch=1 << 14,
+                         steps=4, min_depth=v["min_depth"], nee_at_end=v["nee_at_end"])
         curve = []
         spent = 0.0
         nxt = 1
+        name = "plain" if v is None else f"cached {v}"
         for f in range(1, args.nrc_frames + 1):
             torch.cuda.synchronize()
             t = time.perf_counter()
@@ -180,47 +214,56 @@ def nrc_equal_time(args):
             torch.cuda.synchronize()
             spent += time.perf_counter() - t
             if f == nxt or f == args.nrc_frames:
-                curve.append((f, spent, relmse(film_image(r.film()), ref)))
-                log(f"nrc leg: {'cached' if with_cache else 'plain'} frame {f}: "
-                    f"{spent:.3f} s, relMSE {curve[-1][2]:.4e}")
+                curve.append({"frames": f, "seconds": round(spent, 4),
+                              "relmse": relmse(film_image(r.film()), ref)})
+                log(f"nrc leg: {name} frame {f}: {spent:.3f} s, relMSE {curve[-1]['relmse']:.4e}")
                 nxt *= 2
-        stats = r.cache_stats() if with_cache else None
+        stats = r.cache_stats() if nrc is not None else None
         r.close()
         if nrc is not None:
             nrc.close()
         return curve, stats
-    plain, _ = run(False)
-    cached, cst = run(True)
+
+    plain, _ = run(None)
+    e_plain, t_plain_all = plain[-1]["relmse"], plain[-1]["seconds"]
+    variants = []
+    for v in parse_nrc_variants(args.nrc_variants):
+        cached, cst = run(v)
+        e_c, t_c = cached[-1]["relmse"], cached[-1]["seconds"]
+        # plain error at the cached run's total time
+        e_plain_eq = _loglog_at(plain, "seconds", "relmse", t_c)
+        if e_plain_eq is None:
+            e_plain_eq = e_plain * t_plain_all / t_c          # relMSE ~ 1/t beyond the curve
+        # time plain PT needs to reach the cached run's final error
+        t_match = _loglog_at(sorted(plain, key=lambda c: c["relmse"]), "relmse", "seconds", e_c)
+        if t_match is None and e_plain > e_c:
+            t_match = t_plain_all * e_plain / e_c
+        # the target: time the cached run needs to reach plain's final error,
+        # over plain's time (None when the cached run never gets there)
+        t_reach = _loglog_at(sorted(cached, key=lambda c: c["relmse"]), "relmse", "seconds", e_plain)
+        variants.append({
+            "variant": v,
+            "cached": cached,
+            "relmse_cached": e_c,
+            "relmse_plain_equal_time": e_plain_eq,
+            "time_ratio_to_match": None if t_match is None else round(t_c / t_match, 4),
+            "time_ratio_to_reach_plain_final": None if t_reach is None else round(t_reach / t_plain_all, 4),
+            "cache_stats": cst,
+        })
     acc.close()
-    # time plain PT needs to reach the cache's final error (log-log interpolation)
-    e_target, t_cache = cached[-1][2], cached[-1][1]
-    t_plain = None
-    for (f0, t0_, e0), (f1, t1_, e1) in zip(plain, plain[1:]):
-        if e0 >= e_target >= e1 and e0 > e1:
-            a = (np.log(e_target) - np.log(e0)) / (np.log(e1) - np.log(e0))
-            t_plain = float(np.exp(np.log(t0_) + a * (np.log(t1_) - np.log(t0_))))
-            break
-    if t_plain is None and plain[-1][2] > e_target:
-        # extrapolate with relMSE ~ 1/t from the last plain point
-        t_plain = plain[-1][1] * plain[-1][2] / e_target
-    # plain error at the cache's total time (interpolated on the curve)
-    e_plain_eq = None
-    for (f0, t0_, e0), (f1, t1_, e1) in zip(plain, plain[1:]):
-        if t0_ <= t_cache <= t1_:
-            a = (np.log(t_cache) - np.log(t0_)) / (np.log(t1_) - np.log(t0_))
-            e_plain_eq = float(np.exp(np.log(e0) + a * (np.log(e1) - np.log(e0))))
-    if e_plain_eq is None:
-        e_plain_eq = plain[-1][2] * plain[-1][1] / t_cache
+    best = min(variants, key=lambda x: x["relmse_cached"])
     return {
-        "scene": f"cornell {W}x{H}, 1 spp/frame, {args.nrc_frames} frames, NEE+MIS, depth 8",
+        "scene": f"{label} {W}x{H}, 1 spp/frame, {args.nrc_frames} frames, NEE+MIS, "
+                 f"depth {rr[1]}, RGB",
         "reference_spp": args.nrc_ref_spp,
         "cache": "hash grid 16 x 2^19 x 2 fp16 + MLP 64 x 5 ReLU (bf16 MFMA), 4 x 2^14 pairs/frame",
-        "plain": [{"frames": f, "seconds": round(t, 4), "relmse": e} for f, t, e in plain],
-        "cached": [{"frames": f, "seconds": round(t, 4), "relmse": e} for f, t, e in cached],
-        "relmse_cached": e_target,
-        "relmse_plain_equal_time": e_plain_eq,
-        "time_ratio_to_match": None if t_plain is None else round(t_cache / t_plain, 4),
-        "cache_stats": cst,
+        "plain": plain,
+        "variants": variants,
+        "relmse_plain": e_plain,
+        "relmse_cached": best["relmse_cached"],
+        "relmse_plain_equal_time": best["relmse_plain_equal_time"],
+        "time_ratio_to_match": best["time_ratio_to_match"],
+        "time_ratio_to_reach_plain_final": best["time_ratio_to_reach_plain_final"],
     }
 
 
@@ -381,6 +424,9 @@ def main():
         sys.exit(2)
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.nrc_only:
+        print(json.dumps({"nrc": nrc_equal_time(args)}), flush=True)
         return
     import numpy as np
     from mray_amd import abi, native, scenes

@@ -386,9 +386,15 @@ typedef struct MRayNrcConfig
     uint32_t maxBatch;         /* largest training batch (multiple of 32)     */
     float    learningRate;
     float    beta1, beta2, epsilon;
-    uint32_t reserved;
+    uint32_t flags;            /* MRAY_NRC_LAZY_ADAM                           */
     uint64_t seed;             /* parameter initialisation                    */
 } MRayNrcConfig;
+
+/* Lazy Adam for the hash grid: an entry whose gradient is zero in a step
+ * keeps its parameter and both moments (the MLP is always updated densely).
+ * A batch touches at most 8 corners per level per sample, so most of the
+ * grid is skipped instead of streamed through every step.                   */
+#define MRAY_NRC_LAZY_ADAM 1u
 
 enum
 {
@@ -421,16 +427,23 @@ int mray_nrc_zero_grads(MRayNrc* nrc, void* stream);
 
 /* Renderer + radiance cache (PathTracerRGB).  One path slot in 2^trainShift
  * is a training path (full length, its vertices become training pairs); the
- * others terminate into the cache once the path spread exceeds
- * spreadThreshold x the primary spread (Mueller et al. 2021, sec. 4).  After
- * every frame the cache takes trainSteps Adam steps of trainBatch pairs
+ * others terminate into the cache at a surface vertex of depth >= minDepth
+ * once the path spread exceeds spreadThreshold x the primary spread (Mueller
+ * et al. 2021, sec. 4).  With MRAY_NRC_NEE_AT_END the terminal vertex still
+ * takes its next-event estimate and the cache supplies only the radiance the
+ * path would gather after it (training targets exclude each vertex's visible
+ * NEE); without it the cache supplies the vertex's whole reflected radiance.
+ * After every frame the cache takes trainSteps Adam steps of trainBatch pairs
  * (trainBatch * trainSteps a power of two: the pair ring).                   */
+#define MRAY_NRC_NEE_AT_END 1u
 typedef struct MRayNrcRenderParams
 {
     uint32_t trainShift;
     float    spreadThreshold;
     uint32_t trainBatch;
     uint32_t trainSteps;
+    uint32_t minDepth;         /* >= 1                                        */
+    uint32_t flags;            /* MRAY_NRC_NEE_AT_END                          */
 } MRayNrcRenderParams;
 
 /* nrc == NULL detaches.                                                      */

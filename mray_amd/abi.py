@@ -160,9 +160,13 @@ class MRayNrcConfig(C.Structure):
         ("beta1", C.c_float),
         ("beta2", C.c_float),
         ("epsilon", C.c_float),
-        ("reserved", C.c_uint32),
+        ("flags", C.c_uint32),
         ("seed", C.c_uint64),
     ]
+
+
+NRC_NEE_AT_END = 1
+NRC_LAZY_ADAM = 1
 
 
 class MRayNrcRenderParams(C.Structure):
@@ -171,6 +175,8 @@ class MRayNrcRenderParams(C.Structure):
         ("spreadThreshold", C.c_float),
         ("trainBatch", C.c_uint32),
         ("trainSteps", C.c_uint32),
+        ("minDepth", C.c_uint32),
+        ("flags", C.c_uint32),
     ]
 
 

@@ -214,9 +214,12 @@ KNrcForward(NrcView v, const float* __restrict__ rec, const uint32_t* __restrict
 {
     __shared__ __attribute__((aligned(16))) uint16_t sW[kNrcLayers * kW * kLdsRow];
     __shared__ __attribute__((aligned(16))) uint16_t sX[kFwdWaves][32 * kLdsRow];
+    const uint32_t n = dCount ? *dCount : count;
+    // a block without a tile (few or no queries this bounce) skips the
+    // weight staging: the test is uniform over the block
+    if(blockIdx.x * kFwdWaves * 32u >= n) return;
     StageWeights(v.wT, sW);
     __syncthreads();
-    const uint32_t n = dCount ? *dCount : count;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     uint16_t* X = sX[wave];
     float lossAcc = 0.0f;
@@ -425,26 +428,25 @@ KNrcWgrad(NrcTrainBuf tb, uint32_t batch, uint32_t chunk, float* __restrict__ wg
 }
 
 // ---- grid gradients -----------------------------------------------------------
+// One thread per (sample, level): the 16 atomics of a level's 8 corners.
 __global__ void __launch_bounds__(256)
 KNrcGridBwd(NrcView v, const float* __restrict__ rec, const float* __restrict__ dx0,
             uint32_t batch, float* __restrict__ ggrad)
 {
-    uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t s = t / v.levels, l = t % v.levels;
     if(s >= batch) return;
+    const float g0 = dx0[size_t(s) * 32u + 2u * l], g1 = dx0[size_t(s) * 32u + 2u * l + 1u];
+    if(g0 == 0.0f && g1 == 0.0f) return;
     const float* r = rec + size_t(s) * 11u;
     float pos[3] = {ClampF(r[0], 0.0f, 1.0f), ClampF(r[1], 0.0f, 1.0f), ClampF(r[2], 0.0f, 1.0f)};
-    for(uint32_t l = 0; l < v.levels; l++)
+    Corners c;
+    LevelCorners(v, l, pos, c);
+    float* gl = ggrad + (size_t(l) << v.log2T) * 2u;
+    for(uint32_t k = 0; k < 8; k++)
     {
-        float g0 = dx0[size_t(s) * 32u + 2u * l], g1 = dx0[size_t(s) * 32u + 2u * l + 1u];
-        if(g0 == 0.0f && g1 == 0.0f) continue;
-        Corners c;
-        LevelCorners(v, l, pos, c);
-        float* gl = ggrad + (size_t(l) << v.log2T) * 2u;
-        for(uint32_t k = 0; k < 8; k++)
-        {
-            atomicAdd(gl + size_t(c.idx[k]) * 2u, c.w[k] * g0);
-            atomicAdd(gl + size_t(c.idx[k]) * 2u + 1u, c.w[k] * g1);
-        }
+        atomicAdd(gl + size_t(c.idx[k]) * 2u, c.w[k] * g0);
+        atomicAdd(gl + size_t(c.idx[k]) * 2u + 1u, c.w[k] * g1);
     }
 }
 
@@ -482,17 +484,35 @@ KNrcAdamMlp(float* __restrict__ w, float* __restrict__ m, float* __restrict__ vv
     wT[l * kLayerParams + n * kW + k] = b;
 }
 
+// Four parameters per thread (count is a multiple of 4).  LAZY: entries
+// with a zero gradient keep parameter and moments, and are not written.
+template<bool LAZY>
 __global__ void __launch_bounds__(256)
 KNrcAdamGrid(float* __restrict__ p, float* __restrict__ m, float* __restrict__ vv,
              float* __restrict__ g, __half* __restrict__ h, size_t count, AdamArgs a)
 {
+    const size_t n4 = count / 4u;
     size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    for(; i < count; i += size_t(gridDim.x) * blockDim.x)
+    for(; i < n4; i += size_t(gridDim.x) * blockDim.x)
     {
-        float pp = p[i], mm = m[i], v2 = vv[i], gg = g[i];
-        AdamOne(pp, mm, v2, gg, a);
-        p[i] = pp; m[i] = mm; vv[i] = v2; g[i] = 0.0f;
-        h[i] = __float2half(pp);
+        float4 g4 = reinterpret_cast<const float4*>(g)[i];
+        if(LAZY && g4.x == 0.0f && g4.y == 0.0f && g4.z == 0.0f && g4.w == 0.0f) continue;
+        float4 p4 = reinterpret_cast<const float4*>(p)[i];
+        float4 m4 = reinterpret_cast<const float4*>(m)[i];
+        float4 v4 = reinterpret_cast<const float4*>(vv)[i];
+        float* pp = &p4.x;
+        float* mm = &m4.x;
+        float* v2 = &v4.x;
+        float* gg = &g4.x;
+        for(int k = 0; k < 4; k++)
+            if(!LAZY || gg[k] != 0.0f) AdamOne(pp[k], mm[k], v2[k], gg[k], a);
+        reinterpret_cast<float4*>(p)[i] = p4;
+        reinterpret_cast<float4*>(m)[i] = m4;
+        reinterpret_cast<float4*>(vv)[i] = v4;
+        reinterpret_cast<float4*>(g)[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        __half2 h01 = __floats2half2_rn(p4.x, p4.y), h23 = __floats2half2_rn(p4.z, p4.w);
+        reinterpret_cast<__half2*>(h)[2 * i] = h01;
+        reinterpret_cast<__half2*>(h)[2 * i + 1] = h23;
     }
 }
 
@@ -705,7 +725,8 @@ int mray_nrc_backward(MRayNrc* n, const float* dRecords, const float* dTargets, 
     uint32_t chunk = 512;
     dim3 wg(DivUp(count, chunk), kNrcLayers);
     KNrcWgrad<<<wg, 256, 0, s>>>(n->tb, count, chunk, n->wg);
-    KNrcGridBwd<<<DivUp(count, 256u), 256, 0, s>>>(n->view, dRecords, n->tb.dx0, count, n->gg);
+    KNrcGridBwd<<<DivUp(size_t(count) * n->view.levels, 256u), 256, 0, s>>>(n->view, dRecords,
+                                                                         n->tb.dx0, count, n->gg);
     return NrcLaunchCheck();
 }
 
@@ -722,7 +743,10 @@ int mray_nrc_adam(MRayNrc* n, void* stream)
     a.c1 = float(1.0 / (1.0 - std::pow(double(a.b1), double(n->step))));
     a.c2 = float(1.0 / (1.0 - std::pow(double(a.b2), double(n->step))));
     KNrcAdamMlp<<<DivUp(kMlpParams, 256u), 256, 0, s>>>(n->w, n->wm, n->wv, n->wg, n->wT, n->wK, a);
-    KNrcAdamGrid<<<4096, 256, 0, s>>>(n->grid, n->gm, n->gv, n->gg, n->gridH, n->gridCount, a);
+    if(n->cfg.flags & MRAY_NRC_LAZY_ADAM)
+        KNrcAdamGrid<true><<<4096, 256, 0, s>>>(n->grid, n->gm, n->gv, n->gg, n->gridH, n->gridCount, a);
+    else
+        KNrcAdamGrid<false><<<4096, 256, 0, s>>>(n->grid, n->gm, n->gv, n->gg, n->gridH, n->gridCount, a);
     return NrcLaunchCheck();
 }
 

@@ -87,6 +87,8 @@ struct NrcRenderState
     uint32_t trainShift;
     uint32_t maxVerts;
     uint32_t ringSize;        // training pairs kept (power of two)
+    uint32_t minDepth;        // terminal vertices have depth >= minDepth
+    uint32_t neeAtEnd;        // terminal vertices take their NEE; the cache the rest
     float spreadC;
     float boxMin[3];
     float boxInvExtent;
@@ -112,13 +114,7 @@ struct Sched
     uint32_t fill;
     uint32_t activeCount;
     uint32_t iterations;
-    uint32_t shadowCount;
-    uint32_t traceNext;     // dynamic work counters (64 items per wave fetch)
-    uint32_t shadeNext;
-    uint32_t shadowNext;
     uint32_t queryCount;      // radiance-cache queries of this iteration
-    uint32_t qCount[5];       // material-keyed queue sizes
-    uint32_t qNext[5];
     unsigned long long ringCount;   // training pairs written so far
     unsigned long long queries;     // radiance-cache queries so far
     unsigned long long startClock;  // wall clock at the first refill
@@ -126,14 +122,100 @@ struct Sched
     uint32_t viewBase;              // batch: view of the first path this refill issues
 };
 
-// Persistent kernels take work 64 items at a time: one atomic per wave,
-// issued with every lane of the wave active (loop head), so waves that run
-// out of work early take more instead of idling behind a fixed share.
-__device__ __forceinline__ uint32_t WaveFetch(uint32_t* counter)
+// Work and queue counters.  Atomics on one address serialise at about
+// 90 M/s on this GPU, so every queue and work list is split into kShards
+// shards with their own counters, 128 bytes apart.  Shard s of the active
+// list is its s-th contiguous sixteenth in 64-item chunks (ShardOfChunk);
+// the shard follows an item into the material queues and the shadow queue,
+// so no shard of any queue holds more than ShardCap(N) items, and a shard
+// stays one region of the image.
+constexpr uint32_t kShards = 16;
+constexpr uint32_t kCtrStride = 32;      // uint32 words between counters
+constexpr uint32_t kFetch = 256;         // items a wave takes per fetch atomic
+enum : uint32_t
 {
-    uint32_t base = 0;
-    if((threadIdx.x & 63u) == 0) base = atomicAdd(counter, 64u);
-    return __shfl(base, 0);
+    CTR_TRACE = 0,                         // fetch, per shard of the active list
+    CTR_SHADOW_FETCH = kShards,            // fetch, per shadow-queue shard
+    CTR_SHADOW = 2 * kShards,              // shadow-queue shard sizes
+    CTR_QFETCH = 3 * kShards,              // [5 material queues][kShards] fetch
+    CTR_QCOUNT = 8 * kShards,              // [5 material queues][kShards] sizes
+    CTR_COUNT = 13 * kShards
+};
+__host__ __device__ constexpr uint32_t ShardCap(uint32_t N)
+{
+    return 64u * ((((N + 63u) / 64u) + kShards - 1u) / kShards);
+}
+// shadow-queue shards: the chunk bound plus 63 items per material queue a
+// wave straddling two queue shards may add
+__host__ __device__ constexpr uint32_t ShadowShardCap(uint32_t N) { return ShardCap(N) + 64u * 5u; }
+__device__ __forceinline__ uint32_t* Ctr(uint32_t* ctr, uint32_t idx) { return ctr + size_t(idx) * kCtrStride; }
+// shard of 64-item chunk c of an n-item list (C chunks): contiguous ranges
+__device__ __forceinline__ uint32_t ShardOfChunk(uint32_t c, uint32_t n)
+{
+    const uint32_t C = (n + 63u) / 64u;
+    return uint32_t((uint64_t(c) * kShards) / C);
+}
+// Dynamic work fetch, 64 items per atomic, over kXcds counters: the list
+// [0, n) is cut into kXcds contiguous parts, a wave starts on the part of its
+// XCD (workgroups are dispatched round-robin over the 8 XCDs, each with its
+// own L2, so an XCD keeps to one region of the image) and then takes from
+// the others.  A plain load first skips a part already used up; every wave
+// exits after one pass over the parts.  Wave-uniform values stay scalar.
+// f(k) runs for each item k < n.
+constexpr uint32_t kXcds = 8;
+template<class F>
+__device__ __forceinline__ void FetchLoop(uint32_t* ctr, uint32_t n, F&& f)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t C = (n + 63u) / 64u;                       // 64-item chunks
+    uint32_t part = blockIdx.x % kXcds;
+    for(uint32_t t = 0; t < kXcds; t++)
+    {
+        const uint32_t c0 = __builtin_amdgcn_readfirstlane(uint32_t((uint64_t(C) * part) / kXcds));
+        const uint32_t c1 = __builtin_amdgcn_readfirstlane(uint32_t((uint64_t(C) * (part + 1u)) / kXcds));
+        uint32_t* fc = ctr + size_t(part) * kCtrStride;
+        for(;;)
+        {
+            uint32_t c = c1 - c0;
+            if(lane == 0)
+            {
+                c = __hip_atomic_load(fc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if(c < c1 - c0) c = atomicAdd(fc, 1u);
+            }
+            c = __builtin_amdgcn_readfirstlane(c);
+            if(c >= c1 - c0) break;
+            const uint32_t k = (c0 + c) * 64u + lane;
+            if(k < n) f(k);
+        }
+        part = (part + 1u) % kXcds;
+    }
+}
+
+// kShards queue shards read as one list: shard s holds items
+// [s * cap, s * cap + count_s); virtual index k maps to its shard and slot.
+struct Shards
+{
+    uint32_t pre[kShards + 1];   // exclusive prefix of the shard sizes
+};
+__device__ __forceinline__ void LoadShards(uint32_t* ctr, uint32_t countBase, Shards& g)
+{
+    uint32_t acc = 0;
+    #pragma unroll
+    for(uint32_t sh = 0; sh < kShards; sh++)
+    {
+        g.pre[sh] = acc;
+        acc += __builtin_amdgcn_readfirstlane(*Ctr(ctr, countBase + sh));
+    }
+    g.pre[kShards] = acc;
+}
+__device__ __forceinline__ uint32_t ShardItem(const Shards& g, uint32_t k, uint32_t cap, uint32_t& sh)
+{
+    uint32_t s0 = 0, first = 0;
+    #pragma unroll
+    for(uint32_t t = 1; t < kShards; t++)
+        if(k >= g.pre[t]) { s0 = t; first = g.pre[t]; }
+    sh = s0;
+    return s0 * cap + (k - first);
 }
 
 // One viewport of a batch: its camera and film, its range of the global path
@@ -268,8 +350,10 @@ KRefillCount(const uint32_t* __restrict__ meta, const uint32_t* __restrict__ vie
 __global__ void __launch_bounds__(1024)
 KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sched,
             const unsigned long long* __restrict__ viewBegin, uint32_t viewCount,
-            unsigned long long* __restrict__ viewClock)
+            unsigned long long* __restrict__ viewClock, uint32_t* __restrict__ ctr)
 {
+    // this bounce's work and queue counters start empty
+    if(threadIdx.x < CTR_COUNT) *Ctr(ctr, threadIdx.x) = 0u;
     __shared__ uint32_t sf[1024], sl[1024];
     __shared__ uint32_t carryF, carryL, vmin;
     __shared__ unsigned long long accum;
@@ -332,12 +416,7 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
         sched->counter += fill;
         sched->activeCount = carryL + uint32_t(fill);
         sched->deadTotal += accum;
-        sched->shadowCount = 0;
-        sched->traceNext = 0;
-        sched->shadeNext = 0;
-        sched->shadowNext = 0;
         sched->queryCount = 0;
-        for(int q = 0; q < 5; q++) { sched->qCount[q] = 0; sched->qNext[q] = 0; }
         if(sched->iterations == 0) sched->startClock = wall_clock64();
         sched->iterations += 1;
     }
@@ -381,29 +460,62 @@ __device__ __forceinline__ void SplatToFilm(const SlotSoA& S, const RenderConst&
     atomicAdd(film + 3 * P + px, w);
 }
 
-// A finished training path turns its recorded vertices into training pairs:
-// target = (final radiance - radiance before the vertex) / throughput.
-__device__ __forceinline__ void EmitTrainingPairs(const SlotSoA& S, const RenderConst& rc, uint32_t i,
-                                                  Sched* sched)
+// Training pairs of the training paths that finished in the last bounce:
+// target = (final radiance - radiance before the vertex) / throughput.  One
+// lane per (path, recorded vertex), G lanes per path (G a power of two
+// >= maxVerts, dividing 64); the group's first lane reserves the ring entries.
+__global__ void __launch_bounds__(256)
+KNrcEmit(SlotSoA S, RenderConst rc, Sched* __restrict__ sched, uint32_t log2G)
 {
+    __shared__ uint32_t sWave[256 / 64];
+    __shared__ unsigned long long sBase;
     const NrcRenderState& nr = rc.nrc;
-    uint32_t ts = i >> nr.trainShift;
-    uint32_t nv = nr.trainCount[ts];
-    if(nv == 0) return;
-    float4 fr = S.rad[i];
-    unsigned long long base = atomicAdd(&sched->ringCount, (unsigned long long)nv);
-    for(uint32_t v = 0; v < nv; v++)
+    const uint32_t G = 1u << log2G;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t ts = t >> log2G, v = t & (G - 1u);
+    const uint32_t i = ts << nr.trainShift;
+    const bool fin = (ts << nr.trainShift) < rc.N && (i >> nr.trainShift) == ts &&
+                     MStatus(S.meta[i]) == ST_DEAD;
+    const uint32_t nv = fin ? nr.trainCount[ts] : 0u;
+    // ring entries for the whole block with one atomic: an exclusive scan
+    // of the paths' vertex counts (held by each group's first lane)
+    const uint32_t want = (v == 0u) ? nv : 0u;
+    uint32_t x = want;
+    for(uint32_t o = 1; o < 64; o <<= 1)
     {
-        const float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + v) * kNrcVert;
-        size_t slot = size_t((base + v) & (nr.ringSize - 1u));
-        for(uint32_t k = 0; k < kNrcRec; k++) nr.ringRec[slot * kNrcRec + k] = vx[k];
-        float fin[3] = {fr.x, fr.y, fr.z};
-        for(uint32_t c = 0; c < 3; c++)
+        const uint32_t y = uint32_t(__shfl_up(int(x), o));
+        if(lane >= o) x += y;
+    }
+    if(lane == 63u) sWave[wave] = x;
+    __syncthreads();
+    if(threadIdx.x == 0)
+    {
+        uint32_t total = 0;
+        for(uint32_t w = 0; w < 256u / 64u; w++)
         {
-            float thr = vx[kNrcRec + c], before = vx[kNrcRec + 3 + c];
-            float tgt = (thr > 1e-12f) ? (fin[c] - before) / thr : 0.0f;
-            nr.ringTgt[slot * 3 + c] = isfinite(tgt) ? MaxF(tgt, 0.0f) : 0.0f;
+            const uint32_t c = sWave[w];
+            sWave[w] = total;
+            total += c;
         }
+        sBase = total ? atomicAdd(&sched->ringCount, (unsigned long long)total) : 0ull;
+    }
+    __syncthreads();
+    const uint32_t mine = sWave[wave] + x - want;              // this path's first entry
+    const int src = int(lane & ~(G - 1u));
+    const uint32_t first = uint32_t(__shfl(int(mine), src));
+    if(!fin || v >= nv) return;
+    const unsigned long long base = sBase + first;
+    const float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + v) * kNrcVert;
+    const size_t slot = size_t((base + v) & (nr.ringSize - 1u));
+    for(uint32_t k = 0; k < kNrcRec; k++) nr.ringRec[slot * kNrcRec + k] = vx[k];
+    const float4 fr = S.rad[i];
+    const float fin3[3] = {fr.x, fr.y, fr.z};
+    for(uint32_t c = 0; c < 3; c++)
+    {
+        float thr = vx[kNrcRec + c], before = vx[kNrcRec + 3 + c];
+        float tgt = (thr > 1e-12f) ? (fin3[c] - before) / thr : 0.0f;
+        nr.ringTgt[slot * 3 + c] = isfinite(tgt) ? MaxF(tgt, 0.0f) : 0.0f;
     }
 }
 
@@ -463,8 +575,6 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
     if(inRange && status == ST_DEAD)
     {
         SplatToFilm<SPECTRAL, BATCH>(S, rc, i, m);
-        if(!SPECTRAL && rc.nrc.on && (i & ((1u << rc.nrc.trainShift) - 1u)) == 0u)
-            EmitTrainingPairs(S, rc, i, sched);
     }
     uint64_t b = __ballot(freeSlot);
     const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
@@ -522,31 +632,36 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
 template<int W, bool COUNT, bool BATCH, int TPB = kTraceTPB>
 __global__ void __launch_bounds__(TPB)
 KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __restrict__ sched,
-       const ViewGPU* __restrict__ views)
+       const ViewGPU* __restrict__ views, uint32_t* __restrict__ ctr)
 {
     __shared__ uint2 lstack[kLdsStack * TPB];
-    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = sched->activeCount;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n = sched->activeCount;
     if(gtid == 0) sched->extRays += n;
     uint32_t nodes = 0, tris = 0;
-    for(;;)
+    FetchLoop(Ctr(ctr, CTR_TRACE), n, [&](uint32_t k)
     {
-        uint32_t k = WaveFetch(&sched->traceNext);
-        if(k >= n) break;
-        k += threadIdx.x & 63u;
-        if(k >= n) continue;
-        uint32_t i = active[k];
-        float4 o = S.rayO[i], d = S.rayD[i];
+        const uint32_t i = active[k];
+        const float4 o = S.rayO[i], d = S.rayD[i];
         const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
         HitRec h = TraceW<W, false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                       o.w, d.w, root, &nodes, &tris);
+                                           o.w, d.w, root, &nodes, &tris);
         S.hit[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.a),
                               __float_as_uint(h.b));
-    }
-    if(COUNT && (nodes | tris))
+    });
+    if(COUNT)
     {
-        atomicAdd(&sched->nodeVisits, (unsigned long long)nodes);
-        atomicAdd(&sched->triTests, (unsigned long long)tris);
+        unsigned long long nv = nodes, nt = tris;
+        for(int o = 32; o > 0; o >>= 1)
+        {
+            nv += __shfl_xor(nv, o);
+            nt += __shfl_xor(nt, o);
+        }
+        if((threadIdx.x & 63u) == 0 && (nv | nt))
+        {
+            atomicAdd(&sched->nodeVisits, nv);
+            atomicAdd(&sched->triTests, nt);
+        }
     }
 }
 
@@ -695,9 +810,18 @@ __device__ __forceinline__ float PdfLightBoundary(const SceneView& s)
 // Shading of one slot after its closest hit.  KEY < 0: any hit; KEY = 0:
 // boundary or light hits only; KEY = 1 + material type: surface hits of that
 // material only (the material switch folds away).
+// What one shading step hands to its wave's appends: a shadow ray (written
+// to the slot's sO/sD/sRad) and a radiance-cache query record.
+struct ShadeOut
+{
+    bool shadow;
+    bool query;
+    float rec[kNrcRec];
+};
+
 template<bool SPECTRAL, int KEY, bool BATCH>
 __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& rc, uint32_t i,
-                                          Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
+                                          ShadeOut& out)
 {
     const SceneView s = SlotScene<BATCH>(rc, S, i);
     uint4 hh = S.hit[i];
@@ -798,6 +922,7 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
     else mat.frontIoR = mat.backIoR = 1.0f;
     V3 wO = Normalize(-rdir);
     bool specular = IsSpecular(Specularity(mat));
+    bool terminal = false;   // radiance cache: ends after this vertex's NEE
     if constexpr(!SPECTRAL)
     {
         if(rc.nrc.on)
@@ -833,22 +958,19 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
             rec[9] = M.albedo[1];
             rec[10] = M.albedo[2];
             const bool training = (i & ((1u << nr.trainShift) - 1u)) == 0u;
-            if(!training && depth >= 1u && sp.y * sp.y > nr.spreadC * sp.x)
+            if(!training && depth >= nr.minDepth && sp.y * sp.y > nr.spreadC * sp.x)
             {
-                // terminate into the cache: queue the query, the path ends
-                const uint64_t qm = __ballot(1);
-                const uint32_t ln = threadIdx.x & 63u;
-                const uint32_t leader = uint32_t(__ffsll((unsigned long long)qm) - 1);
-                uint32_t qb = 0;
-                if(ln == leader) qb = atomicAdd(&sched->queryCount, uint32_t(__popcll(qm)));
-                qb = __shfl(qb, int(leader));
-                const uint64_t below = (ln == 0) ? 0ull : ((~0ull) >> (64 - ln));
-                uint32_t q = qb + uint32_t(__popcll(qm & below));
-                for(uint32_t k = 0; k < kNrcRec; k++) nr.queryRec[size_t(q) * kNrcRec + k] = rec[k];
-                nr.querySlot[q] = i;
-                S.rng[i] = st;
-                S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
-                return;
+                // terminate into the cache: a query; the path ends here, or
+                // after this vertex's NEE with neeAtEnd
+                out.query = true;
+                for(uint32_t k = 0; k < kNrcRec; k++) out.rec[k] = rec[k];
+                if(!nr.neeAtEnd)
+                {
+                    S.rng[i] = st;
+                    S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
+                    return;
+                }
+                terminal = true;
             }
             if(training)
             {
@@ -901,18 +1023,15 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
                 S.sO[i] = make_float4(spos.x, spos.y, spos.z, kEps);
                 S.sD[i] = make_float4(sdir.x, sdir.y, sdir.z, length * (1.0f - kLargeEps));
                 S.sRad[i] = make_float4(srad.x, srad.y, srad.z, srad.w);
-                // shadow queue append: one atomic per wave, from the
-                // lowest active lane (only lanes with a shadow ray run this)
-                const uint64_t pm = __ballot(1);
-                const uint32_t lane = threadIdx.x & 63u;
-                const uint32_t leader = uint32_t(__ffsll((unsigned long long)pm) - 1);
-                uint32_t qBase = 0;
-                if(lane == leader) qBase = atomicAdd(&sched->shadowCount, uint32_t(__popcll(pm)));
-                qBase = __shfl(qBase, int(leader));
-                const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-                shadowQ[qBase + uint32_t(__popcll(pm & below))] = i;
+                out.shadow = true;
             }
         }
+    }
+    if(terminal)
+    {
+        S.rng[i] = st;
+        S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
+        return;
     }
     // WorkFunction (PathTracerRendererShaders.h:201-301)
     uint32_t matRN = MatRNCount(mat.type);
@@ -947,33 +1066,87 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
     S.meta[i] = MPack(depth, dead ? ST_DEAD : ST_ALIVE, type, dispersed, false);
 }
 
-// Generic shading over the whole active list.
+// The wave's appends after a shading step (every lane calls it): shadow
+// rays into shadow-queue shard `sh` (one atomic per wave on that shard's
+// counter), cache queries into the query queue.
+__device__ __forceinline__ void ShadeAppend(const ShadeOut& o, uint32_t i, uint32_t sh, uint32_t cap,
+                                            uint32_t* __restrict__ ctr, uint32_t* __restrict__ shadowQ,
+                                            Sched* __restrict__ sched, const NrcRenderState& nr,
+                                            bool queries)
+{
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const uint64_t sm = __ballot(o.shadow);
+    if(sm)
+    {
+        uint32_t b = 0;
+        if(lane == 0) b = atomicAdd(Ctr(ctr, CTR_SHADOW + sh), uint32_t(__popcll(sm)));
+        b = __shfl(b, 0);
+        if(o.shadow) shadowQ[size_t(sh) * cap + b + uint32_t(__popcll(sm & lt))] = i;
+    }
+    if(queries)
+    {
+        const uint64_t qm = __ballot(o.query);
+        if(qm)
+        {
+            uint32_t b = 0;
+            if(lane == 0) b = atomicAdd(&sched->queryCount, uint32_t(__popcll(qm)));
+            b = __shfl(b, 0);
+            if(o.query)
+            {
+                const uint32_t q = b + uint32_t(__popcll(qm & lt));
+                for(uint32_t k = 0; k < kNrcRec; k++) nr.queryRec[size_t(q) * kNrcRec + k] = o.rec[k];
+                nr.querySlot[q] = i;
+            }
+        }
+    }
+}
+
+// Generic shading over the whole active list: static, one 64-item chunk per
+// wave step (shading is short per item, a fetch atomic would cost as much as
+// the work it hands out); the chunk's shard picks the shadow-queue shard.
 template<bool SPECTRAL, bool BATCH>
 __global__ void __launch_bounds__(kSlotTPB)
 KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
-       Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
+       Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ, uint32_t* __restrict__ ctr)
 {
-    // static grid-stride: shading is short per item, so a per-wave atomic
-    // fetch would cost as much as the work it hands out
-    uint32_t n = sched->activeCount;
-    for(uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        ShadeSlot<SPECTRAL, -1, BATCH>(S, rc, active[k], sched, shadowQ);
+    const uint32_t n = sched->activeCount;
+    const uint32_t cap = ShadowShardCap(rc.N);
+    const bool queries = !SPECTRAL && rc.nrc.on;
+    for(uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n;
+        base += gridDim.x * blockDim.x)
+    {
+        const uint32_t k = base + (threadIdx.x & 63u);
+        ShadeOut o;
+        o.shadow = false;
+        o.query = false;
+        uint32_t i = 0;
+        if(k < n)
+        {
+            i = active[k];
+            ShadeSlot<SPECTRAL, -1, BATCH>(S, rc, i, o);
+        }
+        ShadeAppend(o, i, ShardOfChunk(base >> 6, n), cap, ctr, shadowQ, sched, rc.nrc, queries);
+    }
 }
 
 // Material-keyed queues (replacing the reference's work-key sort and
 // partition, RayPartitioner.cu:263-411): key 0 = boundary / light hits,
-// 1 + material type otherwise; one atomic per key per wave.
+// 1 + material type otherwise; one atomic per key per wave, on the shard of
+// the wave's active-list chunk.
 __global__ void __launch_bounds__(kSlotTPB)
 KClassify(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active, Sched* __restrict__ sched,
-          uint32_t* __restrict__ queues, uint32_t N)
+          uint32_t* __restrict__ queues, uint32_t* __restrict__ ctr)
 {
     const SceneView& s = rc.scene;
     const uint32_t n = sched->activeCount;
+    const uint32_t cap = ShardCap(rc.N);
     const uint32_t lane = threadIdx.x & 63u;
     const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
     for(uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n;
         base += gridDim.x * blockDim.x)
     {
+        const uint32_t sh = ShardOfChunk(base >> 6, n);
         uint32_t k = base + lane;
         uint32_t key = kMatQueues;           // no work
         uint32_t i = 0;
@@ -994,22 +1167,48 @@ KClassify(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active, Sched*
             uint64_t m = __ballot(key == q);
             if(!m) continue;
             uint32_t qb = 0;
-            if(lane == 0) qb = atomicAdd(&sched->qCount[q], uint32_t(__popcll(m)));
+            if(lane == 0) qb = atomicAdd(Ctr(ctr, CTR_QCOUNT + q * kShards + sh), uint32_t(__popcll(m)));
             qb = __shfl(qb, 0);
-            if(key == q) queues[size_t(q) * N + qb + uint32_t(__popcll(m & below))] = i;
+            if(key == q)
+                queues[(size_t(q) * kShards + sh) * cap + qb + uint32_t(__popcll(m & below))] = i;
         }
     }
 }
 
 template<bool SPECTRAL, int KEY, bool BATCH>
 __global__ void __launch_bounds__(kSlotTPB)
-KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues, uint32_t N,
-        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ)
+KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues,
+        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ, uint32_t* __restrict__ ctr)
 {
-    const uint32_t n = sched->qCount[KEY];
-    const uint32_t* q = queues + size_t(KEY) * N;
-    for(uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x)
-        ShadeSlot<SPECTRAL, KEY, BATCH>(S, rc, q[k], sched, shadowQ);
+    const uint32_t cap = ShardCap(rc.N), scap = ShadowShardCap(rc.N);
+    const bool queries = !SPECTRAL && rc.nrc.on;
+    Shards g;
+    LoadShards(ctr, CTR_QCOUNT + uint32_t(KEY) * kShards, g);
+    const uint32_t n = g.pre[kShards];
+    const uint32_t* q = queues + size_t(KEY) * kShards * cap;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t* counter = Ctr(ctr, CTR_QFETCH + uint32_t(KEY) * kShards);
+    for(;;)
+    {
+        uint32_t base = 0;
+        if(lane == 0) base = atomicAdd(counter, 64u);
+        base = __builtin_amdgcn_readfirstlane(base);
+        if(base >= n) break;
+        const uint32_t k = base + lane;
+        ShadeOut o;
+        o.shadow = false;
+        o.query = false;
+        uint32_t i = 0, sh = 0;
+        if(k < n)
+        {
+            i = q[ShardItem(g, k, cap, sh)];
+            ShadeSlot<SPECTRAL, KEY, BATCH>(S, rc, i, o);
+        }
+        // the wave's first item names the shadow shard (a wave straddling two
+        // queue shards adds at most 63 items to it: ShadowShardCap's slack)
+        ShadeAppend(o, i, __builtin_amdgcn_readfirstlane(sh), scap, ctr, shadowQ, sched, rc.nrc,
+                    queries);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1020,29 +1219,44 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues, uint32_t
 template<int W, bool BATCH, int TPB = kTraceTPB>
 __global__ void __launch_bounds__(TPB)
 KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __restrict__ sched,
-        const ViewGPU* __restrict__ views)
+        const ViewGPU* __restrict__ views, NrcRenderState nrc, uint32_t* __restrict__ ctr,
+        uint32_t N)
 {
     __shared__ uint2 lstack[kLdsStack * TPB];
-    uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t n = sched->shadowCount;
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t cap = ShadowShardCap(N);
+    Shards g;
+    LoadShards(ctr, CTR_SHADOW, g);
+    const uint32_t n = g.pre[kShards];
     if(gtid == 0) sched->shadowRays += n;
-    for(;;)
+    FetchLoop(Ctr(ctr, CTR_SHADOW_FETCH), n, [&](uint32_t k)
     {
-        uint32_t k = WaveFetch(&sched->shadowNext);
-        if(k >= n) break;
-        k += threadIdx.x & 63u;
-        if(k >= n) continue;
-        uint32_t i = shadowQ[k];
-        float4 o = S.sO[i], d = S.sD[i];
+        uint32_t sh;
+        const uint32_t i = shadowQ[ShardItem(g, k, cap, sh)];
+        const float4 o = S.sO[i], d = S.sD[i];
         const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
         HitRec h = TraceW<W, true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                      o.w, d.w, root);
+                                          o.w, d.w, root);
         if(h.prim == kInvalid)
         {
             float4 a = S.rad[i], b = S.sRad[i];
             S.rad[i] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+            // a training path's vertex recorded this bounce (its last) counts
+            // its visible NEE as radiance gathered before the cache's part
+            if(!BATCH && nrc.neeAtEnd && (i & ((1u << nrc.trainShift) - 1u)) == 0u)
+            {
+                const uint32_t ts = i >> nrc.trainShift;
+                const uint32_t nv = nrc.trainCount[ts];
+                if(nv > 0)
+                {
+                    float* vx = nrc.trainStore + (size_t(ts) * nrc.maxVerts + nv - 1u) * kNrcVert;
+                    vx[kNrcRec + 3] += b.x;
+                    vx[kNrcRec + 4] += b.y;
+                    vx[kNrcRec + 5] += b.z;
+                }
+            }
         }
-    }
+    });
 }
 
 // Radiance-cache answers: a terminated path adds throughput x max(L, 0).
@@ -1093,10 +1307,6 @@ __global__ void KResetSlots(uint32_t* __restrict__ meta, uint32_t* __restrict__ 
         sched->fill = 0;
         sched->activeCount = 0;
         sched->iterations = 0;
-        sched->shadowCount = 0;
-        sched->traceNext = 0;
-        sched->shadeNext = 0;
-        sched->shadowNext = 0;
         sched->queryCount = 0;
         sched->ringCount = 0;
         sched->queries = 0;
@@ -1131,7 +1341,8 @@ struct MRayRenderer
     uint32_t shadeBlocks = 0;
     uint32_t shadeQBlocks[5] = {0, 0, 0, 0, 0};
     uint32_t classifyBlocks = 0;
-    uint32_t* dQueues = nullptr;  // [5][N] material-keyed queues (NULL: generic shading)
+    uint32_t* dQueues = nullptr;  // [5][kShards][ShardCap(N)] material queues (NULL: generic shading)
+    uint32_t* dCtr = nullptr;     // [CTR_COUNT] work and queue counters, kCtrStride apart
     uint32_t shadowBlocks = 0;
     unsigned long long limit = 0;
     Sched last{};                 // stats at the previous mray_renderer_stats()
@@ -1150,6 +1361,7 @@ struct MRayRenderer
     MRayNrc* nrc = nullptr;
     MRayNrcRenderParams nrcParams{};
     DeviceArena nrcArena;
+    uint32_t nrcLog2G = 0;        // lanes per training path in KNrcEmit (log2)
     // frame schedule (mray_renderer_render_frame)
     unsigned long long frameLimit = 0;
     // traversal context: the accelerator's, or the merged batch scene's
@@ -1203,12 +1415,11 @@ static int HarvestEvents(MRayRenderer* r)
 template<bool SPECTRAL, bool BATCH>
 static void LaunchShadeQueues(MRayRenderer* r, hipStream_t s)
 {
-    const uint32_t N = r->rc.N;
-    KShadeQ<SPECTRAL, 0, BATCH><<<r->shadeQBlocks[0], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
-    KShadeQ<SPECTRAL, 1, BATCH><<<r->shadeQBlocks[1], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
-    KShadeQ<SPECTRAL, 2, BATCH><<<r->shadeQBlocks[2], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
-    KShadeQ<SPECTRAL, 3, BATCH><<<r->shadeQBlocks[3], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
-    KShadeQ<SPECTRAL, 4, BATCH><<<r->shadeQBlocks[4], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, N, r->dSched, r->dShadowQ);
+    KShadeQ<SPECTRAL, 0, BATCH><<<r->shadeQBlocks[0], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, r->dSched, r->dShadowQ, r->dCtr);
+    KShadeQ<SPECTRAL, 1, BATCH><<<r->shadeQBlocks[1], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, r->dSched, r->dShadowQ, r->dCtr);
+    KShadeQ<SPECTRAL, 2, BATCH><<<r->shadeQBlocks[2], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, r->dSched, r->dShadowQ, r->dCtr);
+    KShadeQ<SPECTRAL, 3, BATCH><<<r->shadeQBlocks[3], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, r->dSched, r->dShadowQ, r->dCtr);
+    KShadeQ<SPECTRAL, 4, BATCH><<<r->shadeQBlocks[4], kSlotTPB, 0, s>>>(r->S, r->rc, r->dQueues, r->dSched, r->dShadowQ, r->dCtr);
 }
 
 template<bool SPECTRAL, bool BATCH, typename F>
@@ -1231,13 +1442,13 @@ static void LaunchTrace(MRayRenderer* r, hipStream_t s)
     {
         if(r->traceTPB == 128)
         {
-            if(ctx.width == 8) KTrace<8, COUNT, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews);
-            else KTrace<4, COUNT, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews);
+            if(ctx.width == 8) KTrace<8, COUNT, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
+            else KTrace<4, COUNT, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
             return;
         }
     }
-    if(ctx.width == 8) KTrace<8, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews);
-    else KTrace<4, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews);
+    if(ctx.width == 8) KTrace<8, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
+    else KTrace<4, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
 }
 
 template<bool BATCH>
@@ -1249,13 +1460,13 @@ static void LaunchShadow(MRayRenderer* r, hipStream_t s)
     {
         if(r->traceTPB == 128)
         {
-            if(ctx.width == 8) KShadow<8, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews);
-            else KShadow<4, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews);
+            if(ctx.width == 8) KShadow<8, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
+            else KShadow<4, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
             return;
         }
     }
-    if(ctx.width == 8) KShadow<8, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews);
-    else KShadow<4, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews);
+    if(ctx.width == 8) KShadow<8, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
+    else KShadow<4, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
 }
 
 template<bool BATCH>
@@ -1272,7 +1483,14 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     KRefillCount<BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, r->S.view, N, r->dBlockCnt);
     KRefillScan<<<1, 1024, 0, s>>>(r->dBlockCnt, r->nBlocks, r->dSched, r->rc.viewBegin,
-                                   r->rc.viewCount, BATCH ? r->dViewClock : nullptr);
+                                   r->rc.viewCount, BATCH ? r->dViewClock : nullptr, r->dCtr);
+    if(!BATCH && r->nrc)
+    {
+        // training pairs of finished training paths, before the refill reuses their slots
+        const NrcRenderState& nr = r->rc.nrc;
+        const uint64_t lanes = (uint64_t((N - 1u) >> nr.trainShift) + 1u) << r->nrcLog2G;
+        KNrcEmit<<<DivUp(lanes, 256), 256, 0, s>>>(r->S, r->rc, r->dSched, r->nrcLog2G);
+    }
     if(r->rc.spectral)
         KRefillApply<true, BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
                                                                   r->dActive);
@@ -1287,16 +1505,16 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
     {
         // material-keyed queues, then one specialised shading launch per key
         KClassify<<<r->classifyBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
-                                                         r->dQueues, N);
+                                                         r->dQueues, r->dCtr);
         if(r->rc.spectral) LaunchShadeQueues<true, BATCH>(r, s);
         else LaunchShadeQueues<false, BATCH>(r, s);
     }
     else if(r->rc.spectral)
         KShade<true, BATCH><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
-                                                                r->dShadowQ);
+                                                                r->dShadowQ, r->dCtr);
     else
         KShade<false, BATCH><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
-                                                                 r->dShadowQ);
+                                                                 r->dShadowQ, r->dCtr);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     LaunchShadow<BATCH>(r, s);
     if(!BATCH && r->nrc)
@@ -1412,7 +1630,9 @@ static int InitSlots(MRayRenderer* r, size_t filmPixels)
     MRAY_TRY(A((void**)&r->rc.film, filmPixels * 4 * sizeof(float)));
     r->nBlocks = DivUp(N, kSlotTPB);
     MRAY_TRY(A((void**)&r->dBlockCnt, size_t(r->nBlocks) * sizeof(uint4)));
-    MRAY_TRY(A((void**)&r->dShadowQ, size_t(N) * 4));
+    MRAY_TRY(A((void**)&r->dShadowQ, size_t(kShards) * ShadowShardCap(N) * 4));
+    MRAY_TRY(A((void**)&r->dCtr, size_t(CTR_COUNT) * kCtrStride * 4));
+    MRAY_HIP_CHECK(hipMemset(r->dCtr, 0, size_t(CTR_COUNT) * kCtrStride * 4));
     MRAY_TRY(A((void**)&r->dActive, size_t(N) * 4));
     MRAY_TRY(A((void**)&r->dSched, sizeof(Sched)));
     MRAY_TRY(A((void**)&r->dSeeds, size_t(N) * 4));
@@ -1482,7 +1702,8 @@ static int SizeLaunches(MRayRenderer* r)
     }
     if(p->flags & MRAY_RENDER_MATERIAL_QUEUES)
     {
-        if(!r->dQueues) MRAY_TRY(DevAlloc(r->arena, (void**)&r->dQueues, size_t(kMatQueues) * N * 4));
+        if(!r->dQueues)
+            MRAY_TRY(DevAlloc(r->arena, (void**)&r->dQueues, size_t(kMatQueues) * kShards * ShardCap(N) * 4));
         r->classifyBlocks = uint32_t(resident(KClassify, kSlotTPB, 1u << 20));
         if(r->bscene)
         {
@@ -1843,9 +2064,13 @@ int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderP
     const uint32_t N = r->rc.N;
     const uint32_t slots = (N >> p->trainShift) + 1u;
     nr.trainShift = p->trainShift;
-    nr.maxVerts = std::max(1u, r->rc.rr1);
+    nr.maxVerts = std::min(64u, std::max(1u, r->rc.rr1));
+    r->nrcLog2G = 0;
+    while((1u << r->nrcLog2G) < nr.maxVerts) r->nrcLog2G++;
     nr.ringSize = uint32_t(ring);
     nr.spreadC = p->spreadThreshold;
+    nr.minDepth = std::max(1u, p->minDepth);
+    nr.neeAtEnd = (p->flags & MRAY_NRC_NEE_AT_END) ? 1u : 0u;
     auto A = [&](void** ptr, size_t bytes) { return DevAlloc(r->nrcArena, ptr, bytes); };
     MRAY_TRY_IT(A((void**)&nr.trainStore, size_t(slots) * nr.maxVerts * kNrcVert * 4));
     MRAY_TRY_IT(A((void**)&nr.trainCount, size_t(slots) * 4));

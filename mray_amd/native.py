@@ -376,11 +376,14 @@ class Renderer:
         check(lib().mray_renderer_take_film(self.h, out.ctypes.data))
         return out
 
-    def attach_nrc(self, nrc, train_shift=4, spread=0.01, batch=1 << 14, steps=4):
+    def attach_nrc(self, nrc, train_shift=4, spread=0.01, batch=1 << 14, steps=4, min_depth=1,
+                   nee_at_end=False):
         """Terminate paths into `nrc` and train it after every frame (RGB only);
         nrc=None detaches."""
         p = abi.MRayNrcRenderParams()
         p.trainShift, p.spreadThreshold, p.trainBatch, p.trainSteps = train_shift, spread, batch, steps
+        p.minDepth = min_depth
+        p.flags = abi.NRC_NEE_AT_END if nee_at_end else 0
         self.nrc = nrc
         check(lib().mray_renderer_attach_nrc(self.h, nrc.h if nrc is not None else None,
                                              C.byref(p)))
@@ -402,9 +405,10 @@ class Nrc:
     Records are float32 [n, 11] device tensors (see include/mray_amd.h)."""
 
     def __init__(self, levels=16, log2_table=19, min_res=16, max_res=2048, max_batch=1 << 16,
-                 lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-15, seed=1):
+                 lr=1e-2, beta1=0.9, beta2=0.99, eps=1e-15, seed=1, lazy_adam=False):
         require_gpu()
         cfg = abi.MRayNrcConfig()
+        cfg.flags = abi.NRC_LAZY_ADAM if lazy_adam else 0
         cfg.levels, cfg.log2TableSize = levels, log2_table
         cfg.minResolution, cfg.maxResolution = min_res, max_res
         cfg.maxBatch = max_batch

@@ -31,6 +31,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(abi.MRayNrcConfig) == 48
     assert ctypes.sizeof(abi.MRayBatchView) == 76
     assert ctypes.sizeof(abi.MRayBatchTune) == 24
+    assert ctypes.sizeof(abi.MRayNrcRenderParams) == 24
 
 
 def test_error_path_without_gpu_work():

@@ -142,3 +142,41 @@ def test_training_reduces_loss(torch_gpu):
     assert losses[-1] < 0.1 * losses[0]
     assert rel < 0.02
     nrc.close()
+
+
+def test_lazy_grid_adam(torch_gpu):
+    """MRAY_NRC_LAZY_ADAM: over two steps, grid entries with a zero gradient in
+    a step keep parameter (and moments); entries with one follow Adam with
+    their own moments.  Checked against the oracle's Adam applied entrywise
+    under the same masks.  The MLP stays dense (as test_adam_step)."""
+    torch = torch_gpu
+    from mray_amd import native
+    rng = np.random.default_rng(11)
+    n = native.Nrc(max_batch=1024, seed=7, lazy_adam=True)
+    m = v = None
+    touched_before = None
+    for step in (1, 2):
+        p = n.read(abi.NRC_GRID).astype(np.float64)
+        if m is None:
+            m, v = np.zeros_like(p), np.zeros_like(p)
+        rec = _records(rng, 128)
+        tgt = rng.uniform(0.0, 1.0, size=(128, 3)).astype(np.float32)
+        n.backward(torch.from_numpy(rec).cuda(), torch.from_numpy(tgt).cuda())
+        g = n.read(abi.NRC_GRID_GRADS).astype(np.float64)
+        n.adam()
+        got = n.read(abi.NRC_GRID).astype(np.float64)
+        touched = g != 0
+        assert touched.sum() > 1000 and (~touched).sum() > 1000
+        p2, m2, v2 = N.adam(p, g, m, v, step, lr=1e-2)
+        assert np.array_equal(got[~touched], p[~touched])       # not written at all
+        assert np.allclose(got[touched], p2[touched], rtol=1e-5, atol=1e-7)
+        m = np.where(touched, m2, m)
+        v = np.where(touched, v2, v)
+        if touched_before is not None:
+            # entries with momentum from step 1 but no gradient now stay put,
+            # where dense Adam would still move them
+            idle = touched_before & ~touched
+            assert idle.sum() > 100
+        touched_before = touched
+    assert np.all(n.read(abi.NRC_GRID_GRADS) == 0.0)
+    n.close()
