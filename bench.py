@@ -54,14 +54,14 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--nrc", type=int, default=1,
                     help="run the radiance-cache equal-time leg (Cornell box, 1 spp per frame)")
-    ap.add_argument("--nrc-width", type=int, default=480)
-    ap.add_argument("--nrc-height", type=int, default=270)
+    ap.add_argument("--nrc-width", type=int, default=1920)
+    ap.add_argument("--nrc-height", type=int, default=1080)
     ap.add_argument("--nrc-only", action="store_true",
                     help="run only the radiance-cache equal-time leg (its own JSON line)")
     ap.add_argument("--nrc-frames", type=int, default=256)
     ap.add_argument("--nrc-ref-spp", type=int, default=16384)
-    ap.add_argument("--nrc-scene", choices=("cornell", "terrain"), default="cornell")
-    ap.add_argument("--nrc-variants", default="0.01:1:1:3",
+    ap.add_argument("--nrc-scene", choices=("cornell", "terrain"), default="terrain")
+    ap.add_argument("--nrc-variants", default="0.01:0:1:3:0;1.0:1:2:4:1",
                     help="cache variants 'spread:neeAtEnd:minDepth:trainShift[:lazyAdam];...'")
     ap.add_argument("--workload", choices=("frame", "batch"), default="frame",
                     help="frame: the 1080p headline frame; batch: many scenes in one renderer")

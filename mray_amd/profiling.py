@@ -17,7 +17,7 @@ import os
 import re
 import sys
 
-TRACE_KERNEL = "KTrace<4, false>"
+TRACE_KERNEL = re.compile(r"^KTrace<\d+, false[,>]")   # closest hit, not the counting variant
 COPY_KERNEL = "neg"
 
 
@@ -80,8 +80,8 @@ def cmd_traffic(fetch_dir, write_dir, bench_log, copy_mib):
         raise SystemExit(f"expected one copy-probe dispatch, found {len(cf)} / {len(cw)}")
     fscale = copy_bytes / (cf[0] * 1024.0)
     wscale = copy_bytes / (cw[0] * 1024.0)
-    tf = _pick(fetch, lambda n: n == TRACE_KERNEL)
-    tw = _pick(write, lambda n: n == TRACE_KERNEL)
+    tf = _pick(fetch, lambda n: TRACE_KERNEL.match(n) is not None)
+    tw = _pick(write, lambda n: TRACE_KERNEL.match(n) is not None)
     rays = None
     for line in open(bench_log):
         m = re.search(r"step 1/\d+: .* (\d+) extension rays", line)
