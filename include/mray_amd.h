@@ -345,6 +345,46 @@ int mray_batch_autotune(const MRayBatchScene* scene4, const MRayBatchScene* scen
 int mray_batch_set_trace_tpb(MRayRenderer* r, uint32_t tpb);
 
 /* ---------------------------------------------------------------------- */
+/* Instanced scenes: meshes placed many times under a tree over instances  */
+/* (the reference's two-level accelerator: a base accelerator over surface */
+/* instances, AcceleratorLBVH.cu:437-495, above one tree per mesh,          */
+/* AcceleratorWork.kt.h:177-244).  Transforms are similarities (rotation,   */
+/* uniform scale, translation).  A mesh's emissive triangles become one     */
+/* light per instance; its materials are shared by its instances.          */
+/* ---------------------------------------------------------------------- */
+typedef struct MRayInstScene MRayInstScene;
+
+typedef struct MRayInstance
+{
+    uint32_t mesh;             /* index into the meshes                    */
+    float    rotation[4];      /* unit quaternion (w, x, y, z), object->world */
+    float    scale;            /* > 0                                      */
+    float    translation[3];
+} MRayInstance;
+
+typedef struct MRayInstEnv
+{
+    uint32_t   boundaryType;   /* MRAY_BOUNDARY_*                          */
+    float      boundaryRadiance[3];
+    float      sceneDiameter;  /* 0: the world bounds' XZ span             */
+    MRayCamera camera;
+} MRayInstEnv;
+
+/* meshes: accelerators of the meshes in their own space (4-wide trees);
+ * they may be destroyed afterwards.                                        */
+int mray_instanced_scene_create(const MRayAccel* const* meshes, uint32_t meshCount,
+                                const MRayInstance* hInstances, uint32_t instanceCount,
+                                const MRayInstEnv* hEnv, MRayInstScene** out);
+int mray_instanced_scene_destroy(MRayInstScene* scene);
+/* instances, triangles summed over instances, lights, instance-tree nodes */
+int mray_instanced_scene_info(const MRayInstScene* scene, uint32_t* hInstances,
+                              uint64_t* hEffectiveTriangles, uint32_t* hLights,
+                              uint32_t* hTlasNodes);
+/* A renderer over an instanced scene: the camera of hEnv; no material queues. */
+int mray_renderer_create_instanced(const MRayInstScene* scene, const MRayRenderParams* hParams,
+                                   MRayRenderer** out);
+
+/* ---------------------------------------------------------------------- */
 /* Spectral tables (SpectraLUTGen capability, SpectrumContext.cu:298-472):  */
 /* RGB->spectrum coefficients lut [3][N][N][N][4] (c0, c1, c2, 0 in nm,     */
 /* x fastest), observer cmf [471][3] and illuminant [471] at 360..830 nm,   */

@@ -130,6 +130,59 @@ def batch_views(specs):
     return arr
 
 
+class MRayInstance(C.Structure):
+    _fields_ = [
+        ("mesh", C.c_uint32),
+        ("rotation", C.c_float * 4),
+        ("scale", C.c_float),
+        ("translation", C.c_float * 3),
+    ]
+
+
+class MRayInstEnv(C.Structure):
+    _fields_ = [
+        ("boundaryType", C.c_uint32),
+        ("boundaryRadiance", C.c_float * 3),
+        ("sceneDiameter", C.c_float),
+        ("camera", MRayCamera),
+    ]
+
+
+def camera_struct(cam, out=None):
+    c = out if out is not None else MRayCamera()
+    for k in range(3):
+        c.position[k] = cam["position"][k]
+        c.gaze[k] = cam["gaze"][k]
+        c.up[k] = cam["up"][k]
+    for k in range(2):
+        c.fov[k] = cam["fov"][k]
+        c.nearFar[k] = cam["nearFar"][k]
+    return c
+
+
+def instances_array(instances):
+    """MRayInstance array from dicts {mesh, rotation (w,x,y,z), scale, translation}."""
+    arr = (MRayInstance * len(instances))()
+    for k, d in enumerate(instances):
+        arr[k].mesh = d["mesh"]
+        for i in range(4):
+            arr[k].rotation[i] = d["rotation"][i]
+        arr[k].scale = d["scale"]
+        for i in range(3):
+            arr[k].translation[i] = d["translation"][i]
+    return arr
+
+
+def inst_env(env):
+    e = MRayInstEnv()
+    e.boundaryType = env["boundary_type"]
+    for k in range(3):
+        e.boundaryRadiance[k] = env["boundary_radiance"][k]
+    e.sceneDiameter = env.get("scene_diameter") or 0.0
+    camera_struct(env["camera"], e.camera)
+    return e
+
+
 class MRayRenderStats(C.Structure):
     _fields_ = [
         ("deadPaths", C.c_uint64),

@@ -116,7 +116,8 @@ int BuildBatchScene(MRayBatchScene* b, const MRayAccel* const* accels, uint32_t 
     SoupTri* dSoup = nullptr;
     float* dPos = nullptr;
     float4* dTbn = nullptr;
-    uint32_t *dIdx = nullptr, *dKey = nullptr, *dLightTri = nullptr;
+    uint32_t *dIdx = nullptr, *dKey = nullptr;
+    float* dLightPos = nullptr;
     MRayMaterial* dMats = nullptr;
     float4* dLightRad = nullptr;
     int rv;
@@ -128,7 +129,7 @@ int BuildBatchScene(MRayBatchScene* b, const MRayAccel* const* accels, uint32_t 
     if((rv = A((void**)&dIdx, tris * 3 * sizeof(uint32_t))) != MRAY_OK) return rv;
     if((rv = A((void**)&dKey, tris * sizeof(uint32_t))) != MRAY_OK) return rv;
     if((rv = A((void**)&dMats, mats * sizeof(MRayMaterial))) != MRAY_OK) return rv;
-    if((rv = A((void**)&dLightTri, lights * sizeof(uint32_t))) != MRAY_OK) return rv;
+    if((rv = A((void**)&dLightPos, lights * 9 * sizeof(float))) != MRAY_OK) return rv;
     if((rv = A((void**)&dLightRad, lights * sizeof(float4))) != MRAY_OK) return rv;
     const hipMemcpyKind D2D = hipMemcpyDeviceToDevice;
     for(uint32_t k = 0; k < count; k++)
@@ -157,8 +158,8 @@ int BuildBatchScene(MRayBatchScene* b, const MRayAccel* const* accels, uint32_t 
         MRAY_HIP_CHECK(hipMemcpy(dMats + p.matOff, a->dMats, size_t(M) * sizeof(MRayMaterial), D2D));
         if(L)
         {
-            MRAY_HIP_CHECK(hipMemcpy(dLightTri + p.lightOff, a->dLightTri, size_t(L) * sizeof(uint32_t), D2D));
-            KAddU32<<<DivUp(L, 256), 256>>>(dLightTri + p.lightOff, L, p.triOff);
+            MRAY_HIP_CHECK(hipMemcpy(dLightPos + size_t(p.lightOff) * 9, a->dLightPos,
+                                     size_t(L) * 9 * sizeof(float), D2D));
             MRAY_HIP_CHECK(hipMemcpy(dLightRad + p.lightOff, a->dLightRad, size_t(L) * sizeof(float4), D2D));
         }
         MRAY_HIP_CHECK(hipGetLastError());
@@ -170,7 +171,7 @@ int BuildBatchScene(MRayBatchScene* b, const MRayAccel* const* accels, uint32_t 
     v.idx = dIdx;
     v.surfKey = dKey;
     v.mats = dMats;
-    v.lightTri = dLightTri;
+    v.lightPos = dLightPos;
     v.lightRad = dLightRad;
     v.lightCount = 0;           // per view
     v.boundaryType = MRAY_BOUNDARY_NULL;

@@ -71,6 +71,7 @@ struct SlotSoA
     float4* wpdf;
     float2* spread;    // radiance cache: (a0, sum of sqrt spreads) of the path
     uint32_t* view;    // batch: viewport of the path
+    uint32_t* inst;    // instanced scenes: instance of the hit
 };
 
 // Radiance-cache state of the renderer (RGB mode).  One path slot in
@@ -249,6 +250,7 @@ struct RenderConst
     float* film;
     NrcRenderState nrc;
     // batch kernels only
+    const InstGPU* inst;              // instanced scenes: instance table
     const ViewGPU* views;             // ordered by pathBegin
     const unsigned long long* viewBegin;  // [viewCount + 1] path index ranges
     uint32_t viewCount;
@@ -278,7 +280,7 @@ __device__ __forceinline__ SceneView SlotScene(const RenderConst& rc, const Slot
     if constexpr(BATCH)
     {
         const ViewGPU& v = rc.views[S.view[i]];
-        s.lightTri += v.lightOff;
+        s.lightPos += size_t(v.lightOff) * 9u;
         s.lightRad += v.lightOff;
         s.lightCount = v.lightCount;
         s.boundaryType = v.boundaryType;
@@ -629,7 +631,7 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
 // ---------------------------------------------------------------------------
 // Closest hit
 // ---------------------------------------------------------------------------
-template<int W, bool COUNT, bool BATCH, int TPB = kTraceTPB>
+template<int W, bool COUNT, bool BATCH, int TPB = kTraceTPB, bool INST = false>
 __global__ void __launch_bounds__(TPB)
 KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __restrict__ sched,
        const ViewGPU* __restrict__ views, uint32_t* __restrict__ ctr)
@@ -643,9 +645,20 @@ KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __re
     {
         const uint32_t i = active[k];
         const float4 o = S.rayO[i], d = S.rayD[i];
-        const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
-        HitRec h = TraceW<W, false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                           o.w, d.w, root, &nodes, &tris);
+        HitRec h;
+        if constexpr(INST)
+        {
+            uint32_t instId;
+            h = TraceInst<false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                        o.w, d.w, instId, &nodes, &tris);
+            S.inst[i] = instId;
+        }
+        else
+        {
+            const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
+            h = TraceW<W, false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                        o.w, d.w, root, &nodes, &tris);
+        }
         S.hit[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.a),
                               __float_as_uint(h.b));
     });
@@ -703,7 +716,7 @@ __device__ __forceinline__ V4 TriLightEmitViaPoint(const RenderConst& rc, const 
                                                    uint32_t li, V3 wO, V3 point, V4 waves)
 {
     V3 p0, p1, p2;
-    LoadTri(s, s.lightTri[li], p0, p1, p2);
+    LoadLightTri(s, li, p0, p1, p2);
     float ha, hb;
     ProjectedHit(p0, p1, p2, point, ha, hb);
     if(!BaryInRange(ha, hb)) return Splat4(0.0f);
@@ -731,7 +744,7 @@ __device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const 
     if(li < lightCount - 1u)
     {
         V3 p0, p1, p2;
-        LoadTri(s, s.lightTri[li], p0, p1, p2);
+        LoadLightTri(s, li, p0, p1, p2);
         float r1 = sqrtf(rn[0]);
         float r2 = rn[1];
         float a = 1 - r1;
@@ -780,7 +793,7 @@ __device__ __forceinline__ float PdfLightTri(const SceneView& s, uint32_t li, fl
 {
     float selectionPDF = 1.0f / float(s.lightCount + 1u);
     V3 p0, p1, p2;
-    LoadTri(s, s.lightTri[li], p0, p1, p2);
+    LoadLightTri(s, li, p0, p1, p2);
     float pdf;
     if(!BaryInRange(ha, hb)) pdf = 0.0f;
     else
@@ -819,7 +832,7 @@ struct ShadeOut
     float rec[kNrcRec];
 };
 
-template<bool SPECTRAL, int KEY, bool BATCH>
+template<bool SPECTRAL, int KEY, bool BATCH, bool INST = false>
 __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& rc, uint32_t i,
                                           ShadeOut& out)
 {
@@ -842,10 +855,12 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
         waves = {w4.x, w4.y, w4.z, w4.w};
     }
     uint32_t li = kInvalid;
+    const uint32_t instId = (INST && prim != kInvalid) ? S.inst[i] : 0u;
     if(prim != kInvalid)
     {
         uint32_t key = s.surfKey[prim];
-        if(key & MRAY_LIGHT_FLAG) li = key & ~MRAY_LIGHT_FLAG;
+        // an instance's copy of a mesh light sits at its own light offset
+        if(key & MRAY_LIGHT_FLAG) li = (key & ~MRAY_LIGHT_FLAG) + (INST ? rc.inst[instId].lightOff : 0u);
     }
     bool isBoundary = (prim == kInvalid);
     if(KEY <= 0 && (isBoundary || li != kInvalid))
@@ -863,7 +878,7 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
         else
         {
             V3 p0, p1, p2;
-            LoadTri(s, prim, p0, p1, p2);
+            LoadLightTri(s, li, p0, p1, p2);   // the hit triangle, in world space
             V3 nrm = TriNormal(p0, p1, p2);
             float NdL = Dot(nrm, wO);
             float4 lr = s.lightRad[li];
@@ -899,7 +914,8 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
     // Surface hit
     uint32_t st = S.rng[i];
     const MRayMaterial& M = s.mats[s.surfKey[prim]];
-    Surface sf = GenSurface(s, prim, ha, hb, rdir);
+    Surface sf = INST ? GenSurfaceInst(s, prim, ha, hb, rdir, rc.inst[instId])
+                      : GenSurface(s, prim, ha, hb, rdir);
     MatInst mat;
     mat.type = (KEY > 0) ? uint32_t(KEY - 1) : M.type;
     mat.albedo = ConvAlbedo<SPECTRAL>(rc, V3{M.albedo[0], M.albedo[1], M.albedo[2]}, waves);
@@ -1105,7 +1121,7 @@ __device__ __forceinline__ void ShadeAppend(const ShadeOut& o, uint32_t i, uint3
 // Generic shading over the whole active list: static, one 64-item chunk per
 // wave step (shading is short per item, a fetch atomic would cost as much as
 // the work it hands out); the chunk's shard picks the shadow-queue shard.
-template<bool SPECTRAL, bool BATCH>
+template<bool SPECTRAL, bool BATCH, bool INST = false>
 __global__ void __launch_bounds__(kSlotTPB)
 KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ, uint32_t* __restrict__ ctr)
@@ -1124,7 +1140,7 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
         if(k < n)
         {
             i = active[k];
-            ShadeSlot<SPECTRAL, -1, BATCH>(S, rc, i, o);
+            ShadeSlot<SPECTRAL, -1, BATCH, INST>(S, rc, i, o);
         }
         ShadeAppend(o, i, ShardOfChunk(base >> 6, n), cap, ctr, shadowQ, sched, rc.nrc, queries);
     }
@@ -1216,7 +1232,7 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues,
 // ---------------------------------------------------------------------------
 // KCAccumulateShadowRaysPT: a shadow ray that reaches its light point adds
 // its precomputed NEE radiance to the path.
-template<int W, bool BATCH, int TPB = kTraceTPB>
+template<int W, bool BATCH, int TPB = kTraceTPB, bool INST = false>
 __global__ void __launch_bounds__(TPB)
 KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __restrict__ sched,
         const ViewGPU* __restrict__ views, NrcRenderState nrc, uint32_t* __restrict__ ctr,
@@ -1234,9 +1250,19 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
         uint32_t sh;
         const uint32_t i = shadowQ[ShardItem(g, k, cap, sh)];
         const float4 o = S.sO[i], d = S.sD[i];
-        const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
-        HitRec h = TraceW<W, true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                          o.w, d.w, root);
+        HitRec h;
+        if constexpr(INST)
+        {
+            uint32_t instId;
+            h = TraceInst<true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                       o.w, d.w, instId);
+        }
+        else
+        {
+            const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
+            h = TraceW<W, true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                       o.w, d.w, root);
+        }
         if(h.prim == kInvalid)
         {
             float4 a = S.rad[i], b = S.sRad[i];
@@ -1366,6 +1392,8 @@ struct MRayRenderer
     unsigned long long frameLimit = 0;
     // traversal context: the accelerator's, or the merged batch scene's
     TraceCtx ctx{};
+    // instanced scene (mray_renderer_create_instanced)
+    const MRayInstScene* iscene = nullptr;
     // batch (mray_batch_create)
     const MRayBatchScene* bscene = nullptr;
     std::vector<ViewGPU> hViews;
@@ -1433,11 +1461,17 @@ static void ShadeQueueOccupancy(MRayRenderer* r, F&& resident)
 }
 
 // Closest-hit launch; a batch may use 128-thread traversal workgroups.
-template<bool COUNT, bool BATCH>
+template<bool COUNT, bool BATCH, bool INST>
 static void LaunchTrace(MRayRenderer* r, hipStream_t s)
 {
     const TraceCtx& ctx = r->ctx;
     const uint32_t nb = r->traceBlocks;
+    if constexpr(INST)
+    {
+        KTrace<4, COUNT, false, kTraceTPB, true><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched,
+                                                                         r->dViews, r->dCtr);
+        return;
+    }
     if constexpr(BATCH)
     {
         if(r->traceTPB == 128)
@@ -1451,11 +1485,17 @@ static void LaunchTrace(MRayRenderer* r, hipStream_t s)
     else KTrace<4, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
 }
 
-template<bool BATCH>
+template<bool BATCH, bool INST>
 static void LaunchShadow(MRayRenderer* r, hipStream_t s)
 {
     const TraceCtx& ctx = r->ctx;
     const uint32_t nb = r->shadowBlocks;
+    if constexpr(INST)
+    {
+        KShadow<4, false, kTraceTPB, true><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched,
+                                                                   r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
+        return;
+    }
     if constexpr(BATCH)
     {
         if(r->traceTPB == 128)
@@ -1469,7 +1509,7 @@ static void LaunchShadow(MRayRenderer* r, hipStream_t s)
     else KShadow<4, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
 }
 
-template<bool BATCH>
+template<bool BATCH, bool INST>
 static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
 {
     const uint32_t N = r->rc.N;
@@ -1498,8 +1538,8 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
         KRefillApply<false, BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
                                                                    r->dActive);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    if(r->counting) LaunchTrace<true, BATCH>(r, s);
-    else LaunchTrace<false, BATCH>(r, s);
+    if(r->counting) LaunchTrace<true, BATCH, INST>(r, s);
+    else LaunchTrace<false, BATCH, INST>(r, s);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     if(r->dQueues)
     {
@@ -1510,13 +1550,13 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
         else LaunchShadeQueues<false, BATCH>(r, s);
     }
     else if(r->rc.spectral)
-        KShade<true, BATCH><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
-                                                                r->dShadowQ, r->dCtr);
+        KShade<true, BATCH, INST><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
+                                                                      r->dShadowQ, r->dCtr);
     else
-        KShade<false, BATCH><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
-                                                                 r->dShadowQ, r->dCtr);
+        KShade<false, BATCH, INST><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
+                                                                       r->dShadowQ, r->dCtr);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    LaunchShadow<BATCH>(r, s);
+    LaunchShadow<BATCH, INST>(r, s);
     if(!BATCH && r->nrc)
     {
         // radiance-cache answers for the paths that terminated this bounce
@@ -1532,7 +1572,9 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
 
 static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
 {
-    return r->bscene ? EnqueueIterationT<true>(r, s) : EnqueueIterationT<false>(r, s);
+    if(r->bscene) return EnqueueIterationT<true, false>(r, s);
+    if(r->iscene) return EnqueueIterationT<false, true>(r, s);
+    return EnqueueIterationT<false, false>(r, s);
 }
 
 __global__ void KSetLimit(Sched* __restrict__ sched, unsigned long long limit)
@@ -1627,6 +1669,7 @@ static int InitSlots(MRayRenderer* r, size_t filmPixels)
         MRAY_TRY(A((void**)&r->S.wpdf, size_t(N) * 16));
     }
     if(r->bscene) MRAY_TRY(A((void**)&r->S.view, size_t(N) * 4));
+    if(r->iscene) MRAY_TRY(A((void**)&r->S.inst, size_t(N) * 4));
     MRAY_TRY(A((void**)&r->rc.film, filmPixels * 4 * sizeof(float)));
     r->nBlocks = DivUp(N, kSlotTPB);
     MRAY_TRY(A((void**)&r->dBlockCnt, size_t(r->nBlocks) * sizeof(uint4)));
@@ -1679,7 +1722,12 @@ static int SizeLaunches(MRayRenderer* r)
     const uint32_t tpb = r->traceTPB;
     const uint32_t maxTraceBlocks = r->ctx.gridThreads / tpb;
     const bool w8 = r->ctx.width == 8;
-    if(!r->bscene)
+    if(r->iscene)
+    {
+        r->traceBlocks = uint32_t(resident(KTrace<4, false, false, kTraceTPB, true>, tpb, maxTraceBlocks));
+        r->shadowBlocks = uint32_t(resident(KShadow<4, false, kTraceTPB, true>, tpb, maxTraceBlocks));
+    }
+    else if(!r->bscene)
     {
         r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false, false>, tpb, maxTraceBlocks)
                                      : resident(KTrace<4, false, false>, tpb, maxTraceBlocks));
@@ -1713,7 +1761,10 @@ static int SizeLaunches(MRayRenderer* r)
         else if(p->spectral) ShadeQueueOccupancy<true, false>(r, resident);
         else ShadeQueueOccupancy<false, false>(r, resident);
     }
-    if(r->bscene)
+    if(r->iscene)
+        r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, false, true>, kSlotTPB, 1u << 20)
+                                              : resident(KShade<false, false, true>, kSlotTPB, 1u << 20));
+    else if(r->bscene)
         r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, true>, kSlotTPB, 1u << 20)
                                               : resident(KShade<false, true>, kSlotTPB, 1u << 20));
     else
@@ -1743,6 +1794,31 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
     r->rc.scene = a->view;
     float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
     BuildCamera(scene ? scene->camera : a->camera, p->width, p->height, fr, r->rc.cam);
+    r->limit = (unsigned long long)p->totalSPP * P;
+    MRAY_TRY(InitSlots(r, size_t(P)));
+    return ResetRenderer(r);
+}
+
+static int CreateInstanced(MRayRenderer* r, const MRayInstScene* sc, const MRayRenderParams* p)
+{
+    r->iscene = sc;
+    r->ctx = sc->ctx;
+    r->params = *p;
+    MRAY_HIP_CHECK(hipStreamCreate(&r->stream));
+    const uint64_t P = uint64_t(p->width) * p->height;
+    uint32_t N = p->maxRayCount ? p->maxRayCount : uint32_t(P);
+    if(N > P) N = uint32_t(P);
+    r->rc.N = N;
+    r->rc.P = uint32_t(P);
+    r->rc.W = p->width;
+    r->rc.rr0 = p->rrRange[0];
+    r->rc.rr1 = p->rrRange[1];
+    r->rc.mode = p->sampleMode;
+    r->rc.spectral = p->spectral;
+    r->rc.scene = sc->view;
+    r->rc.inst = sc->ctx.inst;
+    const float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
+    BuildCamera(sc->camera, p->width, p->height, fr, r->rc.cam);
     r->limit = (unsigned long long)p->totalSPP * P;
     MRAY_TRY(InitSlots(r, size_t(P)));
     return ResetRenderer(r);
@@ -1865,6 +1941,39 @@ int mray_renderer_create(const MRayAccel* accel, const MRaySceneDesc* hScene,
     MRAY_TRY(CheckSampling(p, "renderer_create"));
     MRayRenderer* r = new MRayRenderer();
     int rv = CreateRenderer(r, accel, hScene, hParams);
+    if(rv != MRAY_OK)
+    {
+        (void)hipDeviceSynchronize();
+        FreeRenderer(r);
+        return rv;
+    }
+    *out = r;
+    return MRAY_OK;
+}
+
+int mray_renderer_create_instanced(const MRayInstScene* scene, const MRayRenderParams* hParams,
+                                   MRayRenderer** out)
+{
+    if(!scene || !hParams || !out)
+    {
+        SetError("renderer_create_instanced: NULL argument");
+        return MRAY_ERR_INVALID;
+    }
+    *out = nullptr;
+    const MRayRenderParams& p = *hParams;
+    if(p.width == 0 || p.height == 0 || p.totalSPP == 0 || uint64_t(p.width) * p.height > (1ull << 31))
+    {
+        SetError("renderer_create_instanced: bad image size or totalSPP");
+        return MRAY_ERR_INVALID;
+    }
+    if(p.flags & MRAY_RENDER_MATERIAL_QUEUES)
+    {
+        SetError("renderer_create_instanced: material queues are not available for instanced scenes");
+        return MRAY_ERR_UNSUPPORTED;
+    }
+    MRAY_TRY(CheckSampling(p, "renderer_create_instanced"));
+    MRayRenderer* r = new MRayRenderer();
+    int rv = CreateInstanced(r, scene, hParams);
     if(rv != MRAY_OK)
     {
         (void)hipDeviceSynchronize();
@@ -2085,7 +2194,8 @@ int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderP
     MRAY_HIP_CHECK(hipMemset(nr.ringRec, 0, size_t(ring) * kNrcRec * 4));
     // scene box -> [0,1]^3 with one scale (keeps the aspect)
     float box[6];
-    MRAY_HIP_CHECK(hipMemcpy(box, r->accel->dSceneBox, sizeof(box), hipMemcpyDeviceToHost));
+    if(r->iscene) std::memcpy(box, r->iscene->box, sizeof(box));
+    else MRAY_HIP_CHECK(hipMemcpy(box, r->accel->dSceneBox, sizeof(box), hipMemcpyDeviceToHost));
     float ext = std::max({box[3] - box[0], box[4] - box[1], box[5] - box[2], 1e-6f});
     for(int k = 0; k < 3; k++) nr.boxMin[k] = box[k] - 1e-3f * ext;
     nr.boxInvExtent = 1.0f / (ext * 1.002f);

@@ -2,6 +2,8 @@
 // surface keys, emissive triangles, boundary light).
 #include "scene.h"
 
+#include <algorithm>
+
 #include <cstring>
 #include <vector>
 
@@ -27,6 +29,14 @@ int UploadSceneData(MRayAccel* a, const MRaySceneDesc* s)
     if(mats.empty()) mats.push_back(MRayMaterial{});
     std::vector<uint32_t> ltri(s->hLightTriangles, s->hLightTriangles + s->lightCount);
     if(ltri.empty()) ltri.push_back(0);
+    // light triangles' world-space vertices, read directly by light sampling
+    std::vector<float> lpos(size_t(std::max<uint32_t>(1u, s->lightCount)) * 9, 0.0f);
+    for(uint32_t l = 0; l < s->lightCount; l++)
+        for(int v = 0; v < 3; v++)
+        {
+            const uint32_t vi = s->hIndices[3 * size_t(s->hLightTriangles[l]) + v];
+            for(int c = 0; c < 3; c++) lpos[size_t(l) * 9 + v * 3 + c] = s->hPositions[3 * size_t(vi) + c];
+        }
     std::vector<float4> lrad(std::max<uint32_t>(1u, s->lightCount));
     for(uint32_t l = 0; l < s->lightCount; l++)
     {
@@ -49,6 +59,8 @@ int UploadSceneData(MRayAccel* a, const MRaySceneDesc* s)
     if((r = up((void**)&a->dLightTri, ltri.data(), ltri.size() * 4)) != MRAY_OK) return r;
     if((r = up((void**)&a->dLightRad, lrad.data(), lrad.size() * sizeof(float4))) != MRAY_OK)
         return r;
+    if((r = up((void**)&a->dLightPos, lpos.data(), lpos.size() * sizeof(float))) != MRAY_OK)
+        return r;
     a->hMats = mats;
     a->camera = s->camera;
     SceneView& v = a->view;
@@ -57,7 +69,7 @@ int UploadSceneData(MRayAccel* a, const MRaySceneDesc* s)
     v.idx = a->dIdx;
     v.surfKey = a->dSurfKey;
     v.mats = a->dMats;
-    v.lightTri = a->dLightTri;
+    v.lightPos = a->dLightPos;
     v.lightRad = a->dLightRad;
     v.lightCount = s->lightCount;
     v.boundaryType = s->boundaryType;

@@ -26,7 +26,7 @@ struct SceneView
     const uint32_t* __restrict__ idx;     // [T][3]
     const uint32_t* __restrict__ surfKey; // [T]
     const MRayMaterial* __restrict__ mats;
-    const uint32_t* __restrict__ lightTri;
+    const float* __restrict__ lightPos;   // [lightCount][9] world-space vertices
     const float4* __restrict__ lightRad;  // rgb, w = two-sided flag
     uint32_t lightCount;                  // emissive triangles (boundary excluded)
     uint32_t boundaryType;
@@ -65,6 +65,7 @@ struct MRayAccel
     MRayMaterial* dMats = nullptr;
     uint32_t* dLightTri = nullptr;
     float4* dLightRad = nullptr;
+    float* dLightPos = nullptr;
     mray::SceneView view{};
     MRayCamera camera{};
     std::vector<MRayMaterial> hMats;
@@ -73,7 +74,7 @@ struct MRayAccel
 // Several scenes merged for one batch of viewports: every scene's triangle
 // soup, traversal tree, vertices and shading tables concatenated on the device
 // with their indices relocated (tree children, leaf ranges, soup triangle
-// ids, vertex indices, material keys, light triangles), so one traversal
+// ids, vertex indices, material keys; light vertices are world-space), so one traversal
 // context and one SceneView serve all of them; a scene is entered at its
 // own root node.
 struct MRayBatchScene
@@ -97,7 +98,25 @@ struct MRayBatchScene
     mray::TraceCtx ctx{};
 };
 
+// Instances of meshes (mray_instanced_scene_create): the meshes merged as
+// for a batch (each keeps its own tree and root), a 4-wide tree over the
+// instances' world boxes, the instance table and the world-space light
+// table (every emissive triangle of every instance).
+struct MRayInstScene
+{
+    MRayBatchScene meshes;
+    mray::DeviceArena arena;
+    uint32_t instanceCount = 0;
+    uint32_t tlasNodes = 0, tlasDepth = 0;
+    uint64_t effectiveTriangles = 0;
+    float box[6] = {0, 0, 0, 0, 0, 0};   // world bounds
+    MRayCamera camera{};
+    mray::SceneView view{};
+    mray::TraceCtx ctx{};
+};
+
 namespace mray
 {
 int UploadSceneData(MRayAccel* a, const MRaySceneDesc* s);
+int BuildBatchScene(MRayBatchScene* b, const MRayAccel* const* accels, uint32_t count);
 }

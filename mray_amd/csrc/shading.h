@@ -311,6 +311,15 @@ __device__ __forceinline__ float CauchyIoR(float wavelength, const float* c)
 // ---------------------------------------------------------------------------
 // Triangle helpers (Core/ShapeFunctions.h, PrimitiveDefaultTriangle.hpp)
 // ---------------------------------------------------------------------------
+// Light triangle li's world-space vertices.
+__device__ __forceinline__ void LoadLightTri(const SceneView& s, uint32_t li, V3& p0, V3& p1, V3& p2)
+{
+    const float* v = s.lightPos + size_t(li) * 9u;
+    p0 = {v[0], v[1], v[2]};
+    p1 = {v[3], v[4], v[5]};
+    p2 = {v[6], v[7], v[8]};
+}
+
 __device__ __forceinline__ void LoadTri(const SceneView& s, uint32_t t, V3& p0, V3& p1, V3& p2)
 {
     uint32_t i0 = s.idx[3 * t], i1 = s.idx[3 * t + 1], i2 = s.idx[3 * t + 2];
@@ -348,9 +357,9 @@ struct Surface
 };
 
 // Triangle::GenerateSurface(DefaultSurface) (PrimitiveDefaultTriangle.hpp:443-630),
-// without the ray-cone / texture-derivative terms (no texture is sampled).
-__device__ __forceinline__ Surface GenSurface(const SceneView& s, uint32_t tri, float a, float b,
-                                              V3 rayDir)
+// without the ray-cone / texture-derivative terms (no texture is sampled):
+// position, geometric normal and interpolated TBN in the mesh's own space.
+__device__ __forceinline__ Surface GenSurfaceRaw(const SceneView& s, uint32_t tri, float a, float b)
 {
     uint32_t i0 = s.idx[3 * tri], i1 = s.idx[3 * tri + 1], i2 = s.idx[3 * tri + 2];
     V3 p0 = {s.pos[3 * i0], s.pos[3 * i0 + 1], s.pos[3 * i0 + 2]};
@@ -359,21 +368,47 @@ __device__ __forceinline__ Surface GenSurface(const SceneView& s, uint32_t tri, 
     float c = 1.0f - a - b;
     Surface r;
     r.position = (p0 * a + p1 * b) + p2 * c;
-    V3 geoN = TriNormal(p0, p1, p2);
+    r.geoNormal = TriNormal(p0, p1, p2);
     float4 f0 = s.tbn[i0], f1 = s.tbn[i1], f2 = s.tbn[i2];
     Q4 q0 = {f0.x, f0.y, f0.z, f0.w}, q1 = {f1.x, f1.y, f1.z, f1.w}, q2 = {f2.x, f2.y, f2.z, f2.w};
-    Q4 tbn = QNormalize(QBarySLerp(q0, q1, q2, a, b));
-    float dDotN = Dot(geoN, Normalize(rayDir));
+    r.tbn = QNormalize(QBarySLerp(q0, q1, q2, a, b));
+    r.backSide = false;
+    return r;
+}
+
+// Back-side hits flip the geometric normal and the tangent frame.
+__device__ __forceinline__ Surface OrientSurface(Surface r, V3 rayDir)
+{
+    float dDotN = Dot(r.geoNormal, Normalize(rayDir));
     bool backSide = dDotN > 0.0f;
     if(backSide)
     {
-        geoN = -geoN;
-        tbn = QMul(Q4{0, 1, 0, 0}, tbn);
+        r.geoNormal = -r.geoNormal;
+        r.tbn = QMul(Q4{0, 1, 0, 0}, r.tbn);
     }
-    r.geoNormal = geoN;
-    r.tbn = tbn;
     r.backSide = backSide;
     return r;
+}
+
+__device__ __forceinline__ Surface GenSurface(const SceneView& s, uint32_t tri, float a, float b,
+                                              V3 rayDir)
+{
+    return OrientSurface(GenSurfaceRaw(s, tri, a, b), rayDir);
+}
+
+// The surface of an instanced mesh triangle in world space: position by
+// the similarity transform, the normal and the tangent frame by its
+// rotation q (world -> tangent = mesh tangent frame after q^-1).
+__device__ __forceinline__ Surface GenSurfaceInst(const SceneView& s, uint32_t tri, float a, float b,
+                                                  V3 rayDir, const InstGPU& I)
+{
+    Surface r = GenSurfaceRaw(s, tri, a, b);
+    const Q4 q = {I.rot.x, I.rot.y, I.rot.z, I.rot.w};
+    const V3 t = {I.transScale.x, I.transScale.y, I.transScale.z};
+    r.position = QRotate(q, r.position) * I.transScale.w + t;
+    r.geoNormal = QRotate(q, r.geoNormal);
+    r.tbn = QMul(r.tbn, Q4{q.w, -q.x, -q.y, -q.z});
+    return OrientSurface(r, rayDir);
 }
 
 // ---------------------------------------------------------------------------

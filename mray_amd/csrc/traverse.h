@@ -19,6 +19,21 @@ namespace mray
 static constexpr uint32_t kTraceTPB = 256;
 static constexpr uint32_t kLdsStack = 12;
 
+// One instance of a mesh in a two-level scene: a similarity transform
+// (rotation quaternion, uniform scale, translation) object -> world, its
+// inverse as three rows, the mesh tree's root and the instance's first
+// light in the world light table.
+struct InstGPU
+{
+    float4 rot;          // (w, x, y, z)
+    float4 transScale;   // translation xyz, uniform scale w
+    float4 inv[3];       // world -> object, rows (3x3 | translation)
+    uint32_t root;
+    uint32_t lightOff;
+    uint32_t mesh;
+    uint32_t pad;
+};
+
 struct TraceCtx
 {
     const BNode4* __restrict__ nodes;   // width 4
@@ -29,6 +44,12 @@ struct TraceCtx
     uint32_t gridThreads;
     uint32_t maxVisits;              // node count: no correct traversal visits more
     uint32_t width;                  // 4 or 8
+    // two-level scenes only: instance tree over world boxes, its leaf
+    // positions' instances, the instance table
+    const BNode4* __restrict__ tlas;
+    const uint32_t* __restrict__ tlasOrder;
+    const InstGPU* __restrict__ inst;
+    uint32_t tlasVisits;
 };
 
 struct HitRec
@@ -197,18 +218,19 @@ __device__ __forceinline__ bool TestLeafRange(const SoupTri* __restrict__ soup, 
 
 // Closest hit; ties in t go to the smaller triangle index.
 // `lstack` = this block's LDS stack base; `gtid` = grid-wide lane id;
-// `root` = the tree's root node (0, or a scene's first node in a batch).
+// `root` = the tree's root node (0, or a scene's first node in a batch);
+// `sp0` = the stack's floor (entries below belong to an enclosing walk).
 template<int W, bool ANY_HIT, bool COUNT>
 __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uint32_t gtid,
                                          V3 o, V3 d, float tMin, float tMax, uint32_t root,
                                          uint32_t* nodeVisits = nullptr,
-                                         uint32_t* triTests = nullptr)
+                                         uint32_t* triTests = nullptr, uint32_t sp0 = 0)
 {
     HitRec best{kInvalid, INFINITY, 0.0f, 0.0f};
     RayPre r = MakeRayPre(o, d);
     const uint32_t tid = threadIdx.x;
     const uint32_t B = blockDim.x;
-    uint32_t sp = 0;
+    uint32_t sp = sp0;
     uint32_t node = root;
     const uint32_t spMax = kLdsStack + ctx.gstackDepth;
     const float kGrow = 1.0000005f;
@@ -289,12 +311,122 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
                 sp++;
             }
         }
-        while(next == kInvalid && sp > 0)
+        while(next == kInvalid && sp > sp0)
         {
             sp--;
             uint2 v = (sp < kLdsStack) ? lstack[sp * B + tid]
                                        : ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
             // drop subtrees that start beyond the closest hit so far
+            if(!ANY_HIT && best.prim != kInvalid && __uint_as_float(v.y) > best.t * kGrow) continue;
+            next = v.x;
+        }
+        if(next == kInvalid) break;
+        node = next;
+    }
+    if(ANY_HIT) best.prim = kInvalid;
+    return best;
+}
+
+// Two-level closest / any hit (the reference's instance tree then mesh tree,
+// AcceleratorLBVH.cu:437-495 and AcceleratorWork.kt.h:177-244): the
+// 4-wide instance tree is walked like a mesh tree; at an instance leaf the
+// ray goes to object space (the transform is affine, so t is unchanged)
+// and the mesh tree is walked on top of the same stack.  instOut names the
+// hit's instance.
+template<bool ANY_HIT, bool COUNT>
+__device__ __forceinline__ HitRec TraceInst(const TraceCtx& ctx, uint2* lstack, uint32_t gtid,
+                                            V3 o, V3 d, float tMin, float tMax, uint32_t& instOut,
+                                            uint32_t* nodeVisits = nullptr,
+                                            uint32_t* triTests = nullptr)
+{
+    HitRec best{kInvalid, INFINITY, 0.0f, 0.0f};
+    instOut = kInvalid;
+    RayPre r = MakeRayPre(o, d);
+    const uint32_t tid = threadIdx.x;
+    const uint32_t B = blockDim.x;
+    uint32_t sp = 0;
+    uint32_t node = 0;
+    const uint32_t spMax = kLdsStack + ctx.gstackDepth;
+    const float kGrow = 1.0000005f;
+    for(uint32_t visits = 0; visits < ctx.tlasVisits; visits++)
+    {
+        if(COUNT) (*nodeVisits)++;
+        float curMax = (best.prim == kInvalid) ? tMax : best.t;
+        float tn[4];
+        uint4 ch;
+        SlabTest4(ctx.tlas + node, r, tMin, curMax, tn, ch);
+        uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
+        auto cs = [&](int i, int j)
+        {
+            if(tn[j] < tn[i])
+            {
+                float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;
+                uint32_t cc = c[i]; c[i] = c[j]; c[j] = cc;
+            }
+        };
+        if(!ANY_HIT) { cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2); }
+        uint32_t inner[4];
+        float innerT[4];
+        int innerCount = 0;
+        for(int k = 0; k < 4; k++)
+        {
+            if(tn[k] == INFINITY) continue;
+            if(c[k] & kLeafFlag)
+            {
+                const uint32_t first = c[k] & kLeafStartMask;
+                const uint32_t cnt = ((c[k] & ~kLeafFlag) >> kLeafCountShift) + 1u;
+                for(uint32_t j = first; j < first + cnt; j++)
+                {
+                    const float lim = (best.prim == kInvalid) ? tMax : best.t;
+                    if(!ANY_HIT && tn[k] > lim * kGrow) break;
+                    const uint32_t id = ctx.tlasOrder[j];
+                    const InstGPU& I = ctx.inst[id];
+                    const float4 a = I.inv[0], b = I.inv[1], e = I.inv[2];
+                    const V3 lo = {a.x * o.x + a.y * o.y + a.z * o.z + a.w,
+                                   b.x * o.x + b.y * o.y + b.z * o.z + b.w,
+                                   e.x * o.x + e.y * o.y + e.z * o.z + e.w};
+                    const V3 ld = {a.x * d.x + a.y * d.y + a.z * d.z,
+                                   b.x * d.x + b.y * d.y + b.z * d.z,
+                                   e.x * d.x + e.y * d.y + e.z * d.z};
+                    HitRec h = TraceW<4, ANY_HIT, COUNT>(ctx, lstack, gtid, lo, ld, tMin, lim, I.root,
+                                                         nodeVisits, triTests, sp);
+                    if(h.prim != kInvalid)
+                    {
+                        instOut = id;
+                        if(ANY_HIT)
+                        {
+                            best.prim = 0;
+                            return best;
+                        }
+                        best = h;     // within [tMin, lim): closer than any earlier hit
+                    }
+                }
+            }
+            else
+            {
+                inner[innerCount] = c[k];
+                innerT[innerCount] = tn[k];
+                innerCount++;
+            }
+        }
+        uint32_t next = kInvalid;
+        if(innerCount > 0)
+        {
+            next = inner[0];
+            for(int k = innerCount - 1; k >= 1; k--)
+            {
+                uint2 v = make_uint2(inner[k], __float_as_uint(innerT[k]));
+                if(sp >= spMax) break;   // unreachable: sized for both trees
+                if(sp < kLdsStack) lstack[sp * B + tid] = v;
+                else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
+                sp++;
+            }
+        }
+        while(next == kInvalid && sp > 0)
+        {
+            sp--;
+            uint2 v = (sp < kLdsStack) ? lstack[sp * B + tid]
+                                       : ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
             if(!ANY_HIT && best.prim != kInvalid && __uint_as_float(v.y) > best.t * kGrow) continue;
             next = v.x;
         }

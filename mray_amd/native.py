@@ -42,6 +42,8 @@ EXPORTS = [
     "mray_batch_set_trace_tpb",
     "mray_spectral_write_mrspectra", "mray_spectral_read_mrspectra", "mray_spectral_use_mrspectra",
     "mray_renderer_take_film", "mray_film_write_exr",
+    "mray_instanced_scene_create", "mray_instanced_scene_destroy", "mray_instanced_scene_info",
+    "mray_renderer_create_instanced",
 ]
 
 
@@ -119,6 +121,11 @@ def lib():
     L.mray_batch_set_trace_tpb.argtypes = [P, u32]
     L.mray_batch_autotune.argtypes = [P, P, P, u32, C.POINTER(abi.MRayRenderParams),
                                       C.POINTER(abi.MRayBatchTune), P, u32, C.POINTER(u32)]
+    L.mray_instanced_scene_create.argtypes = [P, u32, P, u32, C.POINTER(abi.MRayInstEnv), C.POINTER(P)]
+    L.mray_instanced_scene_destroy.argtypes = [P]
+    L.mray_instanced_scene_info.argtypes = [P, C.POINTER(u32), C.POINTER(C.c_uint64), C.POINTER(u32),
+                                            C.POINTER(u32)]
+    L.mray_renderer_create_instanced.argtypes = [P, C.POINTER(abi.MRayRenderParams), C.POINTER(P)]
     _lib = L
     return L
 
@@ -546,3 +553,49 @@ class BatchRenderer(Renderer):
 
     def set_trace_tpb(self, tpb):
         check(lib().mray_batch_set_trace_tpb(self.h, tpb))
+
+
+class InstancedScene:
+    """Meshes placed by instances under a tree over instances (MRayInstScene).
+    instances: dicts {mesh, rotation (w,x,y,z), scale, translation};
+    env: {boundary_type, boundary_radiance, scene_diameter, camera}."""
+
+    def __init__(self, mesh_accels, instances, env):
+        require_gpu()
+        arr = (C.c_void_p * len(mesh_accels))(*[a.h.value for a in mesh_accels])
+        inst = abi.instances_array(instances)
+        e = abi.inst_env(env)
+        h = C.c_void_p()
+        check(lib().mray_instanced_scene_create(arr, len(mesh_accels), inst, len(instances),
+                                                C.byref(e), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mray_instanced_scene_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        n, l, t = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        tri = C.c_uint64()
+        check(lib().mray_instanced_scene_info(self.h, C.byref(n), C.byref(tri), C.byref(l), C.byref(t)))
+        return {"instances": n.value, "effective_triangles": tri.value, "lights": l.value,
+                "tlas_nodes": t.value}
+
+
+class InstancedRenderer(Renderer):
+    """Wavefront path tracer over an InstancedScene."""
+
+    def __init__(self, iscene, params):
+        self.iscene = iscene
+        self.params = params
+        self.width, self.height = params.width, params.height
+        h = C.c_void_p()
+        check(lib().mray_renderer_create_instanced(iscene.h, C.byref(params), C.byref(h)))
+        self.h = h

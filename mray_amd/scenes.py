@@ -598,3 +598,164 @@ def to_mray_json(scene, path):
         f.write("// written by mray_amd.scenes.to_mray_json\n")
         json.dump(doc, f)
 
+
+
+# ---------------------------------------------------------------------------
+# Instanced scenes (mray_instanced_scene_create) and their flattening
+# ---------------------------------------------------------------------------
+def quat_mul(a, b):
+    """Hamilton product of (w, x, y, z) quaternions (broadcasting)."""
+    aw, ax, ay, az = np.moveaxis(np.asarray(a, np.float64), -1, 0)
+    bw, bx, by, bz = np.moveaxis(np.asarray(b, np.float64), -1, 0)
+    return np.stack([aw * bw - ax * bx - ay * by - az * bz,
+                     aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw], axis=-1)
+
+
+def quat_rotate(q, v):
+    """q v q* for a unit quaternion q and vectors v [..., 3]."""
+    q = np.asarray(q, np.float64)
+    v = np.asarray(v, np.float64)
+    u = q[1:]
+    t = 2.0 * np.cross(u, v)
+    return v + q[0] * t + np.cross(u, t)
+
+
+def _unit_quat(r):
+    r = np.asarray(r, np.float64)
+    return r / np.linalg.norm(r)
+
+
+def flatten_instances(meshes, instances, env):
+    """The world-space scene an instanced scene describes: every instance's
+    triangles transformed (positions by the similarity, tangent frames by
+    its rotation), materials of each mesh once, one light per instance of an
+    emissive triangle.  The oracle renders this."""
+    mat_off, mats = [], []
+    for m in meshes:
+        mat_off.append(len(mats))
+        mats += list(m.materials)
+    pos, tbn, idx, keys, cull, ltri, lrad, l2s = [], [], [], [], [], [], [], []
+    vbase = tbase = lbase = 0
+    for inst in instances:
+        m = meshes[inst["mesh"]]
+        q = _unit_quat(inst["rotation"])
+        s = float(inst["scale"])
+        t = np.asarray(inst["translation"], np.float64)
+        pos.append((quat_rotate(q, m.positions) * s + t).astype(np.float32))
+        qc = np.array([q[0], -q[1], -q[2], -q[3]])
+        tb = quat_mul(np.asarray(m.tbn, np.float64), qc)
+        tbn.append((tb / np.linalg.norm(tb, axis=1, keepdims=True)).astype(np.float32))
+        idx.append(np.asarray(m.indices, np.uint32) + np.uint32(vbase))
+        k = np.asarray(m.surface_keys, np.uint32)
+        is_light = (k & abi.LIGHT_FLAG) != 0
+        keys.append(np.where(is_light, abi.LIGHT_FLAG | ((k & ~np.uint32(abi.LIGHT_FLAG)) + np.uint32(lbase)),
+                             k + np.uint32(mat_off[inst["mesh"]])).astype(np.uint32))
+        cull.append(np.asarray(m.cull, np.uint8))
+        ltri.append(np.asarray(m.light_tris, np.uint32) + np.uint32(tbase))
+        lrad.append(np.asarray(m.light_radiance, np.float32).reshape(-1, 3))
+        l2s.append(np.asarray(m.light_two_sided, np.uint8))
+        vbase += len(m.positions)
+        tbase += m.triangle_count
+        lbase += len(m.light_tris)
+    return Scene(positions=np.concatenate(pos), tbn=np.concatenate(tbn), indices=np.concatenate(idx),
+                 surface_keys=np.concatenate(keys), cull=np.concatenate(cull), materials=mats,
+                 light_tris=np.concatenate(ltri), light_radiance=np.concatenate(lrad),
+                 light_two_sided=np.concatenate(l2s), boundary_type=env["boundary_type"],
+                 boundary_radiance=tuple(env["boundary_radiance"]),
+                 scene_diameter=env.get("scene_diameter"), camera=env["camera"], name="instances")
+
+
+def light_panel(count_quads, size, radiance, seed=0, span=1.0):
+    """A mesh of small downward-facing emissive quads (2 triangles each),
+    scattered over [-span, span]^2 at y = 0 (mt19937(seed))."""
+    rng = MT19937(seed)
+    b = _Builder()
+    light_tris = []
+    for k in range(count_quads):
+        cx, cz = rng.uniform(-span, span), rng.uniform(-span, span)
+        li = len(light_tris)
+        first = b.add_mesh(*_quad((cx, 0.0, cz), (size, 0, 0), (0, 0, size)),
+                           key=[abi.LIGHT_FLAG | li, abi.LIGHT_FLAG | (li + 1)], cull=0)
+        light_tris += [first, first + 1]
+    pos, tbn, idx, key, cull = b.finish()
+    nl = len(light_tris)
+    return Scene(positions=pos, tbn=tbn, indices=idx, surface_keys=key, cull=cull,
+                 materials=[{"type": abi.MAT_LAMBERT, "albedo": (0.5, 0.5, 0.5)}],
+                 light_tris=np.array(light_tris, dtype=np.uint32),
+                 light_radiance=np.tile(np.asarray(radiance, np.float32), (nl, 1)),
+                 light_two_sided=np.zeros(nl, dtype=np.uint8), name="light_panel",
+                 camera={"position": (0.0, -2.0, 0.0), "gaze": (0.0, 0.0, 0.0), "up": (0.0, 0.0, 1.0),
+                         "fov": (math.radians(60.0),) * 2, "nearFar": (0.01, 100.0)})
+
+
+def _random_rotation(rng, full=True):
+    """A seeded unit quaternion: uniform over rotations (full) or about +Y."""
+    if full:
+        u1, u2, u3 = rng.uniform(0, 1), rng.uniform(0, 1), rng.uniform(0, 1)
+        a, b = math.sqrt(1 - u1), math.sqrt(u1)
+        return (a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2),
+                b * math.sin(2 * math.pi * u3), b * math.cos(2 * math.pi * u3))
+    ang = rng.uniform(0, 2 * math.pi)
+    return (math.cos(ang / 2), 0.0, math.sin(ang / 2), 0.0)
+
+
+def instanced_test_scene(width=64, height=48, seed=3):
+    """Small instanced scene for parity tests: two terrain meshes placed 12
+    times with arbitrary rotations and scales, and an emissive panel mesh
+    placed twice (its lights exist once per instance)."""
+    meshes = [terrain_spheres(grid=20, spheres=4, width=width, height=height, light_grid=0, seed=1),
+              terrain_spheres(grid=16, spheres=6, width=width, height=height, light_grid=0, seed=2),
+              light_panel(3, 0.15, (9.0, 7.0, 5.0), seed=4, span=0.5)]
+    rng = MT19937(seed)
+    instances = []
+    for k in range(12):
+        instances.append({"mesh": k % 2, "rotation": _random_rotation(rng, full=(k % 3 != 0)),
+                          "scale": rng.uniform(0.5, 1.3),
+                          "translation": (rng.uniform(-2.5, 2.5), rng.uniform(-0.3, 0.3),
+                                          rng.uniform(-2.5, 2.5))})
+    for k in range(2):
+        instances.append({"mesh": 2, "rotation": (1.0, 0.0, 0.0, 0.0) if k == 0 else
+                          _random_rotation(rng, full=False),
+                          "scale": 1.0 + 0.5 * k, "translation": (-1.0 + 2.0 * k, 2.2, 0.3 * k)})
+    fov = math.radians(60.0)
+    cam = {"position": (0.0, 4.5, 6.0), "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
+           "fov": (fov, fov_y_from_x(fov, width / height)), "nearFar": (0.01, 1000.0)}
+    env = {"boundary_type": abi.BOUNDARY_SKYSPHERE_COOCTA, "boundary_radiance": (0.3, 0.3, 0.35),
+           "scene_diameter": None, "camera": cam}
+    flat = flatten_instances(meshes, instances, env)
+    env["scene_diameter"] = flat.diameter
+    flat.scene_diameter = env["scene_diameter"]
+    return meshes, instances, env, flat
+
+
+def instanced_workload(mesh_count=8, instance_count=1000, tri_range=(50_000, 200_000),
+                       emissive_tris=10_000, width=1920, height=1080, seed=5):
+    """Two-level workload: mesh_count terrain meshes (triangles log-uniform in
+    tri_range) placed instance_count times over a 60 x 60 field with seeded
+    rotations about +Y and scales, plus one panel of emissive_tris emissive
+    triangles above it; constant Skysphere."""
+    rng = MT19937(seed)
+    meshes = []
+    for k in range(mesh_count):
+        t = math.exp(rng.uniform(math.log(tri_range[0]), math.log(tri_range[1])))
+        spheres = max(1, int(round(2000 * t / 2253090)))
+        grid = int(round(math.sqrt(max(512, t - 80 * spheres) / 2.0))) + 1
+        meshes.append(terrain_spheres(grid=grid, spheres=spheres, width=width, height=height,
+                                      light_grid=0, seed=20 + k))
+    meshes.append(light_panel(emissive_tris // 2, 0.08, (12.0, 11.0, 9.0), seed=seed, span=28.0))
+    instances = []
+    for k in range(instance_count):
+        instances.append({"mesh": int(rng() % mesh_count), "rotation": _random_rotation(rng, full=False),
+                          "scale": rng.uniform(0.6, 1.6),
+                          "translation": (rng.uniform(-30, 30), rng.uniform(-0.2, 0.2),
+                                          rng.uniform(-30, 30))})
+    instances.append({"mesh": mesh_count, "rotation": (1.0, 0.0, 0.0, 0.0), "scale": 1.0,
+                      "translation": (0.0, 4.0, 0.0)})
+    fov = math.radians(60.0)
+    cam = {"position": (0.0, 9.0, 36.0), "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
+           "fov": (fov, fov_y_from_x(fov, width / height)), "nearFar": (0.01, 1000.0)}
+    env = {"boundary_type": abi.BOUNDARY_SKYSPHERE_COOCTA, "boundary_radiance": (0.5, 0.5, 0.5),
+           "scene_diameter": None, "camera": cam}
+    return meshes, instances, env
