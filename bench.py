@@ -67,6 +67,8 @@ def parse():
                     help="frame: the 1080p headline frame; batch: many scenes in one renderer")
     ap.add_argument("--inst-spp", type=int, default=16,
                     help="samples per pixel of the instanced workload")
+    ap.add_argument("--inst-views", type=int, default=8,
+                    help="viewports of the instanced workload")
     ap.add_argument("--batch-scenes", type=int, default=32)
     ap.add_argument("--batch-spp", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true",
@@ -418,15 +420,68 @@ def bench_batch(args):
     print(json.dumps(result), flush=True)
 
 
+def _timed_render(r, args, torch, label):
+    """Counting frame, warmup, then args.steps timed frames of renderer r."""
+    r.set_measurement(profiling=False, counting=True)
+    cst = r.render_all()
+    log(f"{label} counting frame: {cst}")
+    r.set_measurement(profiling=False, counting=False)
+    for w in range(args.warmup):
+        r.reset()
+        r.render_all()
+        log(f"{label} warmup {w + 1}/{args.warmup} done")
+    r.set_measurement(profiling=True, counting=False)
+    totals = {"dead_paths": 0, "ext_rays": 0, "shadow_rays": 0, "iterations": 0}
+    torch.cuda.synchronize()
+    t_all = time.perf_counter()
+    for k in range(args.steps):
+        t0 = time.perf_counter()
+        r.reset()
+        st = r.render_all()
+        for key in totals:
+            totals[key] += st[key]
+        log(f"{label} step {k + 1}/{args.steps}: {(time.perf_counter() - t0) * 1e3:.1f} ms, "
+            f"{st['dead_paths']} paths, {st['iterations']} bounces")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_all
+    return cst, totals, elapsed, r.phase_times()
+
+
+def _trace_roofline(cst, totals, phases, width, kernel):
+    nodes_per_ray = cst["node_visits"] / max(1, cst["ext_rays"])
+    tris_per_ray = cst["tri_tests"] / max(1, cst["ext_rays"])
+    trace_ms, trace_launches = phases["trace"]
+    trace_avg_ms = trace_ms / max(1, trace_launches)
+    rays_per_launch = totals["ext_rays"] / max(1, trace_launches)
+    bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES[width] + tris_per_ray * TRI_BYTES
+    achieved = rays_per_launch * bytes_per_ray / (trace_avg_ms * 1e-3) / 1e9
+    return {
+        "bound": "hbm",
+        "kernel": kernel,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": None,
+        "bytes_per_ray": round(bytes_per_ray, 1),
+        "nodes_per_ray": round(nodes_per_ray, 2),
+        "tris_per_ray": round(tris_per_ray, 2),
+        "avg_launch_ms": round(trace_avg_ms, 4),
+        "rays_per_launch": round(rays_per_launch, 1),
+    }
+
+
 def bench_instanced(args):
     """Two-level workload: 8 terrain meshes (50k-200k triangles each) placed
     1000 times with seeded rotations about +Y, scales and translations, a
-    panel of 10k emissive triangles, constant Skysphere; one view at the
-    given resolution, spectral, depth 12."""
+    panel of 10k emissive triangles, constant Skysphere; spectral, depth 12.
+    Main line: 8 viewports of uneven sizes (1080p down to 512^2) batched in
+    one renderer; also one 1080p view alone."""
     import numpy as np
     from mray_amd import abi, native, scenes
     torch = native.require_gpu()
     spectral = not args.rgb
+    spp = args.inst_spp
     log("generating instanced scene")
     t0 = time.perf_counter()
     meshes, inst, env = scenes.instanced_workload(width=args.width, height=args.height)
@@ -440,46 +495,29 @@ def bench_instanced(args):
     build_s = time.perf_counter() - t0
     info = isc.info()
     log(f"instanced scene: {info} in {build_s:.2f} s")
-    spp = args.inst_spp
-    params = abi.render_params(args.width, args.height, spp, rr=(12, 12), mode=abi.SAMPLE_NEE_MIS,
-                               spectral=spectral, seed=0)
-    r = native.InstancedRenderer(isc, params)
-    r.set_measurement(profiling=False, counting=True)
-    cst = r.render_all()
-    nodes_per_ray = cst["node_visits"] / max(1, cst["ext_rays"])
-    tris_per_ray = cst["tri_tests"] / max(1, cst["ext_rays"])
-    log(f"counting frame: {cst}")
-    r.set_measurement(profiling=False, counting=False)
-    for w in range(args.warmup):
-        r.reset()
-        r.render_all()
-        log(f"warmup {w + 1}/{args.warmup} done")
-    r.set_measurement(profiling=True, counting=False)
-    paths = args.width * args.height * spp
-    totals = {"dead_paths": 0, "ext_rays": 0, "shadow_rays": 0, "iterations": 0}
-    torch.cuda.synchronize()
-    t_all = time.perf_counter()
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        r.reset()
-        st = r.render_all()
-        for key in totals:
-            totals[key] += st[key]
-        log(f"step {k + 1}/{args.steps}: {(time.perf_counter() - t0) * 1e3:.1f} ms, "
-            f"{st['dead_paths']} paths, {st['iterations']} bounces")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_all
-    phases = r.phase_times()
-    film = r.film()
+    views = scenes.instanced_views(spp=spp, count=args.inst_views)
+    paths = sum(v["width"] * v["height"] * v["spp"] for v in views)
+    params = abi.render_params(1, 1, 1, rr=(12, 12), mode=abi.SAMPLE_NEE_MIS, spectral=spectral,
+                               seed=0, max_ray_count=args.batch_slots)
+    r = native.BatchRenderer(isc, views, params)
+    cst, totals, elapsed, phases = _timed_render(r, args, torch, "viewports")
+    lat = r.view_times()
+    films = r.film()
+    r.close()
     assert totals["dead_paths"] == paths * args.steps, totals
-    assert np.isfinite(film).all()
-    trace_ms, trace_launches = phases["trace"]
-    trace_avg_ms = trace_ms / max(1, trace_launches)
-    rays_per_launch = totals["ext_rays"] / max(1, trace_launches)
-    bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES[4] + tris_per_ray * TRI_BYTES
-    achieved = rays_per_launch * bytes_per_ray / (trace_avg_ms * 1e-3) / 1e9
+    assert all(np.isfinite(f).all() for f in films)
+    # one 1080p view alone (a slot per pixel)
+    p1 = abi.render_params(args.width, args.height, spp, rr=(12, 12), mode=abi.SAMPLE_NEE_MIS,
+                           spectral=spectral, seed=0)
+    r1 = native.InstancedRenderer(isc, p1)
+    cst1, tot1, el1, ph1 = _timed_render(r1, args, torch, "single view")
+    r1.close()
+    isc.close()
+    single_paths = args.width * args.height * spp
+    assert tot1["dead_paths"] == single_paths * args.steps, tot1
     result = {
-        "metric": "Mpaths/s, instanced two-level scene, 1 MI355X",
+        "metric": f"Mpaths/s, {len(views)} viewports of an instanced two-level scene in one "
+                  f"renderer, 1 MI355X",
         "value": round(paths * args.steps / elapsed / 1e6, 3),
         "unit": "Mpaths/s",
         "n_gpus": 1,
@@ -494,34 +532,31 @@ def bench_instanced(args):
         "config": {
             "workload": f"{info['instances']} instances of {len(meshes) - 1} terrain meshes "
                         f"({info['effective_triangles']} effective tris) + "
-                        f"{info['lights']} emissive tris, {args.width}x{args.height}, {spp} spp, "
-                        f"depth 12, {'spectral 4 lambda' if spectral else 'RGB'}, NEE+MIS",
+                        f"{info['lights']} emissive tris; {len(views)} viewports "
+                        f"(1080p..512^2), {spp} spp, depth 12, "
+                        f"{'spectral 4 lambda' if spectral else 'RGB'}, NEE+MIS",
             "mesh_triangles": [m.triangle_count for m in meshes],
             "effective_triangles": info["effective_triangles"],
             "tlas_nodes": info["tlas_nodes"],
+            "views": [[v["width"], v["height"]] for v in views],
             "paths_per_step": paths,
+            "slots": params.maxRayCount,
         },
         "mrays_per_s": round((totals["ext_rays"] + totals["shadow_rays"]) / elapsed / 1e6, 3),
         "bounces_per_step": totals["iterations"] // args.steps,
         "accel_build_s": round(build_s, 3),
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in phases.items()},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "KTrace (closest hit, instance tree + 4-wide mesh trees)",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "bytes_per_ray": round(bytes_per_ray, 1),
-            "nodes_per_ray": round(nodes_per_ray, 2),
-            "tris_per_ray": round(tris_per_ray, 2),
-            "avg_launch_ms": round(trace_avg_ms, 4),
-            "rays_per_launch": round(rays_per_launch, 1),
+        "view_latency_ms": [round(float(x), 3) for x in lat],
+        "roofline": _trace_roofline(cst, totals, phases, 4,
+                                    "KTrace (closest hit, instance tree + 4-wide mesh trees)"),
+        "single_view": {
+            "resolution": [args.width, args.height],
+            "mpaths_per_s": round(single_paths * args.steps / el1 / 1e6, 3),
+            "ms_per_step": round(el1 * 1e3 / args.steps, 2),
+            "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in ph1.items()},
+            "roofline": _trace_roofline(cst1, tot1, ph1, 4, "KTrace (instanced)"),
         },
     }
-    r.close()
-    isc.close()
     print(json.dumps(result), flush=True)
 
 

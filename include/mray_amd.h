@@ -383,6 +383,13 @@ int mray_instanced_scene_info(const MRayInstScene* scene, uint32_t* hInstances,
 /* A renderer over an instanced scene: the camera of hEnv; no material queues. */
 int mray_renderer_create_instanced(const MRayInstScene* scene, const MRayRenderParams* hParams,
                                    MRayRenderer** out);
+/* Several views of one instanced scene in one slot pool (the batch
+ * renderer's schedule, views' scene field 0; useCamera 0 takes hEnv's
+ * camera).  The batch read-film and view-times calls apply; no material
+ * queues, traversal workgroups of 256.                                     */
+int mray_instanced_batch_create(const MRayInstScene* scene, const MRayBatchView* hViews,
+                                uint32_t viewCount, const MRayRenderParams* hParams,
+                                MRayRenderer** out);
 
 /* ---------------------------------------------------------------------- */
 /* Spectral tables (SpectraLUTGen capability, SpectrumContext.cu:298-472):  */

@@ -1394,7 +1394,9 @@ struct MRayRenderer
     TraceCtx ctx{};
     // instanced scene (mray_renderer_create_instanced)
     const MRayInstScene* iscene = nullptr;
-    // batch (mray_batch_create)
+    // batch (mray_batch_create, mray_instanced_batch_create): per-view
+    // cameras and films; the scene is a merged batch scene or an instanced one
+    bool batch = false;
     const MRayBatchScene* bscene = nullptr;
     std::vector<ViewGPU> hViews;
     ViewGPU* dViews = nullptr;
@@ -1468,7 +1470,7 @@ static void LaunchTrace(MRayRenderer* r, hipStream_t s)
     const uint32_t nb = r->traceBlocks;
     if constexpr(INST)
     {
-        KTrace<4, COUNT, false, kTraceTPB, true><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched,
+        KTrace<4, COUNT, BATCH, kTraceTPB, true><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched,
                                                                          r->dViews, r->dCtr);
         return;
     }
@@ -1492,7 +1494,7 @@ static void LaunchShadow(MRayRenderer* r, hipStream_t s)
     const uint32_t nb = r->shadowBlocks;
     if constexpr(INST)
     {
-        KShadow<4, false, kTraceTPB, true><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched,
+        KShadow<4, BATCH, kTraceTPB, true><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched,
                                                                    r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
         return;
     }
@@ -1572,7 +1574,8 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
 
 static int EnqueueIteration(MRayRenderer* r, hipStream_t s)
 {
-    if(r->bscene) return EnqueueIterationT<true, false>(r, s);
+    if(r->batch && r->iscene) return EnqueueIterationT<true, true>(r, s);
+    if(r->batch) return EnqueueIterationT<true, false>(r, s);
     if(r->iscene) return EnqueueIterationT<false, true>(r, s);
     return EnqueueIterationT<false, false>(r, s);
 }
@@ -1589,7 +1592,7 @@ static int ResetRenderer(MRayRenderer* r)
     KZeroFloats<<<1024, 256, 0, s>>>(r->rc.film, filmN);
     KResetSlots<<<DivUp(r->rc.N, 256), 256, 0, s>>>(r->S.meta, r->S.rng, r->dSeeds, r->rc.N,
                                                      r->dSched, r->limit);
-    if(r->bscene)
+    if(r->batch)
     {
         const size_t V = r->hViews.size();
         KZeroFloats<<<DivUp(2 * V, 256), 256, 0, s>>>(reinterpret_cast<float*>(r->dViewClock), 2 * V);
@@ -1638,7 +1641,7 @@ static void BuildCamera(const MRayCamera& c, uint32_t W, uint32_t H, float filte
 static int SizeLaunches(MRayRenderer* r);
 
 // Path-slot state, queues, seeds and launch sizes, shared by the one-scene
-// and batch renderers (r->params, r->rc.N, r->ctx and r->bscene set).
+// and batch renderers (r->params, r->rc.N, r->ctx and r->batch set).
 static int InitSlots(MRayRenderer* r, size_t filmPixels)
 {
     const MRayRenderParams* p = &r->params;
@@ -1668,7 +1671,7 @@ static int InitSlots(MRayRenderer* r, size_t filmPixels)
         MRAY_TRY(A((void**)&r->S.waves, size_t(N) * 16));
         MRAY_TRY(A((void**)&r->S.wpdf, size_t(N) * 16));
     }
-    if(r->bscene) MRAY_TRY(A((void**)&r->S.view, size_t(N) * 4));
+    if(r->batch) MRAY_TRY(A((void**)&r->S.view, size_t(N) * 4));
     if(r->iscene) MRAY_TRY(A((void**)&r->S.inst, size_t(N) * 4));
     MRAY_TRY(A((void**)&r->rc.film, filmPixels * 4 * sizeof(float)));
     r->nBlocks = DivUp(N, kSlotTPB);
@@ -1727,7 +1730,7 @@ static int SizeLaunches(MRayRenderer* r)
         r->traceBlocks = uint32_t(resident(KTrace<4, false, false, kTraceTPB, true>, tpb, maxTraceBlocks));
         r->shadowBlocks = uint32_t(resident(KShadow<4, false, kTraceTPB, true>, tpb, maxTraceBlocks));
     }
-    else if(!r->bscene)
+    else if(!r->batch)
     {
         r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false, false>, tpb, maxTraceBlocks)
                                      : resident(KTrace<4, false, false>, tpb, maxTraceBlocks));
@@ -1761,10 +1764,13 @@ static int SizeLaunches(MRayRenderer* r)
         else if(p->spectral) ShadeQueueOccupancy<true, false>(r, resident);
         else ShadeQueueOccupancy<false, false>(r, resident);
     }
-    if(r->iscene)
+    if(r->iscene && r->batch)
+        r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, true, true>, kSlotTPB, 1u << 20)
+                                              : resident(KShade<false, true, true>, kSlotTPB, 1u << 20));
+    else if(r->iscene)
         r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, false, true>, kSlotTPB, 1u << 20)
                                               : resident(KShade<false, false, true>, kSlotTPB, 1u << 20));
-    else if(r->bscene)
+    else if(r->batch)
         r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, true>, kSlotTPB, 1u << 20)
                                               : resident(KShade<false, true>, kSlotTPB, 1u << 20));
     else
@@ -1840,25 +1846,43 @@ static int CheckSampling(const MRayRenderParams& p, const char* who)
     return MRAY_OK;
 }
 
-static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayBatchView* views,
-                       uint32_t viewCount, const MRayRenderParams* p)
+// The one scene an instanced batch's views look at, as a batch part.
+static MRayBatchScene::Part InstancedPart(const MRayInstScene* is)
 {
+    MRayBatchScene::Part part;
+    part.lightOff = 0;
+    part.lightCount = is->view.lightCount;
+    part.boundaryType = is->view.boundaryType;
+    for(int c = 0; c < 3; c++) part.boundaryRad[c] = is->view.boundaryRad[c];
+    part.sceneDiameter = is->view.sceneDiameter;
+    part.camera = is->camera;
+    return part;
+}
+
+// Views over a merged batch scene (b) or over one instanced scene (is).
+static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayInstScene* is,
+                       const MRayBatchView* views, uint32_t viewCount, const MRayRenderParams* p)
+{
+    r->batch = true;
     r->bscene = b;
-    r->ctx = b->ctx;
+    r->iscene = is;
+    r->ctx = is ? is->ctx : b->ctx;
     r->params = *p;
     MRAY_HIP_CHECK(hipStreamCreate(&r->stream));
     r->rc.rr0 = p->rrRange[0];
     r->rc.rr1 = p->rrRange[1];
     r->rc.mode = p->sampleMode;
     r->rc.spectral = p->spectral;
-    r->rc.scene = b->view;
+    r->rc.scene = is ? is->view : b->view;
+    if(is) r->rc.inst = is->ctx.inst;
     const float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
     uint64_t pixels = 0, paths = 0;
+    const MRayBatchScene::Part instPart = is ? InstancedPart(is) : MRayBatchScene::Part{};
     r->hViews.resize(viewCount);
     for(uint32_t k = 0; k < viewCount; k++)
     {
         const MRayBatchView& v = views[k];
-        const MRayBatchScene::Part& part = b->parts[v.scene];
+        const MRayBatchScene::Part& part = is ? instPart : b->parts[v.scene];
         ViewGPU& g = r->hViews[k];
         g = ViewGPU{};
         BuildCamera(v.useCamera ? v.camera : part.camera, v.width, v.height, fr, g.cam);
@@ -2144,7 +2168,7 @@ int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderP
         r->nrcArena.Release();
         return MRAY_OK;
     }
-    if(r->bscene)
+    if(r->batch)
     {
         SetError("attach_nrc: a batch renderer has no radiance cache");
         return MRAY_ERR_UNSUPPORTED;
@@ -2208,7 +2232,7 @@ int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderP
 int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* hStats)
 {
     if(!r || spp == 0) { SetError("render_frame: NULL renderer or spp == 0"); return MRAY_ERR_INVALID; }
-    if(r->bscene) { SetError("render_frame: not for a batch renderer"); return MRAY_ERR_UNSUPPORTED; }
+    if(r->batch) { SetError("render_frame: not for a batch renderer"); return MRAY_ERR_UNSUPPORTED; }
     hipStream_t s = r->stream;
     r->frameLimit += (unsigned long long)spp * r->rc.P;
     KSetLimit<<<1, 1, 0, s>>>(r->dSched, r->frameLimit);
@@ -2293,7 +2317,56 @@ int mray_batch_create(const MRayBatchScene* scene, const MRayBatchView* hViews, 
         return MRAY_ERR_INVALID;
     }
     MRayRenderer* r = new MRayRenderer();
-    int rv = CreateBatch(r, scene, hViews, viewCount, hParams);
+    int rv = CreateBatch(r, scene, nullptr, hViews, viewCount, hParams);
+    if(rv != MRAY_OK)
+    {
+        (void)hipDeviceSynchronize();
+        FreeRenderer(r);
+        return rv;
+    }
+    *out = r;
+    return MRAY_OK;
+}
+
+int mray_instanced_batch_create(const MRayInstScene* scene, const MRayBatchView* hViews,
+                                uint32_t viewCount, const MRayRenderParams* hParams, MRayRenderer** out)
+{
+    if(!scene || !hViews || !hParams || !out || viewCount == 0)
+    {
+        SetError("instanced_batch_create: NULL argument or no views");
+        return MRAY_ERR_INVALID;
+    }
+    *out = nullptr;
+    MRAY_TRY(CheckSampling(*hParams, "instanced_batch_create"));
+    if(hParams->flags & MRAY_RENDER_MATERIAL_QUEUES)
+    {
+        SetError("instanced_batch_create: material queues are not available for instanced scenes");
+        return MRAY_ERR_UNSUPPORTED;
+    }
+    uint64_t pixels = 0;
+    for(uint32_t k = 0; k < viewCount; k++)
+    {
+        const MRayBatchView& v = hViews[k];
+        if(v.scene != 0)
+        {
+            SetError("instanced_batch_create: view " + std::to_string(k) + " names scene " +
+                     std::to_string(v.scene) + "; an instanced scene is scene 0");
+            return MRAY_ERR_INVALID;
+        }
+        if(v.width == 0 || v.height == 0 || v.spp == 0)
+        {
+            SetError("instanced_batch_create: view width, height and spp must be non-zero");
+            return MRAY_ERR_INVALID;
+        }
+        pixels += uint64_t(v.width) * v.height;
+    }
+    if(pixels > (1ull << 31))
+    {
+        SetError("instanced_batch_create: the views hold more than 2^31 pixels");
+        return MRAY_ERR_INVALID;
+    }
+    MRayRenderer* r = new MRayRenderer();
+    int rv = CreateBatch(r, nullptr, scene, hViews, viewCount, hParams);
     if(rv != MRAY_OK)
     {
         (void)hipDeviceSynchronize();
@@ -2307,7 +2380,7 @@ int mray_batch_create(const MRayBatchScene* scene, const MRayBatchView* hViews, 
 int mray_batch_read_film(MRayRenderer* r, uint32_t view, float* hOut)
 {
     if(!r || !hOut) { SetError("batch_read_film: NULL argument"); return MRAY_ERR_INVALID; }
-    if(!r->bscene || view >= r->hViews.size())
+    if(!r->batch || view >= r->hViews.size())
     {
         SetError("batch_read_film: not a batch renderer or view out of range");
         return MRAY_ERR_INVALID;
@@ -2321,7 +2394,7 @@ int mray_batch_read_film(MRayRenderer* r, uint32_t view, float* hOut)
 int mray_batch_view_times(MRayRenderer* r, double* hMs)
 {
     if(!r || !hMs) { SetError("batch_view_times: NULL argument"); return MRAY_ERR_INVALID; }
-    if(!r->bscene) { SetError("batch_view_times: not a batch renderer"); return MRAY_ERR_INVALID; }
+    if(!r->batch) { SetError("batch_view_times: not a batch renderer"); return MRAY_ERR_INVALID; }
     const size_t V = r->hViews.size();
     std::vector<unsigned long long> clk(V);
     MRAY_HIP_CHECK(hipDeviceSynchronize());
@@ -2341,7 +2414,7 @@ int mray_batch_set_trace_tpb(MRayRenderer* r, uint32_t tpb)
 {
     if(!r || !r->bscene || (tpb != 128 && tpb != 256))
     {
-        SetError("batch_set_trace_tpb: a batch renderer and tpb 128 or 256");
+        SetError("batch_set_trace_tpb: a batch renderer over a merged scene and tpb 128 or 256");
         return MRAY_ERR_INVALID;
     }
     MRAY_HIP_CHECK(hipDeviceSynchronize());

@@ -43,7 +43,7 @@ EXPORTS = [
     "mray_spectral_write_mrspectra", "mray_spectral_read_mrspectra", "mray_spectral_use_mrspectra",
     "mray_renderer_take_film", "mray_film_write_exr",
     "mray_instanced_scene_create", "mray_instanced_scene_destroy", "mray_instanced_scene_info",
-    "mray_renderer_create_instanced",
+    "mray_renderer_create_instanced", "mray_instanced_batch_create",
 ]
 
 
@@ -126,6 +126,7 @@ def lib():
     L.mray_instanced_scene_info.argtypes = [P, C.POINTER(u32), C.POINTER(C.c_uint64), C.POINTER(u32),
                                             C.POINTER(u32)]
     L.mray_renderer_create_instanced.argtypes = [P, C.POINTER(abi.MRayRenderParams), C.POINTER(P)]
+    L.mray_instanced_batch_create.argtypes = [P, P, u32, C.POINTER(abi.MRayRenderParams), C.POINTER(P)]
     _lib = L
     return L
 
@@ -527,7 +528,9 @@ def batch_autotune(scene4, scene8, views, params, max_trials=64):
 
 
 class BatchRenderer(Renderer):
-    """Batch of views over a BatchScene in one renderer (mray_batch_create)."""
+    """Batch of views in one renderer: over a BatchScene (mray_batch_create)
+    or over one InstancedScene, every view's scene 0
+    (mray_instanced_batch_create)."""
 
     def __init__(self, bscene, views, params):
         self.bscene = bscene
@@ -535,7 +538,11 @@ class BatchRenderer(Renderer):
         self.views = [dict(v) for v in views]
         arr = abi.batch_views(views)
         h = C.c_void_p()
-        check(lib().mray_batch_create(bscene.h, arr, len(views), C.byref(params), C.byref(h)))
+        if isinstance(bscene, InstancedScene):
+            check(lib().mray_instanced_batch_create(bscene.h, arr, len(views), C.byref(params),
+                                                    C.byref(h)))
+        else:
+            check(lib().mray_batch_create(bscene.h, arr, len(views), C.byref(params), C.byref(h)))
         self.h = h
 
     def film(self, view=None):

@@ -759,3 +759,20 @@ def instanced_workload(mesh_count=8, instance_count=1000, tri_range=(50_000, 200
     env = {"boundary_type": abi.BOUNDARY_SKYSPHERE_COOCTA, "boundary_radiance": (0.5, 0.5, 0.5),
            "scene_diameter": None, "camera": cam}
     return meshes, instances, env
+
+
+def instanced_views(spp=16, count=8):
+    """count viewports of uneven sizes (1080p down to 512^2) looking at the
+    instanced workload's field from around it."""
+    sizes = [(1920, 1080), (1280, 720), (1280, 720), (960, 540), (960, 540), (640, 360),
+             (640, 360), (512, 512)]
+    views = []
+    for k in range(count):
+        w, h = sizes[k % len(sizes)]
+        ang = 2.0 * math.pi * k / count
+        fov = math.radians(60.0 if k % 2 == 0 else 45.0)
+        cam = {"position": (30.0 * math.sin(ang), 6.0 + 2.0 * (k % 3), 30.0 * math.cos(ang)),
+               "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
+               "fov": (fov, fov_y_from_x(fov, w / h)), "nearFar": (0.01, 1000.0)}
+        views.append({"scene": 0, "width": w, "height": h, "spp": spp, "camera": cam})
+    return views

@@ -24,7 +24,7 @@ from tests.conftest import film_image, relmse
 pytestmark = pytest.mark.gpu
 
 
-def _gpu_instanced(meshes, instances, env, params, **attach):
+def _gpu_instanced(meshes, instances, env, params):
     from mray_amd import native
     accs = [native.Accel(m) for m in meshes]
     isc = native.InstancedScene(accs, instances, env)
@@ -153,3 +153,67 @@ def test_many_instances_of_one_mesh(torch_gpu):
     e = relmse(film_image(film), film_image(ofilm))
     print(f"100 instances: relMSE={e:.3e}", flush=True)
     assert e <= 1e-3
+
+
+def _views():
+    cams = []
+    for pos, fov in (((2.0, 3.0, 5.0), 50.0), ((-4.0, 2.5, 1.0), 70.0),
+                     ((0.5, 6.0, 0.5), 60.0)):
+        cams.append({"position": pos, "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
+                     "fov": (math.radians(fov),) * 2, "nearFar": (0.01, 100.0)})
+    return [dict(scene=0, width=48, height=36, spp=3),
+            dict(scene=0, width=40, height=40, spp=2, camera=cams[0]),
+            dict(scene=0, width=17, height=9, spp=5, camera=cams[1]),
+            dict(scene=0, width=32, height=24, spp=4, camera=cams[2])]
+
+
+@pytest.mark.parametrize("spectral", [False, True])
+def test_instanced_batch_matches_flattened_oracle(torch_gpu, spectral):
+    """Several views of one instanced scene in one slot pool, against the
+    oracle's batch render of the flattened scene."""
+    from mray_amd import native
+    if spectral:
+        refcpu.spectral_load(native.spectral_tables())
+    meshes, inst, env, flat = scenes.instanced_test_scene(width=48, height=36)
+    views = _views()
+    params = abi.render_params(1, 1, 1, rr=(3, 8), seed=13, spectral=spectral, max_ray_count=3000)
+    accs = [native.Accel(m) for m in meshes]
+    isc = native.InstancedScene(accs, inst, env)
+    for a in accs:
+        a.close()
+    r = native.BatchRenderer(isc, views, params)
+    st = r.render_all()
+    films = r.film()
+    times = r.view_times()
+    r.close()
+    isc.close()
+    ofilms, ost = refcpu.render_batch([refcpu.OracleScene(flat)], views, params)
+    print(f"gpu={st} cpu={ost}", flush=True)
+    assert st["dead_paths"] == ost["dead_paths"] == sum(v["width"] * v["height"] * v["spp"]
+                                                        for v in views)
+    for k, (f, o) in enumerate(zip(films, ofilms)):
+        assert np.allclose(f[3], o[3], rtol=1e-5)
+        e = relmse(film_image(f), film_image(o))
+        print(f"view {k}: relMSE={e:.3e}", flush=True)
+        assert e <= 1e-3
+    assert np.all(times >= 0.0)
+
+
+def test_instanced_batch_errors(torch_gpu):
+    from mray_amd import native
+    meshes, inst, env, _ = scenes.instanced_test_scene(width=16, height=16)
+    accs = [native.Accel(m) for m in meshes]
+    isc = native.InstancedScene(accs, inst, env)
+    params = abi.render_params(1, 1, 1, rr=(3, 8), seed=1)
+    with pytest.raises(native.MRayError, match="scene 0"):
+        native.BatchRenderer(isc, [dict(scene=1, width=8, height=8, spp=1)], params)
+    q = abi.render_params(1, 1, 1, rr=(3, 8), seed=1, flags=abi.RENDER_MATERIAL_QUEUES)
+    with pytest.raises(native.MRayError, match="material queues"):
+        native.BatchRenderer(isc, [dict(scene=0, width=8, height=8, spp=1)], q)
+    r = native.BatchRenderer(isc, [dict(scene=0, width=8, height=8, spp=1)], params)
+    with pytest.raises(native.MRayError, match="merged scene"):
+        r.set_trace_tpb(128)
+    r.close()
+    isc.close()
+    for a in accs:
+        a.close()
