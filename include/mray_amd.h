@@ -234,6 +234,21 @@ int mray_accel_cast_closest_counted(const MRayAccel* accel, const float* dRays, 
 int mray_accel_cast_visibility(const MRayAccel* accel, const float* dRays, uint32_t count,
                                uint32_t* dVisible, void* stream);
 
+/* Textured Skysphere (LightGroupSkysphere with a radiance texture,
+ * LightsDefault.hpp:300-421, 860-907): hRgb [height][width][3] linear RGB
+ * texels over concentric-octahedral uv (texel (x, y) centred at
+ * ((x + 1/2) / width, (y + 1/2) / height)), read bilinearly with wrapped
+ * edges; NEE samples directions by texel luminance (PWC-2D,
+ * Distributions.h:117-243).  The scene's boundary must be
+ * MRAY_BOUNDARY_SKYSPHERE_COOCTA (boundaryRadiance is then unused);
+ * renderers and merged scenes created afterwards use the texture.
+ * Sizes 1..8192 per side; texels must be finite, luminance not all zero.   */
+int mray_accel_set_environment(MRayAccel* accel, uint32_t width, uint32_t height,
+                               const float* hRgb);
+/* The luminance distribution: hCdfX [height][width] normalized row CDFs,
+ * hCdfY [height] the row-marginal CDF.                                     */
+int mray_accel_environment_cdfs(const MRayAccel* accel, float* hCdfX, float* hCdfY);
+
 /* ---------------------------------------------------------------------- */
 /* Path tracer renderer ("PathTracerRGB" / "PathTracerSpectral").           */
 /* ---------------------------------------------------------------------- */
@@ -390,6 +405,10 @@ int mray_renderer_create_instanced(const MRayInstScene* scene, const MRayRenderP
 int mray_instanced_batch_create(const MRayInstScene* scene, const MRayBatchView* hViews,
                                 uint32_t viewCount, const MRayRenderParams* hParams,
                                 MRayRenderer** out);
+/* A textured Skysphere for an instanced scene (as mray_accel_set_environment;
+ * the MRayInstEnv boundary must be MRAY_BOUNDARY_SKYSPHERE_COOCTA).        */
+int mray_instanced_scene_set_environment(MRayInstScene* scene, uint32_t width, uint32_t height,
+                                         const float* hRgb);
 
 /* ---------------------------------------------------------------------- */
 /* Spectral tables (SpectraLUTGen capability, SpectrumContext.cu:298-472):  */

@@ -43,12 +43,6 @@ void DeviceArena::Release()
     ptrs.clear();
 }
 
-#define MRAY_TRY(expr)                  \
-    do {                                \
-        int r_ = (expr);                \
-        if(r_ != MRAY_OK) return r_;    \
-    } while(0)
-
 template<class T>
 int Upload(DeviceArena& a, T** d, const T* h, size_t count)
 {

@@ -98,6 +98,8 @@ int BuildBatchScene(MRayBatchScene* b, const MRayAccel* const* accels, uint32_t 
         lights += a->view.lightCount;
         maxDepth = std::max(maxDepth, a->maxDepth);
     }
+    for(uint32_t k = 0; k < count; k++)
+        MRAY_TRY(CopyEnvironment(b->arena, accels[k]->view, b->parts[k].env));
     // leaf ranges hold a soup position in 26 bits; vertex indices are 32-bit
     if(tris > kLeafStartMask || nodes >= kLeafFlag || verts >= kInvalid || mats >= MRAY_LIGHT_FLAG)
     {

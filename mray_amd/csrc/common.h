@@ -34,6 +34,13 @@ void SetError(const std::string& msg);
         }                                                                          \
     } while(0)
 
+// propagate a non-OK status code
+#define MRAY_TRY(expr)                  \
+    do {                                \
+        int r_ = (expr);                \
+        if(r_ != MRAY_OK) return r_;    \
+    } while(0)
+
 static constexpr uint32_t kLeafFlag = 0x80000000u;
 static constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 static constexpr uint32_t kTPB = 256;   // default workgroup: 4 wave64s

@@ -10,11 +10,6 @@
 #include <cmath>
 #include <vector>
 
-#define MRAY_TRY_C(expr)                \
-    do {                                \
-        int r_ = (expr);                \
-        if(r_ != MRAY_OK) return r_;    \
-    } while(0)
 
 namespace mray
 {
@@ -38,7 +33,7 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
                    const MRayInstance* inst, uint32_t n, const MRayInstEnv& env)
 {
     MRayBatchScene& mb = sc->meshes;
-    MRAY_TRY_C(BuildBatchScene(&mb, meshes, meshCount));
+    MRAY_TRY(BuildBatchScene(&mb, meshes, meshCount));
     if(mb.width != 4)
     {
         SetError("instanced_scene_create: meshes need 4-wide trees");
@@ -194,11 +189,11 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
         if(bytes) MRAY_HIP_CHECK(hipMemcpy(*d, h, bytes, hipMemcpyHostToDevice));
         return MRAY_OK;
     };
-    MRAY_TRY_C(up((void**)&dTlas, w.nodes.data(), w.nodes.size() * sizeof(BNode4)));
-    MRAY_TRY_C(up((void**)&dOrder, order.data(), order.size() * 4));
-    MRAY_TRY_C(up((void**)&dInst, table.data(), table.size() * sizeof(InstGPU)));
-    MRAY_TRY_C(up((void**)&dLightPos, lightPos.data(), lightPos.size() * 4));
-    MRAY_TRY_C(up((void**)&dLightRad, lightRad.data(), lightRad.size() * sizeof(float4)));
+    MRAY_TRY(up((void**)&dTlas, w.nodes.data(), w.nodes.size() * sizeof(BNode4)));
+    MRAY_TRY(up((void**)&dOrder, order.data(), order.size() * 4));
+    MRAY_TRY(up((void**)&dInst, table.data(), table.size() * sizeof(InstGPU)));
+    MRAY_TRY(up((void**)&dLightPos, lightPos.data(), lightPos.size() * 4));
+    MRAY_TRY(up((void**)&dLightRad, lightRad.data(), lightRad.size() * sizeof(float4)));
     sc->instanceCount = n;
     sc->tlasNodes = uint32_t(w.nodes.size());
     sc->tlasDepth = w.maxDepth;
@@ -222,7 +217,7 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
     c = mb.ctx;
     const uint32_t need = 3u * (w.maxDepth + maxMeshDepth) + 4u;
     c.gstackDepth = (need > kLdsStack) ? need - kLdsStack : 1u;
-    MRAY_TRY_C(DevAlloc(ar, (void**)&c.gstack, size_t(c.gstackDepth) * c.gridThreads * sizeof(uint2)));
+    MRAY_TRY(DevAlloc(ar, (void**)&c.gstack, size_t(c.gstackDepth) * c.gridThreads * sizeof(uint2)));
     c.tlas = dTlas;
     c.tlasOrder = dOrder;
     c.inst = dInst;

@@ -571,11 +571,6 @@ struct MRayNrc
 
 static int NrcLaunchCheck() { MRAY_HIP_CHECK(hipGetLastError()); return MRAY_OK; }
 
-#define MRAY_TRY_C(expr)                \
-    do {                                \
-        int r_ = (expr);                \
-        if(r_ != MRAY_OK) return r_;    \
-    } while(0)
 
 extern "C" {
 
@@ -753,7 +748,7 @@ int mray_nrc_adam(MRayNrc* n, void* stream)
 int mray_nrc_train(MRayNrc* n, const float* dRecords, const float* dTargets, uint32_t count,
                    void* stream)
 {
-    MRAY_TRY_C(mray_nrc_backward(n, dRecords, dTargets, count, stream));
+    MRAY_TRY(mray_nrc_backward(n, dRecords, dTargets, count, stream));
     return mray_nrc_adam(n, stream);
 }
 

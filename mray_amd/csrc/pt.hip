@@ -234,6 +234,10 @@ struct ViewGPU
     uint32_t boundaryType;
     float boundaryRad[3];
     float sceneDiameter;
+    const float4* envTex;             // textured Skysphere (envW != 0)
+    const float* envCdfX;
+    const float* envCdfY;
+    uint32_t envW, envH;
 };
 
 struct RenderConst
@@ -288,6 +292,11 @@ __device__ __forceinline__ SceneView SlotScene(const RenderConst& rc, const Slot
         s.boundaryRad[1] = v.boundaryRad[1];
         s.boundaryRad[2] = v.boundaryRad[2];
         s.sceneDiameter = v.sceneDiameter;
+        s.envTex = v.envTex;
+        s.envCdfX = v.envCdfX;
+        s.envCdfY = v.envCdfY;
+        s.envW = v.envW;
+        s.envH = v.envH;
     }
     return s;
 }
@@ -698,6 +707,122 @@ __device__ __forceinline__ V3 CoOctaToDir(float s, float t)
     return {cosPhi * xyFactor, sinPhi * xyFactor, z};
 }
 
+// Graphics::DirectionToConcentricOctahedral (GraphicsFunctions.h:318-353),
+// Z-up unit direction -> uv in [0, 1]^2 (the reference's quirk kept: the
+// +Z pole maps to uv (0, 0)).
+__device__ __forceinline__ void DirToCoOcta(V3 d, float& s, float& t)
+{
+    const float twoOvrPi = kInvPi * 2.0f;
+    if(d.x == 0.0f && d.y == 0.0f) { s = 0.0f; t = 0.0f; return; }
+    float xAbs = fabsf(d.x), yAbs = fabsf(d.y);
+    float phiPrime = atanf(yAbs / xAbs);
+    float radius = sqrtf(MaxF(0.0f, 1.0f - fabsf(d.z)));
+    float v = radius * twoOvrPi * phiPrime;
+    float u = radius - v;
+    if(d.z < 0.0f)
+    {
+        float uP = 1.0f - v, vP = 1.0f - u;
+        u = uP;
+        v = vP;
+    }
+    u *= signbit(d.x) ? -1.0f : 1.0f;
+    v *= signbit(d.y) ? -1.0f : 1.0f;
+    s = (u + 1.0f) * 0.5f;
+    t = (v + 1.0f) * 0.5f;
+}
+
+// Textured Skysphere radiance at Y-up direction dirYUp: bilinear, wrapped
+// edges, texel centres at (i + 1/2) / size (the reference's default
+// texture parameters, TracerI.h:71-82), evaluated in fp32.
+__device__ __forceinline__ V3 EnvRadiance(const SceneView& s, V3 dirYUp)
+{
+    float u, v;
+    DirToCoOcta(V3{dirYUp.z, dirYUp.x, dirYUp.y}, u, v);   // YUpToZUp
+    const int W = int(s.envW), H = int(s.envH);
+    float xf = u * float(W) - 0.5f, yf = v * float(H) - 0.5f;
+    float x0f = floorf(xf), y0f = floorf(yf);
+    float fx = xf - x0f, fy = yf - y0f;
+    int x0 = int(x0f), y0 = int(y0f);
+    int x1 = x0 + 1, y1 = y0 + 1;
+    x0 = ((x0 % W) + W) % W; x1 = ((x1 % W) + W) % W;
+    y0 = ((y0 % H) + H) % H; y1 = ((y1 % H) + H) % H;
+    const float4 a = s.envTex[y0 * W + x0], b = s.envTex[y0 * W + x1];
+    const float4 c = s.envTex[y1 * W + x0], d = s.envTex[y1 * W + x1];
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    V3 top = {a.x * gx + b.x * fx, a.y * gx + b.y * fx, a.z * gx + b.z * fx};
+    V3 bot = {c.x * gx + d.x * fx, c.y * gx + d.y * fx, c.z * gx + d.z * fx};
+    return {top.x * gy + bot.x * fy, top.y * gy + bot.y * fy, top.z * gy + bot.z * fy};
+}
+
+// rocprim::lower_bound: the first i with cdf[i] >= xi (n if none).
+__device__ __forceinline__ uint32_t LowerBoundF(const float* cdf, uint32_t n, float xi)
+{
+    uint32_t lo = 0, len = n;
+    while(len > 0)
+    {
+        uint32_t half = len >> 1;
+        if(cdf[lo + half] < xi) { lo += half + 1; len -= half + 1; }
+        else len = half;
+    }
+    return lo;
+}
+
+// DistributionPwC<1>::SampleIndex (Distributions.h:128-148), with its
+// clamp kept as written: an index of 1 or more steps down one ulp.
+__device__ __forceinline__ float PwcSampleIndex(const float* cdf, uint32_t n, float xi, float& pdf)
+{
+    uint32_t index = LowerBoundF(cdf, n, xi);
+    index = min(index, n - 1u);
+    float prevCDF = (index == 0) ? 0.0f : cdf[index - 1];
+    float myCDF = cdf[index];
+    float t = (xi - prevCDF) / (myCDF - prevCDF);
+    float indexF = float(index) + t;
+    indexF = (indexF < 1.0f) ? indexF : __uint_as_float(__float_as_uint(indexF) - 1u);
+    pdf = (myCDF - prevCDF) * float(n);
+    return indexF;
+}
+__device__ __forceinline__ float PwcPdfIndex(const float* cdf, uint32_t n, float index)
+{
+    uint32_t i = uint32_t(index);
+    float prevCDF = (i == 0) ? 0.0f : cdf[i - 1];
+    return (cdf[i] - prevCDF) * float(n);
+}
+
+// DistributionPwC<2>::SampleUV (Distributions.h:181-213): the row from
+// xi.y on the marginal, the column from xi.x on that row.
+__device__ __forceinline__ void EnvSampleUV(const SceneView& s, float xi0, float xi1,
+                                            float& u, float& v, float& pdf)
+{
+    float pdfY, pdfX;
+    float yF = PwcSampleIndex(s.envCdfY, s.envH, xi1, pdfY);
+    uint32_t row = uint32_t(yF);
+    float xF = PwcSampleIndex(s.envCdfX + size_t(row) * s.envW, s.envW, xi0, pdfX);
+    u = xF * (1.0f / float(s.envW));
+    v = yF * (1.0f / float(s.envH));
+    pdf = pdfY * pdfX;
+}
+
+// DistributionPwC<2>::PdfUV (Distributions.h:232-243)
+__device__ __forceinline__ float EnvPdfUV(const SceneView& s, float u, float v)
+{
+    float xF = MinF(u * float(s.envW), float(s.envW) - 1.0f);
+    float yF = MinF(v * float(s.envH), float(s.envH) - 1.0f);
+    uint32_t row = uint32_t(yF);
+    float pdfMarginal = PwcPdfIndex(s.envCdfY, s.envH, yF);
+    float pdfConditional = PwcPdfIndex(s.envCdfX + size_t(row) * s.envW, s.envW, xF);
+    return pdfConditional * pdfMarginal;
+}
+
+// LightSkysphere::PdfSolidAngle (LightsDefault.hpp:359-370) of direction dir
+__device__ __forceinline__ float SkyspherePdf(const SceneView& s, V3 dir)
+{
+    if(s.envW == 0) return 1.0f * 0.25f * kInvPi;
+    V3 d = Normalize(dir);
+    float u, v;
+    DirToCoOcta(V3{d.z, d.x, d.y}, u, v);
+    return EnvPdfUV(s, u, v) * 0.25f * kInvPi;
+}
+
 template<bool SPECTRAL>
 __device__ __forceinline__ V4 ConvRadiance(const RenderConst& rc, V3 rgb, V4 waves)
 {
@@ -767,10 +892,13 @@ __device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const 
     }
     else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
     {
-        V3 dz = CoOctaToDir(rn[0], rn[1]);
+        // LightSkysphere::SampleSolidAngle (LightsDefault.hpp:340-357)
+        float u = rn[0], v = rn[1], uvPdf = 1.0f;
+        if(s.envW != 0) EnvSampleUV(s, rn[0], rn[1], u, v, uvPdf);
+        V3 dz = CoOctaToDir(u, v);
         V3 dirYUp = {dz.y, dz.z, dz.x};
         point = distantPoint + dirYUp * s.sceneDiameter;
-        pointPdf = 1.0f * 0.25f * kInvPi;
+        pointPdf = uvPdf * 0.25f * kInvPi;
     }
     else
     {
@@ -781,8 +909,9 @@ __device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const 
     V4 emission;
     if(li < lightCount - 1u) emission = TriLightEmitViaPoint<SPECTRAL>(rc, s, li, wO, point, waves);
     else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
-        emission = ConvRadiance<SPECTRAL>(rc, V3{s.boundaryRad[0], s.boundaryRad[1],
-                                                 s.boundaryRad[2]}, waves);
+        emission = ConvRadiance<SPECTRAL>(rc, s.envW ? EnvRadiance(s, Normalize(-wO))
+                                                     : V3{s.boundaryRad[0], s.boundaryRad[1],
+                                                          s.boundaryRad[2]}, waves);
     else emission = Splat4(0.0f);
     float selectionPdf = 1.0f / lightCountF;
     return {point, emission, selectionPdf * pointPdf};
@@ -810,10 +939,10 @@ __device__ __forceinline__ float PdfLightTri(const SceneView& s, uint32_t li, fl
     }
     return pdf * selectionPDF;
 }
-__device__ __forceinline__ float PdfLightBoundary(const SceneView& s)
+__device__ __forceinline__ float PdfLightBoundary(const SceneView& s, V3 rayDir)
 {
     float selectionPDF = 1.0f / float(s.lightCount + 1u);
-    float pdf = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA) ? 1.0f * 0.25f * kInvPi : 0.0f;
+    float pdf = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA) ? SkyspherePdf(s, rayDir) : 0.0f;
     return pdf * selectionPDF;
 }
 
@@ -869,9 +998,11 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
         V4 emission;
         if(isBoundary)
         {
+            // LightSkysphere::EmitViaHit (LightsDefault.hpp:405-421)
             emission = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
-                ? ConvRadiance<SPECTRAL>(rc, V3{s.boundaryRad[0], s.boundaryRad[1],
-                                                s.boundaryRad[2]}, waves)
+                ? ConvRadiance<SPECTRAL>(rc, s.envW ? EnvRadiance(s, Normalize(rdir))
+                                                    : V3{s.boundaryRad[0], s.boundaryRad[1],
+                                                         s.boundaryRad[2]}, waves)
                 : Splat4(0.0f);
         }
         else if(!BaryInRange(ha, hb)) emission = Splat4(0.0f);
@@ -897,7 +1028,7 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
             if(switchToMIS)
             {
                 float p0 = S.prevPdf[i];
-                float p1 = isBoundary ? PdfLightBoundary(s)
+                float p1 = isBoundary ? PdfLightBoundary(s, rdir)
                                       : PdfLightTri(s, li, ha, hb, rpos, rdir);
                 float misPdf = Balance2(p0, 1.0f, p1, 1.0f);
                 t2 = t2 * p0;
@@ -1632,12 +1763,6 @@ static void BuildCamera(const MRayCamera& c, uint32_t W, uint32_t H, float filte
     out.height = H;
 }
 
-#define MRAY_TRY(expr)                  \
-    do {                                \
-        int r_ = (expr);                \
-        if(r_ != MRAY_OK) return r_;    \
-    } while(0)
-
 static int SizeLaunches(MRayRenderer* r);
 
 // Path-slot state, queues, seeds and launch sizes, shared by the one-scene
@@ -1856,6 +1981,7 @@ static MRayBatchScene::Part InstancedPart(const MRayInstScene* is)
     for(int c = 0; c < 3; c++) part.boundaryRad[c] = is->view.boundaryRad[c];
     part.sceneDiameter = is->view.sceneDiameter;
     part.camera = is->camera;
+    part.env = is->view;
     return part;
 }
 
@@ -1897,6 +2023,11 @@ static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayInstS
         g.boundaryType = part.boundaryType;
         for(int c = 0; c < 3; c++) g.boundaryRad[c] = part.boundaryRad[c];
         g.sceneDiameter = part.sceneDiameter;
+        g.envTex = part.env.envTex;
+        g.envCdfX = part.env.envCdfX;
+        g.envCdfY = part.env.envCdfY;
+        g.envW = part.env.envW;
+        g.envH = part.env.envH;
         pixels += g.P;
         paths = g.pathEnd;
     }

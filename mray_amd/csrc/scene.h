@@ -32,7 +32,20 @@ struct SceneView
     uint32_t boundaryType;
     float boundaryRad[3];
     float sceneDiameter;
+    // textured Skysphere (envW != 0; else boundaryRad is constant): radiance
+    // texels in concentric-octahedral uv, row y = v, and its luminance
+    // distribution as normalized row CDFs plus the row-marginal CDF
+    const float4* __restrict__ envTex;    // [envH][envW] rgb, w unused
+    const float* __restrict__ envCdfX;    // [envH][envW]
+    const float* __restrict__ envCdfY;    // [envH]
+    uint32_t envW, envH;
 };
+
+// Builds the texture and luminance CDFs of a textured Skysphere into `ar`
+// and points `v` at them (envmap.cpp).
+int BuildEnvironment(DeviceArena& ar, SceneView& v, uint32_t W, uint32_t H, const float* hRgb);
+// Device copy of v's environment into `ar` (a merged scene's own copy).
+int CopyEnvironment(DeviceArena& ar, const SceneView& from, SceneView& to);
 
 } // namespace mray
 
@@ -89,6 +102,8 @@ struct MRayBatchScene
         float boundaryRad[3] = {0, 0, 0};
         float sceneDiameter = 0;
         MRayCamera camera{};
+        // textured Skysphere of the scene (the merged scene's own copy)
+        mray::SceneView env{};
     };
     mray::DeviceArena arena;
     std::vector<Part> parts;

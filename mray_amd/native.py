@@ -44,6 +44,8 @@ EXPORTS = [
     "mray_renderer_take_film", "mray_film_write_exr",
     "mray_instanced_scene_create", "mray_instanced_scene_destroy", "mray_instanced_scene_info",
     "mray_renderer_create_instanced", "mray_instanced_batch_create",
+    "mray_accel_set_environment", "mray_accel_environment_cdfs",
+    "mray_instanced_scene_set_environment",
 ]
 
 
@@ -127,6 +129,9 @@ def lib():
                                             C.POINTER(u32)]
     L.mray_renderer_create_instanced.argtypes = [P, C.POINTER(abi.MRayRenderParams), C.POINTER(P)]
     L.mray_instanced_batch_create.argtypes = [P, P, u32, C.POINTER(abi.MRayRenderParams), C.POINTER(P)]
+    L.mray_accel_set_environment.argtypes = [P, u32, u32, P]
+    L.mray_accel_environment_cdfs.argtypes = [P, P, P]
+    L.mray_instanced_scene_set_environment.argtypes = [P, u32, u32, P]
     _lib = L
     return L
 
@@ -263,6 +268,19 @@ class Accel:
         check(lib().mray_accel_create_ex(C.byref(self.desc), C.byref(opt), C.byref(h)))
         self.h = h
         self.n = scene.triangle_count
+        if getattr(scene, "env_map", None) is not None:
+            self.set_environment(scene.env_map)
+
+    def set_environment(self, rgb):
+        """Textured Skysphere: rgb [H, W, 3] over concentric-octahedral uv."""
+        env = np.ascontiguousarray(rgb, dtype=np.float32)
+        check(lib().mray_accel_set_environment(self.h, env.shape[1], env.shape[0], env.ctypes.data))
+
+    def environment_cdfs(self, height, width):
+        cx = np.zeros((height, width), np.float32)
+        cy = np.zeros(height, np.float32)
+        check(lib().mray_accel_environment_cdfs(self.h, cx.ctypes.data, cy.ctypes.data))
+        return cx, cy
 
     def close(self):
         if getattr(self, "h", None):
@@ -576,6 +594,10 @@ class InstancedScene:
         check(lib().mray_instanced_scene_create(arr, len(mesh_accels), inst, len(instances),
                                                 C.byref(e), C.byref(h)))
         self.h = h
+        if env.get("env_map") is not None:
+            rgb = np.ascontiguousarray(env["env_map"], dtype=np.float32)
+            check(lib().mray_instanced_scene_set_environment(self.h, rgb.shape[1], rgb.shape[0],
+                                                             rgb.ctypes.data))
 
     def close(self):
         if getattr(self, "h", None):

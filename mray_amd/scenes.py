@@ -98,6 +98,7 @@ class Scene:
     scene_diameter: float = None           # None: from the AABB, as the tracer does
     camera: dict = field(default_factory=dict)
     name: str = ""
+    env_map: np.ndarray = None             # [H,W,3] f32 Skysphere texels (CoOcta uv), or None
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -664,7 +665,8 @@ def flatten_instances(meshes, instances, env):
                  light_tris=np.concatenate(ltri), light_radiance=np.concatenate(lrad),
                  light_two_sided=np.concatenate(l2s), boundary_type=env["boundary_type"],
                  boundary_radiance=tuple(env["boundary_radiance"]),
-                 scene_diameter=env.get("scene_diameter"), camera=env["camera"], name="instances")
+                 scene_diameter=env.get("scene_diameter"), camera=env["camera"], name="instances",
+                 env_map=env.get("env_map"))
 
 
 def light_panel(count_quads, size, radiance, seed=0, span=1.0):
@@ -776,3 +778,38 @@ def instanced_views(spp=16, count=8):
                "fov": (fov, fov_y_from_x(fov, w / h)), "nearFar": (0.01, 1000.0)}
         views.append({"scene": 0, "width": w, "height": h, "spp": spp, "camera": cam})
     return views
+
+
+def coocta_to_dir_yup(u, v):
+    """Concentric octahedral uv in [0, 1]^2 -> Y-up unit directions (numpy,
+    float64): the Skysphere's texture parameterization."""
+    a, b = 2.0 * np.asarray(u, np.float64) - 1.0, 2.0 * np.asarray(v, np.float64) - 1.0
+    ua, va = np.abs(a), np.abs(b)
+    d = 1.0 - (ua + va)
+    r = 1.0 - np.abs(d)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        phi = np.where(r != 0.0, ((va - ua) / np.where(r != 0.0, r, 1.0) + 1.0) * (math.pi / 4), 0.0)
+    zs = np.where(np.signbit(d), -1.0, 1.0) * (1.0 - r * r)
+    xy = r * np.sqrt(np.maximum(0.0, 2.0 - r * r))
+    x = np.where(np.signbit(a), -1.0, 1.0) * np.cos(phi) * xy
+    y = np.where(np.signbit(b), -1.0, 1.0) * np.sin(phi) * xy
+    return np.stack([y, zs, x], axis=-1)         # Z-up (x, y, z) -> Y-up (y, z, x)
+
+
+def sky_env(width=128, height=128, sun=(0.4, 0.6, 0.3), sun_size=0.06, sun_radiance=40.0,
+            zenith=(0.15, 0.3, 0.8), horizon=(0.8, 0.85, 0.9), ground=(0.12, 0.1, 0.08)):
+    """A procedural sky texture [height, width, 3] over concentric-octahedral
+    uv: an elevation gradient from ground through horizon to zenith plus a
+    bright sun lobe (cosine power) around `sun`."""
+    ys, xs = np.meshgrid((np.arange(height) + 0.5) / height, (np.arange(width) + 0.5) / width,
+                         indexing="ij")
+    d = coocta_to_dir_yup(xs, ys)
+    el = d[..., 1]
+    t = np.clip(el, 0.0, 1.0)[..., None] ** 0.5
+    sky = np.asarray(horizon) * (1 - t) + np.asarray(zenith) * t
+    rgb = np.where((el >= 0.0)[..., None], sky, np.asarray(ground))
+    s = np.asarray(sun, np.float64) / np.linalg.norm(sun)
+    c = np.clip(d @ s, 0.0, 1.0)
+    lobe = np.exp((c - 1.0) / (sun_size ** 2))
+    rgb = rgb + sun_radiance * lobe[..., None] * np.asarray([1.0, 0.9, 0.75])
+    return rgb.astype(np.float32)

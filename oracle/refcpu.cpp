@@ -597,6 +597,11 @@ struct Scene
     V3 boundaryRad{0, 0, 0};
     float sceneDiameter = 0;
     MRayCamera cam{};
+    // textured Skysphere: texels [envH][envW], luminance CDFs
+    uint32_t envW = 0, envH = 0;
+    std::vector<V3> envTex;
+    std::vector<float> envCdfX;      // [envH][envW], rows normalized
+    std::vector<float> envCdfY;      // [envH]
     // LBVH products
     std::vector<uint64_t> codes;      // triangle order
     std::vector<uint32_t> leafOrder;  // sorted position -> triangle
@@ -1414,6 +1419,177 @@ static inline V3 CoOctaToDir(float s, float t)
     return {cosPhi * xyFactor, sinPhi * xyFactor, z};
 }
 
+// Graphics::DirectionToConcentricOctahedral (GraphicsFunctions.h:318-353):
+// Z-up unit direction -> (s, t) in [0, 1]^2; the +Z pole goes to (0, 0) as
+// in the reference.
+static inline void DirToCoOcta(V3 d, float& s, float& t)
+{
+    const float twoOvrPi = kInvPi * 2.0f;
+    if(d.x == 0.0f && d.y == 0.0f) { s = 0.0f; t = 0.0f; return; }
+    float xAbs = std::fabs(d.x), yAbs = std::fabs(d.y);
+    float phiPrime = std::atan(yAbs / xAbs);
+    float radius = std::sqrt(MaxF(0.0f, 1.0f - std::fabs(d.z)));
+    float v = radius * twoOvrPi * phiPrime;
+    float u = radius - v;
+    if(d.z < 0.0f)
+    {
+        float uPrime = 1.0f - v;
+        float vPrime = 1.0f - u;
+        u = uPrime;
+        v = vPrime;
+    }
+    u *= SignPM1(d.x);
+    v *= SignPM1(d.y);
+    s = (u + 1.0f) * 0.5f;
+    t = (v + 1.0f) * 0.5f;
+}
+
+// Skysphere texture read at a Y-up direction: the reference's default
+// texture parameters (TracerI.h:71-82: linear interpolation, wrapped
+// edges), texel centres at (i + 1/2) / size, bilinear weights in fp32.
+static V3 EnvRadiance(const Scene& s, V3 dirYUp)
+{
+    float u, v;
+    DirToCoOcta(ZUpFromYUp(dirYUp), u, v);
+    const int W = int(s.envW), H = int(s.envH);
+    float xf = u * float(W) - 0.5f, yf = v * float(H) - 0.5f;
+    float x0f = std::floor(xf), y0f = std::floor(yf);
+    float fx = xf - x0f, fy = yf - y0f;
+    auto wrap = [](int i, int n) { return ((i % n) + n) % n; };
+    int x0 = wrap(int(x0f), W), x1 = wrap(int(x0f) + 1, W);
+    int y0 = wrap(int(y0f), H), y1 = wrap(int(y0f) + 1, H);
+    const V3 a = s.envTex[size_t(y0) * W + x0], b = s.envTex[size_t(y0) * W + x1];
+    const V3 c = s.envTex[size_t(y1) * W + x0], d = s.envTex[size_t(y1) * W + x1];
+    const float gx = 1.0f - fx, gy = 1.0f - fy;
+    V3 top = {a.x * gx + b.x * fx, a.y * gx + b.y * fx, a.z * gx + b.z * fx};
+    V3 bot = {c.x * gx + d.x * fx, c.y * gx + d.y * fx, c.z * gx + d.z * fx};
+    return {top.x * gy + bot.x * fy, top.y * gy + bot.y * fy, top.z * gy + bot.z * fy};
+}
+
+// DistributionPwC<1>::SampleIndex (Distributions.h:128-148): lower bound of
+// xi in the CDF, linear position inside the bin, and the clamp as written
+// there (an index of 1 or more steps down one ulp).  Two guards of this
+// restatement: the bin index stays below n, and an empty bin gives t = 0.
+static float PwcSampleIndex(const float* cdf, uint32_t n, float xi, float& pdf)
+{
+    uint32_t index = uint32_t(std::lower_bound(cdf, cdf + n, xi) - cdf);
+    index = std::min(index, n - 1u);
+    float prevCDF = (index == 0) ? 0.0f : cdf[index - 1];
+    float myCDF = cdf[index];
+    float t = (myCDF > prevCDF) ? (xi - prevCDF) / (myCDF - prevCDF) : 0.0f;
+    float indexF = float(index) + t;
+    indexF = (indexF < 1.0f) ? indexF : std::nextafter(indexF, -INFINITY);
+    pdf = (myCDF - prevCDF) * float(n);
+    return indexF;
+}
+// DistributionPwC<1>::PdfIndex (Distributions.h:158-170)
+static float PwcPdfIndex(const float* cdf, uint32_t n, float index)
+{
+    uint32_t i = uint32_t(index);
+    float prevCDF = (i == 0) ? 0.0f : cdf[i - 1];
+    return (cdf[i] - prevCDF) * float(n);
+}
+// DistributionPwC<2>::SampleUV (Distributions.h:181-213)
+static void EnvSampleUV(const Scene& s, float xi0, float xi1, float& u, float& v, float& pdf)
+{
+    float pdfY, pdfX;
+    float yF = PwcSampleIndex(s.envCdfY.data(), s.envH, xi1, pdfY);
+    uint32_t row = uint32_t(yF);
+    float xF = PwcSampleIndex(&s.envCdfX[size_t(row) * s.envW], s.envW, xi0, pdfX);
+    u = xF * (1.0f / float(s.envW));
+    v = yF * (1.0f / float(s.envH));
+    pdf = pdfY * pdfX;
+}
+// DistributionPwC<2>::PdfUV (Distributions.h:232-243)
+static float EnvPdfUV(const Scene& s, float u, float v)
+{
+    float xF = std::min(u * float(s.envW), float(s.envW) - 1.0f);
+    float yF = std::min(v * float(s.envH), float(s.envH) - 1.0f);
+    uint32_t row = uint32_t(yF);
+    float pdfMarginal = PwcPdfIndex(s.envCdfY.data(), s.envH, yF);
+    float pdfConditional = PwcPdfIndex(&s.envCdfX[size_t(row) * s.envW], s.envW, xF);
+    return pdfConditional * pdfMarginal;
+}
+// LightSkysphere::PdfSolidAngle (LightsDefault.hpp:359-370)
+static float SkyspherePdf(const Scene& s, V3 dir)
+{
+    if(s.envW == 0) return 1.0f * 0.25f * kInvPi;
+    float u, v;
+    DirToCoOcta(ZUpFromYUp(Normalize(dir)), u, v);
+    return EnvPdfUV(s, u, v) * 0.25f * kInvPi;
+}
+static V3 SkysphereRadiance(const Scene& s, V3 dirYUp)
+{
+    return s.envW ? EnvRadiance(s, dirYUp) : s.boundaryRad;
+}
+
+// The luminance distribution of a Skysphere texture (LightGroupSkysphere::
+// Finalize, LightsDefault.hpp:860-907; DistributionGroupPwC2D::Construct,
+// Distributions.cu:465-538): Rec.709 Y per texel, rows scanned in double
+// (rounded to float per element), the marginal as a float running sum of
+// row totals, each normalized by the float reciprocal of its last entry.
+// A zero row keeps a uniform CDF (the reference divides by zero there).
+static int BuildEnv(Scene& s, uint32_t W, uint32_t H, const float* rgb)
+{
+    if(!rgb || W == 0 || H == 0 || W > 8192 || H > 8192) return -1;
+    // Y row of the Rec.709 (D65) RGB -> XYZ matrix from the primaries
+    const double Pm[3][3] = {{0.64 / 0.33, 0.30 / 0.60, 0.15 / 0.06},
+                             {1.0, 1.0, 1.0},
+                             {(1 - 0.64 - 0.33) / 0.33, (1 - 0.30 - 0.60) / 0.60,
+                              (1 - 0.15 - 0.06) / 0.06}};
+    const double Wt[3] = {0.3127 / 0.3290, 1.0, (1 - 0.3127 - 0.3290) / 0.3290};
+    auto det3 = [](const double m[3][3])
+    {
+        return m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) -
+               m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+               m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+    };
+    float yRow[3];
+    for(int c = 0; c < 3; c++)
+    {
+        double m[3][3];
+        std::memcpy(m, Pm, sizeof(m));
+        for(int r = 0; r < 3; r++) m[r][c] = Wt[r];
+        yRow[c] = float(det3(m) / det3(Pm));
+    }
+    const size_t P = size_t(W) * H;
+    s.envTex.resize(P);
+    s.envCdfX.assign(P, 0.0f);
+    s.envCdfY.assign(H, 0.0f);
+    for(size_t i = 0; i < P; i++) s.envTex[i] = {rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]};
+    float ySum = 0.0f;
+    for(uint32_t y = 0; y < H; y++)
+    {
+        double sum = 0.0;
+        for(uint32_t x = 0; x < W; x++)
+        {
+            const V3 t = s.envTex[size_t(y) * W + x];
+            const float lum = yRow[0] * t.x + yRow[1] * t.y + yRow[2] * t.z;
+            sum += double(std::fabs(lum));
+            s.envCdfX[size_t(y) * W + x] = float(sum);
+        }
+        ySum += s.envCdfX[size_t(y) * W + W - 1];
+        s.envCdfY[y] = ySum;
+    }
+    if(!(ySum > 0.0f)) return -1;
+    for(uint32_t y = 0; y < H; y++)
+    {
+        float* row = &s.envCdfX[size_t(y) * W];
+        if(row[W - 1] > 0.0f)
+        {
+            const float recip = 1.0f / row[W - 1];
+            for(uint32_t x = 0; x < W; x++) row[x] *= recip;
+        }
+        else
+            for(uint32_t x = 0; x < W; x++) row[x] = float(x + 1) / float(W);
+    }
+    const float recip = 1.0f / s.envCdfY[H - 1];
+    for(uint32_t y = 0; y < H; y++) s.envCdfY[y] *= recip;
+    s.envW = W;
+    s.envH = H;
+    return 0;
+}
+
 struct LightSampleOut
 {
     V3 position;
@@ -1475,10 +1651,13 @@ static LightSampleOut SampleLight(const Scene& s, const float* rn, V3 distantPoi
     }
     else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
     {
-        // LightSkysphere::SampleSolidAngle (LightsDefault.hpp:340-357),
-        // constant radiance: uv = xi, pdf 1; CoOcta pdf / (4 pi)
-        V3 dirYUp = YUpFromZUp(CoOctaToDir(rn[0], rn[1]));
-        float pdf = 1.0f * 0.25f * kInvPi;
+        // LightSkysphere::SampleSolidAngle (LightsDefault.hpp:340-357):
+        // constant radiance: uv = xi, pdf 1; a texture: PWC-2D by
+        // luminance; CoOcta pdf / (4 pi)
+        float u = rn[0], v = rn[1], uvPdf = 1.0f;
+        if(s.envW) EnvSampleUV(s, rn[0], rn[1], u, v, uvPdf);
+        V3 dirYUp = YUpFromZUp(CoOctaToDir(u, v));
+        float pdf = uvPdf * 0.25f * kInvPi;
         point = distantPoint + dirYUp * s.sceneDiameter;
         pointPdf = pdf;
     }
@@ -1493,7 +1672,7 @@ static LightSampleOut SampleLight(const Scene& s, const float* rn, V3 distantPoi
     if(li < lightCount - 1u)
         emission = TriLightEmitViaSurfacePoint(s, li, wO, point, sc);
     else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
-        emission = ConvertRadiance(sc, s.boundaryRad);
+        emission = ConvertRadiance(sc, SkysphereRadiance(s, Normalize(-wO)));
     else
         emission = Splat4(0.0f);
     float selectionPdf = 1.0f / lightCountF;
@@ -1524,12 +1703,12 @@ static float PdfLightTri(const Scene& s, uint32_t li, float ha, float hb, V3 ray
     }
     return pdf * selectionPDF;
 }
-static float PdfLightBoundary(const Scene& s)
+static float PdfLightBoundary(const Scene& s, V3 rayDir)
 {
     uint32_t lightCount = uint32_t(s.lightTri.size()) + 1u;
     float selectionPDF = 1.0f / float(lightCount);
     float pdf = 0.0f;
-    if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA) pdf = 1.0f * 0.25f * kInvPi;
+    if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA) pdf = SkyspherePdf(s, rayDir);
     return pdf * selectionPDF;
 }
 
@@ -1801,8 +1980,9 @@ static void BounceSlot(const Scene& s, const MRayRenderParams& p, Slot& sl, Rend
         V4 emission;
         if(isBoundary)
         {
+            // LightSkysphere::EmitViaHit (LightsDefault.hpp:405-421)
             emission = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
-                ? ConvertRadiance(sc, s.boundaryRad) : Splat4(0.0f);
+                ? ConvertRadiance(sc, SkysphereRadiance(s, Normalize(sl.dir))) : Splat4(0.0f);
         }
         else
         {
@@ -1837,7 +2017,7 @@ static void BounceSlot(const Scene& s, const MRayRenderParams& p, Slot& sl, Rend
         if(switchToMIS)
         {
             float p0 = sl.prevMatPdf;
-            float p1 = isBoundary ? PdfLightBoundary(s)
+            float p1 = isBoundary ? PdfLightBoundary(s, sl.dir)
                                   : PdfLightTri(s, li, h.a, h.b, sl.pos, sl.dir);
             float misPdf = Balance2(p0, 1.0f, p1, 1.0f);
             thr = thr * p0;
@@ -2562,9 +2742,62 @@ void rc_light_probe(void* h, const float* point, const float* rn, float* out)
 float rc_light_pdf(void* h, uint32_t li, float a, float b, const float* rayPos, const float* rayDir)
 {
     const Scene& s = *static_cast<Scene*>(h);
-    if(li >= s.lightTri.size()) return PdfLightBoundary(s);
+    if(li >= s.lightTri.size()) return PdfLightBoundary(s, V3{rayDir[0], rayDir[1], rayDir[2]});
     return PdfLightTri(s, li, a, b, V3{rayPos[0], rayPos[1], rayPos[2]},
                        V3{rayDir[0], rayDir[1], rayDir[2]});
+}
+
+// Textured Skysphere of a scene and probes of its parts.
+int rc_scene_set_environment(void* h, uint32_t W, uint32_t H, const float* rgb)
+{
+    Scene& s = *static_cast<Scene*>(h);
+    if(s.boundaryType != MRAY_BOUNDARY_SKYSPHERE_COOCTA) return -1;
+    return BuildEnv(s, W, H, rgb);
+}
+void rc_env_cdfs(void* h, float* cdfX, float* cdfY)
+{
+    const Scene& s = *static_cast<Scene*>(h);
+    std::memcpy(cdfX, s.envCdfX.data(), s.envCdfX.size() * 4);
+    std::memcpy(cdfY, s.envCdfY.data(), s.envCdfY.size() * 4);
+}
+// xi [n][2] -> out [n][3] = (u, v, pdf over uv)
+void rc_env_sample(void* h, const float* xi, uint32_t n, float* out)
+{
+    const Scene& s = *static_cast<Scene*>(h);
+    for(uint32_t i = 0; i < n; i++)
+        EnvSampleUV(s, xi[2 * i], xi[2 * i + 1], out[3 * i], out[3 * i + 1], out[3 * i + 2]);
+}
+// uv [n][2] -> pdf over uv [n]
+void rc_env_pdf_uv(void* h, const float* uv, uint32_t n, float* out)
+{
+    const Scene& s = *static_cast<Scene*>(h);
+    for(uint32_t i = 0; i < n; i++) out[i] = EnvPdfUV(s, uv[2 * i], uv[2 * i + 1]);
+}
+// Y-up directions [n][3] -> solid-angle pdf [n] and radiance [n][3]
+void rc_env_eval(void* h, const float* dirs, uint32_t n, float* pdf, float* rgb)
+{
+    const Scene& s = *static_cast<Scene*>(h);
+    for(uint32_t i = 0; i < n; i++)
+    {
+        V3 d{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]};
+        pdf[i] = SkyspherePdf(s, d);
+        V3 r = SkysphereRadiance(s, Normalize(d));
+        rgb[3 * i] = r.x; rgb[3 * i + 1] = r.y; rgb[3 * i + 2] = r.z;
+    }
+}
+// concentric octahedral map: uv [n][2] <-> Z-up unit directions [n][3]
+void rc_coocta_to_dir(const float* uv, uint32_t n, float* dirs)
+{
+    for(uint32_t i = 0; i < n; i++)
+    {
+        V3 d = CoOctaToDir(uv[2 * i], uv[2 * i + 1]);
+        dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+    }
+}
+void rc_dir_to_coocta(const float* dirs, uint32_t n, float* uv)
+{
+    for(uint32_t i = 0; i < n; i++)
+        DirToCoOcta(V3{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]}, uv[2 * i], uv[2 * i + 1]);
 }
 
 } // extern "C"

@@ -69,6 +69,14 @@ def lib():
         L.rc_light_probe.argtypes = [P, f32, f32, f32]
         L.rc_light_pdf.restype = C.c_float
         L.rc_light_pdf.argtypes = [P, C.c_uint32, C.c_float, C.c_float, f32, f32]
+        L.rc_scene_set_environment.restype = C.c_int
+        L.rc_scene_set_environment.argtypes = [P, C.c_uint32, C.c_uint32, f32]
+        L.rc_env_cdfs.argtypes = [P, f32, f32]
+        L.rc_env_sample.argtypes = [P, f32, C.c_uint32, f32]
+        L.rc_env_pdf_uv.argtypes = [P, f32, C.c_uint32, f32]
+        L.rc_env_eval.argtypes = [P, f32, C.c_uint32, f32, f32]
+        L.rc_coocta_to_dir.argtypes = [f32, C.c_uint32, f32]
+        L.rc_dir_to_coocta.argtypes = [f32, C.c_uint32, f32]
         _lib = L
     return _lib
 
@@ -151,6 +159,42 @@ class OracleScene:
         self.desc = scene.desc()
         self.h = lib().rc_scene_create(C.byref(self.desc))
         self.n = scene.triangle_count
+        self.env_shape = None
+        if getattr(scene, "env_map", None) is not None:
+            self.set_environment(scene.env_map)
+
+    def set_environment(self, rgb):
+        env = np.ascontiguousarray(rgb, dtype=np.float32)
+        if lib().rc_scene_set_environment(self.h, env.shape[1], env.shape[0], abi.fptr(env)) != 0:
+            raise ValueError("environment rejected (boundary not a Skysphere, bad size or zero luminance)")
+        self.env_shape = env.shape[:2]
+        self._env = env
+
+    def env_cdfs(self):
+        h, w = self.env_shape
+        cx, cy = np.zeros((h, w), np.float32), np.zeros(h, np.float32)
+        lib().rc_env_cdfs(self.h, abi.fptr(cx), abi.fptr(cy))
+        return cx, cy
+
+    def env_sample(self, xi):
+        xi = _f(xi).reshape(-1, 2)
+        out = np.zeros((len(xi), 3), np.float32)
+        lib().rc_env_sample(self.h, abi.fptr(xi), len(xi), abi.fptr(out))
+        return out
+
+    def env_pdf_uv(self, uv):
+        uv = _f(uv).reshape(-1, 2)
+        out = np.zeros(len(uv), np.float32)
+        lib().rc_env_pdf_uv(self.h, abi.fptr(uv), len(uv), abi.fptr(out))
+        return out
+
+    def env_eval(self, dirs):
+        """Solid-angle pdf and radiance of the Skysphere at Y-up directions."""
+        d = _f(dirs).reshape(-1, 3)
+        pdf = np.zeros(len(d), np.float32)
+        rgb = np.zeros((len(d), 3), np.float32)
+        lib().rc_env_eval(self.h, abi.fptr(d), len(d), abi.fptr(pdf), abi.fptr(rgb))
+        return pdf, rgb
 
     def close(self):
         if self.h:
@@ -222,6 +266,20 @@ class OracleScene:
     def light_pdf(self, li, a, b, ray_pos, ray_dir):
         return float(lib().rc_light_pdf(self.h, li, a, b, abi.fptr(_f(ray_pos)),
                                         abi.fptr(_f(ray_dir))))
+
+
+def coocta_to_dir(uv):
+    uv = _f(uv).reshape(-1, 2)
+    out = np.zeros((len(uv), 3), np.float32)
+    lib().rc_coocta_to_dir(abi.fptr(uv), len(uv), abi.fptr(out))
+    return out
+
+
+def dir_to_coocta(dirs):
+    d = _f(dirs).reshape(-1, 3)
+    out = np.zeros((len(d), 2), np.float32)
+    lib().rc_dir_to_coocta(abi.fptr(d), len(d), abi.fptr(out))
+    return out
 
 
 def render_batch(scenes, views, params, threads=0):

@@ -1,0 +1,152 @@
+"""Textured Skysphere in the oracle, pinned by what the mathematics fixes.
+
+* The concentric octahedral map and its inverse (GraphicsFunctions.h:318-381)
+  are inverses of each other and equal-area (uniform uv gives uniform
+  directions: zero mean, E[y^2] = 1/3).
+* The luminance distribution (Distributions.cu:465-538) equals its closed
+  form: per row the double-precision running sum of |Y| rounded per element,
+  times the float reciprocal of the row total; the marginal likewise over
+  row totals; Y is the Rec.709 luminance row (0.2126, 0.7152, 0.0722).
+* Sampling (Distributions.h:128-243): texel hit counts of uniform xi
+  follow luminance (within 4.5 sigma); the returned pdf equals PdfUV at the returned uv;
+  PdfUV at a texel centre is |Y| / mean|Y|; the solid-angle pdf is that
+  over 4 pi.
+* The texture read is bilinear with wrapped edges.
+* Renders: a constant texture gives the constant Skysphere's image; with a
+  peaky sky, next-event estimation (NEE-only and NEE+MIS) and BSDF-only
+  sampling converge to the same image mean, which a wrong pdf breaks.
+"""
+import math
+
+import numpy as np
+from mray_amd import abi, scenes
+from oracle import refcpu
+from tests.conftest import film_image, relmse
+
+Y_ROW = np.array([0.2126390058715103, 0.7151686787677559, 0.0721923153607337])
+
+
+def _sky_scene(env, width=24, height=18, light_grid=0, seed=2):
+    sc = scenes.terrain_spheres(grid=24, spheres=6, width=width, height=height,
+                                light_grid=light_grid, seed=seed)
+    sc.env_map = env
+    return sc
+
+
+def test_coocta_round_trip_and_equal_area():
+    rng = np.random.default_rng(1)
+    uv = rng.uniform(0.01, 0.99, (4000, 2)).astype(np.float32)
+    d = refcpu.coocta_to_dir(uv)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=2e-6)
+    back = refcpu.dir_to_coocta(d)
+    assert np.allclose(back, uv, atol=5e-6)
+    g = (np.arange(256) + 0.5) / 256
+    uu, vv = np.meshgrid(g, g)
+    dd = refcpu.coocta_to_dir(np.stack([uu.ravel(), vv.ravel()], 1))
+    assert np.all(np.abs(dd.mean(axis=0)) < 2e-3)
+    assert np.allclose((dd ** 2).mean(axis=0), 1.0 / 3.0, atol=2e-3)
+    # the reference's pole quirk: +Z maps to uv (0, 0), which is -Z
+    assert np.array_equal(refcpu.dir_to_coocta([[0.0, 0.0, 1.0]])[0], [0.0, 0.0])
+
+
+def _env_and_oracle(w=16, h=8, seed=3):
+    rng = np.random.default_rng(seed)
+    env = rng.uniform(0.0, 1.0, (h, w, 3)).astype(np.float32) ** 3
+    env[2] = 0.0                               # one black row
+    env[5, 3] = (30.0, 20.0, 10.0)             # one bright texel
+    sc = _sky_scene(env)
+    return env, refcpu.OracleScene(sc)
+
+
+def test_cdfs_equal_closed_form():
+    assert np.allclose(Y_ROW, [0.2126, 0.7152, 0.0722], atol=1e-4)
+    env, o = _env_and_oracle()
+    cx, cy = o.env_cdfs()
+    h, w = env.shape[:2]
+    yrow = Y_ROW.astype(np.float32)
+    lum = (yrow[0] * env[..., 0] + yrow[1] * env[..., 1]) + yrow[2] * env[..., 2]
+    rows = np.cumsum(np.abs(lum).astype(np.float64), axis=1).astype(np.float32)
+    tot = rows[:, -1].copy()
+    ycdf = np.add.accumulate(tot, dtype=np.float32)
+    for y in range(h):
+        if tot[y] > 0:
+            ref = rows[y] * (np.float32(1.0) / tot[y])
+        else:
+            ref = (np.arange(1, w + 1) / np.float32(w)).astype(np.float32)
+        np.testing.assert_allclose(cx[y], ref, rtol=2e-7, atol=0)
+    np.testing.assert_allclose(cy, ycdf * (np.float32(1.0) / ycdf[-1]), rtol=2e-7, atol=0)
+    assert cy[2] == cy[1]                      # the black row has zero probability
+
+
+def test_sampling_follows_luminance():
+    env, o = _env_and_oracle()
+    h, w = env.shape[:2]
+    xi = np.random.default_rng(11).uniform(0.0, 1.0, (1 << 18, 2)).astype(np.float32)
+    s = o.env_sample(xi)
+    u, v, pdf = s[:, 0], s[:, 1], s[:, 2]
+    assert np.all((u >= 0) & (u < 1) & (v >= 0) & (v < 1))
+    tx, ty = np.minimum((u * w).astype(int), w - 1), np.minimum((v * h).astype(int), h - 1)
+    counts = np.bincount(ty * w + tx, minlength=w * h).reshape(h, w)
+    lum = np.abs(env @ Y_ROW)
+    expect = lum / lum.sum() * len(xi)
+    assert np.all(np.abs(counts - expect) <= 4.5 * np.sqrt(expect) + 3.0)
+    assert counts[2].sum() == 0
+    # the sample's pdf is PdfUV at its uv (away from bin edges)
+    p2 = o.env_pdf_uv(s[:, :2])
+    inner = (np.abs(u * w - np.round(u * w)) > 1e-3) & (np.abs(v * h - np.round(v * h)) > 1e-3)
+    np.testing.assert_allclose(pdf[inner], p2[inner], rtol=1e-4)
+    # PdfUV at texel centres: |Y| / mean |Y|; integrates to one
+    cu, cv = np.meshgrid((np.arange(w) + 0.5) / w, (np.arange(h) + 0.5) / h)
+    pc = o.env_pdf_uv(np.stack([cu.ravel(), cv.ravel()], 1)).reshape(h, w)
+    np.testing.assert_allclose(pc, lum / lum.mean(), rtol=1e-4, atol=1e-6)
+    assert abs(pc.mean() - 1.0) < 1e-5
+
+
+def test_radiance_is_bilinear_and_pdf_is_per_solid_angle():
+    env, o = _env_and_oracle()
+    h, w = env.shape[:2]
+    rng = np.random.default_rng(5)
+    uv = rng.uniform(0.02, 0.98, (500, 2)).astype(np.float32)
+    dz = refcpu.coocta_to_dir(uv)
+    dyup = dz[:, [1, 2, 0]]
+    pdf, rgb = o.env_eval(dyup)
+    uvr = refcpu.dir_to_coocta(dz)             # the uv the read sees
+    xf, yf = uvr[:, 0] * w - 0.5, uvr[:, 1] * h - 0.5
+    x0, y0 = np.floor(xf).astype(int), np.floor(yf).astype(int)
+    fx, fy = (xf - x0)[:, None], (yf - y0)[:, None]
+    t = lambda x, y: env[y % h, x % w]
+    ref = ((t(x0, y0) * (1 - fx) + t(x0 + 1, y0) * fx) * (1 - fy) +
+           (t(x0, y0 + 1) * (1 - fx) + t(x0 + 1, y0 + 1) * fx) * fy)
+    np.testing.assert_allclose(rgb, ref, rtol=1e-4, atol=1e-6)
+    np.testing.assert_allclose(pdf, o.env_pdf_uv(uvr) / (4 * math.pi), rtol=1e-5)
+
+
+def test_constant_texture_equals_constant_skysphere():
+    sc = scenes.terrain_spheres(grid=24, spheres=6, width=24, height=18, seed=2)
+    p = abi.render_params(24, 18, 8, rr=(3, 8), seed=4)
+    ref, _ = refcpu.OracleScene(sc).render(p)
+    sc.env_map = np.tile(np.asarray(sc.boundary_radiance, np.float32), (8, 16, 1))
+    img, _ = refcpu.OracleScene(sc).render(p)
+    e = relmse(film_image(img), film_image(ref))
+    assert e <= 1e-5, e
+
+
+def test_nee_and_bsdf_sampling_agree_under_a_peaky_sky():
+    """Image means of NEE-only, NEE+MIS and BSDF-only renders of terrain
+    seen from above, lit only by a sky with a sun lobe 100x its dimmest
+    texels: equal within 1.5 % (seeds differ by 0.1-0.4 % here), so a pdf
+    that is off by a few percent fails."""
+    env = scenes.sky_env(64, 64, sun_size=0.35, sun_radiance=4.0)
+    sc = _sky_scene(env, width=16, height=12)
+    sc.camera = {"position": (0.0, 1.5, 0.01), "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
+                 "fov": (math.radians(50),) * 2, "nearFar": (0.01, 100.0)}
+    o = refcpu.OracleScene(sc)
+    means = {}
+    for name, mode, spp in (("nee", abi.SAMPLE_NEE, 256), ("mis", abi.SAMPLE_NEE_MIS, 256),
+                            ("pure", abi.SAMPLE_PURE, 1024)):
+        p = abi.render_params(16, 12, spp, rr=(3, 4), mode=mode, seed=7)
+        f, _ = o.render(p)
+        means[name] = film_image(f).astype(np.float64).mean(axis=(1, 2))
+    for k in ("nee", "mis"):
+        rel = np.abs(means[k] - means["pure"]) / means["pure"]
+        assert np.all(rel < 0.015), (k, means)
