@@ -91,14 +91,14 @@ struct WideF
 };
 
 // Top-down collapse of a binary tree into W-wide nodes: expand the child with
-// the largest surface area until W children; subtrees of <= kLeafMax
-// triangles become leaves over their leaf-order range.  Nodes are numbered in
+// the largest surface area until W children; subtrees of <= leafMax
+// primitives become leaves over their leaf-order range.  Nodes are numbered in
 // depth-first order (root 0).
 template<int W>
 void CollapseWide(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
                   const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
                   const std::vector<uint32_t>& leafOrder, std::vector<WideF<W>>& out,
-                  uint32_t& maxDepth, uint32_t& leafSlots)
+                  uint32_t& maxDepth, uint32_t& leafSlots, uint32_t leafMax = kLeafMax)
 {
     const uint32_t n = uint32_t(leafOrder.size());
     maxDepth = 0;
@@ -169,7 +169,7 @@ void CollapseWide(const std::vector<uint32_t>& binNodes, const std::vector<float
             for(int k = 0; k < count; k++)
             {
                 uint32_t cnt = kids[k].hi - kids[k].lo + 1;
-                if(kids[k].leaf || cnt <= kLeafMax) continue;
+                if(kids[k].leaf || cnt <= leafMax) continue;
                 float ar = Area(kids[k].box);
                 if(ar > bestArea) { bestArea = ar; best = k; }
             }
@@ -187,7 +187,7 @@ void CollapseWide(const std::vector<uint32_t>& binNodes, const std::vector<float
             const BChild& c = kids[k];
             std::copy_n(c.box, 6, nd.box[k]);
             uint32_t cnt = c.hi - c.lo + 1;
-            if(cnt <= kLeafMax)
+            if(cnt <= leafMax)
             {
                 nd.child[k] = kLeafFlag | ((cnt - 1) << kLeafCountShift) | c.lo;
                 leafSlots++;
@@ -221,12 +221,12 @@ int QuantExponent(float lo, float hi)
 
 WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
                        const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
-                       const std::vector<uint32_t>& leafOrder)
+                       const std::vector<uint32_t>& leafOrder, uint32_t leafMax)
 {
     WideBvh out;
     std::vector<WideF<4>> wide;
     CollapseWide<4>(binNodes, binBoxes, binRange, leafAabbs, leafOrder, wide, out.maxDepth,
-                    out.leafSlots);
+                    out.leafSlots, leafMax);
     out.nodes.resize(wide.size());
     for(size_t i = 0; i < wide.size(); i++)
     {

@@ -158,7 +158,9 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
             SetError("instanced_scene_create: instance tree construction failed");
             return MRAY_ERR_INVALID;
         }
-        w = CollapseToBvh4(bn, bb, br, boxes, order);
+        // one instance per leaf: each instance's own world box is tested in
+        // its parent before its mesh tree is entered
+        w = CollapseToBvh4(bn, bb, br, boxes, order, 1u);
     }
     else
     {

@@ -101,10 +101,11 @@ bool BuildSahBinary(const std::vector<float>& aabbs, uint32_t n, std::vector<uin
 
 // Collapse the binary LBVH (reference layout) into BNode4s, top-down,
 // expanding the child with the largest surface area until four children;
-// subtrees of <= kLeafMax triangles become leaves over their leaf-order range.
+// subtrees of <= leafMax primitives become leaves over their leaf-order range.
 WideBvh CollapseToBvh4(const std::vector<uint32_t>& binNodes, const std::vector<float>& binBoxes,
                        const std::vector<uint32_t>& binRange, const std::vector<float>& leafAabbs,
-                       const std::vector<uint32_t>& leafOrder);
+                       const std::vector<uint32_t>& leafOrder,
+                       uint32_t leafMax = kLeafMax);
 
 // Same collapse with eight children per node, then quantised into CNode8s;
 // false if a box cannot be represented conservatively.
