@@ -130,6 +130,11 @@ typedef struct MRayRenderParams
 /* Launch every bounce directly instead of replaying a captured hipGraph of
  * 8 bounces (mray_renderer_iterate on the renderer's own stream).          */
 #define MRAY_RENDER_NO_GRAPH 2u
+/* NEE picks triangle lights in proportion to emitted power (luminance x
+ * area, x2 if two-sided) through an alias table, instead of uniformly
+ * (LightSampler.hpp:30-119); the boundary light keeps the uniform
+ * sampler's probability 1 / (lights + 1).  This build's own sampler.     */
+#define MRAY_RENDER_LIGHT_POWER 4u
 
 /* Per-iteration analytics (Common/AnalyticStructs.h:33-52). */
 typedef struct MRayRenderStats
@@ -358,6 +363,13 @@ int mray_batch_autotune(const MRayBatchScene* scene4, const MRayBatchScene* scen
 /* Use traversal workgroup size tpb (128 or 256) for a batch renderer's
  * closest-hit and shadow launches (the autotuner's knob; default 256).     */
 int mray_batch_set_trace_tpb(MRayRenderer* r, uint32_t tpb);
+
+/* The light-selection tables of a renderer created with
+ * MRAY_RENDER_LIGHT_POWER, one entry per emissive triangle of its scene(s):
+ * hProb [n] the bucket's own share, hAlias [n] its alias (an index within
+ * its scene's lights), hPmf [n] each light's selection probability.  No
+ * reference counterpart (its light sampler is uniform).                     */
+int mray_renderer_light_table(const MRayRenderer* r, float* hProb, uint32_t* hAlias, float* hPmf);
 
 /* ---------------------------------------------------------------------- */
 /* Instanced scenes: meshes placed many times under a tree over instances  */

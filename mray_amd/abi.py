@@ -235,6 +235,7 @@ class MRayNrcRenderParams(C.Structure):
 
 RENDER_MATERIAL_QUEUES = 1
 RENDER_NO_GRAPH = 2
+RENDER_LIGHT_POWER = 4
 
 
 def render_params(width, height, spp, rr=(8, 8), mode=SAMPLE_NEE_MIS, spectral=False,

@@ -297,6 +297,11 @@ __device__ __forceinline__ SceneView SlotScene(const RenderConst& rc, const Slot
         s.envCdfY = v.envCdfY;
         s.envW = v.envW;
         s.envH = v.envH;
+        if(s.lightAlias)
+        {
+            s.lightAlias += v.lightOff;
+            s.lightPmf += v.lightOff;
+        }
     }
     return s;
 }
@@ -864,6 +869,15 @@ __device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const 
     float lightCountF = float(lightCount);
     uint32_t li = uint32_t(rn[2] * lightCountF);
     li = min(li, lightCount - 1u);
+    float selectionPdf = 1.0f / lightCountF;
+    if(s.lightAlias && li < lightCount - 1u)
+    {
+        // power sampling: stratum li of the n + 1 is the alias bucket li
+        const float f = rn[2] * lightCountF - float(li);
+        const uint2 e = s.lightAlias[li];
+        li = (f < __uint_as_float(e.x)) ? li : e.y;
+        selectionPdf = s.lightPmf[li];
+    }
     V3 point;
     float pointPdf;
     if(li < lightCount - 1u)
@@ -913,14 +927,13 @@ __device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const 
                                                      : V3{s.boundaryRad[0], s.boundaryRad[1],
                                                           s.boundaryRad[2]}, waves);
     else emission = Splat4(0.0f);
-    float selectionPdf = 1.0f / lightCountF;
     return {point, emission, selectionPdf * pointPdf};
 }
 
 __device__ __forceinline__ float PdfLightTri(const SceneView& s, uint32_t li, float ha, float hb,
                                              V3 rayPos, V3 rayDir)
 {
-    float selectionPDF = 1.0f / float(s.lightCount + 1u);
+    float selectionPDF = s.lightAlias ? s.lightPmf[li] : 1.0f / float(s.lightCount + 1u);
     V3 p0, p1, p2;
     LoadLightTri(s, li, p0, p1, p2);
     float pdf;
@@ -1525,6 +1538,10 @@ struct MRayRenderer
     TraceCtx ctx{};
     // instanced scene (mray_renderer_create_instanced)
     const MRayInstScene* iscene = nullptr;
+    // light selection by power (MRAY_RENDER_LIGHT_POWER): the tables
+    uint2* dLightAlias = nullptr;
+    float* dLightPmf = nullptr;
+    uint32_t lightTableCount = 0;
     // batch (mray_batch_create, mray_instanced_batch_create): per-view
     // cameras and films; the scene is a merged batch scene or an instanced one
     bool batch = false;
@@ -1904,6 +1921,9 @@ static int SizeLaunches(MRayRenderer* r)
     return MRAY_OK;
 }
 
+static int SetupLightPower(MRayRenderer* r, const std::vector<uint32_t>& offs,
+                           const std::vector<uint32_t>& counts, uint32_t total);
+
 static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDesc* scene,
                           const MRayRenderParams* p)
 {
@@ -1926,8 +1946,25 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
     float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
     BuildCamera(scene ? scene->camera : a->camera, p->width, p->height, fr, r->rc.cam);
     r->limit = (unsigned long long)p->totalSPP * P;
+    MRAY_TRY(SetupLightPower(r, {0u}, {a->view.lightCount}, a->view.lightCount));
     MRAY_TRY(InitSlots(r, size_t(P)));
     return ResetRenderer(r);
+}
+
+// MRAY_RENDER_LIGHT_POWER: alias tables over the given light ranges of the
+// renderer's scene view (one range per scene of a batch).
+static int SetupLightPower(MRayRenderer* r, const std::vector<uint32_t>& offs,
+                           const std::vector<uint32_t>& counts, uint32_t total)
+{
+    r->rc.scene.lightAlias = nullptr;
+    r->rc.scene.lightPmf = nullptr;
+    if(!(r->params.flags & MRAY_RENDER_LIGHT_POWER)) return MRAY_OK;
+    MRAY_TRY(BuildLightPower(r->arena, r->rc.scene, offs, counts, total, &r->dLightAlias,
+                             &r->dLightPmf));
+    r->lightTableCount = total;
+    r->rc.scene.lightAlias = r->dLightAlias;
+    r->rc.scene.lightPmf = r->dLightPmf;
+    return MRAY_OK;
 }
 
 static int CreateInstanced(MRayRenderer* r, const MRayInstScene* sc, const MRayRenderParams* p)
@@ -1951,6 +1988,7 @@ static int CreateInstanced(MRayRenderer* r, const MRayInstScene* sc, const MRayR
     const float fr = (p->filterRadius > 0.0f) ? p->filterRadius : 1.0f;
     BuildCamera(sc->camera, p->width, p->height, fr, r->rc.cam);
     r->limit = (unsigned long long)p->totalSPP * P;
+    MRAY_TRY(SetupLightPower(r, {0u}, {sc->view.lightCount}, sc->view.lightCount));
     MRAY_TRY(InitSlots(r, size_t(P)));
     return ResetRenderer(r);
 }
@@ -2057,6 +2095,17 @@ static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayInstS
     }
     r->rc.viewBegin = r->dViewBegin;
     r->rc.viewClock = r->dViewClock;
+    if(is) MRAY_TRY(SetupLightPower(r, {0u}, {is->view.lightCount}, is->view.lightCount));
+    else
+    {
+        std::vector<uint32_t> offs, counts;
+        for(const MRayBatchScene::Part& part : b->parts)
+        {
+            offs.push_back(part.lightOff);
+            counts.push_back(part.lightCount);
+        }
+        MRAY_TRY(SetupLightPower(r, offs, counts, b->lights));
+    }
     return ResetRenderer(r);
 }
 
@@ -2505,6 +2554,27 @@ int mray_instanced_batch_create(const MRayInstScene* scene, const MRayBatchView*
         return rv;
     }
     *out = r;
+    return MRAY_OK;
+}
+
+int mray_renderer_light_table(const MRayRenderer* r, float* hProb, uint32_t* hAlias, float* hPmf)
+{
+    if(!r || !hProb || !hAlias || !hPmf) { SetError("renderer_light_table: NULL argument"); return MRAY_ERR_INVALID; }
+    if(!r->dLightAlias)
+    {
+        SetError("renderer_light_table: the renderer samples lights uniformly");
+        return MRAY_ERR_INVALID;
+    }
+    std::vector<uint2> packed(r->lightTableCount);
+    MRAY_HIP_CHECK(hipMemcpy(packed.data(), r->dLightAlias, packed.size() * sizeof(uint2),
+                             hipMemcpyDeviceToHost));
+    MRAY_HIP_CHECK(hipMemcpy(hPmf, r->dLightPmf, size_t(r->lightTableCount) * sizeof(float),
+                             hipMemcpyDeviceToHost));
+    for(uint32_t i = 0; i < r->lightTableCount; i++)
+    {
+        std::memcpy(&hProb[i], &packed[i].x, 4);
+        hAlias[i] = packed[i].y;
+    }
     return MRAY_OK;
 }
 

@@ -39,7 +39,21 @@ struct SceneView
     const float* __restrict__ envCdfX;    // [envH][envW]
     const float* __restrict__ envCdfY;    // [envH]
     uint32_t envW, envH;
+    // light selection by power (null: uniform): per light {prob bits, alias}
+    // of its range's alias table, and its selection probability
+    const uint2* __restrict__ lightAlias;
+    const float* __restrict__ lightPmf;
 };
+
+// Alias table of n lights (pos9 [n][9] world vertices, rad [n] rgb + two-sided
+// flag) by emitted power (lightpower.cpp).
+void LightPowerTable(const float* pos9, const float4* rad, uint32_t n, float* prob, uint32_t* alias,
+                     float* pmf);
+// Tables for the light ranges [rangeOff, +rangeCount) of v's light arrays
+// (total lights), uploaded into `ar`.
+int BuildLightPower(DeviceArena& ar, const SceneView& v, const std::vector<uint32_t>& rangeOff,
+                    const std::vector<uint32_t>& rangeCount, uint32_t total, uint2** outAlias,
+                    float** outPmf);
 
 // Builds the texture and luminance CDFs of a textured Skysphere into `ar`
 // and points `v` at them (envmap.cpp).

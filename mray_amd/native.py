@@ -45,7 +45,7 @@ EXPORTS = [
     "mray_instanced_scene_create", "mray_instanced_scene_destroy", "mray_instanced_scene_info",
     "mray_renderer_create_instanced", "mray_instanced_batch_create",
     "mray_accel_set_environment", "mray_accel_environment_cdfs",
-    "mray_instanced_scene_set_environment",
+    "mray_instanced_scene_set_environment", "mray_renderer_light_table",
 ]
 
 
@@ -132,6 +132,7 @@ def lib():
     L.mray_accel_set_environment.argtypes = [P, u32, u32, P]
     L.mray_accel_environment_cdfs.argtypes = [P, P, P]
     L.mray_instanced_scene_set_environment.argtypes = [P, u32, u32, P]
+    L.mray_renderer_light_table.argtypes = [P, P, P, P]
     _lib = L
     return L
 
@@ -401,6 +402,14 @@ class Renderer:
         out = np.zeros((4, self.height, self.width), np.float32)
         check(lib().mray_renderer_take_film(self.h, out.ctypes.data))
         return out
+
+    def light_table(self, n):
+        """(prob, alias, pmf) of the n lights' power-sampling tables."""
+        prob, pmf = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        alias = np.zeros(n, np.uint32)
+        check(lib().mray_renderer_light_table(self.h, prob.ctypes.data, alias.ctypes.data,
+                                              pmf.ctypes.data))
+        return prob, alias, pmf
 
     def attach_nrc(self, nrc, train_shift=4, spread=0.01, batch=1 << 14, steps=4, min_depth=1,
                    nee_at_end=False):

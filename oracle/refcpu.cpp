@@ -602,6 +602,10 @@ struct Scene
     std::vector<V3> envTex;
     std::vector<float> envCdfX;      // [envH][envW], rows normalized
     std::vector<float> envCdfY;      // [envH]
+    // NEE light selection by power (used with MRAY_RENDER_LIGHT_POWER)
+    std::vector<float> lightProb;
+    std::vector<uint32_t> lightAlias;
+    std::vector<float> lightPmf;
     // LBVH products
     std::vector<uint64_t> codes;      // triangle order
     std::vector<uint32_t> leafOrder;  // sorted position -> triangle
@@ -1611,9 +1615,72 @@ static inline V4 TriLightEmitViaSurfacePoint(const Scene& s, uint32_t li, V3 wO,
     return ConvertRadiance(sc, s.lightRad[li]);
 }
 
+// Power sampling (MRAY_RENDER_LIGHT_POWER, the build's own sampler):
+// weights Y(radiance) * area (x2 two-sided), an alias table built with
+// Vose's method in double over the emissive triangles; the boundary light
+// keeps probability 1 / (n + 1); a triangle light gets n / (n + 1) * w / W.
+static void BuildLightPowerTable(Scene& s)
+{
+    const uint32_t n = uint32_t(s.lightTri.size());
+    s.lightProb.assign(n, 1.0f);
+    s.lightAlias.assign(n, 0u);
+    s.lightPmf.assign(n, 0.0f);
+    if(n == 0) return;
+    std::vector<double> w(n);
+    double total = 0.0;
+    for(uint32_t i = 0; i < n; i++)
+    {
+        V3 p0, p1, p2;
+        s.Tri(s.lightTri[i], p0, p1, p2);
+        const double ax = double(p1.x) - p0.x, ay = double(p1.y) - p0.y, az = double(p1.z) - p0.z;
+        const double bx = double(p2.x) - p0.x, by = double(p2.y) - p0.y, bz = double(p2.z) - p0.z;
+        const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+        const double area = 0.5 * std::sqrt(cx * cx + cy * cy + cz * cz);
+        const V3 L = s.lightRad[i];
+        const double lum = std::fabs(0.2126390058715103 * L.x + 0.7151686787677559 * L.y +
+                                     0.0721923153607337 * L.z);
+        double wi = lum * area * (s.lightTwoSided[i] ? 2.0 : 1.0);
+        if(!std::isfinite(wi)) wi = 0.0;
+        w[i] = wi;
+        total += wi;
+    }
+    if(!(total > 0.0) || !std::isfinite(total))
+    {
+        std::fill(w.begin(), w.end(), 1.0);
+        total = double(n);
+    }
+    std::vector<double> q(n);
+    std::vector<uint32_t> under, over;
+    for(uint32_t i = 0; i < n; i++)
+    {
+        q[i] = w[i] * double(n) / total;
+        if(q[i] < 1.0) under.push_back(i); else over.push_back(i);
+    }
+    while(!under.empty() && !over.empty())
+    {
+        uint32_t a = under.back();
+        under.pop_back();
+        uint32_t b = over.back();
+        s.lightProb[a] = float(q[a]);
+        s.lightAlias[a] = b;
+        q[b] = (q[b] + q[a]) - 1.0;
+        if(q[b] < 1.0)
+        {
+            over.pop_back();
+            under.push_back(b);
+        }
+    }
+    for(uint32_t i : over) { s.lightProb[i] = 1.0f; s.lightAlias[i] = i; }
+    for(uint32_t i : under) { s.lightProb[i] = 1.0f; s.lightAlias[i] = i; }
+    const double share = double(n) / double(n + 1);
+    for(uint32_t i = 0; i < n; i++) s.lightPmf[i] = float(share * w[i] / total);
+}
+
 // DirectLightSamplerUniform::SampleLight (LightSampler.hpp:30-74); rn[0..1]
-// sample the point, rn[2] picks the light.
-static LightSampleOut SampleLight(const Scene& s, const float* rn, V3 distantPoint, const SpecCtx& sc)
+// sample the point, rn[2] picks the light (power: the stratum of a triangle
+// light is its alias bucket).
+static LightSampleOut SampleLight(const Scene& s, const float* rn, V3 distantPoint, const SpecCtx& sc,
+                                  bool power = false)
 {
     uint32_t lightCount = uint32_t(s.lightTri.size()) + 1u;
     float lightCountF = float(lightCount);
@@ -1621,6 +1688,13 @@ static LightSampleOut SampleLight(const Scene& s, const float* rn, V3 distantPoi
     float lightIndexF = dXi * lightCountF;
     uint32_t li = uint32_t(lightIndexF);
     li = std::min(std::max(li, 0u), lightCount - 1u);
+    float selectionPdf = 1.0f / lightCountF;
+    if(power && li < lightCount - 1u)
+    {
+        const float f = rn[2] * lightCountF - float(li);
+        li = (f < s.lightProb[li]) ? li : s.lightAlias[li];
+        selectionPdf = s.lightPmf[li];
+    }
     V3 point;
     float pointPdf;
     V4 emission;
@@ -1675,16 +1749,16 @@ static LightSampleOut SampleLight(const Scene& s, const float* rn, V3 distantPoi
         emission = ConvertRadiance(sc, SkysphereRadiance(s, Normalize(-wO)));
     else
         emission = Splat4(0.0f);
-    float selectionPdf = 1.0f / lightCountF;
     return {point, emission, selectionPdf * pointPdf};
 }
 
 // DirectLightSamplerUniform::PdfLight (LightSampler.hpp:76-119) for a hit
 // triangle light or the boundary light; rayPos/rayDir of the path ray.
-static float PdfLightTri(const Scene& s, uint32_t li, float ha, float hb, V3 rayPos, V3 rayDir)
+static float PdfLightTri(const Scene& s, uint32_t li, float ha, float hb, V3 rayPos, V3 rayDir,
+                         bool power = false)
 {
     uint32_t lightCount = uint32_t(s.lightTri.size()) + 1u;
-    float selectionPDF = 1.0f / float(lightCount);
+    float selectionPDF = power ? s.lightPmf[li] : 1.0f / float(lightCount);
     V3 p0, p1, p2;
     s.Tri(s.lightTri[li], p0, p1, p2);
     // LightPrim::PdfSolidAngle (LightsDefault.hpp:46-66)
@@ -2018,7 +2092,8 @@ static void BounceSlot(const Scene& s, const MRayRenderParams& p, Slot& sl, Rend
         {
             float p0 = sl.prevMatPdf;
             float p1 = isBoundary ? PdfLightBoundary(s, sl.dir)
-                                  : PdfLightTri(s, li, h.a, h.b, sl.pos, sl.dir);
+                                  : PdfLightTri(s, li, h.a, h.b, sl.pos, sl.dir,
+                                                (p.flags & MRAY_RENDER_LIGHT_POWER) != 0);
             float misPdf = Balance2(p0, 1.0f, p1, 1.0f);
             thr = thr * p0;
             thr = DivideByPDF(thr, misPdf);
@@ -2042,7 +2117,7 @@ static void BounceSlot(const Scene& s, const MRayRenderParams& p, Slot& sl, Rend
         float rn[3];
         for(int k = 0; k < 3; k++) rn[k] = ToFloat01(PcgNext(sl.rng));
         V3 worldPos = surf.position;
-        LightSampleOut ls = SampleLight(s, rn, worldPos, sc);
+        LightSampleOut ls = SampleLight(s, rn, worldPos, sc, (p.flags & MRAY_RENDER_LIGHT_POWER) != 0);
         // LightSampleOutput::SampledRay (LightSampler.hpp:5-21)
         V3 sdir = Normalize(ls.position - worldPos);
         V3 npos = Nudge(ls.position, -sdir);
@@ -2496,6 +2571,7 @@ void* rc_scene_create(const MRaySceneDesc* d)
     s->sceneDiameter = d->sceneDiameter;
     s->cam = d->camera;
     if(s->tCount) BuildLBVH(*s);
+    BuildLightPowerTable(*s);
     return s;
 }
 
@@ -2745,6 +2821,15 @@ float rc_light_pdf(void* h, uint32_t li, float a, float b, const float* rayPos, 
     if(li >= s.lightTri.size()) return PdfLightBoundary(s, V3{rayDir[0], rayDir[1], rayDir[2]});
     return PdfLightTri(s, li, a, b, V3{rayPos[0], rayPos[1], rayPos[2]},
                        V3{rayDir[0], rayDir[1], rayDir[2]});
+}
+
+// The power-sampling tables of a scene's lights.
+void rc_light_power_table(void* h, float* prob, uint32_t* alias, float* pmf)
+{
+    const Scene& s = *static_cast<Scene*>(h);
+    std::memcpy(prob, s.lightProb.data(), s.lightProb.size() * 4);
+    std::memcpy(alias, s.lightAlias.data(), s.lightAlias.size() * 4);
+    std::memcpy(pmf, s.lightPmf.data(), s.lightPmf.size() * 4);
 }
 
 // Textured Skysphere of a scene and probes of its parts.

@@ -69,6 +69,7 @@ def lib():
         L.rc_light_probe.argtypes = [P, f32, f32, f32]
         L.rc_light_pdf.restype = C.c_float
         L.rc_light_pdf.argtypes = [P, C.c_uint32, C.c_float, C.c_float, f32, f32]
+        L.rc_light_power_table.argtypes = [P, f32, u32, f32]
         L.rc_scene_set_environment.restype = C.c_int
         L.rc_scene_set_environment.argtypes = [P, C.c_uint32, C.c_uint32, f32]
         L.rc_env_cdfs.argtypes = [P, f32, f32]
@@ -169,6 +170,13 @@ class OracleScene:
             raise ValueError("environment rejected (boundary not a Skysphere, bad size or zero luminance)")
         self.env_shape = env.shape[:2]
         self._env = env
+
+    def light_power_table(self):
+        n = len(self.scene.light_tris)
+        prob, pmf = np.zeros(n, np.float32), np.zeros(n, np.float32)
+        alias = np.zeros(n, np.uint32)
+        lib().rc_light_power_table(self.h, abi.fptr(prob), abi.u32ptr(alias), abi.fptr(pmf))
+        return prob, alias, pmf
 
     def env_cdfs(self):
         h, w = self.env_shape
