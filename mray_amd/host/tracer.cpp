@@ -276,6 +276,7 @@ struct RendererRec
     uint32_t burstSize = 1;
     uint32_t sampleMode = MRAY_SAMPLE_NEE_MIS;
     uint32_t rr[2] = {4, 20};
+    bool powerLights = false;   // neeSamplerType "Power"
 };
 
 uint32_t ParseSampleMode(std::string_view s)
@@ -1146,8 +1147,12 @@ class Tracer final : public TracerI
             }
             case 5:
             {
+                // LightSamplerNames (PathTracerRendererBase.h:50-60) hold only
+                // "Uniform"; "Power" is this tracer's alias-table sampler
                 std::string_view s = std::as_const(data).AccessAsString();
-                if(s != "Uniform") Unsupported("Light sampler \"" + std::string(s) + "\"");
+                if(s == "Uniform") r.powerLights = false;
+                else if(s == "Power") r.powerLights = true;
+                else Unsupported("Light sampler \"" + std::string(s) + "\"");
                 break;
             }
             default:
@@ -1192,6 +1197,7 @@ class Tracer final : public TracerI
         p.maxRayCount = params.parallelizationHint;
         p.seed = params.seed;
         p.filterRadius = params.filmFilter.radius;
+        p.flags = rr.powerLights ? MRAY_RENDER_LIGHT_POWER : 0u;
         MRaySceneDesc d = Desc(CameraOf(U(cs)));
         CheckRc(mray_renderer_create(accel, &d, &p, &renderer), "StartRender");
         activeRenderer = U(rid);

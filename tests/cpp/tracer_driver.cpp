@@ -310,6 +310,14 @@ int Errors(TracerI& t)
     RendererId r = t.CreateRenderer("(R)PathTracerRGB");
     expectThrow("unknown renderer attribute", [&] { t.PushRendererAttribute(r, 6, Data(std::vector<uint32_t>{1})); });
     expectThrow("unknown sample mode", [&] { t.PushRendererAttribute(r, 3, Str("Bogus")); });
+    expectThrow("unknown light sampler", [&] { t.PushRendererAttribute(r, 5, Str("Irradiance")); });
+    try
+    {
+        t.PushRendererAttribute(r, 5, Str("Uniform"));
+        t.PushRendererAttribute(r, 5, Str("Power"));
+        expect("light samplers Uniform and Power accepted", true);
+    }
+    catch(const MRayError&) { expect("light samplers Uniform and Power accepted", false); }
     expectThrow("wrong attribute type", [&] { t.PushRendererAttribute(r, 0, Data(std::vector<float>{1.0f})); });
     expectThrow("StartRender before CommitSurfaces", [&] {
         (void)t.StartRender(r, CamSurfaceId(0), RenderImageParams{}, std::nullopt, std::nullopt); });
@@ -424,7 +432,7 @@ int Run(int argc, char** argv)
             t->PushRendererAttribute(r, 2, Str("Throughput"));
             t->PushRendererAttribute(r, 3, Str(modes[sampleMode]));
             t->PushRendererAttribute(r, 4, Data(std::vector<Vector2ui>{Vector2ui{{rr0, rr1}}}));
-            t->PushRendererAttribute(r, 5, Str("Uniform"));
+            t->PushRendererAttribute(r, 5, Str(std::getenv("MRAY_DRIVER_POWER_LIGHTS") ? "Power" : "Uniform"));
             TimelineSemaphore sem(0);
             t->SetupRenderEnv(&sem, 4096, 0);
             RenderImageParams ip{Vector2ui{{W, H}}, Vector2ui{{0, 0}}, Vector2ui{{W, H}}};
