@@ -476,7 +476,8 @@ def bench_instanced(args):
     1000 times with seeded rotations about +Y, scales and translations, a
     panel of 10k emissive triangles, constant Skysphere; spectral, depth 12.
     Main line: 8 viewports of uneven sizes (1080p down to 512^2) batched in
-    one renderer; also one 1080p view alone."""
+    one renderer; also one 1080p view alone.  NEE picks lights by power
+    through an alias table."""
     import numpy as np
     from mray_amd import abi, native, scenes
     torch = native.require_gpu()
@@ -498,7 +499,7 @@ def bench_instanced(args):
     views = scenes.instanced_views(spp=spp, count=args.inst_views)
     paths = sum(v["width"] * v["height"] * v["spp"] for v in views)
     params = abi.render_params(1, 1, 1, rr=(12, 12), mode=abi.SAMPLE_NEE_MIS, spectral=spectral,
-                               seed=0, max_ray_count=args.batch_slots)
+                               seed=0, max_ray_count=args.batch_slots, flags=abi.RENDER_LIGHT_POWER)
     r = native.BatchRenderer(isc, views, params)
     cst, totals, elapsed, phases = _timed_render(r, args, torch, "viewports")
     lat = r.view_times()
@@ -508,7 +509,7 @@ def bench_instanced(args):
     assert all(np.isfinite(f).all() for f in films)
     # one 1080p view alone (a slot per pixel)
     p1 = abi.render_params(args.width, args.height, spp, rr=(12, 12), mode=abi.SAMPLE_NEE_MIS,
-                           spectral=spectral, seed=0)
+                           spectral=spectral, seed=0, flags=abi.RENDER_LIGHT_POWER)
     r1 = native.InstancedRenderer(isc, p1)
     cst1, tot1, el1, ph1 = _timed_render(r1, args, torch, "single view")
     r1.close()
@@ -534,7 +535,8 @@ def bench_instanced(args):
                         f"({info['effective_triangles']} effective tris) + "
                         f"{info['lights']} emissive tris; {len(views)} viewports "
                         f"(1080p..512^2), {spp} spp, depth 12, "
-                        f"{'spectral 4 lambda' if spectral else 'RGB'}, NEE+MIS",
+                        f"{'spectral 4 lambda' if spectral else 'RGB'}, NEE+MIS, "
+                        f"lights chosen by power (alias table)",
             "mesh_triangles": [m.triangle_count for m in meshes],
             "effective_triangles": info["effective_triangles"],
             "tlas_nodes": info["tlas_nodes"],
