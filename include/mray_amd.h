@@ -395,7 +395,16 @@ typedef struct MRayInstEnv
     float      boundaryRadiance[3];
     float      sceneDiameter;  /* 0: the world bounds' XZ span             */
     MRayCamera camera;
+    uint32_t   flags;          /* MRAY_INST_*                              */
 } MRayInstEnv;
+
+/* Shading frames of instanced triangles.  By default they stay in the
+ * mesh's space, as the reference generates surfaces of transformed
+ * triangles: positions and the geometric normal are transformed, the
+ * interpolated TBN quaternion is not (PrimitiveDefaultTriangle.hpp:443-630,
+ * "we can't apply a transform to tbn").  With this flag the frames rotate
+ * with the instance, which is what the geometry implies.                  */
+#define MRAY_INST_ROTATE_FRAMES 1u
 
 /* meshes: accelerators of the meshes in their own space (4-wide trees);
  * they may be destroyed afterwards.                                        */

@@ -145,7 +145,11 @@ class MRayInstEnv(C.Structure):
         ("boundaryRadiance", C.c_float * 3),
         ("sceneDiameter", C.c_float),
         ("camera", MRayCamera),
+        ("flags", C.c_uint32),
     ]
+
+
+INST_ROTATE_FRAMES = 1
 
 
 def camera_struct(cam, out=None):
@@ -180,6 +184,7 @@ def inst_env(env):
         e.boundaryRadiance[k] = env["boundary_radiance"][k]
     e.sceneDiameter = env.get("scene_diameter") or 0.0
     camera_struct(env["camera"], e.camera)
+    e.flags = INST_ROTATE_FRAMES if env.get("rotate_frames") else 0
     return e
 
 

@@ -108,7 +108,7 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
         g.root = mb.parts[in.mesh].root;
         g.lightOff = uint32_t(lightRad.size());
         g.mesh = in.mesh;
-        g.pad = 0;
+        g.flags = env.flags & MRAY_INST_ROTATE_FRAMES;
         // world box of the mesh box's corners
         const float* mbx = &meshBox[size_t(in.mesh) * 6];
         float* wb = &boxes[size_t(i) * 6];

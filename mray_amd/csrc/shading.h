@@ -397,8 +397,10 @@ __device__ __forceinline__ Surface GenSurface(const SceneView& s, uint32_t tri, 
 }
 
 // The surface of an instanced mesh triangle in world space: position by
-// the similarity transform, the normal and the tangent frame by its
-// rotation q (world -> tangent = mesh tangent frame after q^-1).
+// the similarity transform, the geometric normal by its rotation q; the
+// shading frame stays the mesh's, as the reference leaves it
+// (PrimitiveDefaultTriangle.hpp:611-614), unless the instance rotates its
+// frames (world -> tangent = mesh tangent frame after q^-1).
 __device__ __forceinline__ Surface GenSurfaceInst(const SceneView& s, uint32_t tri, float a, float b,
                                                   V3 rayDir, const InstGPU& I)
 {
@@ -407,7 +409,7 @@ __device__ __forceinline__ Surface GenSurfaceInst(const SceneView& s, uint32_t t
     const V3 t = {I.transScale.x, I.transScale.y, I.transScale.z};
     r.position = QRotate(q, r.position) * I.transScale.w + t;
     r.geoNormal = QRotate(q, r.geoNormal);
-    r.tbn = QMul(r.tbn, Q4{q.w, -q.x, -q.y, -q.z});
+    if(I.flags & MRAY_INST_ROTATE_FRAMES) r.tbn = QMul(r.tbn, Q4{q.w, -q.x, -q.y, -q.z});
     return OrientSurface(r, rayDir);
 }
 

@@ -31,7 +31,7 @@ struct InstGPU
     uint32_t root;
     uint32_t lightOff;
     uint32_t mesh;
-    uint32_t pad;
+    uint32_t flags;       // MRAY_INST_ROTATE_FRAMES
 };
 
 struct TraceCtx

@@ -6,7 +6,9 @@
 // is kept on the host until CommitSurfaces, which flattens every surface,
 // light surface and the boundary light into one MRaySceneDesc (triangle soup
 // with per-triangle material / light keys) and builds the accelerator on the
-// GPU.  Transforms are baked into positions and TBN quaternions there.
+// GPU.  Transforms are baked into positions there; TBN quaternions turn with
+// TransformPrimitives only, as in the reference, where a surface's transform
+// moves positions but leaves the shading frame in mesh space.
 #include "mray_tracer.hpp"
 #include "mray_amd.h"
 
@@ -133,7 +135,7 @@ Rigid IdentityRigid()
 }
 
 // Matrix -> rigid transform with uniform scale.  The rotation is taken from
-// the normalised columns (Shepperd's method); TBN frames rotate with it.
+// the normalised columns (Shepperd's method).
 Rigid MakeRigid(const float* m12)
 {
     Rigid r;
@@ -972,7 +974,10 @@ class Tracer final : public TracerI
             {
                 float p[3], q[4];
                 ApplyPoint(r, pb.pos[v].v, p);
-                ApplyTbn(r, pb.tbn[v].v, q);
+                // TransformPrimitives turns the frames with the vertices
+                // (KCApplyTransformsTriangle); a surface's transform does not
+                // (Triangle::GenerateSurface leaves the TBN in mesh space)
+                ApplyTbn(pb.baked, pb.tbn[v].v, q);
                 f.pos.insert(f.pos.end(), p, p + 3);
                 f.tbn.insert(f.tbn.end(), q, q + 4);
             }

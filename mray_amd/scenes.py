@@ -630,9 +630,12 @@ def _unit_quat(r):
 
 def flatten_instances(meshes, instances, env):
     """The world-space scene an instanced scene describes: every instance's
-    triangles transformed (positions by the similarity, tangent frames by
-    its rotation), materials of each mesh once, one light per instance of an
-    emissive triangle.  The oracle renders this."""
+    triangles transformed (positions by the similarity; tangent frames kept
+    as the mesh's, as the reference's transformed triangles keep them, or
+    rotated with the instance when env["rotate_frames"]), materials of each
+    mesh once, one light per instance of an emissive triangle.  The oracle
+    renders this."""
+    rotate_frames = bool(env.get("rotate_frames"))
     mat_off, mats = [], []
     for m in meshes:
         mat_off.append(len(mats))
@@ -645,9 +648,12 @@ def flatten_instances(meshes, instances, env):
         s = float(inst["scale"])
         t = np.asarray(inst["translation"], np.float64)
         pos.append((quat_rotate(q, m.positions) * s + t).astype(np.float32))
-        qc = np.array([q[0], -q[1], -q[2], -q[3]])
-        tb = quat_mul(np.asarray(m.tbn, np.float64), qc)
-        tbn.append((tb / np.linalg.norm(tb, axis=1, keepdims=True)).astype(np.float32))
+        if rotate_frames:
+            qc = np.array([q[0], -q[1], -q[2], -q[3]])
+            tb = quat_mul(np.asarray(m.tbn, np.float64), qc)
+            tbn.append((tb / np.linalg.norm(tb, axis=1, keepdims=True)).astype(np.float32))
+        else:
+            tbn.append(np.asarray(m.tbn, np.float32))
         idx.append(np.asarray(m.indices, np.uint32) + np.uint32(vbase))
         k = np.asarray(m.surface_keys, np.uint32)
         is_light = (k & abi.LIGHT_FLAG) != 0
@@ -703,17 +709,22 @@ def _random_rotation(rng, full=True):
     return (math.cos(ang / 2), 0.0, math.sin(ang / 2), 0.0)
 
 
-def instanced_test_scene(width=64, height=48, seed=3):
+def instanced_test_scene(width=64, height=48, seed=3, rotate_frames=False):
     """Small instanced scene for parity tests: two terrain meshes placed 12
-    times with arbitrary rotations and scales, and an emissive panel mesh
-    placed twice (its lights exist once per instance)."""
+    times with rotations and scales, and an emissive panel mesh placed twice
+    (its lights exist once per instance).  With rotate_frames the shading
+    frames turn with the instances and rotations are arbitrary; without, the
+    frames stay the meshes' (the reference's behaviour) and every rotation
+    is about +Y, so no instance shades around a normal that points into its
+    own surface."""
     meshes = [terrain_spheres(grid=20, spheres=4, width=width, height=height, light_grid=0, seed=1),
               terrain_spheres(grid=16, spheres=6, width=width, height=height, light_grid=0, seed=2),
               light_panel(3, 0.15, (9.0, 7.0, 5.0), seed=4, span=0.5)]
     rng = MT19937(seed)
     instances = []
     for k in range(12):
-        instances.append({"mesh": k % 2, "rotation": _random_rotation(rng, full=(k % 3 != 0)),
+        instances.append({"mesh": k % 2,
+                          "rotation": _random_rotation(rng, full=rotate_frames and (k % 3 != 0)),
                           "scale": rng.uniform(0.5, 1.3),
                           "translation": (rng.uniform(-2.5, 2.5), rng.uniform(-0.3, 0.3),
                                           rng.uniform(-2.5, 2.5))})
@@ -725,7 +736,7 @@ def instanced_test_scene(width=64, height=48, seed=3):
     cam = {"position": (0.0, 4.5, 6.0), "gaze": (0.0, 0.0, 0.0), "up": (0.0, 1.0, 0.0),
            "fov": (fov, fov_y_from_x(fov, width / height)), "nearFar": (0.01, 1000.0)}
     env = {"boundary_type": abi.BOUNDARY_SKYSPHERE_COOCTA, "boundary_radiance": (0.3, 0.3, 0.35),
-           "scene_diameter": None, "camera": cam}
+           "scene_diameter": None, "camera": cam, "rotate_frames": rotate_frames}
     flat = flatten_instances(meshes, instances, env)
     env["scene_diameter"] = flat.diameter
     flat.scene_diameter = env["scene_diameter"]

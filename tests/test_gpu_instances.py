@@ -39,12 +39,14 @@ def _gpu_instanced(meshes, instances, env, params):
     return film, st, info
 
 
-@pytest.mark.parametrize("spectral", [False, True])
-def test_instanced_matches_flattened_oracle(torch_gpu, spectral):
+@pytest.mark.parametrize("spectral,rotate", [(False, False), (True, False), (False, True)])
+def test_instanced_matches_flattened_oracle(torch_gpu, spectral, rotate):
+    """Shading frames kept in mesh space (the reference's behaviour) or
+    rotated with each instance (MRAY_INST_ROTATE_FRAMES)."""
     from mray_amd import native
     if spectral:
         refcpu.spectral_load(native.spectral_tables())
-    meshes, inst, env, flat = scenes.instanced_test_scene(width=64, height=48)
+    meshes, inst, env, flat = scenes.instanced_test_scene(width=64, height=48, rotate_frames=rotate)
     params = abi.render_params(64, 48, 4, rr=(3, 8), seed=5, spectral=spectral, max_ray_count=4096)
     film, st, info = _gpu_instanced(meshes, inst, env, params)
     assert info["instances"] == len(inst)
@@ -55,15 +57,20 @@ def test_instanced_matches_flattened_oracle(torch_gpu, spectral):
     assert st["dead_paths"] == ost["dead_paths"] == 64 * 48 * 4
     assert np.allclose(film[3], ofilm[3], rtol=1e-5)
     e = relmse(film_image(film), film_image(ofilm))
-    print(f"spectral={spectral}: relMSE={e:.3e}", flush=True)
+    print(f"spectral={spectral} rotate={rotate}: relMSE={e:.3e}", flush=True)
     assert e <= 1e-3
 
 
 def test_instanced_matches_flattened_gpu(torch_gpu):
     """Against the one-level GPU renderer of the flattened scene at a larger
-    image: the same samples through the two traversals."""
+    image: the same samples through the two traversals.  Frames rotate with
+    the instances here: with the reference's mesh-space frames, an
+    upside-down instance shades around a normal that points into its own
+    surface, and such paths turn the last-bit differences of the two
+    geometric normals (rotated vs. from world vertices) into different
+    paths, so that mode is compared with the oracle at a smaller size."""
     from mray_amd import native
-    meshes, inst, env, flat = scenes.instanced_test_scene(width=160, height=120)
+    meshes, inst, env, flat = scenes.instanced_test_scene(width=160, height=120, rotate_frames=True)
     params = abi.render_params(160, 120, 8, rr=(3, 8), seed=9)
     film, st, _ = _gpu_instanced(meshes, inst, env, params)
     acc = native.Accel(flat)
@@ -167,14 +174,14 @@ def _views():
             dict(scene=0, width=32, height=24, spp=4, camera=cams[2])]
 
 
-@pytest.mark.parametrize("spectral", [False, True])
-def test_instanced_batch_matches_flattened_oracle(torch_gpu, spectral):
+@pytest.mark.parametrize("spectral,rotate", [(False, False), (True, True)])
+def test_instanced_batch_matches_flattened_oracle(torch_gpu, spectral, rotate):
     """Several views of one instanced scene in one slot pool, against the
     oracle's batch render of the flattened scene."""
     from mray_amd import native
     if spectral:
         refcpu.spectral_load(native.spectral_tables())
-    meshes, inst, env, flat = scenes.instanced_test_scene(width=48, height=36)
+    meshes, inst, env, flat = scenes.instanced_test_scene(width=48, height=36, rotate_frames=rotate)
     views = _views()
     params = abi.render_params(1, 1, 1, rr=(3, 8), seed=13, spectral=spectral, max_ray_count=3000)
     accs = [native.Accel(m) for m in meshes]

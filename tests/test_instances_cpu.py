@@ -2,8 +2,10 @@
 view of a two-level scene, pinned by what geometry fixes:
 
 * an identity instance is the mesh itself;
-* a similarity (rotation, uniform scale, translation) applied to a whole
-  scene and its camera leaves the image unchanged: radiance is invariant
+* by default a transformed instance keeps its mesh's shading frames, as the
+  reference's transformed triangles do (PrimitiveDefaultTriangle.hpp:611-614);
+* with rotated frames, a similarity (rotation, uniform scale, translation)
+  applied to a whole scene and its camera leaves the image unchanged: radiance is invariant
   under it, and the camera rays, path indices and PCG32 streams are the
   same, so only rounding moves paths.  A mistake in the positions' or the
   tangent frames' transform (wrong quaternion order, a missing conjugate,
@@ -67,8 +69,18 @@ def _similar(sc, q, s, t):
     cam["up"] = tuple(scenes.quat_rotate(qa, cam["up"]))
     cam["nearFar"] = (cam["nearFar"][0] * s, cam["nearFar"][1] * s)
     env = {"boundary_type": sc.boundary_type, "boundary_radiance": sc.boundary_radiance,
-           "scene_diameter": sc.diameter * s, "camera": cam}
+           "scene_diameter": sc.diameter * s, "camera": cam, "rotate_frames": True}
     return scenes.flatten_instances([sc], [{"mesh": 0, "rotation": q, "scale": s, "translation": t}], env)
+
+
+def test_default_frames_stay_in_mesh_space():
+    sc = scenes.cornell()
+    q = (np.cos(0.4), 0.0, np.sin(0.4), 0.0)
+    inst = [{"mesh": 0, "rotation": q, "scale": 1.5, "translation": (1.0, 0.0, 0.0)}]
+    flat = scenes.flatten_instances([sc], inst, _env_of(sc))
+    assert np.array_equal(flat.tbn, sc.tbn)
+    rot = scenes.flatten_instances([sc], inst, dict(_env_of(sc), rotate_frames=True))
+    assert not np.allclose(rot.tbn, sc.tbn, atol=1e-3)
 
 
 def test_similarity_leaves_the_image_unchanged():
