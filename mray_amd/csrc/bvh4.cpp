@@ -266,7 +266,7 @@ bool QuantizeCNode8(const float box[8][6], const uint32_t child[8], int count, C
         out.child[k] = (k < count) ? child[k] : kInvalid;
         if(k >= count)
         {
-            // empty slot: lo = 255 > hi = 0 on every axis
+            // empty slot: kInvalid child (traversal masks on it), bytes 255 / 0
             for(int d = 0; d < 3; d++) { q[d][k] = 255; q[3 + d][k] = 0; }
             continue;
         }

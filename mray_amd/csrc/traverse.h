@@ -178,8 +178,10 @@ __device__ __forceinline__ void SlabTest8(const CNode8* __restrict__ np, const R
         float z0 = (nz - r.o.z) * r.inv.z, z1 = (fz - r.o.z) * r.inv.z;
         float tn = MaxF(MaxF(tMin, x0), MaxF(y0, z0));
         float tf = MinF(MinF(tMax, x1), MinF(y1, z1)) * grow;
-        // empty slots decode to lo > hi on every axis: near > far
-        bool valid = lx <= hx;
+        // an empty slot holds kInvalid; its 255/0 bytes do not decode to an
+        // empty box when the node is flat on an axis (exponent -126 makes
+        // 255 steps vanish against the origin), so the child word decides
+        bool valid = child[k] != kInvalid;
         tNear[k] = (valid && tn <= tf) ? tn : INFINITY;
     }
 }

@@ -76,11 +76,7 @@ void SubtreeBounds(const std::vector<CNode8>& nodes, uint32_t c, const std::vect
         }
     }
     for(int k = n.nChild; k < 8; k++)
-    {
-        float dl[3], dh[3];
-        DecodeBox(n, k, dl, dh);
-        Check(n.child[k] == kInvalid && dl[0] > dh[0], "empty CNode8 slot decodes empty");
-    }
+        Check(n.child[k] == kInvalid, "empty CNode8 slot holds kInvalid");
 }
 
 bool HitTri(const float* o, const float* dir, const Tri& t, float& tOut)
