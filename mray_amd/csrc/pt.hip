@@ -302,6 +302,7 @@ __device__ __forceinline__ SceneView SlotScene(const RenderConst& rc, const Slot
             s.lightAlias += v.lightOff;
             s.lightPmf += v.lightOff;
         }
+        if(s.lightCoeff) s.lightCoeff += v.lightOff;
     }
     return s;
 }
@@ -840,6 +841,37 @@ __device__ __forceinline__ V4 ConvAlbedo(const RenderConst& rc, V3 rgb, V4 waves
     if constexpr(SPECTRAL) return SpectralAlbedo(rc.spec, rgb, waves);
     else return V4{rgb.x, rgb.y, rgb.z, 0.0f};
 }
+// A light's radiance at the path's wavelengths: spectral renderers evaluate
+// the coefficients converted once per renderer (KSpectralCoeffs), the same
+// numbers SpectralRadiance computes for that colour on every call.
+template<bool SPECTRAL>
+__device__ __forceinline__ V4 LightRadiance(const RenderConst& rc, const SceneView& s, uint32_t li,
+                                            float4 lr, V4 waves)
+{
+    if constexpr(SPECTRAL) return EvalRadianceCoeffs(rc.spec, s.lightCoeff[li], waves);
+    else return V4{lr.x, lr.y, lr.z, 0.0f};
+}
+
+// Per material the albedo's coefficients, per light its radiance's
+// coefficients and scale (spectral renderers, once at creation).
+__global__ void __launch_bounds__(256)
+KSpectralCoeffs(SpectralView sv, const MRayMaterial* __restrict__ mats, uint32_t nm,
+                const float4* __restrict__ lightRad, uint32_t nl, float4* __restrict__ matCoeff,
+                float4* __restrict__ lightCoeff)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if(i < nm)
+    {
+        const MRayMaterial& m = mats[i];
+        V3 co = AlbedoCoeffs(sv, V3{m.albedo[0], m.albedo[1], m.albedo[2]});
+        matCoeff[i] = make_float4(co.x, co.y, co.z, 0.0f);
+    }
+    if(i < nl)
+    {
+        const float4 lr = lightRad[i];
+        lightCoeff[i] = RadianceCoeffs(sv, V3{lr.x, lr.y, lr.z});
+    }
+}
 
 template<bool SPECTRAL>
 __device__ __forceinline__ V4 TriLightEmitViaPoint(const RenderConst& rc, const SceneView& s,
@@ -855,7 +887,7 @@ __device__ __forceinline__ V4 TriLightEmitViaPoint(const RenderConst& rc, const 
     float4 lr = s.lightRad[li];
     bool twoSided = lr.w != 0.0f;
     if(!twoSided && NdL <= 0.0f) return Splat4(0.0f);
-    return ConvRadiance<SPECTRAL>(rc, V3{lr.x, lr.y, lr.z}, waves);
+    return LightRadiance<SPECTRAL>(rc, s, li, lr, waves);
 }
 
 struct LightSample { V3 position; V4 emission; float pdf; };
@@ -1027,7 +1059,7 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
             float NdL = Dot(nrm, wO);
             float4 lr = s.lightRad[li];
             emission = (lr.w == 0.0f && NdL <= 0.0f)
-                ? Splat4(0.0f) : ConvRadiance<SPECTRAL>(rc, V3{lr.x, lr.y, lr.z}, waves);
+                ? Splat4(0.0f) : LightRadiance<SPECTRAL>(rc, s, li, lr, waves);
         }
         if(rc.mode == MRAY_SAMPLE_PURE)
         {
@@ -1062,7 +1094,12 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
                       : GenSurface(s, prim, ha, hb, rdir);
     MatInst mat;
     mat.type = (KEY > 0) ? uint32_t(KEY - 1) : M.type;
-    mat.albedo = ConvAlbedo<SPECTRAL>(rc, V3{M.albedo[0], M.albedo[1], M.albedo[2]}, waves);
+    if constexpr(SPECTRAL)
+    {
+        const float4 co = s.matCoeff[s.surfKey[prim]];
+        mat.albedo = EvalCoeffs(V3{co.x, co.y, co.z}, waves);
+    }
+    else mat.albedo = V4{M.albedo[0], M.albedo[1], M.albedo[2], 0.0f};
     mat.roughness = M.roughness;
     mat.metallic = M.metallic;
     mat.specular = M.specular;
@@ -1924,6 +1961,27 @@ static int SizeLaunches(MRayRenderer* r)
 static int SetupLightPower(MRayRenderer* r, const std::vector<uint32_t>& offs,
                            const std::vector<uint32_t>& counts, uint32_t total);
 
+// Spectral renderers: convert the scene's constant colours (material
+// albedos, light radiances) to sigmoid coefficients once.
+static int SetupSpectralCoeffs(MRayRenderer* r, uint32_t nm, uint32_t nl)
+{
+    r->rc.scene.matCoeff = nullptr;
+    r->rc.scene.lightCoeff = nullptr;
+    if(!r->rc.spectral) return MRAY_OK;
+    float4 *dm = nullptr, *dl = nullptr;
+    MRAY_TRY(DevAlloc(r->arena, (void**)&dm, std::max<size_t>(nm, 1) * sizeof(float4)));
+    MRAY_TRY(DevAlloc(r->arena, (void**)&dl, std::max<size_t>(nl, 1) * sizeof(float4)));
+    const uint32_t n = std::max(nm, nl);
+    if(n)
+        KSpectralCoeffs<<<DivUp(n, 256), 256, 0, r->stream>>>(r->rc.spec, r->rc.scene.mats, nm,
+                                                              r->rc.scene.lightRad, nl, dm, dl);
+    MRAY_HIP_CHECK(hipGetLastError());
+    MRAY_HIP_CHECK(hipStreamSynchronize(r->stream));
+    r->rc.scene.matCoeff = dm;
+    r->rc.scene.lightCoeff = dl;
+    return MRAY_OK;
+}
+
 static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDesc* scene,
                           const MRayRenderParams* p)
 {
@@ -1948,6 +2006,7 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
     r->limit = (unsigned long long)p->totalSPP * P;
     MRAY_TRY(SetupLightPower(r, {0u}, {a->view.lightCount}, a->view.lightCount));
     MRAY_TRY(InitSlots(r, size_t(P)));
+    MRAY_TRY(SetupSpectralCoeffs(r, uint32_t(a->hMats.size()), a->view.lightCount));
     return ResetRenderer(r);
 }
 
@@ -1990,6 +2049,7 @@ static int CreateInstanced(MRayRenderer* r, const MRayInstScene* sc, const MRayR
     r->limit = (unsigned long long)p->totalSPP * P;
     MRAY_TRY(SetupLightPower(r, {0u}, {sc->view.lightCount}, sc->view.lightCount));
     MRAY_TRY(InitSlots(r, size_t(P)));
+    MRAY_TRY(SetupSpectralCoeffs(r, sc->meshes.mats, sc->view.lightCount));
     return ResetRenderer(r);
 }
 
@@ -2106,6 +2166,7 @@ static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayInstS
         }
         MRAY_TRY(SetupLightPower(r, offs, counts, b->lights));
     }
+    MRAY_TRY(SetupSpectralCoeffs(r, is ? is->meshes.mats : b->mats, is ? is->view.lightCount : b->lights));
     return ResetRenderer(r);
 }
 

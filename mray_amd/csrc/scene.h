@@ -43,6 +43,10 @@ struct SceneView
     // of its range's alias table, and its selection probability
     const uint2* __restrict__ lightAlias;
     const float* __restrict__ lightPmf;
+    // spectral renderers: per material its albedo's sigmoid coefficients,
+    // per light its radiance coefficients and scale (null in RGB mode)
+    const float4* __restrict__ matCoeff;
+    const float4* __restrict__ lightCoeff;
 };
 
 // Alias table of n lights (pos9 [n][9] world vertices, rad [n] rgb + two-sided

@@ -101,7 +101,9 @@ __device__ __forceinline__ float EvalSigmoidPoly(V3 c, float lambda)
     return Fma(0.5f * x, denomRecip, 0.5f);
 }
 
-__device__ __forceinline__ V4 SpectralAlbedo(const SpectralView& sv, V3 rgb, V4 waves)
+// The sigmoid-polynomial coefficients of an RGB albedo (the table lookup
+// half of SpectralAlbedo); constant colours are converted once per scene.
+__device__ __forceinline__ V3 AlbedoCoeffs(const SpectralView& sv, V3 rgb)
 {
     float c[3] = {rgb.x, rgb.y, rgb.z};
     uint32_t maxI = 0;
@@ -117,9 +119,16 @@ __device__ __forceinline__ V4 SpectralAlbedo(const SpectralView& sv, V3 rgb, V4 
     }
     float slice = ClampF(maxC, 0.0f, 1.0f);
     float z = InvSmoothstep(InvSmoothstep(slice));
-    V3 co = LutFetch(sv, maxI, x, y, z);
+    return LutFetch(sv, maxI, x, y, z);
+}
+__device__ __forceinline__ V4 EvalCoeffs(V3 co, V4 waves)
+{
     return {EvalSigmoidPoly(co, waves.x), EvalSigmoidPoly(co, waves.y),
             EvalSigmoidPoly(co, waves.z), EvalSigmoidPoly(co, waves.w)};
+}
+__device__ __forceinline__ V4 SpectralAlbedo(const SpectralView& sv, V3 rgb, V4 waves)
+{
+    return EvalCoeffs(AlbedoCoeffs(sv, rgb), waves);
 }
 
 __device__ __forceinline__ float Tex1D(const float* __restrict__ t, float lambda)
@@ -138,17 +147,28 @@ __device__ __forceinline__ V3 Tex1D3(const float4* __restrict__ t, float lambda)
     return {a.x * (1 - u) + b.x * u, a.y * (1 - u) + b.y * u, a.z * (1 - u) + b.z * u};
 }
 
-__device__ __forceinline__ V4 SpectralRadiance(const SpectralView& sv, V3 rgb, V4 waves)
+// Radiance = albedo of rgb / (2 max) times the illuminant, times 2 max:
+// coefficients (c0, c1, c2) and the scale, then their evaluation.
+__device__ __forceinline__ float4 RadianceCoeffs(const SpectralView& sv, V3 rgb)
 {
     float mx = MaxF(MaxF(rgb.x, rgb.y), rgb.z);
     float scale = mx * 2.0f;
     V3 n = (scale == 0.0f) ? V3{0, 0, 0} : V3{rgb.x / scale, rgb.y / scale, rgb.z / scale};
-    V4 s = SpectralAlbedo(sv, n, waves);
+    V3 co = AlbedoCoeffs(sv, n);
+    return make_float4(co.x, co.y, co.z, scale);
+}
+__device__ __forceinline__ V4 EvalRadianceCoeffs(const SpectralView& sv, float4 rc, V4 waves)
+{
+    V4 s = EvalCoeffs(V3{rc.x, rc.y, rc.z}, waves);
     s.x *= Tex1D(sv.illum, waves.x);
     s.y *= Tex1D(sv.illum, waves.y);
     s.z *= Tex1D(sv.illum, waves.z);
     s.w *= Tex1D(sv.illum, waves.w);
-    return s * scale;
+    return s * rc.w;
+}
+__device__ __forceinline__ V4 SpectralRadiance(const SpectralView& sv, V3 rgb, V4 waves)
+{
+    return EvalRadianceCoeffs(sv, RadianceCoeffs(sv, rgb), waves);
 }
 
 __device__ __forceinline__ V4 SpectrumToRGB(const SpectralView& sv, V4 value, V4 waves, V4 pdf,
