@@ -371,36 +371,66 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
 {
     // this bounce's work and queue counters start empty
     if(threadIdx.x < CTR_COUNT) *Ctr(ctr, threadIdx.x) = 0u;
-    __shared__ uint32_t sf[1024], sl[1024];
-    __shared__ uint32_t carryF, carryL, vmin;
+    // exclusive scans of the free (x) and live (y) counts: 8 blocks per
+    // thread, a wave scan of the thread totals, then a scan of the 16 wave
+    // totals (two barriers per 8192 blocks)
+    constexpr uint32_t kPer = 8;
+    __shared__ uint32_t wsF[16], wsL[16];
+    __shared__ uint32_t vmin;
     __shared__ unsigned long long accum;
-    if(threadIdx.x == 0) { carryF = 0; carryL = 0; accum = 0; vmin = 0xFFFFFFFFu; }
+    if(threadIdx.x == 0) { accum = 0; vmin = 0xFFFFFFFFu; }
     __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t carryF = 0, carryL = 0;
     unsigned long long myAccum = 0;
     uint32_t myMin = 0xFFFFFFFFu;
-    for(uint32_t base = 0; base < count; base += 1024)
+    for(uint32_t base = 0; base < count; base += 1024u * kPer)
     {
-        uint32_t i = base + threadIdx.x;
-        uint4 v = (i < count) ? blockCnt[i] : make_uint4(0, 0, 0, 0xFFFFFFFFu);
-        myAccum += v.z;
-        myMin = min(myMin, v.w);
-        sf[threadIdx.x] = v.x;
-        sl[threadIdx.x] = v.y;
-        __syncthreads();
-        for(uint32_t off = 1; off < 1024; off <<= 1)
+        const uint32_t i0 = base + threadIdx.x * kPer;
+        uint32_t f[kPer], l[kPer], z[kPer];
+        uint32_t tf = 0, tl = 0;
+        #pragma unroll
+        for(uint32_t k = 0; k < kPer; k++)
         {
-            uint32_t af = (threadIdx.x >= off) ? sf[threadIdx.x - off] : 0u;
-            uint32_t al = (threadIdx.x >= off) ? sl[threadIdx.x - off] : 0u;
-            __syncthreads();
-            sf[threadIdx.x] += af;
-            sl[threadIdx.x] += al;
-            __syncthreads();
+            const uint32_t i = i0 + k;
+            const uint4 v = (i < count) ? blockCnt[i] : make_uint4(0, 0, 0, 0xFFFFFFFFu);
+            myAccum += v.z;
+            myMin = min(myMin, v.w);
+            f[k] = v.x; l[k] = v.y; z[k] = v.z;
+            tf += v.x; tl += v.y;
         }
-        uint32_t inF = sf[threadIdx.x], inL = sl[threadIdx.x];
-        if(i < count) blockCnt[i] = make_uint4(carryF + inF - v.x, carryL + inL - v.y, v.z, 0u);
+        uint32_t sF = tf, sL = tl;      // inclusive scan over the wave
+        for(uint32_t o = 1; o < 64; o <<= 1)
+        {
+            const uint32_t a = __shfl_up(sF, o), b = __shfl_up(sL, o);
+            if(lane >= o) { sF += a; sL += b; }
+        }
+        if(lane == 63) { wsF[wave] = sF; wsL[wave] = sL; }
         __syncthreads();
-        if(threadIdx.x == 1023) { carryF += inF; carryL += inL; }
+        if(wave == 0)
+        {
+            uint32_t x = (lane < 16) ? wsF[lane] : 0u, y = (lane < 16) ? wsL[lane] : 0u;
+            for(uint32_t o = 1; o < 16; o <<= 1)
+            {
+                const uint32_t a = __shfl_up(x, o), b = __shfl_up(y, o);
+                if(lane >= o) { x += a; y += b; }
+            }
+            if(lane < 16) { wsF[lane] = x; wsL[lane] = y; }
+        }
         __syncthreads();
+        uint32_t exF = carryF + (wave ? wsF[wave - 1] : 0u) + sF - tf;
+        uint32_t exL = carryL + (wave ? wsL[wave - 1] : 0u) + sL - tl;
+        #pragma unroll
+        for(uint32_t k = 0; k < kPer; k++)
+        {
+            const uint32_t i = i0 + k;
+            if(i < count) blockCnt[i] = make_uint4(exF, exL, z[k], 0u);
+            exF += f[k];
+            exL += l[k];
+        }
+        carryF += wsF[15];
+        carryL += wsL[15];
+        __syncthreads();                // wsF / wsL are rewritten next pass
     }
     // number of finished paths: wave reduction, then one LDS atomic per wave
     for(uint32_t o = 32; o > 0; o >>= 1) myAccum += __shfl_down(myAccum, o);
