@@ -192,7 +192,8 @@ __device__ __forceinline__ bool TestLeafRange(const SoupTri* __restrict__ soup, 
                                               HitRec& best, uint32_t* triTests)
 {
     uint32_t first = leaf & kLeafStartMask;
-    uint32_t cnt = ((leaf & ~kLeafFlag) >> kLeafCountShift) + 1u;
+    // every tree is collapsed with kLeafMax triangles per leaf at most
+    uint32_t cnt = (kLeafMax == 1u) ? 1u : ((leaf & ~kLeafFlag) >> kLeafCountShift) + 1u;
     for(uint32_t k = 0; k < cnt; k++)
     {
         const float4* tp = reinterpret_cast<const float4*>(soup + first + k);
