@@ -247,7 +247,9 @@ int mray_accel_cast_visibility(const MRayAccel* accel, const float* dRays, uint3
  * Distributions.h:117-243).  The scene's boundary must be
  * MRAY_BOUNDARY_SKYSPHERE_COOCTA (boundaryRadiance is then unused);
  * renderers and merged scenes created afterwards use the texture.
- * Sizes 1..8192 per side; texels must be finite, luminance not all zero.   */
+ * Sizes 1..8192 per side; texels must be finite, luminance not all zero.
+ * A second call replaces the texture for renderers created afterwards; the
+ * earlier tables stay allocated until the accelerator is destroyed.       */
 int mray_accel_set_environment(MRayAccel* accel, uint32_t width, uint32_t height,
                                const float* hRgb);
 /* The luminance distribution: hCdfX [height][width] normalized row CDFs,

@@ -432,7 +432,7 @@ int Run(int argc, char** argv)
             t->PushRendererAttribute(r, 2, Str("Throughput"));
             t->PushRendererAttribute(r, 3, Str(modes[sampleMode]));
             t->PushRendererAttribute(r, 4, Data(std::vector<Vector2ui>{Vector2ui{{rr0, rr1}}}));
-            t->PushRendererAttribute(r, 5, Str(std::getenv("MRAY_DRIVER_POWER_LIGHTS") ? "Power" : "Uniform"));
+            t->PushRendererAttribute(r, 5, Str("Uniform"));
             TimelineSemaphore sem(0);
             t->SetupRenderEnv(&sem, 4096, 0);
             RenderImageParams ip{Vector2ui{{W, H}}, Vector2ui{{0, 0}}, Vector2ui{{W, H}}};
