@@ -1245,7 +1245,11 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
             V3 nn = pt ? -sf.geoNormal : sf.geoNormal;
             V3 spos = Nudge(worldPos, nn);
             type = RT_SHADOW;
-            if((depth + 2u) <= rc.rr1)
+            // a visible shadow ray only adds srad: one whose contribution is
+            // exactly zero (light facing away, BxDF zero below the surface)
+            // cannot change the image and is not cast
+            const bool zero = srad.x == 0.0f && srad.y == 0.0f && srad.z == 0.0f && srad.w == 0.0f;
+            if((depth + 2u) <= rc.rr1 && !zero)
             {
                 S.sO[i] = make_float4(spos.x, spos.y, spos.z, kEps);
                 S.sD[i] = make_float4(sdir.x, sdir.y, sdir.z, length * (1.0f - kLargeEps));
