@@ -2,7 +2,7 @@
 
 Paths terminate into the cache once their spread is large; 1 in 16 paths is
 a full-length training path whose vertices train the cache after every frame.
-Judge: a 4096-spp plain render.  The trained cache must beat the same
+Reference image: a 4096-spp plain render.  The trained cache must beat the same
 schedule with a frozen (untrained, lr = 0) cache by a wide margin, i.e. the
 network's training is what carries the estimate, and terminated paths must
 have cut the extension rays against plain path tracing at equal samples."""
