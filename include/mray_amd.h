@@ -377,8 +377,8 @@ int mray_renderer_light_table(const MRayRenderer* r, float* hProb, uint32_t* hAl
 /* Instanced scenes: meshes placed many times under a tree over instances  */
 /* (the reference's two-level accelerator: a base accelerator over surface */
 /* instances, AcceleratorLBVH.cu:437-495, above one tree per mesh,          */
-/* AcceleratorWork.kt.h:177-244).  Transforms are similarities (rotation,   */
-/* uniform scale, translation).  A mesh's emissive triangles become one     */
+/* AcceleratorWork.kt.h:177-244).  Transforms are rotation, per-axis scale  */
+/* and translation (TRS).  A mesh's emissive triangles become one           */
 /* light per instance; its materials are shared by its instances.          */
 /* ---------------------------------------------------------------------- */
 typedef struct MRayInstScene MRayInstScene;
@@ -387,7 +387,7 @@ typedef struct MRayInstance
 {
     uint32_t mesh;             /* index into the meshes                    */
     float    rotation[4];      /* unit quaternion (w, x, y, z), object->world */
-    float    scale;            /* > 0                                      */
+    float    scale[3];         /* per axis, > 0 (object space, applied first) */
     float    translation[3];
 } MRayInstance;
 

@@ -134,7 +134,7 @@ class MRayInstance(C.Structure):
     _fields_ = [
         ("mesh", C.c_uint32),
         ("rotation", C.c_float * 4),
-        ("scale", C.c_float),
+        ("scale", C.c_float * 3),
         ("translation", C.c_float * 3),
     ]
 
@@ -165,13 +165,17 @@ def camera_struct(cam, out=None):
 
 
 def instances_array(instances):
-    """MRayInstance array from dicts {mesh, rotation (w,x,y,z), scale, translation}."""
+    """MRayInstance array from dicts {mesh, rotation (w,x,y,z), scale (one
+    number or three), translation}."""
     arr = (MRayInstance * len(instances))()
     for k, d in enumerate(instances):
         arr[k].mesh = d["mesh"]
         for i in range(4):
             arr[k].rotation[i] = d["rotation"][i]
-        arr[k].scale = d["scale"]
+        sc = d["scale"]
+        sc = (sc, sc, sc) if isinstance(sc, (int, float)) else tuple(sc)
+        for i in range(3):
+            arr[k].scale[i] = sc[i]
         for i in range(3):
             arr[k].translation[i] = d["translation"][i]
     return arr

@@ -70,7 +70,8 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
     for(uint32_t i = 0; i < n; i++)
     {
         const MRayInstance& in = inst[i];
-        if(in.mesh >= meshCount || !(in.scale > 0.0f))
+        if(in.mesh >= meshCount || !(in.scale[0] > 0.0f) || !(in.scale[1] > 0.0f) ||
+           !(in.scale[2] > 0.0f))
         {
             SetError("instanced_scene_create: instance " + std::to_string(i) +
                      " names a missing mesh or has a non-positive scale");
@@ -84,11 +85,13 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
             return MRAY_ERR_INVALID;
         }
         q = {q.w / len, q.x / len, q.y / len, q.z / len};
-        const double sc_ = in.scale;
+        const double sc_[3] = {in.scale[0], in.scale[1], in.scale[2]};
+        const bool uniform = (in.scale[0] == in.scale[1]) && (in.scale[1] == in.scale[2]);
         const double t[3] = {in.translation[0], in.translation[1], in.translation[2]};
         InstGPU& g = table[i];
         g.rot = make_float4(float(q.w), float(q.x), float(q.y), float(q.z));
-        g.transScale = make_float4(float(t[0]), float(t[1]), float(t[2]), float(sc_));
+        g.transScale = make_float4(float(t[0]), float(t[1]), float(t[2]), uniform ? 1.0f : 0.0f);
+        g.scale = make_float4(in.scale[0], in.scale[1], in.scale[2], 0.0f);
         // world -> object: x_o = R^-1 (x_w - t) / s
         const Quat qi{q.w, -q.x, -q.y, -q.z};
         double rows[3][4];
@@ -98,7 +101,7 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
             e[c] = 1.0;
             double col[3];
             Rotate(qi, e, col);
-            for(int r = 0; r < 3; r++) rows[r][c] = col[r] / sc_;
+            for(int r = 0; r < 3; r++) rows[r][c] = col[r] / sc_[r];
         }
         for(int r = 0; r < 3; r++)
             rows[r][3] = -(rows[r][0] * t[0] + rows[r][1] * t[1] + rows[r][2] * t[2]);
@@ -115,12 +118,13 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
         for(int k = 0; k < 3; k++) { wb[k] = INFINITY; wb[3 + k] = -INFINITY; }
         for(int corner = 0; corner < 8; corner++)
         {
-            const double p[3] = {mbx[(corner & 1) ? 3 : 0], mbx[(corner & 2) ? 4 : 1], mbx[(corner & 4) ? 5 : 2]};
+            const double p[3] = {mbx[(corner & 1) ? 3 : 0] * sc_[0], mbx[(corner & 2) ? 4 : 1] * sc_[1],
+                                  mbx[(corner & 4) ? 5 : 2] * sc_[2]};
             double w[3];
             Rotate(q, p, w);
             for(int k = 0; k < 3; k++)
             {
-                const float v = float(w[k] * sc_ + t[k]);
+                const float v = float(w[k] + t[k]);
                 // widen by one float ulp each way: the box stays conservative
                 wb[k] = std::min(wb[k], std::nextafter(v, -INFINITY));
                 wb[3 + k] = std::max(wb[3 + k], std::nextafter(v, INFINITY));
@@ -137,10 +141,11 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
         {
             for(int v = 0; v < 3; v++)
             {
-                const double p[3] = {lp[l * 9 + v * 3], lp[l * 9 + v * 3 + 1], lp[l * 9 + v * 3 + 2]};
+                const double p[3] = {lp[l * 9 + v * 3] * sc_[0], lp[l * 9 + v * 3 + 1] * sc_[1],
+                                     lp[l * 9 + v * 3 + 2] * sc_[2]};
                 double w[3];
                 Rotate(q, p, w);
-                for(int k = 0; k < 3; k++) lightPos.push_back(float(w[k] * sc_ + t[k]));
+                for(int k = 0; k < 3; k++) lightPos.push_back(float(w[k] + t[k]));
             }
             lightRad.push_back(meshLightRad[in.mesh][l]);
         }

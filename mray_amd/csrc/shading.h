@@ -397,7 +397,9 @@ __device__ __forceinline__ Surface GenSurface(const SceneView& s, uint32_t tri, 
 }
 
 // The surface of an instanced mesh triangle in world space: position by
-// the similarity transform, the geometric normal by its rotation q; the
+// the TRS transform, the geometric normal by its rotation q (and, with a
+// non-uniform scale, by the inverse scale first, as the normal of the
+// transformed vertices); the
 // shading frame stays the mesh's, as the reference leaves it
 // (PrimitiveDefaultTriangle.hpp:611-614), unless the instance rotates its
 // frames (world -> tangent = mesh tangent frame after q^-1).
@@ -407,8 +409,12 @@ __device__ __forceinline__ Surface GenSurfaceInst(const SceneView& s, uint32_t t
     Surface r = GenSurfaceRaw(s, tri, a, b);
     const Q4 q = {I.rot.x, I.rot.y, I.rot.z, I.rot.w};
     const V3 t = {I.transScale.x, I.transScale.y, I.transScale.z};
-    r.position = QRotate(q, r.position) * I.transScale.w + t;
-    r.geoNormal = QRotate(q, r.geoNormal);
+    const V3 sc = {I.scale.x, I.scale.y, I.scale.z};
+    r.position = QRotate(q, V3{r.position.x * sc.x, r.position.y * sc.y, r.position.z * sc.z}) + t;
+    if(I.transScale.w != 0.0f) r.geoNormal = QRotate(q, r.geoNormal);
+    else
+        r.geoNormal = Normalize(QRotate(q, V3{r.geoNormal.x / sc.x, r.geoNormal.y / sc.y,
+                                              r.geoNormal.z / sc.z}));
     if(I.flags & MRAY_INST_ROTATE_FRAMES) r.tbn = QMul(r.tbn, Q4{q.w, -q.x, -q.y, -q.z});
     return OrientSurface(r, rayDir);
 }

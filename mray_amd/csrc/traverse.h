@@ -19,14 +19,15 @@ namespace mray
 static constexpr uint32_t kTraceTPB = 256;
 static constexpr uint32_t kLdsStack = 12;
 
-// One instance of a mesh in a two-level scene: a similarity transform
-// (rotation quaternion, uniform scale, translation) object -> world, its
-// inverse as three rows, the mesh tree's root and the instance's first
-// light in the world light table.
+// One instance of a mesh in a two-level scene: a TRS transform (rotation
+// quaternion, per-axis scale, translation) object -> world, its inverse as
+// three rows, the mesh tree's root and the instance's first light in the
+// world light table.
 struct InstGPU
 {
     float4 rot;          // (w, x, y, z)
-    float4 transScale;   // translation xyz, uniform scale w
+    float4 transScale;   // translation xyz, w = 1 if the scale is uniform else 0
+    float4 scale;        // per-axis scale xyz
     float4 inv[3];       // world -> object, rows (3x3 | translation)
     uint32_t root;
     uint32_t lightOff;

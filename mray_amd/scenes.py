@@ -645,9 +645,9 @@ def flatten_instances(meshes, instances, env):
     for inst in instances:
         m = meshes[inst["mesh"]]
         q = _unit_quat(inst["rotation"])
-        s = float(inst["scale"])
+        s = np.broadcast_to(np.asarray(inst["scale"], np.float64), (3,))
         t = np.asarray(inst["translation"], np.float64)
-        pos.append((quat_rotate(q, m.positions) * s + t).astype(np.float32))
+        pos.append((quat_rotate(q, np.asarray(m.positions, np.float64) * s) + t).astype(np.float32))
         if rotate_frames:
             qc = np.array([q[0], -q[1], -q[2], -q[3]])
             tb = quat_mul(np.asarray(m.tbn, np.float64), qc)
@@ -711,7 +711,7 @@ def _random_rotation(rng, full=True):
 
 def instanced_test_scene(width=64, height=48, seed=3, rotate_frames=False):
     """Small instanced scene for parity tests: two terrain meshes placed 12
-    times with rotations and scales, and an emissive panel mesh placed twice
+    times with rotations and (some per-axis) scales, and an emissive panel mesh placed twice
     (its lights exist once per instance).  With rotate_frames the shading
     frames turn with the instances and rotations are arbitrary; without, the
     frames stay the meshes' (the reference's behaviour) and every rotation
@@ -725,7 +725,8 @@ def instanced_test_scene(width=64, height=48, seed=3, rotate_frames=False):
     for k in range(12):
         instances.append({"mesh": k % 2,
                           "rotation": _random_rotation(rng, full=rotate_frames and (k % 3 != 0)),
-                          "scale": rng.uniform(0.5, 1.3),
+                          "scale": (rng.uniform(0.5, 1.3) if k % 4 != 1 else
+                                    (rng.uniform(0.5, 1.3), rng.uniform(0.5, 1.3), rng.uniform(0.5, 1.3))),
                           "translation": (rng.uniform(-2.5, 2.5), rng.uniform(-0.3, 0.3),
                                           rng.uniform(-2.5, 2.5))})
     for k in range(2):
@@ -747,8 +748,9 @@ def instanced_workload(mesh_count=8, instance_count=1000, tri_range=(50_000, 200
                        emissive_tris=10_000, width=1920, height=1080, seed=5):
     """Two-level workload: mesh_count terrain meshes (triangles log-uniform in
     tri_range) placed instance_count times over a 60 x 60 field with seeded
-    rotations about +Y and scales, plus one panel of emissive_tris emissive
-    triangles above it; constant Skysphere."""
+    TRS transforms (rotations about +Y, per-axis scales in [0.6, 1.6]), plus
+    one panel of emissive_tris emissive triangles above it; constant
+    Skysphere."""
     rng = MT19937(seed)
     meshes = []
     for k in range(mesh_count):
@@ -761,7 +763,7 @@ def instanced_workload(mesh_count=8, instance_count=1000, tri_range=(50_000, 200
     instances = []
     for k in range(instance_count):
         instances.append({"mesh": int(rng() % mesh_count), "rotation": _random_rotation(rng, full=False),
-                          "scale": rng.uniform(0.6, 1.6),
+                          "scale": (rng.uniform(0.6, 1.6), rng.uniform(0.6, 1.6), rng.uniform(0.6, 1.6)),
                           "translation": (rng.uniform(-30, 30), rng.uniform(-0.2, 0.2),
                                           rng.uniform(-30, 30))})
     instances.append({"mesh": mesh_count, "rotation": (1.0, 0.0, 0.0, 0.0), "scale": 1.0,
