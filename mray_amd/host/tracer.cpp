@@ -135,12 +135,23 @@ Rigid IdentityRigid()
 }
 
 // Matrix -> rigid transform with uniform scale.  The rotation is taken from
-// the normalised columns (Shepperd's method).
-Rigid MakeRigid(const float* m12)
+// the normalised columns (Shepperd's method).  With `affine` any
+// non-singular matrix is kept as it is (a surface's transform moves only
+// positions; its rotation part is not needed).
+Rigid MakeRigid(const float* m12, bool affine = false)
 {
     Rigid r;
     std::copy_n(m12, 12, r.m);
     r.identity = false;
+    if(affine)
+    {
+        const double det = double(m12[0]) * (double(m12[5]) * m12[10] - double(m12[6]) * m12[9]) -
+                           double(m12[1]) * (double(m12[4]) * m12[10] - double(m12[6]) * m12[8]) +
+                           double(m12[2]) * (double(m12[4]) * m12[9] - double(m12[5]) * m12[8]);
+        if(det == 0.0 || !std::isfinite(det)) throw MRayError("Transform: singular matrix");
+        r.q[0] = 1; r.q[1] = r.q[2] = r.q[3] = 0;
+        return r;
+    }
     double c[3][3];
     double len[3];
     for(int j = 0; j < 3; j++)
@@ -208,7 +219,7 @@ void ApplyTbn(const Rigid& r, const float* q, float* out)
     out[3] = aw * bz + ax * by - ay * bx + az * bw;
 }
 
-Rigid Compose(const Rigid& outer, const Rigid& inner)
+Rigid Compose(const Rigid& outer, const Rigid& inner, bool affine = false)
 {
     if(inner.identity) return outer;
     if(outer.identity) return inner;
@@ -223,7 +234,7 @@ Rigid Compose(const Rigid& outer, const Rigid& inner)
             m[i * 4 + j] = v;
         }
     }
-    return MakeRigid(m);
+    return MakeRigid(m, affine);
 }
 
 // ---- host scene storage ---------------------------------------------------
@@ -764,7 +775,7 @@ class Tracer final : public TracerI
             const float* m = s[k].m;
             if(m[12] != 0.0f || m[13] != 0.0f || m[14] != 0.0f || m[15] != 1.0f)
                 Unsupported("A projective transform");
-            grp.xf[IndexOf(r[0]) + k] = MakeRigid(m);
+            grp.xf[IndexOf(r[0]) + k] = MakeRigid(m, true);
         }
     }
 
@@ -968,7 +979,7 @@ class Tracer final : public TracerI
         std::vector<uint8_t> used;
         auto emitBatch = [&](const PrimBatch& pb, const Rigid& xf, uint32_t key, bool cull)
         {
-            Rigid r = Compose(xf, pb.baked);
+            Rigid r = Compose(xf, pb.baked, true);
             uint32_t base = uint32_t(f.pos.size() / 3);
             for(uint32_t v = 0; v < pb.vertexCount; v++)
             {

@@ -349,10 +349,20 @@ int Errors(TracerI& t)
     TransGroupId tg = t.CreateTransformGroup("(T)Single");
     TransformId tr = t.ReserveTransformation(tg, {});
     t.CommitTransReservations(tg);
+    // a surface transform may be any non-singular affine matrix (positions
+    // only); a singular one is an error
     Matrix4x4 shear;
     shear.m[1] = 0.5f;
-    expectThrow("sheared transform", [&] {
-        t.PushTransAttribute(tg, One(uint32_t(tr)), 0, Data(std::vector<Matrix4x4>{shear})); });
+    try
+    {
+        t.PushTransAttribute(tg, One(uint32_t(tr)), 0, Data(std::vector<Matrix4x4>{shear}));
+        expect("sheared transform accepted", true);
+    }
+    catch(const MRayError&) { expect("sheared transform accepted", false); }
+    Matrix4x4 flat;
+    flat.m[10] = 0.0f;
+    expectThrow("singular transform", [&] {
+        t.PushTransAttribute(tg, One(uint32_t(tr)), 0, Data(std::vector<Matrix4x4>{flat})); });
     return bad == 0 ? 0 : 1;
 }
 

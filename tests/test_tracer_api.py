@@ -182,11 +182,14 @@ def test_json_scene_transforms_and_errors(tmp_path):
                        {"id": 1, "type": "Single", "layout": "trs", "translate": [1, 2, 3],
                         "rotate": [0, 0, 90], "scale": [2, 2, 2]},
                        {"id": 2, "type": "Single", "layout": "matrix",
-                        "matrix": [1, 0, 0, 5, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]}],
+                        "matrix": [1, 0, 0, 5, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1]},
+                       {"id": 3, "type": "Single", "layout": "trs", "translate": [0, 0, 0],
+                        "rotate": [0, 0, 0], "scale": [1, 2, 3]}],
         "Materials": [{"id": [0, 1], "type": "Lambert", "albedo": [[0.5, 0.5, 0.5], [0.2, 0.3, 0.4]]}],
         "Primitives": [quad],
         "Surfaces": [{"transform": 1, "material": 0, "primitive": 0},
-                     {"transform": 2, "material": 1, "primitive": 0, "cullBackFace": False}],
+                     {"transform": 2, "material": 1, "primitive": 0, "cullBackFace": False},
+                     {"transform": 3, "material": 0, "primitive": 0}],
         "LightSurfaces": [], "CameraSurfaces": [{"camera": 0}],
         "Boundary": {"light": 0},
     }
@@ -203,8 +206,12 @@ def test_json_scene_transforms_and_errors(tmp_path):
     assert np.allclose(p[0, 0], [2.0, 1.0, 3.0], atol=1e-5)
     # matrix: translate x by 5
     assert np.allclose(p[2, 0], [4.5, -0.5, 0.0], atol=1e-6)
-    assert list(f["cull"]) == [1, 1, 0, 0]
-    assert list(f["keys"]) == [0, 0, 1, 1]
+    # trs with a non-uniform scale moves positions only; shading frames stay
+    # the mesh's, as for every surface transform
+    assert np.allclose(p[4, 0], [-0.5, -1.0, 0.0], atol=1e-6)
+    assert np.allclose(f["tbn"][f["idx"][4]], f["tbn"][f["idx"][2]], atol=1e-7)
+    assert list(f["cull"]) == [1, 1, 0, 0, 1, 1]
+    assert list(f["keys"]) == [0, 0, 1, 1, 0, 0]
     doc["Materials"][0]["type"] = "Glass"
     with open(src, "w") as fh:
         json.dump(doc, fh)
