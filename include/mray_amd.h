@@ -53,7 +53,9 @@ enum
 enum
 {
     MRAY_BOUNDARY_NULL             = 0,   /* LightNull                       */
-    MRAY_BOUNDARY_SKYSPHERE_COOCTA = 1    /* LightSkysphere<CoOcta>, constant */
+    MRAY_BOUNDARY_SKYSPHERE_COOCTA = 1,   /* LightSkysphere<CoOcta>          */
+    MRAY_BOUNDARY_SKYSPHERE_SPHERICAL = 2 /* LightSkysphere<Spherical>: texture
+                                             uv latitude-longitude             */
 };
 
 /* Surface key of a triangle: a material index, or a light index with this flag. */
@@ -245,7 +247,9 @@ int mray_accel_cast_visibility(const MRayAccel* accel, const float* dRays, uint3
  * ((x + 1/2) / width, (y + 1/2) / height)), read bilinearly with wrapped
  * edges; NEE samples directions by texel luminance (PWC-2D,
  * Distributions.h:117-243).  The scene's boundary must be
- * MRAY_BOUNDARY_SKYSPHERE_COOCTA (boundaryRadiance is then unused);
+ * a Skysphere, MRAY_BOUNDARY_SKYSPHERE_COOCTA (texels over concentric-
+ * octahedral uv) or _SPHERICAL (latitude-longitude uv); boundaryRadiance is
+ * then unused;
  * renderers and merged scenes created afterwards use the texture.
  * Sizes 1..8192 per side; texels must be finite, luminance not all zero.
  * A second call replaces the texture for renderers created afterwards; the
@@ -429,7 +433,7 @@ int mray_instanced_batch_create(const MRayInstScene* scene, const MRayBatchView*
                                 uint32_t viewCount, const MRayRenderParams* hParams,
                                 MRayRenderer** out);
 /* A textured Skysphere for an instanced scene (as mray_accel_set_environment;
- * the MRayInstEnv boundary must be MRAY_BOUNDARY_SKYSPHERE_COOCTA).        */
+ * the MRayInstEnv boundary must be a Skysphere).                           */
 int mray_instanced_scene_set_environment(MRayInstScene* scene, uint32_t width, uint32_t height,
                                          const float* hRgb);
 

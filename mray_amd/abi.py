@@ -15,6 +15,7 @@ MAT_UNREAL = 3
 
 BOUNDARY_NULL = 0
 BOUNDARY_SKYSPHERE_COOCTA = 1
+BOUNDARY_SKYSPHERE_SPHERICAL = 2
 
 LIGHT_FLAG = 0x80000000
 

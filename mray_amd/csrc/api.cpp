@@ -95,7 +95,7 @@ static int ValidateScene(const MRaySceneDesc* s)
             SetError("unknown material type");
             return MRAY_ERR_INVALID;
         }
-    if(s->boundaryType > MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+    if(s->boundaryType > MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
     {
         SetError("unknown boundary light type");
         return MRAY_ERR_INVALID;

@@ -53,7 +53,8 @@ static void LuminanceRow(float out[3])
 
 int BuildEnvironment(DeviceArena& ar, SceneView& v, uint32_t W, uint32_t H, const float* hRgb)
 {
-    if(v.boundaryType != MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+    if(v.boundaryType != MRAY_BOUNDARY_SKYSPHERE_COOCTA &&
+       v.boundaryType != MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
     {
         SetError("set_environment: the scene's boundary is not a Skysphere");
         return MRAY_ERR_INVALID;

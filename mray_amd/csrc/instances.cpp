@@ -248,7 +248,7 @@ int mray_instanced_scene_create(const MRayAccel* const* meshes, uint32_t meshCou
         return MRAY_ERR_INVALID;
     }
     *out = nullptr;
-    if(hEnv->boundaryType > MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+    if(hEnv->boundaryType > MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
     {
         SetError("instanced_scene_create: unknown boundaryType");
         return MRAY_ERR_INVALID;

@@ -767,13 +767,50 @@ __device__ __forceinline__ void DirToCoOcta(V3 d, float& s, float& t)
     t = (v + 1.0f) * 0.5f;
 }
 
+__device__ __forceinline__ bool IsSkysphere(uint32_t t)
+{
+    return t == MRAY_BOUNDARY_SKYSPHERE_COOCTA || t == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL;
+}
+// SphericalCoordConverter (LightsDefault.hpp:177-222): Z-up spherical
+// angles, u = (azimuth + pi) / (2 pi), v = 1 - inclination / pi.
+__device__ __forceinline__ void DirToSpherical(V3 dz, float& u, float& v)
+{
+    float azimuth = atan2f(dz.y, dz.x);
+    float incl = acosf(ClampF(dz.z, -1.0f, 1.0f));
+    u = (azimuth + kPi) * 0.5f / kPi;
+    v = 1.0f - incl * kInvPi;
+}
+__device__ __forceinline__ V3 SphericalToDir(float u, float v, float& sinIncl)
+{
+    float theta = (u * kPi * 2.0f) - kPi;
+    float phi = (1.0f - v) * kPi;
+    float sT = sinf(theta), cT = cosf(theta), sP = sinf(phi), cP = cosf(phi);
+    sinIncl = sP;
+    return {cT * sP, sT * sP, cP};
+}
+// Skysphere texture coordinates of a Y-up direction for the boundary type
+__device__ __forceinline__ void SkyUV(uint32_t type, V3 dirYUp, float& u, float& v)
+{
+    const V3 dz = {dirYUp.z, dirYUp.x, dirYUp.y};           // YUpToZUp
+    if(type == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL) DirToSpherical(dz, u, v);
+    else DirToCoOcta(dz, u, v);
+}
+// Solid-angle pdf of a uv pdf: equal-area CoOcta 1 / (4 pi); spherical
+// 1 / (2 pi^2 sin(inclination)), zero at the poles
+__device__ __forceinline__ float SkySolidAnglePdf(uint32_t type, float uvPdf, float sinIncl)
+{
+    if(type == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
+        return (sinIncl <= 0.0f) ? 0.0f : uvPdf / (2.0f * kPi * kPi * sinIncl);
+    return uvPdf * 0.25f * kInvPi;
+}
+
 // Textured Skysphere radiance at Y-up direction dirYUp: bilinear, wrapped
 // edges, texel centres at (i + 1/2) / size (the reference's default
 // texture parameters, TracerI.h:71-82), evaluated in fp32.
 __device__ __forceinline__ V3 EnvRadiance(const SceneView& s, V3 dirYUp)
 {
     float u, v;
-    DirToCoOcta(V3{dirYUp.z, dirYUp.x, dirYUp.y}, u, v);   // YUpToZUp
+    SkyUV(s.boundaryType, dirYUp, u, v);
     const int W = int(s.envW), H = int(s.envH);
     float xf = u * float(W) - 0.5f, yf = v * float(H) - 0.5f;
     float x0f = floorf(xf), y0f = floorf(yf);
@@ -852,11 +889,17 @@ __device__ __forceinline__ float EnvPdfUV(const SceneView& s, float u, float v)
 // LightSkysphere::PdfSolidAngle (LightsDefault.hpp:359-370) of direction dir
 __device__ __forceinline__ float SkyspherePdf(const SceneView& s, V3 dir)
 {
-    if(s.envW == 0) return 1.0f * 0.25f * kInvPi;
-    V3 d = Normalize(dir);
-    float u, v;
-    DirToCoOcta(V3{d.z, d.x, d.y}, u, v);
-    return EnvPdfUV(s, u, v) * 0.25f * kInvPi;
+    float uvPdf = 1.0f, sinIncl = 1.0f;
+    const V3 d = Normalize(dir);
+    if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
+        sinIncl = sinf(acosf(ClampF(d.y, -1.0f, 1.0f)));    // Z-up z = Y-up y
+    if(s.envW != 0)
+    {
+        float u, v;
+        SkyUV(s.boundaryType, d, u, v);
+        uvPdf = EnvPdfUV(s, u, v);
+    }
+    return SkySolidAnglePdf(s.boundaryType, uvPdf, sinIncl);
 }
 
 template<bool SPECTRAL>
@@ -966,15 +1009,16 @@ __device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const 
         point = position;
         pointPdf = pdf;
     }
-    else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+    else if(IsSkysphere(s.boundaryType))
     {
         // LightSkysphere::SampleSolidAngle (LightsDefault.hpp:340-357)
-        float u = rn[0], v = rn[1], uvPdf = 1.0f;
+        float u = rn[0], v = rn[1], uvPdf = 1.0f, sinIncl = 1.0f;
         if(s.envW != 0) EnvSampleUV(s, rn[0], rn[1], u, v, uvPdf);
-        V3 dz = CoOctaToDir(u, v);
+        V3 dz = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL) ? SphericalToDir(u, v, sinIncl)
+                                                                       : CoOctaToDir(u, v);
         V3 dirYUp = {dz.y, dz.z, dz.x};
         point = distantPoint + dirYUp * s.sceneDiameter;
-        pointPdf = uvPdf * 0.25f * kInvPi;
+        pointPdf = SkySolidAnglePdf(s.boundaryType, uvPdf, sinIncl);
     }
     else
     {
@@ -984,7 +1028,7 @@ __device__ __forceinline__ LightSample SampleLight(const RenderConst& rc, const 
     V3 wO = Normalize(distantPoint - point);
     V4 emission;
     if(li < lightCount - 1u) emission = TriLightEmitViaPoint<SPECTRAL>(rc, s, li, wO, point, waves);
-    else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+    else if(IsSkysphere(s.boundaryType))
         emission = ConvRadiance<SPECTRAL>(rc, s.envW ? EnvRadiance(s, Normalize(-wO))
                                                      : V3{s.boundaryRad[0], s.boundaryRad[1],
                                                           s.boundaryRad[2]}, waves);
@@ -1017,7 +1061,7 @@ __device__ __forceinline__ float PdfLightTri(const SceneView& s, uint32_t li, fl
 __device__ __forceinline__ float PdfLightBoundary(const SceneView& s, V3 rayDir)
 {
     float selectionPDF = 1.0f / float(s.lightCount + 1u);
-    float pdf = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA) ? SkyspherePdf(s, rayDir) : 0.0f;
+    float pdf = IsSkysphere(s.boundaryType) ? SkyspherePdf(s, rayDir) : 0.0f;
     return pdf * selectionPDF;
 }
 
@@ -1074,7 +1118,7 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
         if(isBoundary)
         {
             // LightSkysphere::EmitViaHit (LightsDefault.hpp:405-421)
-            emission = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+            emission = IsSkysphere(s.boundaryType)
                 ? ConvRadiance<SPECTRAL>(rc, s.envW ? EnvRadiance(s, Normalize(rdir))
                                                     : V3{s.boundaryRad[0], s.boundaryRad[1],
                                                          s.boundaryRad[2]}, waves)

@@ -809,14 +809,25 @@ def coocta_to_dir_yup(u, v):
     return np.stack([y, zs, x], axis=-1)         # Z-up (x, y, z) -> Y-up (y, z, x)
 
 
+def spherical_to_dir_yup(u, v):
+    """Latitude-longitude uv in [0, 1]^2 -> Y-up unit directions (numpy):
+    azimuth (u * 2 pi - pi) and inclination ((1 - v) pi) about Z-up."""
+    th = np.asarray(u, np.float64) * 2.0 * math.pi - math.pi
+    ph = (1.0 - np.asarray(v, np.float64)) * math.pi
+    x, y, z = np.cos(th) * np.sin(ph), np.sin(th) * np.sin(ph), np.cos(ph)
+    return np.stack([y, z, x], axis=-1)
+
+
 def sky_env(width=128, height=128, sun=(0.4, 0.6, 0.3), sun_size=0.06, sun_radiance=40.0,
-            zenith=(0.15, 0.3, 0.8), horizon=(0.8, 0.85, 0.9), ground=(0.12, 0.1, 0.08)):
+            zenith=(0.15, 0.3, 0.8), horizon=(0.8, 0.85, 0.9), ground=(0.12, 0.1, 0.08),
+            mapping="coocta"):
     """A procedural sky texture [height, width, 3] over concentric-octahedral
-    uv: an elevation gradient from ground through horizon to zenith plus a
-    bright sun lobe (cosine power) around `sun`."""
+    (or, with mapping="spherical", latitude-longitude) uv: an elevation
+    gradient from ground through horizon to zenith plus a bright sun lobe
+    (cosine power) around `sun`."""
     ys, xs = np.meshgrid((np.arange(height) + 0.5) / height, (np.arange(width) + 0.5) / width,
                          indexing="ij")
-    d = coocta_to_dir_yup(xs, ys)
+    d = spherical_to_dir_yup(xs, ys) if mapping == "spherical" else coocta_to_dir_yup(xs, ys)
     el = d[..., 1]
     t = np.clip(el, 0.0, 1.0)[..., None] ** 0.5
     sky = np.asarray(horizon) * (1 - t) + np.asarray(zenith) * t

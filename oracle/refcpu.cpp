@@ -1448,13 +1448,50 @@ static inline void DirToCoOcta(V3 d, float& s, float& t)
     t = (v + 1.0f) * 0.5f;
 }
 
+static inline bool IsSkysphere(uint32_t t)
+{
+    return t == MRAY_BOUNDARY_SKYSPHERE_COOCTA || t == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL;
+}
+// SphericalCoordConverter (LightsDefault.hpp:177-222) over
+// CartesianToUnitSpherical / UVToSphericalAngles (GraphicsFunctions.h:253-285,
+// 447-470): Z-up azimuth and inclination; u = (azimuth + pi) / (2 pi),
+// v = 1 - inclination / pi.
+static inline void DirToSpherical(V3 dz, float& u, float& v)
+{
+    float azimuth = std::atan2(dz.y, dz.x);
+    float incl = std::acos(std::min(std::max(dz.z, -1.0f), 1.0f));
+    u = (azimuth + kPi) * 0.5f / kPi;
+    v = 1.0f - incl * kInvPi;
+}
+static inline V3 SphericalToDir(float u, float v, float& sinIncl)
+{
+    float theta = (u * kPi * 2.0f) - kPi;
+    float phi = (1.0f - v) * kPi;
+    float sT = std::sin(theta), cT = std::cos(theta), sP = std::sin(phi), cP = std::cos(phi);
+    sinIncl = sP;
+    return {cT * sP, sT * sP, cP};
+}
+static inline void SkyUV(uint32_t type, V3 dirYUp, float& u, float& v)
+{
+    if(type == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL) DirToSpherical(ZUpFromYUp(dirYUp), u, v);
+    else DirToCoOcta(ZUpFromYUp(dirYUp), u, v);
+}
+// ToSolidAnglePdf: CoOcta uvPdf / (4 pi); spherical uvPdf / (2 pi^2 sin),
+// zero where sin <= 0 (LightsDefault.hpp:195-222, 270-281)
+static inline float SkySolidAnglePdf(uint32_t type, float uvPdf, float sinIncl)
+{
+    if(type == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
+        return (sinIncl <= 0.0f) ? 0.0f : uvPdf / (2.0f * kPi * kPi * sinIncl);
+    return uvPdf * 0.25f * kInvPi;
+}
+
 // Skysphere texture read at a Y-up direction: the reference's default
 // texture parameters (TracerI.h:71-82: linear interpolation, wrapped
 // edges), texel centres at (i + 1/2) / size, bilinear weights in fp32.
 static V3 EnvRadiance(const Scene& s, V3 dirYUp)
 {
     float u, v;
-    DirToCoOcta(ZUpFromYUp(dirYUp), u, v);
+    SkyUV(s.boundaryType, dirYUp, u, v);
     const int W = int(s.envW), H = int(s.envH);
     float xf = u * float(W) - 0.5f, yf = v * float(H) - 0.5f;
     float x0f = std::floor(xf), y0f = std::floor(yf);
@@ -1517,10 +1554,17 @@ static float EnvPdfUV(const Scene& s, float u, float v)
 // LightSkysphere::PdfSolidAngle (LightsDefault.hpp:359-370)
 static float SkyspherePdf(const Scene& s, V3 dir)
 {
-    if(s.envW == 0) return 1.0f * 0.25f * kInvPi;
-    float u, v;
-    DirToCoOcta(ZUpFromYUp(Normalize(dir)), u, v);
-    return EnvPdfUV(s, u, v) * 0.25f * kInvPi;
+    float uvPdf = 1.0f, sinIncl = 1.0f;
+    const V3 d = Normalize(dir);
+    if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
+        sinIncl = std::sin(std::acos(std::min(std::max(d.y, -1.0f), 1.0f)));
+    if(s.envW)
+    {
+        float u, v;
+        SkyUV(s.boundaryType, d, u, v);
+        uvPdf = EnvPdfUV(s, u, v);
+    }
+    return SkySolidAnglePdf(s.boundaryType, uvPdf, sinIncl);
 }
 static V3 SkysphereRadiance(const Scene& s, V3 dirYUp)
 {
@@ -1723,15 +1767,16 @@ static LightSampleOut SampleLight(const Scene& s, const float* rn, V3 distantPoi
         point = position;
         pointPdf = pdf;
     }
-    else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+    else if(IsSkysphere(s.boundaryType))
     {
         // LightSkysphere::SampleSolidAngle (LightsDefault.hpp:340-357):
         // constant radiance: uv = xi, pdf 1; a texture: PWC-2D by
-        // luminance; CoOcta pdf / (4 pi)
-        float u = rn[0], v = rn[1], uvPdf = 1.0f;
+        // luminance; then the mapping's solid-angle pdf
+        float u = rn[0], v = rn[1], uvPdf = 1.0f, sinIncl = 1.0f;
         if(s.envW) EnvSampleUV(s, rn[0], rn[1], u, v, uvPdf);
-        V3 dirYUp = YUpFromZUp(CoOctaToDir(u, v));
-        float pdf = uvPdf * 0.25f * kInvPi;
+        V3 dirYUp = YUpFromZUp((s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
+                                   ? SphericalToDir(u, v, sinIncl) : CoOctaToDir(u, v));
+        float pdf = SkySolidAnglePdf(s.boundaryType, uvPdf, sinIncl);
         point = distantPoint + dirYUp * s.sceneDiameter;
         pointPdf = pdf;
     }
@@ -1745,7 +1790,7 @@ static LightSampleOut SampleLight(const Scene& s, const float* rn, V3 distantPoi
     wO = Normalize(wO);
     if(li < lightCount - 1u)
         emission = TriLightEmitViaSurfacePoint(s, li, wO, point, sc);
-    else if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+    else if(IsSkysphere(s.boundaryType))
         emission = ConvertRadiance(sc, SkysphereRadiance(s, Normalize(-wO)));
     else
         emission = Splat4(0.0f);
@@ -1782,7 +1827,7 @@ static float PdfLightBoundary(const Scene& s, V3 rayDir)
     uint32_t lightCount = uint32_t(s.lightTri.size()) + 1u;
     float selectionPDF = 1.0f / float(lightCount);
     float pdf = 0.0f;
-    if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA) pdf = SkyspherePdf(s, rayDir);
+    if(IsSkysphere(s.boundaryType)) pdf = SkyspherePdf(s, rayDir);
     return pdf * selectionPDF;
 }
 
@@ -2055,7 +2100,7 @@ static void BounceSlot(const Scene& s, const MRayRenderParams& p, Slot& sl, Rend
         if(isBoundary)
         {
             // LightSkysphere::EmitViaHit (LightsDefault.hpp:405-421)
-            emission = (s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_COOCTA)
+            emission = IsSkysphere(s.boundaryType)
                 ? ConvertRadiance(sc, SkysphereRadiance(s, Normalize(sl.dir))) : Splat4(0.0f);
         }
         else
@@ -2836,7 +2881,7 @@ void rc_light_power_table(void* h, float* prob, uint32_t* alias, float* pmf)
 int rc_scene_set_environment(void* h, uint32_t W, uint32_t H, const float* rgb)
 {
     Scene& s = *static_cast<Scene*>(h);
-    if(s.boundaryType != MRAY_BOUNDARY_SKYSPHERE_COOCTA) return -1;
+    if(!IsSkysphere(s.boundaryType)) return -1;
     return BuildEnv(s, W, H, rgb);
 }
 void rc_env_cdfs(void* h, float* cdfX, float* cdfY)

@@ -112,3 +112,25 @@ def test_errors(torch_gpu):
     with pytest.raises(native.MRayError, match="non-finite"):
         acc.set_environment(bad)
     acc.close()
+
+
+@pytest.mark.parametrize("textured", [False, True])
+def test_spherical_skysphere_matches_oracle(torch_gpu, textured):
+    """The latitude-longitude Skysphere, constant and textured."""
+    from mray_amd import native
+    sc = _scene(env=scenes.sky_env(96, 48, mapping="spherical") if textured else None)
+    if not textured:
+        sc.env_map = None
+    sc.boundary_type = abi.BOUNDARY_SKYSPHERE_SPHERICAL
+    params = abi.render_params(48, 32, 4, rr=(3, 8), seed=11, max_ray_count=1500)
+    acc = native.Accel(sc)
+    r = native.Renderer(acc, params)
+    st = r.render_all()
+    film = r.film()
+    r.close()
+    acc.close()
+    ofilm, ost = refcpu.OracleScene(sc).render(params)
+    assert st["dead_paths"] == ost["dead_paths"]
+    e = relmse(film_image(film), film_image(ofilm))
+    print(f"spherical textured={textured}: relMSE={e:.3e}", flush=True)
+    assert e <= 1e-3
