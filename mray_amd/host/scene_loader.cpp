@@ -4,8 +4,8 @@
 //
 // Loads what the path tracer supports: inline "Triangle" primitives (tags
 // nodeTriangle / nodeTriangleIndexed), Lambert / Reflect / Refract / Unreal
-// materials with constant parameters, Null / Primitive / Skysphere_CoOcta
-// lights with constant radiance, Pinhole cameras, Identity / Single
+// materials with constant parameters, Null / Primitive / Skysphere_CoOcta /
+// Skysphere_Spherical lights with constant radiance, Pinhole cameras, Identity / Single
 // transforms (layout "matrix" or "trs"), and the Surfaces / LightSurfaces /
 // CameraSurfaces / Boundary definitions.  Only entities the definitions use
 // are loaded, as in the reference.  Anything else throws MRayError.
@@ -365,8 +365,8 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
 
     // ---- lights -------------------------------------------------------------------
     std::map<uint32_t, LightId> lightId;
-    LightGroupId primLights{0}, sky{0};
-    bool havePrimLights = false, haveSky = false;
+    LightGroupId primLights{0}, sky{0}, skySph{0};
+    bool havePrimLights = false, haveSky = false, haveSkySph = false;
     struct LightPush { LightGroupId g; LightId id; Vector3 rad; bool twoSided; bool prim; };
     std::vector<LightPush> lpush;
     for(uint32_t l : usedLights)
@@ -392,10 +392,19 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
             lpush.push_back({sky, id, Vec3(PerId(*node, "radiance", i, cnt)), false, false});
             continue;
         }
+        if(type == "Skysphere_Spherical")
+        {
+            if(!haveSkySph) { skySph = tracer.CreateLightGroup("(L)Skysphere_Spherical"); haveSkySph = true; }
+            LightId id = tracer.ReserveLight(skySph, {});
+            lightId[l] = id;
+            lpush.push_back({skySph, id, Vec3(PerId(*node, "radiance", i, cnt)), false, false});
+            continue;
+        }
         Unsupported_("light type " + type);
     }
     if(havePrimLights) tracer.CommitLightReservations(primLights);
     if(haveSky) tracer.CommitLightReservations(sky);
+    if(haveSkySph) tracer.CommitLightReservations(skySph);
     for(const LightPush& lp : lpush)
     {
         CommonIdRange r = {uint32_t(lp.id), uint32_t(lp.id)};

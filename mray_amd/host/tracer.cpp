@@ -45,6 +45,7 @@ constexpr std::string_view kPrimTriangle = "(P)Triangle";
 constexpr std::string_view kMatNames[4] = {"(Mt)Lambert", "(Mt)Reflect", "(Mt)Refract", "(Mt)Unreal"};
 constexpr std::string_view kLightPrimTri = "(L)Prim(P)Triangle";
 constexpr std::string_view kLightSky     = "(L)Skysphere_CoOcta";
+constexpr std::string_view kLightSkySph  = "(L)Skysphere_Spherical";
 constexpr std::string_view kLightNull    = "(L)Null";
 constexpr std::string_view kCamPinhole   = "(C)Pinhole";
 constexpr std::string_view kTransIdentity = "(T)Identity";
@@ -414,7 +415,7 @@ class Tracer final : public TracerI
     TypeNameList TransformGroups() const override { return {kTransIdentity, kTransSingle}; }
     TypeNameList CameraGroups() const override { return {kCamPinhole}; }
     TypeNameList MediumGroups() const override { return {}; }
-    TypeNameList LightGroups() const override { return {kLightNull, kLightPrimTri, kLightSky}; }
+    TypeNameList LightGroups() const override { return {kLightNull, kLightPrimTri, kLightSky, kLightSkySph}; }
     TypeNameList Renderers() const override { return {kRendRGB, kRendSpectral}; }
 
     PrimAttributeInfoList AttributeInfo(PrimGroupId g) const override
@@ -444,7 +445,8 @@ class Tracer final : public TracerI
         switch(lightGroups[U(g)].kind)
         {
             case 1: return kPrimLightAttribs;
-            case 2: return kSkyAttribs;
+            case 2:
+            case 3: return kSkyAttribs;
             default: return {};
         }
     }
@@ -476,7 +478,8 @@ class Tracer final : public TracerI
         switch(LightKind(n))
         {
             case 1: return kPrimLightAttribs;
-            case 2: return kSkyAttribs;
+            case 2:
+            case 3: return kSkyAttribs;
             default: return {};
         }
     }
@@ -503,7 +506,7 @@ class Tracer final : public TracerI
     {
         if(U(g) >= lightGroups.size()) throw MRayError("Unknown LightGroupId");
         uint32_t k = lightGroups[U(g)].kind;
-        return std::string(k == 1 ? kLightPrimTri : k == 2 ? kLightSky : kLightNull);
+        return std::string(k == 1 ? kLightPrimTri : k == 2 ? kLightSky : k == 3 ? kLightSkySph : kLightNull);
     }
     std::string TypeName(RendererId r) const override
     {
@@ -523,6 +526,7 @@ class Tracer final : public TracerI
         if(n == kLightNull) return 0;
         if(n == kLightPrimTri) return 1;
         if(n == kLightSky) return 2;
+        if(n == kLightSkySph) return 3;
         throw MRayError("Unknown light type \"" + std::string(n) + "\"");
     }
     static bool RendererKind(std::string_view n)
@@ -940,8 +944,8 @@ class Tracer final : public TracerI
     LightSurfaceId SetBoundarySurface(LightId l, TransformId t) override
     {
         uint32_t g = GroupOf(U(l));
-        if(g >= lightGroups.size() || (g != 0 && lightGroups[g].kind != 2))
-            throw MRayError("Boundary light must be a (L)Null or (L)Skysphere_CoOcta light");
+        if(g >= lightGroups.size() || (g != 0 && lightGroups[g].kind < 2))
+            throw MRayError("Boundary light must be a (L)Null or (L)Skysphere light");
         if(g != 0 && IndexOf(U(l)) >= lightGroups[g].lights.size()) throw MRayError("Unknown LightId");
         TransformOf(t);
         hasBoundary = true;
@@ -1032,7 +1036,8 @@ class Tracer final : public TracerI
         if(hasBoundary && GroupOf(boundaryLight) != 0)
         {
             const LightEntry& le = lightGroups[GroupOf(boundaryLight)].lights[IndexOf(boundaryLight)];
-            f.boundaryType = MRAY_BOUNDARY_SKYSPHERE_COOCTA;
+            f.boundaryType = lightGroups[GroupOf(boundaryLight)].kind == 3
+                             ? MRAY_BOUNDARY_SKYSPHERE_SPHERICAL : MRAY_BOUNDARY_SKYSPHERE_COOCTA;
             std::copy_n(le.radiance, 3, f.boundaryRad);
             // a constant sky is invariant under the boundary transform
         }

@@ -573,10 +573,11 @@ def to_mray_json(scene, path):
             e["cauchyFront"] = list(m.get("cauchyFront", (1.0, 0.0, 0.0)))
         mats.append(e)
     boundary_light = 0
-    if scene.boundary_type == abi.BOUNDARY_SKYSPHERE_COOCTA:
+    if scene.boundary_type in (abi.BOUNDARY_SKYSPHERE_COOCTA, abi.BOUNDARY_SKYSPHERE_SPHERICAL):
         boundary_light = len(lights)
-        lights.append({"id": boundary_light, "type": "Skysphere_CoOcta",
-                       "radiance": list(scene.boundary_radiance)})
+        sky = ("Skysphere_CoOcta" if scene.boundary_type == abi.BOUNDARY_SKYSPHERE_COOCTA
+               else "Skysphere_Spherical")
+        lights.append({"id": boundary_light, "type": sky, "radiance": list(scene.boundary_radiance)})
     cam = scene.camera
     fx, fy = cam["fov"]
     doc = {

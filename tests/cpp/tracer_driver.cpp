@@ -239,9 +239,10 @@ CamSurfaceId Feed(TracerI& t, const SceneFile& s)
             t.PushLightAttribute(lg, One(uint32_t(ids[j])), 1, std::move(two));
         }
     }
-    if(s.boundaryType == 1)
+    if(s.boundaryType == 1 || s.boundaryType == 2)
     {
-        LightGroupId sg = t.CreateLightGroup("(L)Skysphere_CoOcta");
+        LightGroupId sg = t.CreateLightGroup(s.boundaryType == 1 ? "(L)Skysphere_CoOcta"
+                                                                 : "(L)Skysphere_Spherical");
         LightId sky = t.ReserveLight(sg, {});
         t.CommitLightReservations(sg);
         Vector3 rad;
@@ -297,7 +298,8 @@ int Errors(TracerI& t)
     auto has = [](const TypeNameList& l, std::string_view n) { return std::find(l.begin(), l.end(), n) != l.end(); };
     expect("prim types", has(t.PrimitiveGroups(), "(P)Triangle"));
     expect("material types", has(t.MaterialGroups(), "(Mt)Unreal") && t.MaterialGroups().size() == 4);
-    expect("light types", has(t.LightGroups(), "(L)Prim(P)Triangle") && has(t.LightGroups(), "(L)Skysphere_CoOcta"));
+    expect("light types", has(t.LightGroups(), "(L)Prim(P)Triangle") && has(t.LightGroups(), "(L)Skysphere_CoOcta") &&
+           has(t.LightGroups(), "(L)Skysphere_Spherical"));
     expect("renderer types", has(t.Renderers(), "(R)PathTracerRGB") && has(t.Renderers(), "(R)PathTracerSpectral"));
     auto ua = t.AttributeInfoMat("(Mt)Unreal");
     expect("Unreal attributes", ua.size() == 5 && ua[0].name == "albedo" && ua[2].name == "roughness" &&

@@ -113,6 +113,16 @@ def test_flatten_terrain_with_skysphere(tmp_path):
     _check_same(s, _flatten(s, tmp_path))
 
 
+def test_flatten_spherical_skysphere(tmp_path):
+    """A (L)Skysphere_Spherical boundary flattens to the latitude-longitude
+    boundary type, through direct TracerI calls and through a JSON file."""
+    s = scenes.terrain_spheres(grid=16, spheres=4, width=32, height=18)
+    s.boundary_type = abi.BOUNDARY_SKYSPHERE_SPHERICAL
+    _check_same(s, _flatten(s, tmp_path))
+    (tmp_path / "json").mkdir()
+    _check_same_json(s, _flatten_json(s, tmp_path / "json"))
+
+
 def test_error_behaviour():
     _built()
     r = subprocess.run([DRIVER, TRACER_LIB, "errors"], capture_output=True, text=True, timeout=120)
