@@ -61,8 +61,11 @@ def test_tracer_api_matches_cabi_cornell(torch_gpu, tmp_path, spectral):
     assert e < 1e-8
 
 
-def test_tracer_api_matches_cabi_terrain(torch_gpu, tmp_path):
+@pytest.mark.parametrize("sky", [abi.BOUNDARY_SKYSPHERE_COOCTA, abi.BOUNDARY_SKYSPHERE_SPHERICAL])
+def test_tracer_api_matches_cabi_terrain(torch_gpu, tmp_path, sky):
+    """Under a (L)Skysphere_CoOcta and a (L)Skysphere_Spherical boundary."""
     scene = scenes.terrain_spheres(grid=64, spheres=60, width=80, height=45)
+    scene.boundary_type = sky
     params = abi.render_params(80, 45, 2, rr=(10, 10), mode=abi.SAMPLE_NEE_MIS, seed=1)
     a = _driver_render(scene, tmp_path, 80, 45, 2, False, abi.SAMPLE_NEE_MIS, (10, 10), 1)
     b = _cabi_render(scene, params)
