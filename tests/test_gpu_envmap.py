@@ -134,3 +134,48 @@ def test_spherical_skysphere_matches_oracle(torch_gpu, textured):
     e = relmse(film_image(film), film_image(ofilm))
     print(f"spherical textured={textured}: relMSE={e:.3e}", flush=True)
     assert e <= 1e-3
+
+
+def test_spherical_skysphere_batch_and_instanced(torch_gpu):
+    """The latitude-longitude Skysphere in a batch beside a concentric-
+    octahedral one, and under an instanced scene."""
+    from mray_amd import native
+    a = _scene(env=scenes.sky_env(64, 32, mapping="spherical"), seed=3)
+    a.boundary_type = abi.BOUNDARY_SKYSPHERE_SPHERICAL
+    scs = [a, _scene(seed=4, env=scenes.sky_env(32, 32))]
+    views = [dict(scene=0, width=32, height=24, spp=3), dict(scene=1, width=24, height=16, spp=2)]
+    params = abi.render_params(1, 1, 1, rr=(3, 8), seed=12, max_ray_count=1500)
+    accs = [native.Accel(s) for s in scs]
+    bs = native.BatchScene(accs)
+    for acc in accs:
+        acc.close()
+    r = native.BatchRenderer(bs, views, params)
+    st = r.render_all()
+    films = r.film()
+    r.close()
+    bs.close()
+    ofilms, ost = refcpu.render_batch([refcpu.OracleScene(s) for s in scs], views, params)
+    assert st["dead_paths"] == ost["dead_paths"]
+    for k, (f, o) in enumerate(zip(films, ofilms)):
+        e = relmse(film_image(f), film_image(o))
+        print(f"batch view {k}: relMSE={e:.3e}", flush=True)
+        assert e <= 1e-3
+    meshes, inst, env, _ = scenes.instanced_test_scene(width=40, height=30)
+    env["boundary_type"] = abi.BOUNDARY_SKYSPHERE_SPHERICAL
+    env["env_map"] = scenes.sky_env(48, 24, sun=(0.2, 0.8, -0.4), mapping="spherical")
+    flat = scenes.flatten_instances(meshes, inst, env)
+    assert flat.boundary_type == abi.BOUNDARY_SKYSPHERE_SPHERICAL
+    ip = abi.render_params(40, 30, 4, rr=(3, 8), seed=13, max_ray_count=1200)
+    accs = [native.Accel(m) for m in meshes]
+    isc = native.InstancedScene(accs, inst, env)
+    for acc in accs:
+        acc.close()
+    ir = native.InstancedRenderer(isc, ip)
+    ir.render_all()
+    film = ir.film()
+    ir.close()
+    isc.close()
+    ofilm, _ = refcpu.OracleScene(flat).render(ip)
+    e = relmse(film_image(film), film_image(ofilm))
+    print(f"instanced spherical sky: relMSE={e:.3e}", flush=True)
+    assert e <= 1e-3
