@@ -9,8 +9,8 @@
 //                       fetch of three coefficients; sigmoid of a quadratic in nm
 //   SpectralRadiance    ConvertRadiance (SpectrumContext.hpp:127-150)
 //   SpectrumToRGB       ConvertSpectraToRGBSingle (SpectrumContext.cu:137-171)
-// Tables (coefficient LUT, CIE observer, illuminant, XYZ->RGB) are this
-// build's own; spectral_lut.cpp generates them.
+// Tables: the CIE 1931 observer and D65 illuminant (colour_data.cpp), the
+// Rec.709 XYZ->RGB matrix, and the coefficient LUT that spectral_lut.cpp fits.
 #pragma once
 
 #include "common.h"

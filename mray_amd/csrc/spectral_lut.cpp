@@ -1,6 +1,6 @@
 // RGB -> spectrum coefficient tables (the capability of the reference's
 // SpectraLUTGen, Source/SpectraLUTGen/main.cpp, loaded at
-// Tracer/SpectrumContext.cu:298-472), built from the published method:
+// Tracer/SpectrumContext.cu:298-472), built by the published method:
 //
 //   Jakob & Hanika 2019, "A Low-Dimensional Function Space for Efficient
 //   Spectral Upsampling": a reflectance is S(c0 l^2 + c1 l + c2) with
@@ -11,13 +11,25 @@
 //   value on a twice-smoothstepped axis; each (x, y) column is solved along z
 //   from N/5 outwards with warm starts.
 //
-// Colour data (this build's own, no tables are copied):
-//   observer    CIE 1931 2-degree colour matching functions, multi-lobe
-//               piecewise-Gaussian fit of Wyman, Sloan & Shirley 2013 (JCGT 2(2))
-//   illuminant  Planck radiator at 6504 K, normalised to Y = 1
-//   RGB space   Rec.709 primaries with the illuminant as white point
+// The fit follows the reference generator's choices step for step, so the
+// table matches the one the reference renders with up to rounding
+// (SpectraLUTGen/main.cpp:80-436):
+//   quadrature   Simpson's 3/8 rule over the 471 1-nm samples (:99-110)
+//   integrand    XYZ->RGB(observer) * D65 / 10566.864 * weight (:121-131)
+//   Lab white    the illuminant's XYZ under the same quadrature (:130-135)
+//   variable     normalised wavelength i / 471 (:160-168)
+//   solver       15 passes of Gauss-Newton, central differences with step
+//                1e-4, LU solve, coefficients scaled down when the largest
+//                exceeds 200, early exit when |residual|^2 < 1e-7 (:186-260)
+//   storage      coefficients in nanometres through l' = (l - 360) / 470
+//                (:350-356: the stored polynomial is the fitted one
+//                stretched by 471/470 in wavelength, as the reference's is)
+//
+// Colour data: the CIE 1931 2-degree observer and the D65 SPD
+// (colour_data.cpp), Rec.709 primaries with the D65 white point.
 // Stored coefficients evaluate in nanometres, as Converter::ConvertAlbedo does.
 #include "spectral.h"
+#include "colour_data.h"
 
 #include <array>
 #include <atomic>
@@ -38,26 +50,7 @@ namespace mray
 namespace
 {
 constexpr int kLutN = 64;
-constexpr double kLmin = 360.0, kLmax = 830.0;
-
-double Lobe(double x, double mu, double s1, double s2)
-{
-    double t = (x - mu) / ((x < mu) ? s1 : s2);
-    return std::exp(-0.5 * t * t);
-}
-double XBar(double l)
-{
-    return 1.056 * Lobe(l, 599.8, 37.9, 31.0) + 0.362 * Lobe(l, 442.0, 16.0, 26.7) -
-           0.065 * Lobe(l, 501.1, 20.4, 26.2);
-}
-double YBar(double l) { return 0.821 * Lobe(l, 568.8, 46.9, 40.5) + 0.286 * Lobe(l, 530.9, 16.3, 31.1); }
-double ZBar(double l) { return 1.217 * Lobe(l, 437.0, 11.8, 36.0) + 0.681 * Lobe(l, 459.0, 26.0, 13.8); }
-double Planck(double lnm, double T)
-{
-    const double c1 = 3.741771852e-16, c2 = 1.438776877e-2;
-    double l = lnm * 1e-9;
-    return c1 / (l * l * l * l * l * (std::exp(c2 / (l * T)) - 1.0));
-}
+constexpr int kFitPasses = 15;
 
 using M3 = std::array<double, 9>;
 M3 Inverse(const M3& m)
@@ -70,175 +63,197 @@ M3 Inverse(const M3& m)
     for(double& v : r) v /= det;
     return r;
 }
+std::array<double, 3> Mul(const M3& m, const double v[3])
+{
+    return {m[0] * v[0] + m[1] * v[1] + m[2] * v[2], m[3] * v[0] + m[4] * v[1] + m[5] * v[2],
+            m[6] * v[0] + m[7] * v[1] + m[8] * v[2]};
+}
+
+// RGB -> XYZ of a colour space from its primaries and white point
+// (Color::GenRGBToXYZ, Core/ColorFunctions.h:347-361, after Lindbloom): the
+// primaries' XYZ at Y = 1 as columns, scaled so RGB (1, 1, 1) is the white.
+// The reference evaluates it in float; the entries are rounded to float here.
+M3 RgbToXyz(const double p[8])
+{
+    auto xyz = [](double x, double y) { return std::array<double, 3>{x / y, 1.0, (1.0 - x - y) / y}; };
+    auto r = xyz(p[0], p[1]), g = xyz(p[2], p[3]), b = xyz(p[4], p[5]), w = xyz(p[6], p[7]);
+    M3 P = {r[0], g[0], b[0], r[1], g[1], b[1], r[2], g[2], b[2]};
+    M3 Pi = Inverse(P);
+    auto S = Mul(Pi, w.data());
+    M3 out;
+    for(int row = 0; row < 3; row++)
+        for(int k = 0; k < 3; k++) out[3 * row + k] = double(float(P[3 * row + k] * S[k]));
+    return out;
+}
 
 struct Colour
 {
-    // fit quadrature (1 nm)
-    std::vector<double> lam, wx, wy, wz;   // CMF * illuminant * dl, normalised
-    double white[3];
-    M3 rgbToXyz, xyzToRgb;
-    std::vector<float> cmf;                // [471][3]
-    std::vector<float> illum;              // [471]
+    std::vector<double> toRgb;   // [471][3] fit integrand: XYZ->RGB(observer) * D65 * weight
+    double white[3];             // Lab white point: the illuminant's XYZ, same quadrature
+    M3 rgbToXyz, xyzToRgb;       // Rec.709 with the D65 white
+    std::vector<float> cmf;      // [471][3] device observer: each channel over its integral
+    std::vector<float> illum;    // [471] device illuminant: D65 * integral(ybar) / 10566.864
 };
 
-// Observer and illuminant normalised as the reference does at load time
-// (SpectrumContext.cu:409-426): each CMF channel over its own integral, the
-// illuminant scaled by integral(ybar) / integral(illum * ybar), so the
-// illuminant's Y is 1 and a flat unit spectrum has XYZ = (1, 1, 1).
+// Simpson's 3/8 weight of sample i of n (n - 1 divisible by 3): 1 3 3 2 3 3 2 ... 3 3 1
+double Simpson38(int i, int n)
+{
+    const double W = 3.0 / 8.0;
+    if(i == 0 || i == n - 1) return W;
+    return ((i - 1) % 3 == 2) ? 2.0 * W : 3.0 * W;
+}
+
 Colour MakeColour()
 {
     Colour c;
-    const double T = 6504.0;
-    double ix = 0, iy = 0, iz = 0, illumY = 0;
-    for(int i = 0; i <= 470; i++)
-    {
-        double l = kLmin + i;
-        ix += XBar(l);
-        iy += YBar(l);
-        iz += ZBar(l);
-        illumY += Planck(l, T) * YBar(l);
-    }
-    const double illumScale = iy / illumY;
-    c.cmf.resize(471 * 3);
-    c.illum.resize(471);
+    c.rgbToXyz = RgbToXyz(kRec709Primaries);
+    M3 inv = Inverse(c.rgbToXyz);
+    for(double& v : inv) v = double(float(v));
+    c.xyzToRgb = inv;
+    c.toRgb.resize(size_t(kCieRows) * 3);
     c.white[0] = c.white[1] = c.white[2] = 0.0;
-    for(int i = 0; i <= 470; i++)
+    c.cmf.resize(size_t(kCieRows) * 3);
+    c.illum.resize(kCieRows);
+    // device tables, in float as the reference computes them at load time
+    // (SpectrumContext.cu:409-426)
+    const float wX = 1.0f / kCieIntegralX, wY = 1.0f / kCieIntegralY, wZ = 1.0f / kCieIntegralZ;
+    float illumFactor = kCieIntegralY;
+    illumFactor /= kD65NormFactor;
+    for(int i = 0; i < kCieRows; i++)
     {
-        double l = kLmin + i;
-        double cx = XBar(l) / ix, cy = YBar(l) / iy, cz = ZBar(l) / iz;
-        double il = Planck(l, T) * illumScale;
-        c.cmf[3 * i] = float(cx);
-        c.cmf[3 * i + 1] = float(cy);
-        c.cmf[3 * i + 2] = float(cz);
-        c.illum[i] = float(il);
-        c.lam.push_back(l);
-        c.wx.push_back(cx * il);
-        c.wy.push_back(cy * il);
-        c.wz.push_back(cz * il);
-        c.white[0] += cx * il;
-        c.white[1] += cy * il;
-        c.white[2] += cz * il;
+        const float* row = kCieObserverD65[i];
+        c.cmf[3 * i] = row[0] * wX;
+        c.cmf[3 * i + 1] = row[1] * wY;
+        c.cmf[3 * i + 2] = row[2] * wZ;
+        c.illum[i] = row[3] * illumFactor;
+        // fit integrand
+        const double xyz[3] = {double(row[0]), double(row[1]), double(row[2])};
+        const double I = double(row[3]) / double(kD65NormFactor);
+        const double w = Simpson38(i, kCieRows);
+        auto rgb = Mul(c.xyzToRgb, xyz);
+        for(int k = 0; k < 3; k++)
+        {
+            c.toRgb[3 * i + k] = rgb[k] * I * w;
+            c.white[k] += xyz[k] * I * w;
+        }
     }
-    // Rec.709 primaries, white = the illuminant's XYZ
-    const double xr = 0.64, yr = 0.33, xg = 0.30, yg = 0.60, xb = 0.15, yb = 0.06;
-    M3 P = {xr / yr, xg / yg, xb / yb, 1.0, 1.0, 1.0, (1 - xr - yr) / yr, (1 - xg - yg) / yg,
-            (1 - xb - yb) / yb};
-    M3 Pi = Inverse(P);
-    double S[3];
-    for(int r = 0; r < 3; r++)
-        S[r] = Pi[3 * r] * c.white[0] + Pi[3 * r + 1] * c.white[1] + Pi[3 * r + 2] * c.white[2];
-    for(int r = 0; r < 3; r++)
-        for(int k = 0; k < 3; k++) c.rgbToXyz[3 * r + k] = P[3 * r + k] * S[k];
-    c.xyzToRgb = Inverse(c.rgbToXyz);
     return c;
 }
 
-double Sigmoid(double x) { return 0.5 + x / (2.0 * std::sqrt(1.0 + x * x)); }
-
-void Lab(const Colour& c, const double xyz[3], double out[3])
+// XYZ -> CIELAB against a white point (Color::XYZToCIELab, ColorFunctions.h:259-284)
+void Lab(const double white[3], const double xyz[3], double out[3])
 {
     auto f = [](double t)
     {
         const double d = 6.0 / 29.0;
-        return (t > d * d * d) ? std::cbrt(t) : t / (3 * d * d) + 4.0 / 29.0;
+        if(t > d * d * d) return std::cbrt(t);
+        const double k = 1.0 / (d * d * 3.0);
+        return t * k + 4.0 / 29.0;
     };
-    double fx = f(xyz[0] / c.white[0]), fy = f(xyz[1] / c.white[1]), fz = f(xyz[2] / c.white[2]);
+    double fx = f(xyz[0] / white[0]), fy = f(xyz[1] / white[1]), fz = f(xyz[2] / white[2]);
     out[0] = 116.0 * fy - 16.0;
     out[1] = 500.0 * (fx - fy);
     out[2] = 200.0 * (fy - fz);
 }
 
-// residual in Lab of normalised-wavelength coefficients
+// target Lab minus the Lab of the reflectance of normalised-wavelength
+// coefficients
 void Residual(const Colour& c, const double coef[3], const double targetLab[3], double r[3])
 {
-    double xyz[3] = {0, 0, 0};
-    const size_t n = c.lam.size();
-    for(size_t i = 0; i < n; i++)
+    double rgb[3] = {0, 0, 0};
+    const double norm = 1.0 / double(kCieRows);
+    for(int i = 0; i < kCieRows; i++)
     {
-        double t = (c.lam[i] - kLmin) / (kLmax - kLmin);
-        double s = Sigmoid((coef[0] * t + coef[1]) * t + coef[2]);
-        xyz[0] += s * c.wx[i];
-        xyz[1] += s * c.wy[i];
-        xyz[2] += s * c.wz[i];
+        double l = double(i) * norm;
+        double x = coef[0];
+        x = x * l + coef[1];
+        x = x * l + coef[2];
+        double s = 0.5 * x;
+        s /= std::sqrt(1.0 + x * x);
+        s += 0.5;
+        rgb[0] += c.toRgb[3 * i] * s;
+        rgb[1] += c.toRgb[3 * i + 1] * s;
+        rgb[2] += c.toRgb[3 * i + 2] * s;
     }
+    auto xyz = Mul(c.rgbToXyz, rgb);
     double lab[3];
-    Lab(c, xyz, lab);
-    for(int k = 0; k < 3; k++) r[k] = lab[k] - targetLab[k];
+    Lab(c.white, xyz.data(), lab);
+    for(int k = 0; k < 3; k++) r[k] = targetLab[k] - lab[k];
 }
 
-bool Solve3(double A[9], double b[3], double x[3])
+// A x = b by LU with partial pivoting; false when A is singular
+bool Solve3(const double Ain[9], const double b[3], double x[3])
 {
-    // Gaussian elimination with partial pivoting
-    double M[3][4];
+    double A[3][3];
+    int perm[3] = {0, 1, 2};
     for(int r = 0; r < 3; r++)
-    {
-        for(int k = 0; k < 3; k++) M[r][k] = A[3 * r + k];
-        M[r][3] = b[r];
-    }
+        for(int k = 0; k < 3; k++) A[r][k] = Ain[3 * r + k];
     for(int col = 0; col < 3; col++)
     {
         int piv = col;
         for(int r = col + 1; r < 3; r++)
-            if(std::fabs(M[r][col]) > std::fabs(M[piv][col])) piv = r;
-        if(std::fabs(M[piv][col]) < 1e-30) return false;
-        std::swap(M[col], M[piv]);
-        for(int r = 0; r < 3; r++)
+            if(std::fabs(A[r][col]) > std::fabs(A[piv][col])) piv = r;
+        if(A[piv][col] == 0.0) return false;
+        if(piv != col)
         {
-            if(r == col) continue;
-            double f = M[r][col] / M[col][col];
-            for(int k = col; k < 4; k++) M[r][k] -= f * M[col][k];
+            for(int k = 0; k < 3; k++) std::swap(A[col][k], A[piv][k]);
+            std::swap(perm[col], perm[piv]);
+        }
+        for(int r = col + 1; r < 3; r++)
+        {
+            A[r][col] /= A[col][col];
+            for(int k = col + 1; k < 3; k++) A[r][k] -= A[r][col] * A[col][k];
         }
     }
-    for(int r = 0; r < 3; r++) x[r] = M[r][3] / M[r][r];
+    double y[3];
+    for(int r = 0; r < 3; r++)
+    {
+        y[r] = b[perm[r]];
+        for(int k = 0; k < r; k++) y[r] -= A[r][k] * y[k];
+    }
+    for(int r = 2; r >= 0; r--)
+    {
+        x[r] = y[r];
+        for(int k = r + 1; k < 3; k++) x[r] -= A[r][k] * x[k];
+        x[r] /= A[r][r];
+    }
     return true;
 }
 
+// Gauss-Newton from coef (in place), normalised-wavelength coefficients
 void GaussNewton(const Colour& c, const double rgb[3], double coef[3])
 {
-    double xyz[3];
-    for(int r = 0; r < 3; r++)
-        xyz[r] = c.rgbToXyz[3 * r] * rgb[0] + c.rgbToXyz[3 * r + 1] * rgb[1] +
-                 c.rgbToXyz[3 * r + 2] * rgb[2];
+    auto xyz = Mul(c.rgbToXyz, rgb);
     double target[3];
-    Lab(c, xyz, target);
-    auto norm = [](const double r[3]) { return std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]); };
-    double r[3];
-    Residual(c, coef, target, r);
-    for(int it = 0; it < 30; it++)
+    Lab(c.white, xyz.data(), target);
+    const double eps = 1e-4;
+    for(int pass = 0; pass < kFitPasses; pass++)
     {
-        double rn = norm(r);
-        if(rn < 1e-6) break;
+        double r[3];
+        Residual(c, coef, target, r);
         double J[9];
-        const double eps = 1e-5;
         for(int k = 0; k < 3; k++)
         {
-            double cp[3] = {coef[0], coef[1], coef[2]};
-            cp[k] += eps;
-            double r1[3];
-            Residual(c, cp, target, r1);
-            for(int q = 0; q < 3; q++) J[3 * q + k] = (r1[q] - r[q]) / eps;
+            double lo[3] = {coef[0], coef[1], coef[2]}, hi[3] = {coef[0], coef[1], coef[2]};
+            lo[k] -= eps;
+            hi[k] += eps;
+            double r0[3], r1[3];
+            Residual(c, lo, target, r0);
+            Residual(c, hi, target, r1);
+            for(int q = 0; q < 3; q++) J[3 * q + k] = (r1[q] - r0[q]) * (0.5 / eps);
         }
-        double delta[3];
-        double rhs[3] = {r[0], r[1], r[2]};
-        if(!Solve3(J, rhs, delta)) break;
-        // backtracking: accept the first step length that lowers the residual
-        double step = 1.0;
-        bool improved = false;
-        for(int ls = 0; ls < 12; ls++, step *= 0.5)
+        double step[3];
+        if(!Solve3(J, r, step)) { coef[0] = coef[1] = coef[2] = NAN; return; }
+        for(int k = 0; k < 3; k++) coef[k] -= step[k];
+        int mi = 0;
+        if(coef[1] > coef[mi]) mi = 1;
+        if(coef[2] > coef[mi]) mi = 2;
+        if(coef[mi] > 200.0)
         {
-            double cand[3] = {coef[0] - step * delta[0], coef[1] - step * delta[1],
-                              coef[2] - step * delta[2]};
-            double rc[3];
-            Residual(c, cand, target, rc);
-            if(norm(rc) < rn)
-            {
-                coef[0] = cand[0]; coef[1] = cand[1]; coef[2] = cand[2];
-                r[0] = rc[0]; r[1] = rc[1]; r[2] = rc[2];
-                improved = true;
-                break;
-            }
+            const double f = 200.0 / coef[mi];
+            for(int k = 0; k < 3; k++) coef[k] *= f;
         }
-        if(!improved) break;
+        if(r[0] * r[0] + r[1] * r[1] + r[2] * r[2] < 1e-7) break;
     }
 }
 
@@ -249,6 +264,7 @@ struct HostTables
     std::vector<float> lut;   // [3][N][N][N][4]
     Colour colour;
     bool built = false;
+    bool failed = false;      // the fit produced a non-finite coefficient
 };
 
 // The table file lives where the reference looks for it, relative to the
@@ -328,15 +344,19 @@ bool ReadMrSpectra(const std::string& path, std::vector<float>& lut, std::string
     return err.empty();
 }
 
-void BuildLut(HostTables& h)
+// Fills the table; false if a cell's fit is not finite (the reference's
+// generator fails the same way, SpectraLUTGen/main.cpp:312-334).
+bool BuildLut(HostTables& h)
 {
     const int N = kLutN;
     const Colour& c = h.colour;
     h.lut.assign(size_t(3) * N * N * N * 4, 0.0f);
     std::vector<double> scale(N);
     for(int k = 0; k < N; k++) scale[k] = Smoothstep(Smoothstep(double(k) / double(N - 1)));
-    const double range = kLmax - kLmin;
+    // normalised wavelength i / 471 -> nanometres through (l - 360) / 470
+    const double p0 = double(kCieFirstNm), p1 = 1.0 / double(kCieRows - 1);
     std::atomic<int> next{0};
+    std::atomic<bool> ok{true};
     auto worker = [&]()
     {
         for(;;)
@@ -347,44 +367,41 @@ void BuildLut(HostTables& h)
             int yi = (job / N) % N;
             int xi = job % N;
             double x = double(xi) / double(N - 1), y = double(yi) / double(N - 1);
-            auto solveRange = [&](int k0, int k1, int step)
+            auto cell = [&](int k, double coef[3])
             {
-                double coef[3] = {0, 0, 0};
-                for(int k = k0; k != k1; k += step)
-                {
-                    double z = scale[k];
-                    double rgb[3];
-                    rgb[ch] = z;
-                    rgb[(ch + 1) % 3] = x * z;
-                    rgb[(ch + 2) % 3] = y * z;
-                    if(k == 0)
-                    {
-                        // black: the sigmoid only reaches 0 asymptotically;
-                        // S(-50) ~ 1e-4 everywhere
-                        coef[0] = 0.0; coef[1] = 0.0; coef[2] = -50.0;
-                    }
-                    else GaussNewton(c, rgb, coef);
-                    // normalised wavelength -> nanometres
-                    double A = coef[0], B = coef[1], Cc = coef[2];
-                    double c0 = A / (range * range);
-                    double c1 = B / range - 2.0 * A * kLmin / (range * range);
-                    double c2 = A * kLmin * kLmin / (range * range) - B * kLmin / range + Cc;
-                    size_t o = ((((size_t(ch) * N + k) * N + yi) * N + xi) * 4);
-                    h.lut[o] = float(c0);
-                    h.lut[o + 1] = float(c1);
-                    h.lut[o + 2] = float(c2);
-                    h.lut[o + 3] = 0.0f;
-                }
+                double z = scale[k];
+                double rgb[3];
+                rgb[ch] = z;
+                rgb[(ch + 1) % 3] = x * z;
+                rgb[(ch + 2) % 3] = y * z;
+                GaussNewton(c, rgb, coef);
+                if(!std::isfinite(coef[0]) || !std::isfinite(coef[1]) || !std::isfinite(coef[2]))
+                    ok = false;
+                const double A = coef[0], B = coef[1], Cc = coef[2];
+                size_t o = ((((size_t(ch) * N + k) * N + yi) * N + xi) * 4);
+                h.lut[o] = float(A * p1 * p1);
+                h.lut[o + 1] = float(B * p1 - 2.0 * A * p0 * p1 * p1);
+                h.lut[o + 2] = float(Cc - B * p0 * p1 + A * p0 * p1 * p0 * p1);
+                h.lut[o + 3] = 0.0f;
             };
-            int start = N / 5;
-            solveRange(start, N, 1);
-            solveRange(start - 1, -1, -1);
+            // from N/5 upwards with warm starts, then from N/5 - 1 down,
+            // warm-started from the N/5 cell
+            const int mid = N / 5;
+            double coef[3] = {0, 0, 0}, midCoef[3] = {0, 0, 0};
+            for(int k = mid; k < N; k++)
+            {
+                cell(k, coef);
+                if(k == mid) std::memcpy(midCoef, coef, sizeof(coef));
+            }
+            std::memcpy(coef, midCoef, sizeof(coef));
+            for(int k = mid - 1; k >= 0; k--) cell(k, coef);
         }
     };
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<std::thread> pool;
     for(unsigned t = 0; t < nt; t++) pool.emplace_back(worker);
     for(auto& t : pool) t.join();
+    return ok;
 }
 
 HostTables& Host()
@@ -397,10 +414,13 @@ HostTables& Host()
     std::string err;
     if(!ReadMrSpectra(CachePath(), h.lut, err))
     {
-        BuildLut(h);
-        (void)::mkdir((LibDir() + "/SpectraLUT").c_str(), 0755);
-        std::string werr;
-        (void)WriteMrSpectra(h.lut, CachePath(), werr);   // cache only: failure is harmless
+        h.failed = !BuildLut(h);
+        if(!h.failed)
+        {
+            (void)::mkdir((LibDir() + "/SpectraLUT").c_str(), 0755);
+            std::string werr;
+            (void)WriteMrSpectra(h.lut, CachePath(), werr);   // cache only: failure is harmless
+        }
     }
     h.built = true;
     return h;
@@ -418,8 +438,13 @@ int GetSpectralTables(SpectralTables& out)
     std::lock_guard<std::mutex> lk(gDevMutex);
     if(!dev.ready)
     {
-        gDevUploaded = true;
         HostTables& h = Host();
+        if(h.failed)
+        {
+            SetError("spectral tables: the RGB->spectrum fit produced a non-finite coefficient");
+            return MRAY_ERR_INVALID;
+        }
+        gDevUploaded = true;
         float4* dLut = nullptr;
         float4* dCmf = nullptr;
         float* dIllum = nullptr;
@@ -457,6 +482,11 @@ int mray_spectral_tables(int* hN, float* hLut, float* hCmf, float* hIllum, float
 {
     using namespace mray;
     HostTables& h = Host();
+    if(h.failed)
+    {
+        SetError("spectral tables: the RGB->spectrum fit produced a non-finite coefficient");
+        return MRAY_ERR_INVALID;
+    }
     if(hN) *hN = kLutN;
     if(hLut) std::memcpy(hLut, h.lut.data(), h.lut.size() * sizeof(float));
     if(hCmf) std::memcpy(hCmf, h.colour.cmf.data(), h.colour.cmf.size() * sizeof(float));

@@ -21,8 +21,12 @@
 //   materials          furnace, reciprocity, pdf integrals (tests/test_oracle_bsdf.py)
 //   path tracer        closed-form scenes: white furnace, directly seen emitter
 //                      (tests/test_oracle_pt.py)
-//   spectral LUT fit   parity unpinned against the reference coefficients (the
-//                      reference generates them at build time); round trip only.
+//   colour data        CIE 1931 observer and D65 values of the standard at
+//                      5-nm knots, D65 chromaticity, trapezoid integrals
+//                      (tests/test_spectral.py)
+//   spectral LUT fit   round trip of T_Spectrum.cu:54-236; the coefficient
+//                      values are parity unpinned (the reference generates
+//                      its table at build time and ships none)
 
 #include <algorithm>
 #include <atomic>
@@ -35,6 +39,7 @@
 #include <vector>
 
 #include "../include/mray_amd.h"
+#include "../mray_amd/csrc/colour_data.h"
 
 namespace rc
 {
@@ -2755,17 +2760,81 @@ int rc_render_batch(void* const* scenes, uint32_t nScenes, const MRayBatchView* 
     return 0;
 }
 
-// Spectral tables: LUT [3][res][res][res][3] + z scale [res], CMF [471][3],
-// illuminant [471], XYZ->RGB [9]
-void rc_spectral_load(int N, const float* lut, const float* cmf, const float* illum,
-                      const float* xyzToRgb)
+// Spectral tables.  The observer, illuminant and XYZ->RGB matrix are built
+// here from the CIE 1931 / D65 data (public standard data, the same values the
+// reference renders with), restating the reference's load-time normalisation
+// (Tracer/SpectrumContext.cu:401-426): each observer channel times the float
+// reciprocal of its integral, the illuminant times integral(ybar) / 10566.864;
+// and Color::GenRGBToXYZ (Core/ColorFunctions.h:347-361) in float for Rec.709
+// with the D65 white (0.31272, 0.32903).  Only the coefficient LUT comes from
+// the caller: [3][N][N][N][4] (c0, c1, c2, 0), x fastest.
+static void Inverse3(const float m[9], float out[9])
+{
+    const float a = m[0], b = m[1], c = m[2], d = m[3], e = m[4], f = m[5], g = m[6], h = m[7],
+                i = m[8];
+    const float A = e * i - f * h, B = f * g - d * i, Cc = d * h - e * g;
+    const float det = a * A + b * B + c * Cc;
+    const float r[9] = {A, c * h - b * i, b * f - c * e, B, a * i - c * g, c * d - a * f,
+                        Cc, b * g - a * h, a * e - b * d};
+    const float inv = 1.0f / det;
+    for(int k = 0; k < 9; k++) out[k] = r[k] * inv;
+}
+
+static void BuildColourTables()
+{
+    using namespace mray;
+    gSpec.cmf.resize(size_t(kCieRows) * 3);
+    gSpec.illum.resize(kCieRows);
+    const float w[3] = {1.0f / kCieIntegralX, 1.0f / kCieIntegralY, 1.0f / kCieIntegralZ};
+    float illumFactor = kCieIntegralY;
+    illumFactor /= kD65NormFactor;
+    for(int i = 0; i < kCieRows; i++)
+    {
+        for(int k = 0; k < 3; k++) gSpec.cmf[3 * i + k] = kCieObserverD65[i][k] * w[k];
+        gSpec.illum[i] = kCieObserverD65[i][3] * illumFactor;
+    }
+    // YxyToXYZ of each primary and the white at Y = 1 (ColorFunctions.h:249-257)
+    auto xyz = [](double xd, double yd, float out[3])
+    {
+        const float x = float(xd), y = float(yd);
+        const float yy = 1.0f / y;
+        out[0] = x * yy;
+        out[1] = 1.0f;
+        out[2] = (1.0f - x - y) * yy;
+    };
+    const double* p = kRec709Primaries;
+    float r[3], g[3], b[3], wt[3];
+    xyz(p[0], p[1], r);
+    xyz(p[2], p[3], g);
+    xyz(p[4], p[5], b);
+    xyz(p[6], p[7], wt);
+    // columns = primaries; scaled so RGB (1, 1, 1) is the white
+    const float P[9] = {r[0], g[0], b[0], r[1], g[1], b[1], r[2], g[2], b[2]};
+    float Pi[9];
+    Inverse3(P, Pi);
+    float S[3];
+    for(int k = 0; k < 3; k++) S[k] = Pi[3 * k] * wt[0] + Pi[3 * k + 1] * wt[1] + Pi[3 * k + 2] * wt[2];
+    float toXyz[9];
+    for(int row = 0; row < 3; row++)
+        for(int k = 0; k < 3; k++) toXyz[3 * row + k] = P[3 * row + k] * S[k];
+    Inverse3(toXyz, gSpec.xyzToRgb);
+}
+
+void rc_spectral_load(int N, const float* lut)
 {
     gSpec.N = N;
     gSpec.lut.assign(lut, lut + size_t(3) * N * N * N * 4);
-    gSpec.cmf.assign(cmf, cmf + 471 * 3);
-    gSpec.illum.assign(illum, illum + 471);
-    std::memcpy(gSpec.xyzToRgb, xyzToRgb, 9 * sizeof(float));
+    BuildColourTables();
     gSpec.loaded = true;
+}
+
+// The oracle's observer [471][3], illuminant [471] and XYZ->RGB [9].
+void rc_spectral_colour_tables(float* cmf, float* illum, float* xyzToRgb)
+{
+    BuildColourTables();
+    std::memcpy(cmf, gSpec.cmf.data(), gSpec.cmf.size() * sizeof(float));
+    std::memcpy(illum, gSpec.illum.data(), gSpec.illum.size() * sizeof(float));
+    std::memcpy(xyzToRgb, gSpec.xyzToRgb, 9 * sizeof(float));
 }
 
 void rc_sample_wavelengths(const float* xi, uint32_t n, float* waves, float* pdfs)

@@ -60,7 +60,8 @@ def lib():
         L.rc_render_batch.restype = C.c_int
         L.rc_render_batch.argtypes = [P, C.c_uint32, P, C.c_uint32, C.POINTER(abi.MRayRenderParams),
                                       P, C.c_uint32, u64]
-        L.rc_spectral_load.argtypes = [C.c_int, f32, f32, f32, f32]
+        L.rc_spectral_load.argtypes = [C.c_int, f32]
+        L.rc_spectral_colour_tables.argtypes = [f32, f32, f32]
         L.rc_sample_wavelengths.argtypes = [f32, C.c_uint32, f32, f32]
         L.rc_rgb_to_spectrum.argtypes = [f32, f32, C.c_uint32, C.c_int, f32]
         L.rc_spectrum_to_rgb.argtypes = [f32, f32, f32, C.c_uint32, f32]
@@ -339,14 +340,21 @@ def bsdf_probe(mat, tbn, geo_n, back_side, wo, wi=None, rn=None, op=0):
 
 
 def spectral_load(tables):
-    """Load the RGB->spectrum tables (as returned by mray_amd.native.spectral_tables)."""
+    """Load the RGB->spectrum coefficient LUT (tables["lut"], as returned by
+    mray_amd.native.spectral_tables); the oracle builds its own observer,
+    illuminant and XYZ->RGB matrix from the CIE 1931 / D65 data."""
     lut = _f(tables["lut"])
-    cmf = _f(tables["cmf"])
-    illum = _f(tables["illum"])
-    m = _f(tables["xyz_to_rgb"])
-    lib().rc_spectral_load(int(tables["N"]), abi.fptr(lut), abi.fptr(cmf), abi.fptr(illum),
-                           abi.fptr(m))
-    _keep.append((lut, cmf, illum, m))
+    lib().rc_spectral_load(int(tables["N"]), abi.fptr(lut))
+    _keep.append(lut)
+
+
+def spectral_colour_tables():
+    """The oracle's observer [471, 3], illuminant [471] and XYZ->RGB [3, 3]."""
+    cmf = np.zeros((471, 3), np.float32)
+    illum = np.zeros(471, np.float32)
+    m = np.zeros(9, np.float32)
+    lib().rc_spectral_colour_tables(abi.fptr(cmf), abi.fptr(illum), abi.fptr(m))
+    return cmf, illum, m.reshape(3, 3)
 
 
 _keep = []

@@ -6,10 +6,25 @@ Pins:
   the derivative of the inverse-CDF sampler (checked numerically) and
   integrates to 1 over the sampled range; the four wavelengths are the
   stratified offsets -1/2, -1/4, 0, 1/4 of one random number, wrapped;
-* the tables: white light maps to RGB (1, 1, 1); the round trip of
+* the colour data (public standard data, the values the reference renders
+  with, Core/ColorFunctions.cpp): the CIE 1931 2-degree observer at 5-nm
+  knots of the standard table, ybar = 1 at 555 nm, the trapezoid integrals the
+  reference normalises by (Core/ColorFunctions.h:70-72); the D65 SPD within
+  0.5 % of the CIE table at its knots (the reference's D65 is the CIE
+  daylight model at 6504 K), its chromaticity within 2e-4 of D65's
+  (0.3127, 0.3290) and its integral against ybar within 2e-5 of the
+  normalisation factor 10566.864 (Core/ColorFunctions.h:85);
+* the tables the library uploads equal the oracle's own restatement of the
+  reference's load-time normalisation (SpectrumContext.cu:401-426) bit for
+  bit, and its XYZ->RGB matrix equals the oracle's float GenRGBToXYZ and the
+  published sRGB matrix (IEC 61966-2-1) to the white-point difference;
+* white light maps to RGB (1, 1, 1); the round trip of
   Tests/Tracer/T_Spectrum.cu:54-236 (its 7 fixed colours + 5 random ones,
   1024 stratified samples, tolerance 0.1) for reflectance under the
-  illuminant and for emission -- tightened here to 0.03.
+  illuminant and for emission -- tightened here to 0.015, except a
+  reflectance of exactly 1: there the fit runs into the generator's clamp of
+  the largest coefficient at 200 (SpectraLUTGen/main.cpp:248-252), which
+  leaves a dip of a few percent mid-spectrum, within the reference's 0.1.
 The coefficient values themselves are parity unpinned: the reference generates
 its LUT at build time and ships none.
 """
@@ -47,12 +62,70 @@ def test_wavelength_sampling_closed_form():
     assert np.trapezoid(pdf, lam) == pytest.approx(1.0, abs=2e-3)
 
 
+# CIE 1931 2-degree observer, knots of the standard's 5-nm table
+CIE_KNOTS = {400: (0.014310, 0.000396, 0.067850), 450: (0.33620, 0.03800, 1.77211),
+             500: (0.00490, 0.32300, 0.27200), 550: (0.433450, 0.994950, 0.008750),
+             555: (0.5120501, 1.0, 0.00574999), 600: (1.06220, 0.63100, 0.00080),
+             650: (0.28350, 0.10700, 0.0), 700: (0.011359, 0.004102, 0.0)}
+# CIE D65 relative SPD (CIE S 014-2), 100 at 560 nm
+D65_KNOTS = {360: 46.6383, 380: 49.9755, 400: 82.7549, 460: 117.812, 500: 109.354,
+             560: 100.0, 600: 90.0062, 700: 71.6091, 780: 63.5927, 830: 60.3125}
+
+
+def _raw_colour_data(tables):
+    """Observer and D65 back from the library's normalised tables."""
+    cmf = tables["cmf"].astype(np.float64) * np.array([106.86540391770, 106.85691492064,
+                                                     106.89194822846])
+    d65 = tables["illum"].astype(np.float64) * 10566.864 / 106.85691492064
+    return cmf, d65
+
+
+def test_colour_data_is_the_standard(tables):
+    cmf, d65 = _raw_colour_data(tables)
+    for nm, v in CIE_KNOTS.items():
+        assert np.allclose(cmf[nm - 360], v, rtol=3e-5, atol=1e-8), (nm, cmf[nm - 360], v)
+    assert int(np.argmax(cmf[:, 1])) == 555 - 360
+    # the integrals the reference normalises by are the trapezoid integrals
+    for k, ref in enumerate((106.86540391770, 106.85691492064, 106.89194822846)):
+        assert np.float32(np.trapezoid(cmf[:, k])) == pytest.approx(np.float32(ref), rel=3e-6)
+    for nm, v in D65_KNOTS.items():
+        assert d65[nm - 360] == pytest.approx(v, rel=5e-3), nm
+    X, Y, Z = (cmf * d65[:, None]).sum(0)
+    assert X / (X + Y + Z) == pytest.approx(0.3127, abs=2e-4)
+    assert Y / (X + Y + Z) == pytest.approx(0.3290, abs=2e-4)
+    assert np.trapezoid(cmf[:, 1] * d65) == pytest.approx(10566.864, rel=2e-5)
+
+
+def test_planted_wrong_illuminant_fails_the_pins(tables):
+    """A plausible mistake -- an equal-energy or a D50-like illuminant, or the
+    observer shifted by one row -- fails the checks above."""
+    cmf, d65 = _raw_colour_data(tables)
+    flat = np.full_like(d65, 100.0)
+    X, Y, Z = (cmf * flat[:, None]).sum(0)
+    assert abs(X / (X + Y + Z) - 0.3127) > 2e-4
+    shifted = np.roll(cmf, 1, axis=0)
+    assert not np.allclose(shifted[555 - 360], CIE_KNOTS[555], rtol=1e-5)
+
+
+def test_library_tables_equal_oracle_restatement(tables):
+    cmf, illum, m = refcpu.spectral_colour_tables()
+    assert np.array_equal(tables["cmf"], cmf)
+    assert np.array_equal(tables["illum"], illum)
+    lib_m = tables["xyz_to_rgb"].reshape(3, 3)
+    assert np.allclose(lib_m, m, rtol=2e-6, atol=2e-7)
+    srgb = np.array([[3.2406, -1.5372, -0.4986], [-0.9689, 1.8758, 0.0415],
+                     [0.0557, -0.2040, 1.0570]])
+    assert np.allclose(lib_m, srgb, atol=6e-4)
+
+
 def test_white_point(tables):
     cmf, il = tables["cmf"].astype(np.float64), tables["illum"].astype(np.float64)
     M = tables["xyz_to_rgb"].reshape(3, 3).astype(np.float64)
     white = (cmf * il[:, None]).sum(0)
-    assert white[1] == pytest.approx(1.0, rel=1e-5)
-    assert np.allclose(M @ white, 1.0, atol=1e-5)
+    # Y of the normalised illuminant: integral(ybar D65) over 10566.864
+    assert white[1] == pytest.approx(1.0, rel=2e-5)
+    # the illuminant's chromaticity is within 1e-4 of the matrix white point
+    assert np.allclose(M @ white, 1.0, atol=1e-3)
     assert np.allclose(cmf.sum(0), 1.0, atol=1e-5)
 
 
@@ -80,4 +153,6 @@ def test_round_trip(tables, radiance):
         if not radiance:
             spec = spec * illum_w            # reflectance under the illuminant
         out = refcpu.spectrum_to_rgb(spec, waves, pdfs).mean(axis=0)
-        assert np.allclose(out, c, atol=0.03), (c, out)
+        assert np.allclose(out, c, atol=0.1), (c, out)
+        if radiance or c.min() < 1.0:
+            assert np.allclose(out, c, atol=0.015), (c, out)
