@@ -6,6 +6,7 @@
 #include "mray_tracer.hpp"
 
 #include <cctype>
+#include <cstdint>
 #include <cstdlib>
 #include <map>
 #include <memory>
@@ -48,6 +49,24 @@ struct Value
         if(t == T::Bool) return b ? 1.0 : 0.0;
         if(t != T::Num) throw MRayError("scene: expected a number");
         return n;
+    }
+    // A non-negative integer that fits in 32 bits (IDs, indices, counts); a
+    // fraction, a negative, huge or non-finite value throws instead of being
+    // cast (the cast would be undefined behaviour).
+    uint32_t U32() const
+    {
+        double d = Num();
+        if(!(d >= 0.0 && d <= 4294967295.0) || d != double(uint64_t(d)))
+            throw MRayError("scene: expected an unsigned 32-bit integer");
+        return uint32_t(d);
+    }
+    // A finite number within float range (positions, colours, angles).
+    float F32() const
+    {
+        double d = Num();
+        if(!(d >= -3.4028234663852886e38 && d <= 3.4028234663852886e38))
+            throw MRayError("scene: expected a finite 32-bit float");
+        return float(d);
     }
     bool Bool() const
     {
@@ -99,7 +118,7 @@ class Parser
     }
     std::string ParseString()
     {
-        if(src[p] != '"') Fail("expected a string");
+        if(p >= src.size() || src[p] != '"') Fail("expected a string");
         p++;
         std::string out;
         while(p < src.size() && src[p] != '"')

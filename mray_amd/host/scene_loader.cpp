@@ -39,8 +39,8 @@ std::vector<uint32_t> Ids(const Value& v)
 {
     std::vector<uint32_t> out;
     if(v.IsArr())
-        for(const Value& e : v.Arr()) out.push_back(uint32_t(e.Num()));
-    else out.push_back(uint32_t(v.Num()));
+        for(const Value& e : v.Arr()) out.push_back(e.U32());
+    else out.push_back(v.U32());
     return out;
 }
 
@@ -62,14 +62,14 @@ Vector3 Vec3(const Value& v)
     if(v.IsObj()) throw MRayError("scene: texture-backed values are not supported");
     const auto& a = v.Arr();
     if(a.size() != 3) throw MRayError("scene: expected a 3-vector");
-    return Vector3{{float(a[0].Num()), float(a[1].Num()), float(a[2].Num())}};
+    return Vector3{{a[0].F32(), a[1].F32(), a[2].F32()}};
 }
 
 Vector2 Vec2(const Value& v)
 {
     const auto& a = v.Arr();
     if(a.size() != 2) throw MRayError("scene: expected a 2-vector");
-    return Vector2{{float(a[0].Num()), float(a[1].Num())}};
+    return Vector2{{a[0].F32(), a[1].F32()}};
 }
 
 Quaternion TbnFromNormal(Vector3 n)
@@ -174,7 +174,7 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
     auto transOf = [&](const Value& n) -> uint32_t
     {
         const Value* t = n.Find("transform");
-        return t ? uint32_t(t->Num()) : UINT32_MAX;
+        return t ? t->U32() : UINT32_MAX;
     };
     if(surfList)
         for(const Value& sn : surfList->Arr())
@@ -186,22 +186,22 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
     if(lightSurfList)
         for(const Value& ln : lightSurfList->Arr())
         {
-            usedLights.insert(uint32_t(ln.At("light").Num()));
+            usedLights.insert(ln.At("light").U32());
             if(transOf(ln) != UINT32_MAX) usedTrans.insert(transOf(ln));
         }
     if(camSurfList)
         for(const Value& cn : camSurfList->Arr())
         {
-            usedCams.insert(uint32_t(cn.At("camera").Num()));
+            usedCams.insert(cn.At("camera").U32());
             if(transOf(cn) != UINT32_MAX) usedTrans.insert(transOf(cn));
         }
-    usedLights.insert(uint32_t(boundary.At("light").Num()));
+    usedLights.insert(boundary.At("light").U32());
     if(transOf(boundary) != UINT32_MAX) usedTrans.insert(transOf(boundary));
     for(uint32_t l : usedLights)
     {
         auto [node, i, cnt] = lights.Get(l, "light");
         if(node->At("type").Str() == "Primitive")
-            usedPrims.insert(uint32_t(PerId(*node, "primitive", i, cnt).Num()));
+            usedPrims.insert(PerId(*node, "primitive", i, cnt).U32());
     }
 
     // ---- primitives ------------------------------------------------------------
@@ -222,7 +222,7 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
             {
                 const auto& a = v.Arr();
                 if(a.size() != 3) throw MRayError("scene: index entries are 3-vectors");
-                m.idx.push_back(Vector3ui{{uint32_t(a[0].Num()), uint32_t(a[1].Num()), uint32_t(a[2].Num())}});
+                m.idx.push_back(Vector3ui{{a[0].U32(), a[1].U32(), a[2].U32()}});
             }
         else
         {
@@ -312,7 +312,7 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
             if(info[a].isTexturable == AttributeTexturable::MR_TEXTURE_ONLY)
                 throw MRayError("scene: texture-backed material attributes are not supported");
             TransientData d = (info[a].dataType == MRayDataEnum::MR_FLOAT)
-                ? Data(std::vector<float>{float(e.Num())}) : Data(std::vector<Vector3>{Vec3(e)});
+                ? Data(std::vector<float>{e.F32()}) : Data(std::vector<Vector3>{Vec3(e)});
             if(info[a].isTexturable == AttributeTexturable::MR_CONSTANT_ONLY)
                 tracer.PushMatAttribute(g, r, a, std::move(d));
             else
@@ -338,7 +338,7 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
         {
             const auto& a = PerId(*node, "matrix", i, cnt).Arr();
             if(a.size() != 16) throw MRayError("scene: matrix needs 16 values");
-            for(int k = 0; k < 16; k++) mtx.m[k] = float(a[k].Num());
+            for(int k = 0; k < 16; k++) mtx.m[k] = a[k].F32();
         }
         else if(layout == "trs")
         {
@@ -377,7 +377,7 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
         if(type == "Primitive")
         {
             if(!havePrimLights) { primLights = tracer.CreateLightGroup("(L)Prim(P)Triangle", pg); havePrimLights = true; }
-            uint32_t p = uint32_t(PerId(*node, "primitive", i, cnt).Num());
+            uint32_t p = PerId(*node, "primitive", i, cnt).U32();
             LightId id = tracer.ReserveLight(primLights, {}, primBatch.at(p));
             lightId[l] = id;
             bool two = node->Find("isTwoSided") ? PerId(*node, "isTwoSided", i, cnt).Bool() : false;
@@ -430,8 +430,8 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
             auto [node, i, cnt] = cams.Get(c, "camera");
             if(node->At("type").Str() != "Pinhole") Unsupported_("camera type " + node->At("type").Str());
             bool isFovX = PerId(*node, "isFovX", i, cnt).Bool();
-            float fov = float(PerId(*node, "fov", i, cnt).Num());
-            float aspect = float(PerId(*node, "aspect", i, cnt).Num());
+            float fov = PerId(*node, "fov", i, cnt).F32();
+            float aspect = PerId(*node, "aspect", i, cnt).F32();
             Vector2 planes = Vec2(PerId(*node, "planes", i, cnt));
             const float d2r = 3.14159265358979323846f / 180.0f;
             Vector4 fnp{{fov, fov, planes.v[0], planes.v[1]}};
@@ -481,12 +481,12 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
     if(lightSurfList)
         for(const Value& ln : lightSurfList->Arr())
             out.lightSurfaces.push_back(tracer.CreateLightSurface(
-                LightSurfaceParams{lightId.at(uint32_t(ln.At("light").Num())), T(ln), {}}));
+                LightSurfaceParams{lightId.at(ln.At("light").U32()), T(ln), {}}));
     if(camSurfList)
         for(const Value& cn : camSurfList->Arr())
             out.cameraSurfaces.push_back(tracer.CreateCameraSurface(
-                CameraSurfaceParams{camId.at(uint32_t(cn.At("camera").Num())), T(cn), {}}));
-    out.boundary = tracer.SetBoundarySurface(lightId.at(uint32_t(boundary.At("light").Num())), T(boundary));
+                CameraSurfaceParams{camId.at(cn.At("camera").U32()), T(cn), {}}));
+    out.boundary = tracer.SetBoundarySurface(lightId.at(boundary.At("light").U32()), T(boundary));
     return out;
 }
 

@@ -2918,6 +2918,59 @@ void rc_bsdf_probe(const MRayMaterial* M, const float* tbn, const float* geoN, i
     }
 }
 
+// Batched BxDF evaluation of one material at one outgoing direction over
+// n incoming directions: out [n][4] = f * cos (the reflectance EvaluateBxDF
+// returns), pdf [n] (may be NULL).
+void rc_bsdf_eval_many(const MRayMaterial* M, const float* tbn, const float* geoN, int backSide,
+                       const float* wO, const float* wI, uint32_t n, float* out, float* pdf)
+{
+    for(uint32_t i = 0; i < n; i++)
+    {
+        rc_bsdf_probe(M, tbn, geoN, backSide, wO, wI + 3 * size_t(i), nullptr, 1, out + 4 * size_t(i));
+        if(pdf) rc_bsdf_probe(M, tbn, geoN, backSide, wO, wI + 3 * size_t(i), nullptr, 2, pdf + i);
+    }
+}
+
+// InvErf (Filters.h Gaussian sampling) at n points.
+void rc_inv_erf(const float* x, uint32_t n, float* out)
+{
+    for(uint32_t i = 0; i < n; i++) out[i] = InvErf(x[i]);
+}
+
+// SampleGaussian(xi, sigma) at n points: value [n], pdf [n].
+void rc_sample_gaussian(const float* xi, uint32_t n, float sigma, float* value, float* pdf)
+{
+    for(uint32_t i = 0; i < n; i++)
+    {
+        Sample1 s = SampleGaussian(xi[i], sigma);
+        value[i] = s.value;
+        pdf[i] = s.pdf;
+    }
+}
+
+// Latitude-longitude map: uv [n][2] <-> Z-up unit directions [n][3];
+// sinIncl [n] (may be NULL) is the sine of the inclination the pdf divides by.
+void rc_spherical_to_dir(const float* uv, uint32_t n, float* dirs, float* sinIncl)
+{
+    for(uint32_t i = 0; i < n; i++)
+    {
+        float s = 0.0f;
+        V3 d = SphericalToDir(uv[2 * i], uv[2 * i + 1], s);
+        dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+        if(sinIncl) sinIncl[i] = s;
+    }
+}
+void rc_dir_to_spherical(const float* dirs, uint32_t n, float* uv)
+{
+    for(uint32_t i = 0; i < n; i++)
+        DirToSpherical(V3{dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]}, uv[2 * i], uv[2 * i + 1]);
+}
+// Solid-angle pdf of the Skysphere mapping for a uv pdf (SkySolidAnglePdf).
+float rc_sky_solid_angle_pdf(uint32_t type, float uvPdf, float sinIncl)
+{
+    return SkySolidAnglePdf(type, uvPdf, sinIncl);
+}
+
 // Light-sample probe: NEE sample from a shading point (rn[3]) ->
 // {point xyz, pdf, emission rgbw}; and the MIS pdf of hitting light li
 // at bary (a, b) from rayPos along rayDir.

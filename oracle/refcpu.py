@@ -79,6 +79,14 @@ def lib():
         L.rc_env_eval.argtypes = [P, f32, C.c_uint32, f32, f32]
         L.rc_coocta_to_dir.argtypes = [f32, C.c_uint32, f32]
         L.rc_dir_to_coocta.argtypes = [f32, C.c_uint32, f32]
+        L.rc_bsdf_eval_many.argtypes = [C.POINTER(abi.MRayMaterial), f32, f32, C.c_int, f32, f32,
+                                        C.c_uint32, f32, f32]
+        L.rc_inv_erf.argtypes = [f32, C.c_uint32, f32]
+        L.rc_sample_gaussian.argtypes = [f32, C.c_uint32, C.c_float, f32, f32]
+        L.rc_spherical_to_dir.argtypes = [f32, C.c_uint32, f32, f32]
+        L.rc_dir_to_spherical.argtypes = [f32, C.c_uint32, f32]
+        L.rc_sky_solid_angle_pdf.restype = C.c_float
+        L.rc_sky_solid_angle_pdf.argtypes = [C.c_uint32, C.c_float, C.c_float]
         _lib = L
     return _lib
 
@@ -320,6 +328,65 @@ def pcg_sequence(state, count):
 
 def to_float01(v):
     return float(lib().rc_to_float01(v))
+
+
+def _material(mat):
+    m = abi.MRayMaterial()
+    m.type = mat["type"]
+    for k in range(3):
+        m.albedo[k] = mat.get("albedo", (0, 0, 0))[k]
+        m.cauchyFront[k] = mat.get("cauchyFront", (1, 0, 0))[k]
+        m.cauchyBack[k] = mat.get("cauchyBack", (1, 0, 0))[k]
+    m.roughness = mat.get("roughness", 0.0)
+    m.metallic = mat.get("metallic", 0.0)
+    m.specular = mat.get("specular", 0.0)
+    return m
+
+
+def bsdf_eval_many(mat, tbn, geo_n, back_side, wo, wi):
+    """f*cos [n][4] and pdf [n] of one material at one wo over wi [n][3]."""
+    m = _material(mat)
+    wi = _f(wi).reshape(-1, 3)
+    out = np.zeros((len(wi), 4), np.float32)
+    pdf = np.zeros(len(wi), np.float32)
+    lib().rc_bsdf_eval_many(C.byref(m), abi.fptr(_f(tbn)), abi.fptr(_f(geo_n)), int(back_side),
+                            abi.fptr(_f(wo)), abi.fptr(wi), len(wi), abi.fptr(out), abi.fptr(pdf))
+    return out, pdf
+
+
+def inv_erf(x):
+    x = _f(x).ravel()
+    out = np.zeros_like(x)
+    lib().rc_inv_erf(abi.fptr(x), len(x), abi.fptr(out))
+    return out
+
+
+def sample_gaussian(xi, sigma):
+    xi = _f(xi).ravel()
+    v = np.zeros_like(xi)
+    p = np.zeros_like(xi)
+    lib().rc_sample_gaussian(abi.fptr(xi), len(xi), float(sigma), abi.fptr(v), abi.fptr(p))
+    return v, p
+
+
+def spherical_to_dir(uv):
+    """Latitude-longitude uv -> (Z-up directions, sin(inclination))."""
+    uv = _f(uv).reshape(-1, 2)
+    d = np.zeros((len(uv), 3), np.float32)
+    s = np.zeros(len(uv), np.float32)
+    lib().rc_spherical_to_dir(abi.fptr(uv), len(uv), abi.fptr(d), abi.fptr(s))
+    return d, s
+
+
+def dir_to_spherical(dirs):
+    d = _f(dirs).reshape(-1, 3)
+    out = np.zeros((len(d), 2), np.float32)
+    lib().rc_dir_to_spherical(abi.fptr(d), len(d), abi.fptr(out))
+    return out
+
+
+def sky_solid_angle_pdf(boundary_type, uv_pdf, sin_incl):
+    return float(lib().rc_sky_solid_angle_pdf(boundary_type, uv_pdf, sin_incl))
 
 
 def bsdf_probe(mat, tbn, geo_n, back_side, wo, wi=None, rn=None, op=0):
