@@ -13,7 +13,6 @@ synchronisation inside the frame.  Output: one JSON line (see --help).
 import argparse
 import json
 import os
-import platform
 import sys
 import time
 
@@ -60,16 +59,21 @@ def parse():
                     help="run only the radiance-cache equal-time leg (its own JSON line)")
     ap.add_argument("--nrc-frames", type=int, default=256)
     ap.add_argument("--nrc-ref-spp", type=int, default=16384)
-    ap.add_argument("--nrc-scene", choices=("cornell", "terrain"), default="terrain")
+    ap.add_argument("--nrc-scene", choices=("cornell", "terrain"), default="cornell")
     ap.add_argument("--nrc-variants", default="0.01:0:1:3:0;1.0:1:2:4:1",
                     help="cache variants 'spread:neeAtEnd:minDepth:trainShift[:lazyAdam];...'")
     ap.add_argument("--workload", choices=("frame", "batch", "instanced"), default="frame",
                     help="frame: the 1080p headline frame; batch: many scenes in one renderer")
+    ap.add_argument("--legs", default="batch,instanced",
+                    help="frame workload: extra legs reported in the same JSON line "
+                         "(comma list of batch, instanced; empty for none)")
     ap.add_argument("--inst-spp", type=int, default=16,
                     help="samples per pixel of the instanced workload")
     ap.add_argument("--inst-views", type=int, default=8,
                     help="viewports of the instanced workload")
     ap.add_argument("--batch-scenes", type=int, default=32)
+    ap.add_argument("--batch-generator", choices=("cornell", "terrain"), default="cornell",
+                    help="batch: tessellated Cornell scenes (the batched workload) or terrains")
     ap.add_argument("--batch-spp", type=int, default=16)
     ap.add_argument("--no-autotune", action="store_true",
                     help="batch: skip the autotuner and use --batch-slots / --batch-tpb, "
@@ -81,6 +85,49 @@ def parse():
     ap.add_argument("--probe-copy-mib", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     return ap.parse_args()
+
+
+# Algorithmic bytes of one shaded path (KShade, surface hit, NEE+MIS): path
+# state read -- active index 4, meta 4, hit 16, ray origin 16, direction 16,
+# throughput 16, radiance 16, previous pdf 4, PCG state 4 (+ wavelengths 16
+# spectral); scene gathers -- surface key 4, material 52, indices 12, three
+# positions 36, three TBN quaternions 48, light triangle 4 + positions 36 +
+# radiance 16 (+ colour coefficients 16 spectral); written -- throughput 16,
+# previous pdf 4, ray 32, PCG state 4, meta 4, shadow ray 32 + radiance 16 +
+# queue index 4.
+SHADE_BYTES = {False: 96 + 208 + 112, True: 112 + 224 + 112}
+# any-hit ray outside the tree walk: queue index 4, shadow ray 32, its
+# radiance 16, path radiance read and written 32
+SHADOW_RAY_BYTES = 4 + 32 + 16 + 32
+
+
+def kernel_rooflines(cst, ptotals, phases, width, spectral, queues):
+    """Launch-averaged algorithmic GB/s of the shading and any-hit kernels
+    (HIP events around each phase on the render stream)."""
+    out = []
+    shade_ms, shade_n = phases["shade"]
+    if shade_n and not queues:
+        per = shade_ms / shade_n
+        paths = ptotals["ext_rays"] / shade_n
+        gbs = paths * SHADE_BYTES[spectral] / (per * 1e-3) / 1e9
+        out.append({"kernel": "KShade", "bound": "hbm", "achieved": round(gbs, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "bytes_per_path": SHADE_BYTES[spectral], "avg_launch_ms": round(per, 4),
+                    "paths_per_launch": round(paths, 1)})
+    sh_ms, sh_n = phases["shadow"]
+    if sh_n and cst.get("shadow_rays"):
+        per = sh_ms / sh_n
+        rays = ptotals["shadow_rays"] / sh_n
+        nodes = cst["shadow_node_visits"] / cst["shadow_rays"]
+        tris = cst["shadow_tri_tests"] / cst["shadow_rays"]
+        b = SHADOW_RAY_BYTES + nodes * NODE_BYTES[width] + tris * TRI_BYTES
+        gbs = rays * b / (per * 1e-3) / 1e9
+        out.append({"kernel": "KShadow", "bound": "hbm", "achieved": round(gbs, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+                    "bytes_per_ray": round(b, 1), "nodes_per_ray": round(nodes, 2),
+                    "tris_per_ray": round(tris, 2), "avg_launch_ms": round(per, 4),
+                    "rays_per_launch": round(rays, 1)})
+    return out
 
 
 def probe_copy(torch, mib):
@@ -115,6 +162,8 @@ def cpu_baseline(scene, params, budget_s):
     box's host cores, over a bounded number of paths of the same workload."""
     import numpy as np  # noqa: F401
     from oracle import refcpu
+    isa = refcpu.use_native_isa()
+    model, _ = refcpu.host_cpu()
     if params.spectral:
         from mray_amd import native
         refcpu.spectral_load(native.spectral_tables())
@@ -130,11 +179,10 @@ def cpu_baseline(scene, params, budget_s):
         if dt >= 0.5 * budget_s or sample >= total:
             break
         sample = int(min(total, max(2 * sample, sample * budget_s / max(dt, 1e-3))))
-    cpu = platform.processor() or platform.machine()
     return {"value": st["dead_paths"] / dt / 1e6, "unit": "Mpaths/s", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu": model, "build": f"g++ -O3 -march={isa}",
             "sample": f"first {st['dead_paths']} paths of the same frame (reference schedule), "
-                      f"{dt:.1f} s on {threads} threads ({cpu})"}
+                      f"{dt:.1f} s on {threads} threads of {model}, g++ -O3 -march={isa}"}
 
 
 def relmse(img, ref):
@@ -272,16 +320,20 @@ def nrc_equal_time(args):
 
 
 def bench_batch(args):
-    """Batched workload: independent terrain scenes (triangles log-uniform in
-    [10k, 2M], resolutions from 256^2 / 512^2 / 720p / 1080p, 16 spp, depth 8,
-    spectral) rendered by one batch renderer; settings from the autotuner."""
+    """Batched workload: 32 independent Cornell scenes tessellated to
+    triangle counts log-uniform in [10k, 2M] (or terrain scenes), resolutions
+    from 256^2 / 512^2 / 720p / 1080p, 16 spp, depth 8, spectral, rendered by
+    one batch renderer; settings from the autotuner."""
     import numpy as np
     from mray_amd import abi, native, scenes
     torch = native.require_gpu()
     spectral = not args.rgb
     log(f"generating {args.batch_scenes} scenes")
     t0 = time.perf_counter()
-    scs, views = scenes.batch_workload(count=args.batch_scenes, spp=args.batch_spp)
+    if args.batch_generator == "cornell":
+        scs, views = scenes.cornell_batch_workload(count=args.batch_scenes, spp=args.batch_spp)
+    else:
+        scs, views = scenes.batch_workload(count=args.batch_scenes, spp=args.batch_spp)
     tris = sum(s.triangle_count for s in scs)
     log(f"scenes: {tris} triangles in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
@@ -316,41 +368,14 @@ def bench_batch(args):
     bs = merged[best["width"]]
     r = native.BatchRenderer(bs, views, params)
     r.set_trace_tpb(best["trace_tpb"])
-    r.set_measurement(profiling=False, counting=True)
-    cst = r.render_all()
-    nodes_per_ray = cst["node_visits"] / max(1, cst["ext_rays"])
-    tris_per_ray = cst["tri_tests"] / max(1, cst["ext_rays"])
-    log(f"counting frame: {cst}")
-    r.set_measurement(profiling=False, counting=False)
-    for w in range(args.warmup):
-        r.reset()
-        r.render_all()
-        log(f"warmup {w + 1}/{args.warmup} done")
-    r.set_measurement(profiling=True, counting=False)
     paths = sum(v["width"] * v["height"] * v["spp"] for v in views)
-    totals = {"dead_paths": 0, "ext_rays": 0, "shadow_rays": 0, "iterations": 0}
-    torch.cuda.synchronize()
-    t_all = time.perf_counter()
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        r.reset()
-        st = r.render_all()
-        for key in totals:
-            totals[key] += st[key]
-        log(f"step {k + 1}/{args.steps}: {(time.perf_counter() - t0) * 1e3:.1f} ms, "
-            f"{st['dead_paths']} paths, {st['iterations']} bounces")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_all
+    m = _timed_render(r, args, torch, "batch")
+    totals, elapsed, phases = m["totals"], m["elapsed"], m["phases"]
     lat = r.view_times()
-    phases = r.phase_times()
     films = r.film()
     assert totals["dead_paths"] == paths * args.steps, totals
+    assert m["ptotals"]["dead_paths"] == paths * args.steps, m["ptotals"]
     assert all(np.isfinite(f).all() for f in films)
-    trace_ms, trace_launches = phases["trace"]
-    trace_avg_ms = trace_ms / max(1, trace_launches)
-    rays_per_launch = totals["ext_rays"] / max(1, trace_launches)
-    bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES[best["width"]] + tris_per_ray * TRI_BYTES
-    achieved = rays_per_launch * bytes_per_ray / (trace_avg_ms * 1e-3) / 1e9
     result = {
         "metric": f"Mpaths/s, {len(views)} scenes batched in one renderer, 1 MI355X",
         "value": round(paths * args.steps / elapsed / 1e6, 3),
@@ -365,7 +390,7 @@ def bench_batch(args):
         "dtype": "f32",
         "data": "synthetic (seeded procedural scenes)",
         "config": {
-            "workload": f"{len(views)} terrain scenes ({tris} tris, 10k-2M each), "
+            "workload": f"{len(views)} {args.batch_generator} scenes ({tris} tris, 10k-2M each), "
                         f"256^2/512^2/720p/1080p views, {args.batch_spp} spp, depth 8, "
                         f"{'spectral 4 lambda' if spectral else 'RGB'}, NEE+MIS",
             "triangles": tris,
@@ -380,20 +405,12 @@ def bench_batch(args):
         "accel_build_s": round(build_s, 3),
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in phases.items()},
         "view_latency_ms": [round(float(x), 3) for x in lat],
-        "roofline": {
-            "bound": "hbm",
-            "kernel": f"KTrace (closest hit, {best['width']}-wide BVH, batch)",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "bytes_per_ray": round(bytes_per_ray, 1),
-            "nodes_per_ray": round(nodes_per_ray, 2),
-            "tris_per_ray": round(tris_per_ray, 2),
-            "avg_launch_ms": round(trace_avg_ms, 4),
-            "rays_per_launch": round(rays_per_launch, 1),
-        },
+        "host_waits_per_step": totals["host_waits"] / args.steps,
+        "device_drains_per_step": totals["device_drains"] / args.steps,
+        "profiled_ms_per_step": round(m["prof_elapsed"] * 1e3 / args.steps, 2),
+        "roofline": _trace_roofline(m, best["width"], f"KTrace (closest hit, {best['width']}-wide BVH, batch)"),
+        "kernel_rooflines": kernel_rooflines(m["cst"], m["ptotals"], phases, best["width"],
+                                             spectral, bool(best["queues"])),
     }
     r.close()
     bs.close()
@@ -417,11 +434,13 @@ def bench_batch(args):
         result["sequential"] = {"ms_per_step": round(seq_s * 1e3, 2),
                                 "mpaths_per_s": round(paths / seq_s / 1e6, 3)}
         log(f"one view at a time: {result['sequential']}")
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def _timed_render(r, args, torch, label):
-    """Counting frame, warmup, then args.steps timed frames of renderer r."""
+    """Counting frame, warmup, args.steps timed frames of renderer r on the
+    production path (hipGraph bursts, no phase events), then the same number
+    of frames with HIP events around each phase for per-kernel times."""
     r.set_measurement(profiling=False, counting=True)
     cst = r.render_all()
     log(f"{label} counting frame: {cst}")
@@ -430,8 +449,8 @@ def _timed_render(r, args, torch, label):
         r.reset()
         r.render_all()
         log(f"{label} warmup {w + 1}/{args.warmup} done")
-    r.set_measurement(profiling=True, counting=False)
-    totals = {"dead_paths": 0, "ext_rays": 0, "shadow_rays": 0, "iterations": 0}
+    totals = {"dead_paths": 0, "ext_rays": 0, "shadow_rays": 0, "iterations": 0,
+              "host_waits": 0, "device_drains": 0}
     torch.cuda.synchronize()
     t_all = time.perf_counter()
     for k in range(args.steps):
@@ -441,18 +460,35 @@ def _timed_render(r, args, torch, label):
         for key in totals:
             totals[key] += st[key]
         log(f"{label} step {k + 1}/{args.steps}: {(time.perf_counter() - t0) * 1e3:.1f} ms, "
-            f"{st['dead_paths']} paths, {st['iterations']} bounces")
+            f"{st['dead_paths']} paths, {st['iterations']} bounces, "
+            f"{st['host_waits']} host waits, {st['device_drains']} drain")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_all
-    return cst, totals, elapsed, r.phase_times()
+    r.set_measurement(profiling=True, counting=False)
+    ptotals = {"dead_paths": 0, "ext_rays": 0, "shadow_rays": 0, "iterations": 0}
+    t_prof = time.perf_counter()
+    for k in range(args.steps):
+        r.reset()
+        st = r.render_all()
+        for key in ptotals:
+            ptotals[key] += st[key]
+    torch.cuda.synchronize()
+    prof_elapsed = time.perf_counter() - t_prof
+    phases = r.phase_times()
+    r.set_measurement(profiling=False, counting=False)
+    log(f"{label} profiled frames: {prof_elapsed * 1e3 / args.steps:.1f} ms per frame")
+    return {"cst": cst, "totals": totals, "elapsed": elapsed, "ptotals": ptotals,
+            "phases": phases, "prof_elapsed": prof_elapsed}
 
 
-def _trace_roofline(cst, totals, phases, width, kernel):
+def _trace_roofline(m, width, kernel):
+    """KTrace's algorithmic GB/s from a _timed_render measurement."""
+    cst, phases = m["cst"], m["phases"]
     nodes_per_ray = cst["node_visits"] / max(1, cst["ext_rays"])
     tris_per_ray = cst["tri_tests"] / max(1, cst["ext_rays"])
     trace_ms, trace_launches = phases["trace"]
     trace_avg_ms = trace_ms / max(1, trace_launches)
-    rays_per_launch = totals["ext_rays"] / max(1, trace_launches)
+    rays_per_launch = m["ptotals"]["ext_rays"] / max(1, trace_launches)
     bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES[width] + tris_per_ray * TRI_BYTES
     achieved = rays_per_launch * bytes_per_ray / (trace_avg_ms * 1e-3) / 1e9
     return {
@@ -501,7 +537,8 @@ def bench_instanced(args):
     params = abi.render_params(1, 1, 1, rr=(12, 12), mode=abi.SAMPLE_NEE_MIS, spectral=spectral,
                                seed=0, max_ray_count=args.batch_slots, flags=abi.RENDER_LIGHT_POWER)
     r = native.BatchRenderer(isc, views, params)
-    cst, totals, elapsed, phases = _timed_render(r, args, torch, "viewports")
+    m = _timed_render(r, args, torch, "viewports")
+    totals, elapsed, phases = m["totals"], m["elapsed"], m["phases"]
     lat = r.view_times()
     films = r.film()
     r.close()
@@ -511,7 +548,8 @@ def bench_instanced(args):
     p1 = abi.render_params(args.width, args.height, spp, rr=(12, 12), mode=abi.SAMPLE_NEE_MIS,
                            spectral=spectral, seed=0, flags=abi.RENDER_LIGHT_POWER)
     r1 = native.InstancedRenderer(isc, p1)
-    cst1, tot1, el1, ph1 = _timed_render(r1, args, torch, "single view")
+    m1 = _timed_render(r1, args, torch, "single view")
+    tot1, el1, ph1 = m1["totals"], m1["elapsed"], m1["phases"]
     r1.close()
     isc.close()
     single_paths = args.width * args.height * spp
@@ -549,17 +587,20 @@ def bench_instanced(args):
         "accel_build_s": round(build_s, 3),
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in phases.items()},
         "view_latency_ms": [round(float(x), 3) for x in lat],
-        "roofline": _trace_roofline(cst, totals, phases, 4,
-                                    "KTrace (closest hit, instance tree + 4-wide mesh trees)"),
+        "host_waits_per_step": totals["host_waits"] / args.steps,
+        "device_drains_per_step": totals["device_drains"] / args.steps,
+        "profiled_ms_per_step": round(m["prof_elapsed"] * 1e3 / args.steps, 2),
+        "roofline": _trace_roofline(m, 4, "KTrace (closest hit, instance tree + 4-wide mesh trees)"),
+        "kernel_rooflines": kernel_rooflines(m["cst"], m["ptotals"], phases, 4, spectral, False),
         "single_view": {
             "resolution": [args.width, args.height],
             "mpaths_per_s": round(single_paths * args.steps / el1 / 1e6, 3),
             "ms_per_step": round(el1 * 1e3 / args.steps, 2),
             "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in ph1.items()},
-            "roofline": _trace_roofline(cst1, tot1, ph1, 4, "KTrace (instanced)"),
+            "roofline": _trace_roofline(m1, 4, "KTrace (instanced)"),
         },
     }
-    print(json.dumps(result), flush=True)
+    return result
 
 
 def main():
@@ -569,10 +610,10 @@ def main():
                                    "designed (one film reduction per frame) but not run"}))
         sys.exit(2)
     if args.workload == "batch":
-        bench_batch(args)
+        print(json.dumps(bench_batch(args)), flush=True)
         return
     if args.workload == "instanced":
-        bench_instanced(args)
+        print(json.dumps(bench_instanced(args)), flush=True)
         return
     if args.nrc_only:
         print(json.dumps({"nrc": nrc_equal_time(args)}), flush=True)
@@ -601,48 +642,21 @@ def main():
     pixels = args.width * args.height
     paths_per_frame = pixels * args.spp
 
-    # counting frame (untimed): nodes visited / triangles tested per ray
-    r.set_measurement(profiling=False, counting=True)
-    cst = r.render_all()
-    nodes_per_ray = cst["node_visits"] / max(1, cst["ext_rays"])
-    tris_per_ray = cst["tri_tests"] / max(1, cst["ext_rays"])
-    log(f"counting frame: {cst}")
-    r.set_measurement(profiling=False, counting=False)
-
-    for w in range(args.warmup):
-        r.reset()
-        r.render_all()
-        log(f"warmup {w + 1}/{args.warmup} done")
-    torch.cuda.synchronize()
-
-    r.set_measurement(profiling=True, counting=False)
-    totals = {"dead_paths": 0, "ext_rays": 0, "shadow_rays": 0, "iterations": 0}
-    step_ms = []
-    torch.cuda.synchronize()
-    t_all = time.perf_counter()
-    for k in range(args.steps):
-        t0 = time.perf_counter()
-        r.reset()
-        st = r.render_all()
-        torch.cuda.synchronize()
-        step_ms.append((time.perf_counter() - t0) * 1e3)
-        for key in totals:
-            totals[key] += st[key]
-        log(f"step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms, {st['dead_paths']} paths, "
-            f"{st['iterations']} bounces, {st['ext_rays']} extension rays, "
-            f"{st['shadow_rays']} shadow rays")
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_all
-    phases = r.phase_times()
+    m = _timed_render(r, args, torch, "frame")
+    cst, totals, ptotals, phases = m["cst"], m["totals"], m["ptotals"], m["phases"]
+    elapsed, prof_elapsed = m["elapsed"], m["prof_elapsed"]
     film = r.film()
     assert totals["dead_paths"] == paths_per_frame * args.steps, totals
+    assert ptotals["dead_paths"] == paths_per_frame * args.steps, ptotals
     assert np.isfinite(film).all()
+    nodes_per_ray = cst["node_visits"] / max(1, cst["ext_rays"])
+    tris_per_ray = cst["tri_tests"] / max(1, cst["ext_rays"])
 
     mpaths = totals["dead_paths"] / elapsed / 1e6
     mrays = (totals["ext_rays"] + totals["shadow_rays"]) / elapsed / 1e6
     trace_ms, trace_launches = phases["trace"]
     trace_avg_ms = trace_ms / max(1, trace_launches)
-    rays_per_launch = totals["ext_rays"] / max(1, trace_launches)
+    rays_per_launch = ptotals["ext_rays"] / max(1, trace_launches)
     bytes_per_ray = RAY_BYTES + nodes_per_ray * NODE_BYTES[args.bvh_width] + tris_per_ray * TRI_BYTES
     achieved = rays_per_launch * bytes_per_ray / (trace_avg_ms * 1e-3) / 1e9
     traffic = traffic_from_profiles(rays_per_launch)
@@ -673,6 +687,11 @@ def main():
         },
         "mrays_per_s": round(mrays, 3),
         "bounces_per_step": totals["iterations"] // args.steps,
+        "host_waits_per_step": totals["host_waits"] / args.steps,
+        "device_drains_per_step": totals["device_drains"] / args.steps,
+        "timed_path": "hipGraph bursts of 8 bounces, scheduler snapshots read 2 bursts behind, "
+                      "one device drain per frame",
+        "profiled_ms_per_step": round(prof_elapsed * 1e3 / args.steps, 2),
         "accel_build_s": round(build_s, 3),
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in phases.items()},
         "roofline": {
@@ -683,18 +702,31 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": None if traffic is None else round(traffic),
+            "bytes": "algorithmic: every node and triangle fetch counted; most are served by "
+                     "L2 / Infinity Cache, the DRAM share is `traffic`",
+            "dram_frac": None if traffic is None
+            else round(traffic / (trace_avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "bytes_per_ray": round(bytes_per_ray, 1),
             "nodes_per_ray": round(nodes_per_ray, 2),
             "tris_per_ray": round(tris_per_ray, 2),
             "avg_launch_ms": round(trace_avg_ms, 4),
             "rays_per_launch": round(rays_per_launch, 1),
         },
+        "kernel_rooflines": kernel_rooflines(cst, ptotals, phases, args.bvh_width, spectral,
+                                             args.queues),
     }
     r.close()
     accel.close()
     if not args.no_cpu:
         log("cpu baseline")
         result["cpu_baseline"] = cpu_baseline(scene, params, args.cpu_seconds)
+    legs = set(args.legs.split(",")) if args.legs else set()
+    if "batch" in legs:
+        log("batched leg")
+        result["batch"] = bench_batch(args)
+    if "instanced" in legs:
+        log("instanced leg")
+        result["instanced"] = bench_instanced(args)
     if args.nrc:
         log("radiance-cache equal-time leg")
         result["nrc"] = nrc_equal_time(args)

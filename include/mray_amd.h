@@ -149,6 +149,12 @@ typedef struct MRayRenderStats
     uint32_t finished;        /* all totalSPP samples are in the film    */
     uint64_t nodeVisits;      /* BVH nodes visited (counting mode)       */
     uint64_t triTests;        /* triangles tested (counting mode)        */
+    uint32_t hostWaits;       /* render_all / render_frame: host waits on
+                                 a burst's scheduler snapshot while later
+                                 bursts stay queued (the GPU keeps working) */
+    uint32_t deviceDrains;    /* waits that leave the GPU idle: 1, at the end */
+    uint64_t shadowNodeVisits;  /* BVH nodes visited by any-hit walks (counting mode) */
+    uint64_t shadowTriTests;    /* triangles tested by any-hit walks (counting mode) */
 } MRayRenderStats;
 
 /* ---------------------------------------------------------------------- */
@@ -516,6 +522,14 @@ int mray_nrc_param_counts(const MRayNrc* nrc, uint64_t* hMlp, uint64_t* hGrid);
 int mray_nrc_read(MRayNrc* nrc, uint32_t which, float* hOut);
 int mray_nrc_write(MRayNrc* nrc, uint32_t which, const float* hIn);
 int mray_nrc_zero_grads(MRayNrc* nrc, void* stream);
+/* Per-kernel timing of the cache (HIP events around each launch on its
+ * stream; no reference counterpart, it instruments the learned part for the
+ * MFMA roofline): while on, every launch is timed; mray_nrc_kernel_times
+ * synchronises and returns accumulated ms and launch counts [6] for
+ * inference forward, training forward, backward, weight gradients, grid
+ * scatter and Adam.                                                        */
+int mray_nrc_set_profiling(MRayNrc* nrc, uint32_t on);
+int mray_nrc_kernel_times(MRayNrc* nrc, double* hMs, uint64_t* hLaunches);
 
 
 /* Renderer + radiance cache (PathTracerRGB).  One path slot in 2^trainShift

@@ -203,6 +203,10 @@ class MRayRenderStats(C.Structure):
         ("finished", C.c_uint32),
         ("nodeVisits", C.c_uint64),
         ("triTests", C.c_uint64),
+        ("hostWaits", C.c_uint32),
+        ("deviceDrains", C.c_uint32),
+        ("shadowNodeVisits", C.c_uint64),
+        ("shadowTriTests", C.c_uint64),
     ]
 
 

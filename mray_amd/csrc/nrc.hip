@@ -14,7 +14,7 @@
 // layer's columns 3..63 stay zero), ReLU after the first five.
 //
 // Kernels (one wave = 32 samples as one 32x64 activation tile in LDS; the
-// MFMA is v_mfma_f32_32x32x8_bf16):
+// MFMA is gfx950's v_mfma_f32_32x32x16_bf16, 16 k per instruction):
 //   KNrcForward<TRAIN>  encode + 6 layers; inference writes RGB, training also
 //                       stores every layer input (bf16) and dL/dy
 //   KNrcBackward        dZ of every layer (bf16) and dL/d(grid features)
@@ -35,17 +35,20 @@
 namespace mray
 {
 
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
 
 constexpr uint32_t kNrcLayers = 6;
 constexpr uint32_t kW = 64;                  // width of every layer
-constexpr uint32_t kLdsRow = 68;             // padded LDS row (conflict-free b64 reads)
+constexpr uint32_t kLdsRow = 72;             // padded LDS row: 16-B aligned, conflict-free b128 reads
 constexpr uint32_t kNrcFeat = 2;
 constexpr uint32_t kNrcMaxLevels = 16;
 constexpr uint32_t kFwdWaves = 4;            // 128 samples per block
 constexpr uint32_t kLayerParams = kW * kW;
 constexpr uint32_t kMlpParams = kNrcLayers * kLayerParams;
+// kernel kinds timed by mray_nrc_kernel_times: inference forward, training
+// forward, backward (data gradients), weight gradients, grid scatter, Adam
+constexpr uint32_t kNrcKernelKinds = 6;
 
 __device__ __forceinline__ uint16_t F2Bf(float f)
 {
@@ -162,19 +165,21 @@ __device__ __forceinline__ void EncodeHalf(const NrcView& v, const float* rec, u
 // column n and reduction index k is bRow(n)[k] (4 consecutive k per lane).
 __device__ __forceinline__ void TileProduct(const uint16_t* sA, const uint16_t* sB, v16f acc[2])
 {
+    // v_mfma_f32_32x32x16_bf16: lane l supplies row (l mod 32) of A and
+    // column (l mod 32) of B over k = 8 (l / 32) .. +7 of the 16-k step
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t r = lane & 31u, kh = (lane >> 5) * 4u;
+    const uint32_t r = lane & 31u, kh = (lane >> 5) * 8u;
     acc[0] = v16f{0};
     acc[1] = v16f{0};
     #pragma unroll
-    for(uint32_t kt = 0; kt < kW / 8u; kt++)
+    for(uint32_t kt = 0; kt < kW / 16u; kt++)
     {
-        const uint32_t k = kt * 8u + kh;
-        v4s a = *reinterpret_cast<const v4s*>(sA + r * kLdsRow + k);
-        v4s b0 = *reinterpret_cast<const v4s*>(sB + r * kLdsRow + k);
-        v4s b1 = *reinterpret_cast<const v4s*>(sB + (r + 32u) * kLdsRow + k);
-        acc[0] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b0, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b1, acc[1], 0, 0, 0);
+        const uint32_t k = kt * 16u + kh;
+        v8bf a = *reinterpret_cast<const v8bf*>(sA + r * kLdsRow + k);
+        v8bf b0 = *reinterpret_cast<const v8bf*>(sB + r * kLdsRow + k);
+        v8bf b1 = *reinterpret_cast<const v8bf*>(sB + (r + 32u) * kLdsRow + k);
+        acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b0, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b1, acc[1], 0, 0, 0);
     }
 }
 
@@ -374,8 +379,8 @@ KNrcBackward(NrcView v, NrcTrainBuf tb, uint32_t batch)
 __global__ void __launch_bounds__(256)
 KNrcWgrad(NrcTrainBuf tb, uint32_t batch, uint32_t chunk, float* __restrict__ wgrad)
 {
-    __shared__ uint16_t sXc[64 * kLdsRow];
-    __shared__ uint16_t sGc[64 * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t sXc[64 * kLdsRow];
+    __shared__ __attribute__((aligned(16))) uint16_t sGc[64 * kLdsRow];
     const uint32_t l = blockIdx.y;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t mt = wave >> 1, nt = wave & 1u;
@@ -401,19 +406,19 @@ KNrcWgrad(NrcTrainBuf tb, uint32_t batch, uint32_t chunk, float* __restrict__ wg
         const uint32_t kRow = mt * 32u + (lane & 31u);
         const uint32_t nCol = nt * 32u + (lane & 31u);
         #pragma unroll
-        for(uint32_t kt = 0; kt < 8; kt++)
+        for(uint32_t kt = 0; kt < 4; kt++)
         {
-            uint32_t sr = kt * 8u + (lane >> 5) * 4u;
-            v4s a, b;
-            a[0] = short(sXc[(sr + 0) * kLdsRow + kRow]);
-            a[1] = short(sXc[(sr + 1) * kLdsRow + kRow]);
-            a[2] = short(sXc[(sr + 2) * kLdsRow + kRow]);
-            a[3] = short(sXc[(sr + 3) * kLdsRow + kRow]);
-            b[0] = short(sGc[(sr + 0) * kLdsRow + nCol]);
-            b[1] = short(sGc[(sr + 1) * kLdsRow + nCol]);
-            b[2] = short(sGc[(sr + 2) * kLdsRow + nCol]);
-            b[3] = short(sGc[(sr + 3) * kLdsRow + nCol]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+            // reduction over 16 samples per instruction: lane half h takes
+            // samples 8h .. 8h+7 of the step
+            const uint32_t sr = kt * 16u + (lane >> 5) * 8u;
+            v8bf a, b;
+            #pragma unroll
+            for(uint32_t j = 0; j < 8; j++)
+            {
+                a[j] = __builtin_bit_cast(__bf16, sXc[(sr + j) * kLdsRow + kRow]);
+                b[j] = __builtin_bit_cast(__bf16, sGc[(sr + j) * kLdsRow + nCol]);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
         }
         __syncthreads();
     }
@@ -567,6 +572,41 @@ struct MRayNrc
     uint64_t step = 0;
     float* hLoss = nullptr;       // pinned
     uint32_t fwdBlocks = 0;
+    // per-kernel timing (mray_nrc_set_profiling): event pairs around each
+    // launch, harvested by mray_nrc_kernel_times
+    bool profiling = false;
+    std::vector<hipEvent_t> evPool;
+    std::vector<std::pair<uint32_t, std::pair<hipEvent_t, hipEvent_t>>> evPending;
+    double kMs[kNrcKernelKinds] = {};
+    uint64_t kLaunches[kNrcKernelKinds] = {};
+};
+
+// Event pair around one launch of kernel kind k (profiling only).
+struct NrcTimer
+{
+    MRayNrc* n;
+    uint32_t k;
+    hipStream_t s;
+    hipEvent_t a = nullptr, b = nullptr;
+    NrcTimer(MRayNrc* n_, uint32_t k_, hipStream_t s_) : n(n_), k(k_), s(s_)
+    {
+        if(!n->profiling) return;
+        a = Take();
+        b = Take();
+        if(a && b) (void)hipEventRecord(a, s);
+    }
+    ~NrcTimer()
+    {
+        if(!a || !b) return;
+        (void)hipEventRecord(b, s);
+        n->evPending.push_back({k, {a, b}});
+    }
+    hipEvent_t Take()
+    {
+        if(!n->evPool.empty()) { hipEvent_t e = n->evPool.back(); n->evPool.pop_back(); return e; }
+        hipEvent_t e = nullptr;
+        return (hipEventCreate(&e) == hipSuccess) ? e : nullptr;
+    }
 };
 
 static int NrcLaunchCheck() { MRAY_HIP_CHECK(hipGetLastError()); return MRAY_OK; }
@@ -684,6 +724,12 @@ int mray_nrc_destroy(MRayNrc* n)
     if(!n) return MRAY_OK;
     (void)hipDeviceSynchronize();
     if(n->hLoss) (void)hipHostFree(n->hLoss);
+    for(auto& p : n->evPending)
+    {
+        (void)hipEventDestroy(p.second.first);
+        (void)hipEventDestroy(p.second.second);
+    }
+    for(hipEvent_t e : n->evPool) (void)hipEventDestroy(e);
     n->arena.Release();
     delete n;
     return MRAY_OK;
@@ -696,8 +742,11 @@ int mray_nrc_infer(MRayNrc* n, const float* dRecords, uint32_t count, const uint
     hipStream_t s = static_cast<hipStream_t>(stream);
     uint32_t blocks = dCount ? n->fwdBlocks
                              : std::min(n->fwdBlocks, DivUp(std::max(count, 1u), 32u * kFwdWaves));
-    KNrcForward<false><<<blocks, kFwdWaves * 64, 0, s>>>(n->view, dRecords, dCount, count, dOut,
-                                                         nullptr, NrcTrainBuf{}, 0);
+    {
+        NrcTimer t(n, 0, s);
+        KNrcForward<false><<<blocks, kFwdWaves * 64, 0, s>>>(n->view, dRecords, dCount, count, dOut,
+                                                             nullptr, NrcTrainBuf{}, 0);
+    }
     return NrcLaunchCheck();
 }
 
@@ -714,14 +763,26 @@ int mray_nrc_backward(MRayNrc* n, const float* dRecords, const float* dTargets, 
     KNrcZero<<<1, 64, 0, s>>>(n->tb.loss, 1);
     uint32_t tiles = count / 32u;
     uint32_t blocks = std::min(n->fwdBlocks, DivUp(tiles, kFwdWaves));
-    KNrcForward<true><<<blocks, kFwdWaves * 64, 0, s>>>(n->view, dRecords, nullptr, count, nullptr,
-                                                        dTargets, n->tb, count);
-    KNrcBackward<<<blocks, kFwdWaves * 64, 0, s>>>(n->view, n->tb, count);
+    {
+        NrcTimer t(n, 1, s);
+        KNrcForward<true><<<blocks, kFwdWaves * 64, 0, s>>>(n->view, dRecords, nullptr, count, nullptr,
+                                                            dTargets, n->tb, count);
+    }
+    {
+        NrcTimer t(n, 2, s);
+        KNrcBackward<<<blocks, kFwdWaves * 64, 0, s>>>(n->view, n->tb, count);
+    }
     uint32_t chunk = 512;
     dim3 wg(DivUp(count, chunk), kNrcLayers);
-    KNrcWgrad<<<wg, 256, 0, s>>>(n->tb, count, chunk, n->wg);
-    KNrcGridBwd<<<DivUp(size_t(count) * n->view.levels, 256u), 256, 0, s>>>(n->view, dRecords,
-                                                                         n->tb.dx0, count, n->gg);
+    {
+        NrcTimer t(n, 3, s);
+        KNrcWgrad<<<wg, 256, 0, s>>>(n->tb, count, chunk, n->wg);
+    }
+    {
+        NrcTimer t(n, 4, s);
+        KNrcGridBwd<<<DivUp(size_t(count) * n->view.levels, 256u), 256, 0, s>>>(n->view, dRecords,
+                                                                             n->tb.dx0, count, n->gg);
+    }
     return NrcLaunchCheck();
 }
 
@@ -737,6 +798,7 @@ int mray_nrc_adam(MRayNrc* n, void* stream)
     a.eps = n->cfg.epsilon;
     a.c1 = float(1.0 / (1.0 - std::pow(double(a.b1), double(n->step))));
     a.c2 = float(1.0 / (1.0 - std::pow(double(a.b2), double(n->step))));
+    NrcTimer t(n, 5, s);
     KNrcAdamMlp<<<DivUp(kMlpParams, 256u), 256, 0, s>>>(n->w, n->wm, n->wv, n->wg, n->wT, n->wK, a);
     if(n->cfg.flags & MRAY_NRC_LAZY_ADAM)
         KNrcAdamGrid<true><<<4096, 256, 0, s>>>(n->grid, n->gm, n->gv, n->gg, n->gridH, n->gridCount, a);
@@ -798,6 +860,35 @@ int mray_nrc_write(MRayNrc* n, uint32_t which, const float* hIn)
     KNrcRefresh<<<2048, 256>>>(n->w, n->wT, n->wK, n->grid, n->gridH, n->gridCount);
     MRAY_HIP_CHECK(hipGetLastError());
     MRAY_HIP_CHECK(hipDeviceSynchronize());
+    return MRAY_OK;
+}
+
+int mray_nrc_set_profiling(MRayNrc* n, uint32_t on)
+{
+    if(!n) { SetError("nrc_set_profiling: NULL argument"); return MRAY_ERR_INVALID; }
+    n->profiling = on != 0;
+    return MRAY_OK;
+}
+
+int mray_nrc_kernel_times(MRayNrc* n, double* hMs, uint64_t* hLaunches)
+{
+    if(!n || !hMs || !hLaunches) { SetError("nrc_kernel_times: NULL argument"); return MRAY_ERR_INVALID; }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    for(auto& p : n->evPending)
+    {
+        float ms = 0.0f;
+        MRAY_HIP_CHECK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+        n->kMs[p.first] += ms;
+        n->kLaunches[p.first] += 1;
+        n->evPool.push_back(p.second.first);
+        n->evPool.push_back(p.second.second);
+    }
+    n->evPending.clear();
+    for(uint32_t k = 0; k < kNrcKernelKinds; k++)
+    {
+        hMs[k] = n->kMs[k];
+        hLaunches[k] = n->kLaunches[k];
+    }
     return MRAY_OK;
 }
 

@@ -121,6 +121,8 @@ struct Sched
     unsigned long long startClock;  // wall clock at the first refill
     uint32_t viewsDone;             // batch: views [0, viewsDone) are complete
     uint32_t viewBase;              // batch: view of the first path this refill issues
+    unsigned long long shadowNodeVisits;  // counting mode only: any-hit walks
+    unsigned long long shadowTriTests;
 };
 
 // Work and queue counters.  Atomics on one address serialise at about
@@ -310,6 +312,11 @@ __device__ __forceinline__ SceneView SlotScene(const RenderConst& rc, const Slot
 static constexpr uint32_t kSlotTPB = 256;
 static constexpr uint32_t kMatQueues = 5;   // boundary/light + 4 material types
 static constexpr uint32_t kGraphIters = 8;  // bounces per captured hipGraph
+// Bursts queued ahead of the one whose scheduler snapshot the host reads: the
+// GPU always holds kLookahead bursts of queued work while the host decides
+// whether the frame is done.
+static constexpr uint32_t kLookahead = 2;
+static constexpr uint32_t kRingBursts = kLookahead + 1;
 
 // ---------------------------------------------------------------------------
 // Refill
@@ -464,8 +471,13 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
         sched->activeCount = carryL + uint32_t(fill);
         sched->deadTotal += accum;
         sched->queryCount = 0;
-        if(sched->iterations == 0) sched->startClock = wall_clock64();
-        sched->iterations += 1;
+        // a bounce after the frame has drained (queued ahead by the host)
+        // does nothing and is not counted
+        if(sched->activeCount != 0 || accum != 0)
+        {
+            if(sched->iterations == 0) sched->startClock = wall_clock64();
+            sched->iterations += 1;
+        }
     }
 }
 
@@ -735,7 +747,8 @@ __device__ __forceinline__ V3 CoOctaToDir(float s, float t)
     float radius = 1.0f - fabsf(d);
     float phiPrime = 0.0f;
     if(radius != 0.0f) phiPrime = ((va - ua) / radius + 1) * piOvr4;
-    float sinP = sinf(phiPrime), cosP = cosf(phiPrime);
+    float sinP, cosP;
+    fm::SinCos(phiPrime, sinP, cosP);
     float cosPhi = (signbit(u) ? -1.0f : 1.0f) * cosP;
     float sinPhi = (signbit(v) ? -1.0f : 1.0f) * sinP;
     float z = (signbit(d) ? -1.0f : 1.0f) * (1.0f - radius * radius);
@@ -775,8 +788,8 @@ __device__ __forceinline__ bool IsSkysphere(uint32_t t)
 // angles, u = (azimuth + pi) / (2 pi), v = 1 - inclination / pi.
 __device__ __forceinline__ void DirToSpherical(V3 dz, float& u, float& v)
 {
-    float azimuth = atan2f(dz.y, dz.x);
-    float incl = acosf(ClampF(dz.z, -1.0f, 1.0f));
+    float azimuth = fm::Atan2(dz.y, dz.x);
+    float incl = fm::Acos(ClampF(dz.z, -1.0f, 1.0f));
     u = (azimuth + kPi) * 0.5f / kPi;
     v = 1.0f - incl * kInvPi;
 }
@@ -784,7 +797,9 @@ __device__ __forceinline__ V3 SphericalToDir(float u, float v, float& sinIncl)
 {
     float theta = (u * kPi * 2.0f) - kPi;
     float phi = (1.0f - v) * kPi;
-    float sT = sinf(theta), cT = cosf(theta), sP = sinf(phi), cP = cosf(phi);
+    float sT, cT, sP, cP;
+    fm::SinCos(theta, sT, cT);
+    fm::SinCos(phi, sP, cP);
     sinIncl = sP;
     return {cT * sP, sT * sP, cP};
 }
@@ -892,7 +907,7 @@ __device__ __forceinline__ float SkyspherePdf(const SceneView& s, V3 dir)
     float uvPdf = 1.0f, sinIncl = 1.0f;
     const V3 d = Normalize(dir);
     if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
-        sinIncl = sinf(acosf(ClampF(d.y, -1.0f, 1.0f)));    // Z-up z = Y-up y
+        sinIncl = fm::Sin(fm::Acos(ClampF(d.y, -1.0f, 1.0f)));    // Z-up z = Y-up y
     if(s.envW != 0)
     {
         float u, v;
@@ -1219,8 +1234,8 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
             rec[2] = (sf.position.z - nr.boxMin[2]) * nr.boxInvExtent;
             auto sph = [](V3 d, float& th, float& ph)
             {
-                th = acosf(ClampF(d.z, -1.0f, 1.0f)) * kInvPi;
-                ph = (atan2f(d.y, d.x) + kPi) * (0.5f * kInvPi);
+                th = fm::Acos(ClampF(d.z, -1.0f, 1.0f)) * kInvPi;
+                ph = (fm::Atan2(d.y, d.x) + kPi) * (0.5f * kInvPi);
             };
             sph(wO, rec[3], rec[4]);
             sph(sf.geoNormal, rec[5], rec[6]);
@@ -1491,7 +1506,7 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues,
 // ---------------------------------------------------------------------------
 // KCAccumulateShadowRaysPT: a shadow ray that reaches its light point adds
 // its precomputed NEE radiance to the path.
-template<int W, bool BATCH, int TPB = kTraceTPB, bool INST = false>
+template<int W, bool BATCH, int TPB = kTraceTPB, bool INST = false, bool COUNT = false>
 __global__ void __launch_bounds__(TPB)
 KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __restrict__ sched,
         const ViewGPU* __restrict__ views, NrcRenderState nrc, uint32_t* __restrict__ ctr,
@@ -1504,6 +1519,7 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
     LoadShards(ctr, CTR_SHADOW, g);
     const uint32_t n = g.pre[kShards];
     if(gtid == 0) sched->shadowRays += n;
+    uint32_t nodes = 0, tris = 0;
     FetchLoop(Ctr(ctr, CTR_SHADOW_FETCH), n, [&](uint32_t k)
     {
         uint32_t sh;
@@ -1513,14 +1529,14 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
         if constexpr(INST)
         {
             uint32_t instId;
-            h = TraceInst<true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                       o.w, d.w, instId);
+            h = TraceInst<true, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                       o.w, d.w, instId, &nodes, &tris);
         }
         else
         {
             const uint32_t root = BATCH ? views[S.view[i]].root : 0u;
-            h = TraceW<W, true, false>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
-                                       o.w, d.w, root);
+            h = TraceW<W, true, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+                                       o.w, d.w, root, &nodes, &tris);
         }
         if(h.prim == kInvalid)
         {
@@ -1542,6 +1558,20 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
             }
         }
     });
+    if(COUNT)
+    {
+        unsigned long long nv = nodes, nt = tris;
+        for(int o = 32; o > 0; o >>= 1)
+        {
+            nv += __shfl_xor(nv, o);
+            nt += __shfl_xor(nt, o);
+        }
+        if((threadIdx.x & 63u) == 0 && (nv | nt))
+        {
+            atomicAdd(&sched->shadowNodeVisits, nv);
+            atomicAdd(&sched->shadowTriTests, nt);
+        }
+    }
 }
 
 // Radiance-cache answers: a terminated path adds throughput x max(L, 0).
@@ -1589,6 +1619,8 @@ __global__ void KResetSlots(uint32_t* __restrict__ meta, uint32_t* __restrict__ 
         sched->shadowRays = 0;
         sched->nodeVisits = 0;
         sched->triTests = 0;
+        sched->shadowNodeVisits = 0;
+        sched->shadowTriTests = 0;
         sched->fill = 0;
         sched->activeCount = 0;
         sched->iterations = 0;
@@ -1617,6 +1649,9 @@ struct MRayRenderer
     RenderConst rc{};
     Sched* dSched = nullptr;
     Sched* hSched = nullptr;      // pinned mirror
+    Sched* hRing = nullptr;       // pinned scheduler snapshots, one per burst in flight
+    hipEvent_t ringEv[kRingBursts] = {};
+    uint32_t hostWaits = 0;       // event waits of the last RunToLimit (none drains the GPU)
     uint4* dBlockCnt = nullptr;
     uint32_t* dShadowQ = nullptr;
     uint32_t* dActive = nullptr;
@@ -1750,28 +1785,28 @@ static void LaunchTrace(MRayRenderer* r, hipStream_t s)
     else KTrace<4, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
 }
 
-template<bool BATCH, bool INST>
+template<bool COUNT, bool BATCH, bool INST>
 static void LaunchShadow(MRayRenderer* r, hipStream_t s)
 {
     const TraceCtx& ctx = r->ctx;
     const uint32_t nb = r->shadowBlocks;
     if constexpr(INST)
     {
-        KShadow<4, BATCH, kTraceTPB, true><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched,
-                                                                   r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
+        KShadow<4, BATCH, kTraceTPB, true, COUNT><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched,
+                                                                          r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
         return;
     }
     if constexpr(BATCH)
     {
         if(r->traceTPB == 128)
         {
-            if(ctx.width == 8) KShadow<8, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
-            else KShadow<4, true, 128><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
+            if(ctx.width == 8) KShadow<8, true, 128, false, COUNT><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
+            else KShadow<4, true, 128, false, COUNT><<<nb, 128, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
             return;
         }
     }
-    if(ctx.width == 8) KShadow<8, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
-    else KShadow<4, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
+    if(ctx.width == 8) KShadow<8, BATCH, kTraceTPB, false, COUNT><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
+    else KShadow<4, BATCH, kTraceTPB, false, COUNT><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dShadowQ, r->dSched, r->dViews, r->rc.nrc, r->dCtr, r->rc.N);
 }
 
 template<bool BATCH, bool INST>
@@ -1821,7 +1856,8 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
         KShade<false, BATCH, INST><<<r->shadeBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dActive, r->dSched,
                                                                        r->dShadowQ, r->dCtr);
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
-    LaunchShadow<BATCH, INST>(r, s);
+    if(r->counting) LaunchShadow<true, BATCH, INST>(r, s);
+    else LaunchShadow<false, BATCH, INST>(r, s);
     if(!BATCH && r->nrc)
     {
         // radiance-cache answers for the paths that terminated this bounce
@@ -2254,6 +2290,9 @@ static void FreeRenderer(MRayRenderer* r)
     if(r->stream) (void)hipStreamDestroy(r->stream);
     for(hipEvent_t e : r->events) (void)hipEventDestroy(e);
     if(r->hSched) (void)hipHostFree(r->hSched);
+    if(r->hRing) (void)hipHostFree(r->hRing);
+    for(hipEvent_t e : r->ringEv)
+        if(e) (void)hipEventDestroy(e);
     r->nrcArena.Release();
     r->arena.Release();
     delete r;
@@ -2380,6 +2419,51 @@ int mray_renderer_iterate(MRayRenderer* r, uint32_t iterations, void* stream)
     return MRAY_OK;
 }
 
+// Queue bursts of kGraphIters bounces until deadTotal reaches `limit`.
+// After each burst the scheduler record is copied asynchronously into a
+// pinned ring slot; the host reads the snapshot of the burst kLookahead
+// bursts back, so the GPU never waits on the host and the device drains once,
+// when the caller reads the statistics.  Bounces queued after the frame has
+// finished find no work and are cheap (every kernel reads its counts from
+// device memory).
+static int RunToLimit(MRayRenderer* r, unsigned long long limit, const char* who)
+{
+    hipStream_t s = r->stream;
+    if(!r->hRing)
+    {
+        MRAY_HIP_CHECK(hipHostMalloc((void**)&r->hRing, sizeof(Sched) * kRingBursts,
+                                     hipHostMallocDefault));
+        for(hipEvent_t& e : r->ringEv)
+            MRAY_HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    r->hostWaits = 0;
+    unsigned long long prevDead = 0, prevExt = 0;
+    bool havePrev = false;
+    for(uint64_t b = 0;; b++)
+    {
+        MRAY_TRY(mray_renderer_iterate(r, kGraphIters, nullptr));
+        const uint32_t slot = uint32_t(b % kRingBursts);
+        MRAY_HIP_CHECK(hipMemcpyAsync(&r->hRing[slot], r->dSched, sizeof(Sched),
+                                      hipMemcpyDeviceToHost, s));
+        MRAY_HIP_CHECK(hipEventRecord(r->ringEv[slot], s));
+        if(b < kLookahead) continue;
+        const uint32_t old = uint32_t((b - kLookahead) % kRingBursts);
+        MRAY_HIP_CHECK(hipEventSynchronize(r->ringEv[old]));
+        r->hostWaits++;
+        const Sched& c = r->hRing[old];
+        if(c.deadTotal >= limit) break;
+        if(havePrev && c.deadTotal == prevDead && c.extRays == prevExt)
+        {
+            SetError(std::string(who) + ": no progress");
+            return MRAY_ERR_INVALID;
+        }
+        prevDead = c.deadTotal;
+        prevExt = c.extRays;
+        havePrev = true;
+    }
+    return MRAY_OK;
+}
+
 int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* out)
 {
     if(!r || !out) { SetError("renderer_stats: NULL argument"); return MRAY_ERR_INVALID; }
@@ -2394,6 +2478,8 @@ int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* out)
     out->finished = (c.deadTotal >= r->limit) ? 1u : 0u;
     out->nodeVisits = c.nodeVisits - r->last.nodeVisits;
     out->triTests = c.triTests - r->last.triTests;
+    out->shadowNodeVisits = c.shadowNodeVisits - r->last.shadowNodeVisits;
+    out->shadowTriTests = c.shadowTriTests - r->last.shadowTriTests;
     r->last = c;
     if(r->profiling) return HarvestEvents(r);
     return MRAY_OK;
@@ -2402,26 +2488,12 @@ int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* out)
 int mray_renderer_render_all(MRayRenderer* r, MRayRenderStats* hStats)
 {
     if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }
+    MRAY_TRY(RunToLimit(r, r->limit, "render_all"));
     MRayRenderStats total{};
-    for(;;)
-    {
-        MRAY_TRY(mray_renderer_iterate(r, kGraphIters, nullptr));
-        MRayRenderStats st{};
-        MRAY_TRY(mray_renderer_stats(r, &st));
-        total.deadPaths += st.deadPaths;
-        total.extensionRays += st.extensionRays;
-        total.shadowRays += st.shadowRays;
-        total.iterations += st.iterations;
-        total.nodeVisits += st.nodeVisits;
-        total.triTests += st.triTests;
-        total.totalDeadPaths = st.totalDeadPaths;
-        if(st.finished) { total.finished = 1; break; }
-        if(st.deadPaths == 0 && st.extensionRays == 0)
-        {
-            SetError("render_all: no progress");
-            return MRAY_ERR_INVALID;
-        }
-    }
+    MRAY_TRY(mray_renderer_stats(r, &total));
+    total.finished = 1;
+    total.hostWaits = r->hostWaits;
+    total.deviceDrains = 1;
     if(hStats) *hStats = total;
     return MRAY_OK;
 }
@@ -2556,26 +2628,11 @@ int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* h
     r->frameLimit += (unsigned long long)spp * r->rc.P;
     KSetLimit<<<1, 1, 0, s>>>(r->dSched, r->frameLimit);
     MRAY_HIP_CHECK(hipGetLastError());
+    MRAY_TRY_IT(RunToLimit(r, r->frameLimit, "render_frame"));
     MRayRenderStats total{};
-    for(;;)
-    {
-        MRAY_TRY_IT(mray_renderer_iterate(r, kGraphIters, nullptr));
-        MRayRenderStats st{};
-        MRAY_TRY_IT(mray_renderer_stats(r, &st));
-        total.deadPaths += st.deadPaths;
-        total.extensionRays += st.extensionRays;
-        total.shadowRays += st.shadowRays;
-        total.iterations += st.iterations;
-        total.nodeVisits += st.nodeVisits;
-        total.triTests += st.triTests;
-        total.totalDeadPaths = st.totalDeadPaths;
-        if(st.totalDeadPaths >= r->frameLimit) break;
-        if(st.deadPaths == 0 && st.extensionRays == 0)
-        {
-            SetError("render_frame: no progress");
-            return MRAY_ERR_INVALID;
-        }
-    }
+    MRAY_TRY_IT(mray_renderer_stats(r, &total));
+    total.hostWaits = r->hostWaits;
+    total.deviceDrains = 1;
     if(r->nrc)
     {
         // online training on the most recent training pairs

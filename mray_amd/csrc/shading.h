@@ -6,6 +6,7 @@
 #pragma once
 
 #include "common.h"
+#include "fmath.h"
 #include "scene.h"
 
 namespace mray
@@ -82,10 +83,10 @@ __device__ __forceinline__ Q4 QSLerp(Q4 s, Q4 e, float t)
     float s0, s1;
     if(cosFlipped < (1 - kEps))
     {
-        float angle = acosf(cosFlipped);
-        float sinAngleRecip = 1.0f / sinf(angle);
-        s0 = sinf(angle * (1 - t)) * sinAngleRecip;
-        s1 = sinf(angle * t) * sinAngleRecip;
+        float angle = fm::Acos(cosFlipped);
+        float sinAngleRecip = 1.0f / fm::Sin(angle);
+        s0 = fm::Sin(angle * (1 - t)) * sinAngleRecip;
+        s1 = fm::Sin(angle * t) * sinAngleRecip;
     }
     else
     {
@@ -126,7 +127,7 @@ __device__ __forceinline__ V3 Nudge(V3 p, V3 n) { return {NudgeC(p.x, n.x), Nudg
 // Math::InvErrFunc host polynomial (Math.h:787-833)
 __device__ __forceinline__ float InvErf(float x)
 {
-    float t = logf(Fma(x, 0.0f - x, 1.0f));
+    float t = fm::Log(Fma(x, 0.0f - x, 1.0f));
     float p;
     if(fabsf(t) > 6.125f)
     {
@@ -161,7 +162,7 @@ __device__ __forceinline__ float Gaussian(float x, float sigma)
     float sigmaInv = 1.0f / sigma;
     float result = invSqrt2Pi * sigmaInv;
     float p = (x - 0.0f) * sigmaInv;
-    result *= expf(-0.5f * p * p);
+    result *= fm::Exp(-0.5f * p * p);
     return result;
 }
 __device__ __forceinline__ float SampleGaussian(float xi, float sigma)
@@ -180,7 +181,8 @@ __device__ __forceinline__ V3 SampleCosDir(float xi0, float xi1, float& pdf)
 {
     float ang = 2.0f * kPi * xi1;
     float r = sqrtf(xi0);
-    float s = sinf(ang), c = cosf(ang);
+    float s, c;
+    fm::SinCos(ang, s, c);
     V3 d;
     d.x = r * c;
     d.y = r * s;
@@ -264,7 +266,8 @@ __device__ __forceinline__ V3 VNDFSample(V3 V, float a, float xi0, float xi1, fl
     V3 T2 = Cross(VH, T1);
     float r = sqrtf(xi0);
     float phi = 2.0f * kPi * xi1;
-    float sinPhi = sinf(phi), cosPhi = cosf(phi);
+    float sinPhi, cosPhi;
+    fm::SinCos(phi, sinPhi, cosPhi);
     float t1 = r * cosPhi;
     float t2 = r * sinPhi;
     float s = 0.5f * (1.0f + VH.z);

@@ -13,6 +13,8 @@
 // Rec.709 XYZ->RGB matrix, and the coefficient LUT that spectral_lut.cpp fits.
 #pragma once
 
+#include "fmath.h"
+
 #include "common.h"
 
 namespace mray
@@ -49,8 +51,8 @@ __host__ __device__ __forceinline__ void SampleWavelengths(float xi0, V4& waves,
         if(x < 0.0f) x += 1.0f;
         if(x >= 1.0f) x -= 1.0f;
         float a = 0.85691062f - 1.82750197f * x;
-        float lam = 538.0f - 138.888889f * atanhf(a);
-        float denom = coshf(0.0072f * (lam - 538.0f));
+        float lam = 538.0f - 138.888889f * fm::Atanh(a);
+        float denom = fm::Cosh(0.0072f * (lam - 538.0f));
         denom = denom * denom;
         w[i] = lam;
         p[i] = 0.0039398042f / denom;
@@ -62,7 +64,7 @@ __host__ __device__ __forceinline__ void SampleWavelengths(float xi0, V4& waves,
 // Math::InvSmoothstep (Core/Math.h:1224-1232)
 __device__ __forceinline__ float InvSmoothstep(float y)
 {
-    return 0.5f - sinf(asinf(1.0f - 2.0f * y) * (1.0f / 3.0f));
+    return 0.5f - fm::Sin(fm::Asin(1.0f - 2.0f * y) * (1.0f / 3.0f));
 }
 
 __device__ __forceinline__ V3 LutFetch(const SpectralView& sv, uint32_t ch, float x, float y, float z)

@@ -1376,15 +1376,20 @@ extern "C" int mray_tracer_flatten_to_file(mray_amd::TracerI* tracer, const char
     std::fwrite(hdr, 4, 5, fp);
     std::fwrite(f.boundaryRad, 4, 3, fp);
     std::fwrite(&f.sceneDiameter, 4, 1, fp);
-    std::fwrite(f.pos.data(), 4, f.pos.size(), fp);
-    std::fwrite(f.tbn.data(), 4, f.tbn.size(), fp);
-    std::fwrite(f.idx.data(), 4, f.idx.size(), fp);
-    std::fwrite(f.keys.data(), 4, f.keys.size(), fp);
-    std::fwrite(f.cull.data(), 1, f.cull.size(), fp);
-    std::fwrite(mats.data(), sizeof(MRayMaterial), mats.size(), fp);
-    std::fwrite(f.lightTris.data(), 4, f.lightTris.size(), fp);
-    std::fwrite(f.lightRad.data(), 4, f.lightRad.size(), fp);
-    std::fwrite(f.lightTwoSided.data(), 1, f.lightTwoSided.size(), fp);
+    // fwrite's buffer must not be NULL, even for zero items (an empty vector's data())
+    auto put = [fp](const auto& v)
+    {
+        if(!v.empty()) std::fwrite(v.data(), sizeof(v[0]), v.size(), fp);
+    };
+    put(f.pos);
+    put(f.tbn);
+    put(f.idx);
+    put(f.keys);
+    put(f.cull);
+    put(mats);
+    put(f.lightTris);
+    put(f.lightRad);
+    put(f.lightTwoSided);
     std::fclose(fp);
     return MRAY_OK;
 }

@@ -34,7 +34,7 @@ EXPORTS = [
     "mray_renderer_set_measurement", "mray_renderer_phase_times",
     "mray_nrc_create", "mray_nrc_destroy", "mray_nrc_infer", "mray_nrc_backward", "mray_nrc_adam",
     "mray_nrc_train", "mray_nrc_last_loss", "mray_nrc_param_counts", "mray_nrc_read",
-    "mray_nrc_write", "mray_nrc_zero_grads",
+    "mray_nrc_write", "mray_nrc_zero_grads", "mray_nrc_set_profiling", "mray_nrc_kernel_times",
     "mray_renderer_attach_nrc", "mray_renderer_render_frame", "mray_renderer_cache_stats",
     "mray_device_memory",
     "mray_batch_scene_create", "mray_batch_scene_destroy", "mray_batch_scene_info",
@@ -108,6 +108,8 @@ def lib():
     L.mray_nrc_read.argtypes = [P, u32, P]
     L.mray_nrc_write.argtypes = [P, u32, P]
     L.mray_nrc_zero_grads.argtypes = [P, P]
+    L.mray_nrc_set_profiling.argtypes = [P, u32]
+    L.mray_nrc_kernel_times.argtypes = [P, P, P]
     L.mray_renderer_attach_nrc.argtypes = [P, P, C.POINTER(abi.MRayNrcRenderParams)]
     L.mray_renderer_render_frame.argtypes = [P, u32, C.POINTER(abi.MRayRenderStats)]
     L.mray_renderer_cache_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
@@ -376,7 +378,9 @@ class Renderer:
         return {"dead_paths": st.deadPaths, "ext_rays": st.extensionRays,
                 "shadow_rays": st.shadowRays, "iterations": st.iterations,
                 "finished": bool(st.finished), "node_visits": st.nodeVisits,
-                "tri_tests": st.triTests}
+                "tri_tests": st.triTests, "host_waits": st.hostWaits,
+                "device_drains": st.deviceDrains, "shadow_node_visits": st.shadowNodeVisits,
+                "shadow_tri_tests": st.shadowTriTests}
 
     def set_measurement(self, profiling=False, counting=False):
         check(lib().mray_renderer_set_measurement(self.h, int(profiling), int(counting)))
@@ -427,7 +431,8 @@ class Renderer:
         st = abi.MRayRenderStats()
         check(lib().mray_renderer_render_frame(self.h, spp, C.byref(st)))
         return {"dead_paths": st.deadPaths, "ext_rays": st.extensionRays,
-                "shadow_rays": st.shadowRays, "iterations": st.iterations}
+                "shadow_rays": st.shadowRays, "iterations": st.iterations,
+                "host_waits": st.hostWaits, "device_drains": st.deviceDrains}
 
     def cache_stats(self):
         q, t = C.c_uint64(), C.c_uint64()
@@ -491,6 +496,19 @@ class Nrc:
 
     def zero_grads(self):
         check(lib().mray_nrc_zero_grads(self.h, None))
+
+    NRC_KERNELS = ("infer_forward", "train_forward", "backward", "wgrad", "grid_bwd", "adam")
+
+    def set_profiling(self, on=True):
+        check(lib().mray_nrc_set_profiling(self.h, int(bool(on))))
+
+    def kernel_times(self):
+        """{kernel: (accumulated ms, launches)} since profiling was turned on."""
+        import numpy as np
+        ms = np.zeros(6, np.float64)
+        n = np.zeros(6, np.uint64)
+        check(lib().mray_nrc_kernel_times(self.h, ms.ctypes.data, n.ctypes.data))
+        return {k: (float(ms[i]), int(n[i])) for i, k in enumerate(self.NRC_KERNELS)}
 
     def read(self, which):
         import numpy as np

@@ -216,8 +216,27 @@ def _quad(c, u, v):
     return p, np.array([[0, 1, 2], [0, 2, 3]])
 
 
-def _box(center, half, angle_y):
-    """12 outward-facing triangles of a box rotated about +Y."""
+def _quad_grid(c, u, v, n):
+    """The quad c +- u +- v split into n x n cells, two CCW triangles each
+    (2 n^2); n = 1 is _quad."""
+    if n <= 1:
+        return _quad(c, u, v)
+    c, u, v = (np.asarray(x, dtype=np.float64) for x in (c, u, v))
+    g = np.linspace(-1.0, 1.0, n + 1)
+    p = (c[None, None] + g[:, None, None] * u[None, None] + g[None, :, None] * v[None, None])
+    p = p.reshape(-1, 3)                           # point (i, j) at i * (n + 1) + j
+    i, j = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    a = (i * (n + 1) + j).ravel()
+    b = ((i + 1) * (n + 1) + j).ravel()
+    cc = ((i + 1) * (n + 1) + j + 1).ravel()
+    d = (i * (n + 1) + j + 1).ravel()
+    t = np.stack([np.stack([a, b, cc], 1), np.stack([a, cc, d], 1)], 1).reshape(-1, 3)
+    return p, t
+
+
+def _box(center, half, angle_y, n=1):
+    """12 outward-facing triangles of a box rotated about +Y (each face split
+    into n x n cells: 12 n^2 triangles)."""
     cx, cy, cz = center
     hx, hy, hz = half
     ca, sa = math.cos(angle_y), math.sin(angle_y)
@@ -230,10 +249,10 @@ def _box(center, half, angle_y):
              ((0, 0, -1), (-1, 0, 0), (0, 1, 0))]
     pts, tris = [], []
     h = np.array([hx, hy, hz])
-    for n, u, v in faces:
-        n, u, v = np.array(n, float), np.array(u, float), np.array(v, float)
-        p, t = _quad(n * h, u * h, v * h)
-        tris.append(t + len(pts) * 4)
+    for n_, u_, v_ in faces:
+        nn, u, v = np.array(n_, float), np.array(u_, float), np.array(v_, float)
+        p, t = _quad_grid(nn * h, u * h, v * h, n)
+        tris.append(t + sum(len(q) for q in pts))
         pts.append(p)
     p = np.concatenate(pts) @ rot.T + np.array([cx, cy, cz])
     return p, np.concatenate(tris)
@@ -249,8 +268,11 @@ def fov_y_from_x(fov_x, aspect):
 # by mt19937(1) angles in [0, 30 deg], one 0.25 x 0.25 ceiling light (2 tris,
 # radiance (17, 12, 4)); Lambert red / green / white 0.73.
 # ---------------------------------------------------------------------------
-def cornell(aspect=1.0, fov_deg=40.0):
-    rng = MT19937(1)
+def cornell(aspect=1.0, fov_deg=40.0, tess=1, seed=1):
+    """The Cornell workload; tess > 1 splits every wall and box face into
+    tess x tess cells (34 tess^2 + 2 triangles; the light stays two), seed
+    draws the box angles (the workload itself is seed 1)."""
+    rng = MT19937(seed)
     a0 = math.radians(rng.uniform(0.0, 30.0))
     a1 = math.radians(rng.uniform(0.0, 30.0))
     mats = [
@@ -258,15 +280,16 @@ def cornell(aspect=1.0, fov_deg=40.0):
         {"type": abi.MAT_LAMBERT, "albedo": (0.63, 0.065, 0.05)},  # 1 red
         {"type": abi.MAT_LAMBERT, "albedo": (0.14, 0.45, 0.09)},   # 2 green
     ]
+    n = max(1, int(tess))
     b = _Builder()
     # walls, normals facing into the box
-    b.add_mesh(*_quad((0, -1, 0), (1, 0, 0), (0, 0, -1)), key=0, cull=1)   # floor, +Y
-    b.add_mesh(*_quad((0, 1, 0), (1, 0, 0), (0, 0, 1)), key=0, cull=1)     # ceiling, -Y
-    b.add_mesh(*_quad((0, 0, -1), (1, 0, 0), (0, 1, 0)), key=0, cull=1)    # back, +Z
-    b.add_mesh(*_quad((-1, 0, 0), (0, 0, -1), (0, 1, 0)), key=1, cull=1)   # left red, +X
-    b.add_mesh(*_quad((1, 0, 0), (0, 0, 1), (0, 1, 0)), key=2, cull=1)     # right green, -X
-    b.add_mesh(*_box((0.33, -0.7, 0.3), (0.3, 0.3, 0.3), a0), key=0, cull=1)
-    b.add_mesh(*_box((-0.33, -0.4, -0.3), (0.3, 0.6, 0.3), a1), key=0, cull=1)
+    b.add_mesh(*_quad_grid((0, -1, 0), (1, 0, 0), (0, 0, -1), n), key=0, cull=1)   # floor, +Y
+    b.add_mesh(*_quad_grid((0, 1, 0), (1, 0, 0), (0, 0, 1), n), key=0, cull=1)     # ceiling, -Y
+    b.add_mesh(*_quad_grid((0, 0, -1), (1, 0, 0), (0, 1, 0), n), key=0, cull=1)    # back, +Z
+    b.add_mesh(*_quad_grid((-1, 0, 0), (0, 0, -1), (0, 1, 0), n), key=1, cull=1)   # left red, +X
+    b.add_mesh(*_quad_grid((1, 0, 0), (0, 0, 1), (0, 1, 0), n), key=2, cull=1)     # right green, -X
+    b.add_mesh(*_box((0.33, -0.7, 0.3), (0.3, 0.3, 0.3), a0, n), key=0, cull=1)
+    b.add_mesh(*_box((-0.33, -0.4, -0.3), (0.3, 0.6, 0.3), a1, n), key=0, cull=1)
     # ceiling light, facing -Y, just below the ceiling
     first = b.add_mesh(*_quad((0, 0.999, 0), (0.125, 0, 0), (0, 0, 0.125)),
                        key=[abi.LIGHT_FLAG | 0, abi.LIGHT_FLAG | 1], cull=0)
@@ -484,6 +507,26 @@ def batch_workload(count=32, seed=1, min_tris=10_000, max_tris=2_000_000, spp=16
         t = math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))
         w, h = res[rng() % 4]
         out.append(terrain_spheres_sized(int(t), w, h, seed=10 + k))
+        views.append({"scene": k, "width": w, "height": h, "spp": spp})
+    return out, views
+
+
+def cornell_batch_workload(count=32, seed=5, min_tris=10_000, max_tris=2_000_000, spp=16):
+    """The batched workload as specified: `count` Cornell scenes (the Cornell
+    generator, box angles from their own seeds) tessellated to triangle
+    counts log-uniform in [min_tris, max_tris], resolutions drawn from 256^2,
+    512^2, 1280x720 and 1920x1080, all from mt19937(seed).  Returns the
+    scenes and the view dicts for mray_batch_create."""
+    rng = MT19937(seed)
+    res = [(256, 256), (512, 512), (1280, 720), (1920, 1080)]
+    out, views = [], []
+    for k in range(count):
+        t = math.exp(rng.uniform(math.log(min_tris), math.log(max_tris)))
+        w, h = res[rng() % 4]
+        n = max(1, int(round(math.sqrt(max(t - 2.0, 34.0) / 34.0))))
+        sc = cornell(aspect=w / h, tess=n, seed=100 + k)
+        sc.name = f"cornell-t{n}"
+        out.append(sc)
         views.append({"scene": k, "width": w, "height": h, "spp": spp})
     return out, views
 

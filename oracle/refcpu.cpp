@@ -40,9 +40,11 @@
 
 #include "../include/mray_amd.h"
 #include "../mray_amd/csrc/colour_data.h"
+#include "../mray_amd/csrc/fmath.h"
 
 namespace rc
 {
+namespace fm = mray::fm;   // the elementary functions the GPU kernels also use
 
 // ------------------------------------------------------------------------
 // Vector math (Core/Vector.hpp, Core/Math.h)
@@ -183,10 +185,10 @@ static inline Quat QSLerp(Quat s, Quat e, float t)
     float s0, s1;
     if(cosFlipped < (1 - kEps))
     {
-        float angle = std::acos(cosFlipped);
-        float sinAngleRecip = 1.0f / std::sin(angle);
-        s0 = std::sin(angle * (1 - t)) * sinAngleRecip;
-        s1 = std::sin(angle * t) * sinAngleRecip;
+        float angle = fm::Acos(cosFlipped);
+        float sinAngleRecip = 1.0f / fm::Sin(angle);
+        s0 = fm::Sin(angle * (1 - t)) * sinAngleRecip;
+        s1 = fm::Sin(angle * t) * sinAngleRecip;
     }
     else
     {
@@ -818,7 +820,7 @@ static void SeedStates(uint64_t seed, uint32_t count, uint32_t* out)
 // Math::InvErrFunc host path (Math.h:787-833)
 static inline float InvErf(float x)
 {
-    float t = std::log(std::fma(x, 0.0f - x, 1.0f));
+    float t = fm::Log(std::fma(x, 0.0f - x, 1.0f));
     float p;
     if(std::fabs(t) > 6.125f)
     {
@@ -855,7 +857,7 @@ static inline float Gaussian(float x, float sigma)
     float sigmaInv = 1.0f / sigma;
     float result = invSqrt2Pi * sigmaInv;
     float p = (x - 0.0f) * sigmaInv;
-    result *= std::exp(-0.5f * p * p);
+    result *= fm::Exp(-0.5f * p * p);
     return result;
 }
 
@@ -880,7 +882,8 @@ static inline DirSample SampleCosDirection(float xi0, float xi1)
 {
     float xi1Angle = 2.0f * kPi * xi1;
     float xi0Sqrt = std::sqrt(xi0);
-    float s = std::sin(xi1Angle), c = std::cos(xi1Angle);
+    float s, c;
+    fm::SinCos(xi1Angle, s, c);
     V3 dir;
     dir.x = xi0Sqrt * c;
     dir.y = xi0Sqrt * s;
@@ -974,7 +977,8 @@ static inline DirSample VNDFGGXSmithSample(V3 V, float a, float xi0, float xi1)
     V3 T2 = Cross(VHemi, T1);
     float r = std::sqrt(xi0);
     float phi = 2.0f * kPi * xi1;
-    float sinPhi = std::sin(phi), cosPhi = std::cos(phi);
+    float sinPhi, cosPhi;
+    fm::SinCos(phi, sinPhi, cosPhi);
     float t1 = r * cosPhi;
     float t2 = r * sinPhi;
     float s = 0.5f * (1.0f + VHemi.z);
@@ -1420,7 +1424,8 @@ static inline V3 CoOctaToDir(float s, float t)
     float radius = 1.0f - std::fabs(d);
     float phiPrime = 0.0f;
     if(radius != 0.0f) phiPrime = ((va - ua) / radius + 1) * piOvr4;
-    float sinP = std::sin(phiPrime), cosP = std::cos(phiPrime);
+    float sinP, cosP;
+    fm::SinCos(phiPrime, sinP, cosP);
     float cosPhi = SignPM1(u) * cosP;
     float sinPhi = SignPM1(v) * sinP;
     float z = SignPM1(d) * (1.0f - radius * radius);
@@ -1463,8 +1468,8 @@ static inline bool IsSkysphere(uint32_t t)
 // v = 1 - inclination / pi.
 static inline void DirToSpherical(V3 dz, float& u, float& v)
 {
-    float azimuth = std::atan2(dz.y, dz.x);
-    float incl = std::acos(std::min(std::max(dz.z, -1.0f), 1.0f));
+    float azimuth = fm::Atan2(dz.y, dz.x);
+    float incl = fm::Acos(std::min(std::max(dz.z, -1.0f), 1.0f));
     u = (azimuth + kPi) * 0.5f / kPi;
     v = 1.0f - incl * kInvPi;
 }
@@ -1472,7 +1477,9 @@ static inline V3 SphericalToDir(float u, float v, float& sinIncl)
 {
     float theta = (u * kPi * 2.0f) - kPi;
     float phi = (1.0f - v) * kPi;
-    float sT = std::sin(theta), cT = std::cos(theta), sP = std::sin(phi), cP = std::cos(phi);
+    float sT, cT, sP, cP;
+    fm::SinCos(theta, sT, cT);
+    fm::SinCos(phi, sP, cP);
     sinIncl = sP;
     return {cT * sP, sT * sP, cP};
 }
@@ -1562,7 +1569,7 @@ static float SkyspherePdf(const Scene& s, V3 dir)
     float uvPdf = 1.0f, sinIncl = 1.0f;
     const V3 d = Normalize(dir);
     if(s.boundaryType == MRAY_BOUNDARY_SKYSPHERE_SPHERICAL)
-        sinIncl = std::sin(std::acos(std::min(std::max(d.y, -1.0f), 1.0f)));
+        sinIncl = fm::Sin(fm::Acos(std::min(std::max(d.y, -1.0f), 1.0f)));
     if(s.envW)
     {
         float u, v;
@@ -1911,7 +1918,7 @@ static SpectralData gSpec;
 static inline float InvSmoothstep(float y)
 {
     const float factor = 1.0f / 3.0f;
-    return 0.5f - std::sin(std::asin(1.0f - 2.0f * y) * factor);
+    return 0.5f - fm::Sin(fm::Asin(1.0f - 2.0f * y) * factor);
 }
 
 // 3-D texture fetch with normalised coordinates and texel centres at
@@ -2014,8 +2021,8 @@ static void SampleWavelengths(float xi0, V4& waves, V4& pdf)
         if(x < 0.0f) x += 1.0f;
         if(x >= 1.0f) x -= 1.0f;
         float a = 0.85691062f - 1.82750197f * x;
-        float lam = 538.0f - 138.888889f * std::atanh(a);
-        float denom = std::cosh(0.0072f * (lam - 538.0f));
+        float lam = 538.0f - 138.888889f * fm::Atanh(a);
+        float denom = fm::Cosh(0.0072f * (lam - 538.0f));
         denom = denom * denom;
         w[i] = lam;
         p[i] = 0.0039398042f / denom;
@@ -2928,6 +2935,29 @@ void rc_bsdf_eval_many(const MRayMaterial* M, const float* tbn, const float* geo
     {
         rc_bsdf_probe(M, tbn, geoN, backSide, wO, wI + 3 * size_t(i), nullptr, 1, out + 4 * size_t(i));
         if(pdf) rc_bsdf_probe(M, tbn, geoN, backSide, wO, wI + 3 * size_t(i), nullptr, 2, pdf + i);
+    }
+}
+
+// The shared elementary functions (fmath.h) at n points, for their accuracy
+// tests: fn 0 sin, 1 cos, 2 exp, 3 log, 4 atan2(x, y), 5 acos, 6 asin,
+// 7 atanh, 8 cosh.
+void rc_fmath(int fn, const float* x, const float* y, uint32_t n, float* out)
+{
+    for(uint32_t i = 0; i < n; i++)
+    {
+        float a = x[i];
+        switch(fn)
+        {
+            case 0: out[i] = fm::Sin(a); break;
+            case 1: out[i] = fm::Cos(a); break;
+            case 2: out[i] = fm::Exp(a); break;
+            case 3: out[i] = fm::Log(a); break;
+            case 4: out[i] = fm::Atan2(a, y[i]); break;
+            case 5: out[i] = fm::Acos(a); break;
+            case 6: out[i] = fm::Asin(a); break;
+            case 7: out[i] = fm::Atanh(a); break;
+            default: out[i] = fm::Cosh(a); break;
+        }
     }
 }
 

@@ -20,6 +20,37 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+_V4_FLAGS = ("avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl")
+
+
+def host_cpu():
+    """(model name, True when the host has the x86-64-v4 AVX-512 set)."""
+    model, flags = "unknown x86_64", set()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name") and model == "unknown x86_64":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("flags") and not flags:
+                flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    return model, all(f in flags for f in _V4_FLAGS)
+
+
+def use_native_isa():
+    """Load the x86-64-v4 build when this host supports it (the CPU
+    baseline's build); call before the first lib().  Returns the ISA level."""
+    global _LIB
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    _, v4 = host_cpu()
+    v4_lib = os.path.join(_HERE, "librefcpu_v4.so")
+    if v4 and os.path.exists(v4_lib):
+        _LIB = v4_lib
+        return "x86-64-v4"
+    return "x86-64-v3"
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -82,6 +113,7 @@ def lib():
         L.rc_bsdf_eval_many.argtypes = [C.POINTER(abi.MRayMaterial), f32, f32, C.c_int, f32, f32,
                                         C.c_uint32, f32, f32]
         L.rc_inv_erf.argtypes = [f32, C.c_uint32, f32]
+        L.rc_fmath.argtypes = [C.c_int, f32, f32, C.c_uint32, f32]
         L.rc_sample_gaussian.argtypes = [f32, C.c_uint32, C.c_float, f32, f32]
         L.rc_spherical_to_dir.argtypes = [f32, C.c_uint32, f32, f32]
         L.rc_dir_to_spherical.argtypes = [f32, C.c_uint32, f32]
@@ -352,6 +384,20 @@ def bsdf_eval_many(mat, tbn, geo_n, back_side, wo, wi):
     lib().rc_bsdf_eval_many(C.byref(m), abi.fptr(_f(tbn)), abi.fptr(_f(geo_n)), int(back_side),
                             abi.fptr(_f(wo)), abi.fptr(wi), len(wi), abi.fptr(out), abi.fptr(pdf))
     return out, pdf
+
+
+FMATH = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "atan2": 4, "acos": 5, "asin": 6,
+         "atanh": 7, "cosh": 8}
+
+
+def fmath(name, x, y=None):
+    """The shared elementary function `name` (fmath.h) at float32 points x
+    (atan2 takes (x, y) as (y-coordinate, x-coordinate))."""
+    x = _f(x).ravel()
+    yy = _f(y).ravel() if y is not None else x
+    out = np.zeros_like(x)
+    lib().rc_fmath(FMATH[name], abi.fptr(x), abi.fptr(yy), len(x), abi.fptr(out))
+    return out
 
 
 def inv_erf(x):

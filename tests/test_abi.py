@@ -27,7 +27,7 @@ def test_struct_layouts():
     assert ctypes.sizeof(abi.MRayMaterial) == 52
     assert ctypes.sizeof(abi.MRayCamera) == 52
     assert ctypes.sizeof(abi.MRayRenderParams) == 48
-    assert ctypes.sizeof(abi.MRayRenderStats) == 56
+    assert ctypes.sizeof(abi.MRayRenderStats) == 80
     assert ctypes.sizeof(abi.MRayNrcConfig) == 48
     assert ctypes.sizeof(abi.MRayBatchView) == 76
     assert ctypes.sizeof(abi.MRayBatchTune) == 24

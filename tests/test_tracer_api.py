@@ -228,3 +228,40 @@ def test_json_scene_transforms_and_errors(tmp_path):
     r = subprocess.run([DRIVER, TRACER_LIB, src, "flatten", dst], capture_output=True, text=True,
                        timeout=120)
     assert r.returncode != 0 and "Unknown material type" in r.stderr
+
+
+GOLDEN_SCENES = os.path.join(ROOT, "tests", "golden", "scenes")
+
+
+def _load_fixture(name, tmp_path):
+    _built()
+    dst = str(tmp_path / f"{name}.bin")
+    r = subprocess.run([DRIVER, TRACER_LIB, os.path.join(GOLDEN_SCENES, f"{name}.json"),
+                        "flatten", dst], capture_output=True, text=True, timeout=120)
+    return r, dst
+
+
+def test_reference_scene_fixtures(tmp_path):
+    """The reference loader test's scenes (T_SceneLoaderMRay.cpp:58-88): the
+    empty scene fails to load; MinimalValidScene and BasicScene load, with
+    the surfaces they define; BasicScene's one triangle arrives unchanged."""
+    r, _ = _load_fixture("empty", tmp_path)
+    assert r.returncode != 0 and "MRayError" in (r.stdout + r.stderr)
+    r, _ = _load_fixture("minimal_valid", tmp_path)
+    assert "loaded: 0 surfaces, 0 light surfaces, 1 camera surfaces" in r.stdout, r.stdout + r.stderr
+    r, dst = _load_fixture("basic", tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "loaded: 1 surfaces, 0 light surfaces, 1 camera surfaces" in r.stdout
+    f = read_flat_file(dst)
+    tri = f["pos"][f["idx"]].reshape(-1, 3, 3)
+    assert tri.shape == (1, 3, 3)
+    assert np.array_equal(tri[0], np.array([[0, 0.5, 0], [-0.5, -0.5, 0], [0.5, -0.5, 0]],
+                                           np.float32))
+    mats = f["mats"].view(np.uint32).reshape(-1, 13)
+    assert mats[0, 0] == abi.MAT_LAMBERT
+    assert np.allclose(mats[0, 1:4].view(np.float32), [0, 1, 0])
+    # normal +Z at every vertex: the TBN quaternion takes +Z to the tangent-space z
+    from mray_amd import scenes as sc
+    want = sc.quat_world_to_tangent(np.array([[0.0, 0.0, 1.0]]))[0]
+    for q in f["tbn"][f["idx"]].reshape(-1, 4):
+        assert np.allclose(np.abs(q), np.abs(want), atol=1e-6)
