@@ -198,6 +198,17 @@ int mray_lbvh_build(const uint64_t* dSortedCodes, const uint32_t* dLeafOrder,
                     const float* dLeafAabbs, uint32_t count, uint32_t* dNodes,
                     float* dNodeAabbs, void* dTemp, size_t tempBytes, void* stream);
 size_t mray_lbvh_temp_bytes(uint32_t count);
+/* The radix sort above with no workspace argument and no allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Its scratch (12.3 MiB, sized for
+ * MRAY_NOALLOC_SORT_MAX pairs) is a static device array, one copy per GPU,
+ * held until the process ends; larger counts return MRAY_ERR_INVALID.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on one
+ * GPU (two streams, two threads) race and must be serialised by the caller. */
+#define MRAY_NOALLOC_SORT_MAX (1u << 20)
+int mray_radix_sort_u64_u32_noalloc(uint64_t* dKeys, uint32_t* dValues, uint32_t count,
+                                    void* stream);
 
 /* ---------------------------------------------------------------------- */
 /* Accelerator: scene upload, LBVH build, wide-BVH collapse, casting.       */

@@ -4,6 +4,9 @@
 #include "common.h"
 #include "lbvh.h"
 
+#include <mutex>
+#include <string>
+
 namespace mray
 {
 
@@ -513,6 +516,53 @@ int LaunchRadixSort(uint64_t* dKeys, uint32_t* dVals, uint32_t n, void* dTemp, s
     return MRAY_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free sort (mray_radix_sort_u64_u32_noalloc): the same passes over
+// a scratch area that lives in the code object, one copy per GPU, held until
+// the process ends.  Its size is RadixSortTempBytes(kNoAllocSortMax).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kNoAllocSortMax = MRAY_NOALLOC_SORT_MAX;
+static_assert(kNoAllocSortMax == (1u << 20), "scratch size is stated as 2^20 pairs");
+constexpr size_t kNoAllocSortHistBytes =
+    ((size_t(256) * ((kNoAllocSortMax + kRsTile - 1) / kRsTile) * sizeof(uint32_t) + 255) / 256) * 256;
+constexpr size_t kNoAllocSortBytes =
+    kNoAllocSortHistBytes + size_t(kNoAllocSortMax) * (sizeof(uint64_t) + sizeof(uint32_t)) + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocSortBytes];
+
+namespace
+{
+constexpr int kMaxDevices = 64;
+std::mutex gScratchLock;
+void* gScratchPtr[kMaxDevices] = {};
+}
+
+// The device address of `scratch` on the current GPU, looked up once per GPU
+// (the array's name is only a host-side handle; kernels need this address).
+static int ScratchOnCurrentDevice(void** out)
+{
+    int dev = 0;
+    MRAY_HIP_CHECK(hipGetDevice(&dev));
+    if(dev < 0 || dev >= kMaxDevices)
+    {
+        SetError("radix sort (no-alloc): device index out of range");
+        return MRAY_ERR_INVALID;
+    }
+    std::lock_guard<std::mutex> g(gScratchLock);
+    if(!gScratchPtr[dev])
+    {
+        void* p = nullptr;
+        MRAY_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if(!p)
+        {
+            SetError("radix sort (no-alloc): scratch symbol has no device address");
+            return MRAY_ERR_HIP;
+        }
+        gScratchPtr[dev] = p;
+    }
+    *out = gScratchPtr[dev];
+    return MRAY_OK;
+}
+
 size_t LbvhTempBytes(uint32_t n)
 {
     // leaf parents [n] + arrival counters [n]
@@ -546,3 +596,27 @@ int LaunchLbvhBuild(const uint64_t* dSortedCodes, const uint32_t* dLeafOrder,
 }
 
 } // namespace mray
+
+using namespace mray;
+
+extern "C" int mray_radix_sort_u64_u32_noalloc(uint64_t* dKeys, uint32_t* dValues, uint32_t count,
+                                                void* stream)
+{
+    if(count > kNoAllocSortMax)
+    {
+        SetError("radix sort (no-alloc): " + std::to_string(count) + " pairs exceed the maximum of " +
+                 std::to_string(kNoAllocSortMax));
+        return MRAY_ERR_INVALID;
+    }
+    if(count > 1 && (!dKeys || !dValues))
+    {
+        SetError("radix sort (no-alloc): NULL keys or values");
+        return MRAY_ERR_INVALID;
+    }
+    if(count <= 1) return MRAY_OK;
+    void* tmp = nullptr;
+    MRAY_TRY(ScratchOnCurrentDevice(&tmp));
+    return LaunchRadixSort(dKeys, dValues, count, tmp, kNoAllocSortBytes,
+                           static_cast<hipStream_t>(stream));
+}
+

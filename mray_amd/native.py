@@ -25,7 +25,7 @@ EXPORTS = [
     "mray_last_error", "mray_version", "mray_device_count",
     "mray_prim_centers_aabbs", "mray_reduce_aabb", "mray_morton63", "mray_morton30",
     "mray_radix_sort_temp_bytes", "mray_radix_sort_u64_u32", "mray_lbvh_build",
-    "mray_lbvh_temp_bytes",
+    "mray_lbvh_temp_bytes", "mray_radix_sort_u64_u32_noalloc",
     "mray_accel_create", "mray_accel_create_ex", "mray_accel_destroy", "mray_accel_info", "mray_accel_read_lbvh",
     "mray_accel_cast_closest", "mray_accel_cast_closest_counted", "mray_accel_cast_visibility",
     "mray_renderer_create", "mray_renderer_destroy", "mray_renderer_iterate",
@@ -70,6 +70,7 @@ def lib():
     L.mray_radix_sort_temp_bytes.restype = C.c_size_t
     L.mray_radix_sort_temp_bytes.argtypes = [u32]
     L.mray_radix_sort_u64_u32.argtypes = [P, P, u32, P, C.c_size_t, P]
+    L.mray_radix_sort_u64_u32_noalloc.argtypes = [P, P, u32, P]
     L.mray_lbvh_temp_bytes.restype = C.c_size_t
     L.mray_lbvh_temp_bytes.argtypes = [u32]
     L.mray_lbvh_build.argtypes = [P, P, P, u32, P, P, P, C.c_size_t, P]
@@ -204,6 +205,17 @@ def radix_sort_u64_u32(d_keys, d_vals):
     nb = lib().mray_radix_sort_temp_bytes(n)
     tmp = torch.empty(max(1, nb), dtype=torch.uint8, device=d_keys.device)
     check(lib().mray_radix_sort_u64_u32(_ptr(d_keys), _ptr(d_vals), n, _ptr(tmp), nb, _stream()))
+
+
+NOALLOC_SORT_MAX = 1 << 20     # MRAY_NOALLOC_SORT_MAX
+
+
+def radix_sort_u64_u32_noalloc(d_keys, d_vals):
+    """The same sort with the library's static scratch: no allocation at all
+    (at most NOALLOC_SORT_MAX pairs; calls on one GPU must not overlap)."""
+    require_gpu()
+    n = d_keys.shape[0]
+    check(lib().mray_radix_sort_u64_u32_noalloc(_ptr(d_keys), _ptr(d_vals), n, _stream()))
 
 
 def lbvh_build(d_sorted_codes, d_leaf_order, d_leaf_aabbs):
