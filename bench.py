@@ -49,6 +49,8 @@ def parse():
                     help="material-keyed shading queues instead of one shading kernel")
     ap.add_argument("--bvh-width", type=int, choices=(4, 8), default=4,
                     help="BVH node width: 4 (fp32 boxes) or 8 (compressed, 8-bit boxes)")
+    ap.add_argument("--lane-refill", type=int, choices=(0, 1), default=0,
+                    help="closest hit with lane refill (4-wide trees): idle lanes take new rays")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--nrc", type=int, default=1,
@@ -64,9 +66,11 @@ def parse():
                     help="cache variants 'spread:neeAtEnd:minDepth:trainShift[:lazyAdam];...'")
     ap.add_argument("--workload", choices=("frame", "batch", "instanced"), default="frame",
                     help="frame: the 1080p headline frame; batch: many scenes in one renderer")
-    ap.add_argument("--legs", default="batch,instanced",
+    ap.add_argument("--inst-cache-frames", type=int, default=64)
+    ap.add_argument("--inst-cache-ref-spp", type=int, default=256)
+    ap.add_argument("--legs", default="batch,instanced,instanced_cache",
                     help="frame workload: extra legs reported in the same JSON line "
-                         "(comma list of batch, instanced; empty for none)")
+                         "(comma list of batch, instanced, instanced_cache; empty for none)")
     ap.add_argument("--inst-spp", type=int, default=16,
                     help="samples per pixel of the instanced workload")
     ap.add_argument("--inst-views", type=int, default=8,
@@ -146,14 +150,14 @@ def probe_copy(torch, mib):
 
 def traffic_from_profiles(rays_per_launch):
     """HBM bytes per KTrace launch from the committed rocprofv3 PMC summary
-    (DRAM bytes per traced ray, FETCH_SIZE + WRITE_SIZE scaled by the copy
+    (FETCH_SIZE + WRITE_SIZE per launch of the same frame, scaled by the copy
     probe), or None when no summary is present."""
     path = os.path.join(ROOT, "profiles", "ktrace_hbm_traffic_1080p.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("dram_bytes_per_ray:"):
-            return rays_per_launch * float(line.split(":", 1)[1])
+        if line.startswith("dram_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
     return None
 
 
@@ -183,6 +187,36 @@ def cpu_baseline(scene, params, budget_s):
             "kind": "port", "cpu": model, "build": f"g++ -O3 -march={isa}",
             "sample": f"first {st['dead_paths']} paths of the same frame (reference schedule), "
                       f"{dt:.1f} s on {threads} threads of {model}, g++ -O3 -march={isa}"}
+
+
+# The cache's MLP: six 64 x 64 layers (the input padded to 64, the output
+# layer's 61 zero columns included, as the MFMA tiles compute them).
+MLP_FLOP_PER_SAMPLE = 2 * 6 * 64 * 64          # one forward pass
+BF16_PEAK_TFLOPS = 2500.0                       # MI355X dense bf16 MFMA
+
+
+def mlp_roofline(kernel_ms, batch):
+    """MFMA roofline of the training GEMM kernels from per-kernel HIP-event
+    times: forward (with the hash encoding fused in), data backward and
+    weight gradients each do one 64 x 64 product per layer and sample."""
+    out = {}
+    flops = 0.0
+    ms = 0.0
+    for k in ("train_forward", "backward", "wgrad"):
+        t, n = kernel_ms[k]
+        if n:
+            f = MLP_FLOP_PER_SAMPLE * batch
+            out[k] = {"avg_ms": round(t / n, 4),
+                      "tflops": round(f / (t / n * 1e-3) / 1e12, 2)}
+            flops += f * n
+            ms += t
+    if ms <= 0:
+        return None
+    achieved = flops / (ms * 1e-3) / 1e12
+    out.update({"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 5),
+                "flop_per_step": 3 * MLP_FLOP_PER_SAMPLE * batch})
+    return out
 
 
 def relmse(img, ref):
@@ -271,6 +305,14 @@ def nrc_equal_time(args):
                 log(f"nrc leg: {name} frame {f}: {spent:.3f} s, relMSE {curve[-1]['relmse']:.4e}")
                 nxt *= 2
         stats = r.cache_stats() if nrc is not None else None
+        if nrc is not None:
+            # one more frame with HIP events around every cache kernel (outside
+            # the timed curve): the MLP kernels' MFMA roofline
+            nrc.set_profiling(True)
+            r.render_frame(1)
+            stats["kernel_ms"] = nrc.kernel_times()
+            stats["mlp_roofline"] = mlp_roofline(stats["kernel_ms"], 1 << 14)
+            nrc.set_profiling(False)
         r.close()
         if nrc is not None:
             nrc.close()
@@ -316,6 +358,7 @@ def nrc_equal_time(args):
         "relmse_plain_equal_time": best["relmse_plain_equal_time"],
         "time_ratio_to_match": best["time_ratio_to_match"],
         "time_ratio_to_reach_plain_final": best["time_ratio_to_reach_plain_final"],
+        "mlp_roofline": (best["cache_stats"] or {}).get("mlp_roofline"),
     }
 
 
@@ -507,6 +550,95 @@ def _trace_roofline(m, width, kernel):
     }
 
 
+def bench_instanced_cache(args):
+    """Batched + cache: the instanced two-level workload's 8 uneven viewports
+    in one renderer, rendered frame by frame (1 spp of every view per frame),
+    one radiance cache per viewport trained online after every frame, against
+    plain path tracing with the same frames; both judged per view by a plain
+    render of --inst-cache-ref-spp spp (frame order).  Spectral, depth 12,
+    NEE+MIS, lights chosen by power."""
+    import numpy as np
+    from mray_amd import abi, native, scenes
+    torch = native.require_gpu()
+    meshes, inst, env = scenes.instanced_workload(width=args.width, height=args.height)
+    accs = [native.Accel(m) for m in meshes]
+    isc = native.InstancedScene(accs, inst, env)
+    for a in accs:
+        a.close()
+    views = scenes.instanced_views(spp=1, count=args.inst_views)
+    pix = np.array([v["width"] * v["height"] for v in views], np.float64)
+    params = abi.render_params(1, 1, 1, rr=(12, 12), mode=abi.SAMPLE_NEE_MIS, spectral=not args.rgb,
+                               seed=0, max_ray_count=args.batch_slots, flags=abi.RENDER_LIGHT_POWER)
+    t0 = time.perf_counter()
+    r = native.BatchRenderer(isc, views, params)
+    done = 0
+    while done < args.inst_cache_ref_spp:
+        step = min(64, args.inst_cache_ref_spp - done)
+        r.render_frame(step)
+        done += step
+    ref = [film_image(f) for f in r.film()]
+    r.close()
+    log(f"instanced cache leg: judge {args.inst_cache_ref_spp} spp in {time.perf_counter() - t0:.1f} s")
+
+    def err(films):
+        e = np.array([relmse(film_image(f), g) for f, g in zip(films, ref)])
+        return e, float((e * pix).sum() / pix.sum())
+
+    def run(cached):
+        params.seed = 1
+        r = native.BatchRenderer(isc, views, params)
+        caches = []
+        if cached:
+            caches = [native.Nrc(seed=11 + k, lazy_adam=True) for k in range(len(views))]
+            r.attach_nrcs(caches, train_shift=4, spread=1.0, batch=1 << 14, steps=4, min_depth=2,
+                          nee_at_end=True)
+        curve = []
+        spent, nxt = 0.0, 1
+        for f in range(1, args.inst_cache_frames + 1):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r.render_frame(1)
+            torch.cuda.synchronize()
+            spent += time.perf_counter() - t
+            if f == nxt or f == args.inst_cache_frames:
+                per, avg = err(r.film())
+                curve.append({"frames": f, "seconds": round(spent, 4), "relmse": avg,
+                              "relmse_per_view": [round(float(x), 6) for x in per]})
+                log(f"instanced cache leg: {'cached' if cached else 'plain'} frame {f}: "
+                    f"{spent:.3f} s, relMSE {avg:.4e}")
+                nxt *= 2
+        cs = r.cache_stats() if cached else None
+        r.close()
+        for c in caches:
+            c.close()
+        return curve, cs
+
+    plain, _ = run(False)
+    cached, cs = run(True)
+    isc.close()
+    e_p, t_p = plain[-1]["relmse"], plain[-1]["seconds"]
+    e_c, t_c = cached[-1]["relmse"], cached[-1]["seconds"]
+    e_p_eq = _loglog_at(plain, "seconds", "relmse", t_c)
+    if e_p_eq is None:
+        e_p_eq = e_p * t_p / t_c
+    t_reach = _loglog_at(sorted(cached, key=lambda c: c["relmse"]), "relmse", "seconds", e_p)
+    return {
+        "workload": f"instanced two-level scene, {len(views)} viewports "
+                    f"({int(pix.sum())} pixels), 1 spp of every view per frame, "
+                    f"{args.inst_cache_frames} frames, spectral, depth 12, NEE+MIS, "
+                    f"one cache per viewport (16 x 2^19 x 2 grid, 5 x 64 MLP, 4 x 2^14 pairs "
+                    f"per frame each)",
+        "judge_spp": args.inst_cache_ref_spp,
+        "plain": plain,
+        "cached": cached,
+        "cache_stats": cs,
+        "relmse_plain": e_p,
+        "relmse_cached": e_c,
+        "relmse_plain_equal_time": e_p_eq,
+        "time_ratio_to_reach_plain_final": None if t_reach is None else round(t_reach / t_p, 4),
+    }
+
+
 def bench_instanced(args):
     """Two-level workload: 8 terrain meshes (50k-200k triangles each) placed
     1000 times with seeded rotations about +Y, scales and translations, a
@@ -637,7 +769,8 @@ def main():
     log(f"accel: {info} in {build_s:.2f} s")
     params = abi.render_params(args.width, args.height, args.spp, rr=(10, 10),
                                mode=abi.SAMPLE_NEE_MIS, spectral=spectral, seed=0,
-                               flags=abi.RENDER_MATERIAL_QUEUES if args.queues else 0)
+                               flags=(abi.RENDER_MATERIAL_QUEUES if args.queues else 0)
+                               | (abi.RENDER_LANE_REFILL if args.lane_refill else 0))
     r = native.Renderer(accel, params)
     pixels = args.width * args.height
     paths_per_frame = pixels * args.spp
@@ -691,6 +824,7 @@ def main():
         "device_drains_per_step": totals["device_drains"] / args.steps,
         "timed_path": "hipGraph bursts of 8 bounces, scheduler snapshots read 2 bursts behind, "
                       "one device drain per frame",
+        "lane_refill": bool(args.lane_refill),
         "profiled_ms_per_step": round(prof_elapsed * 1e3 / args.steps, 2),
         "accel_build_s": round(build_s, 3),
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in phases.items()},
@@ -727,6 +861,9 @@ def main():
     if "instanced" in legs:
         log("instanced leg")
         result["instanced"] = bench_instanced(args)
+    if "instanced_cache" in legs:
+        log("instanced + per-viewport cache leg")
+        result["instanced_cache"] = bench_instanced_cache(args)
     if args.nrc:
         log("radiance-cache equal-time leg")
         result["nrc"] = nrc_equal_time(args)

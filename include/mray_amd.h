@@ -137,6 +137,11 @@ typedef struct MRayRenderParams
  * (LightSampler.hpp:30-119); the boundary light keeps the uniform
  * sampler's probability 1 / (lights + 1).  This build's own sampler.     */
 #define MRAY_RENDER_LIGHT_POWER 4u
+/* Closest-hit traversal with lane refill on 4-wide trees of one scene or a
+ * batch: a lane whose walk has ended takes the next ray of its wave's chunk
+ * once 16 of the wave's 64 lanes are idle (Aila and Laine 2009), instead of
+ * idling until the wave's longest walk ends.  Same hits either way.        */
+#define MRAY_RENDER_LANE_REFILL 8u
 
 /* Per-iteration analytics (Common/AnalyticStructs.h:33-52). */
 typedef struct MRayRenderStats
@@ -543,7 +548,10 @@ int mray_nrc_set_profiling(MRayNrc* nrc, uint32_t on);
 int mray_nrc_kernel_times(MRayNrc* nrc, double* hMs, uint64_t* hLaunches);
 
 
-/* Renderer + radiance cache (PathTracerRGB).  One path slot in 2^trainShift
+/* Renderer + radiance cache (PathTracerRGB or PathTracerSpectral: the cache
+ * answers RGB, which spectral paths take at their wavelengths as an emitter's
+ * RGB radiance; spectral training targets are converted to RGB as the film
+ * converts a path).  One path slot in 2^trainShift
  * is a training path (full length, its vertices become training pairs); the
  * others terminate into the cache at a surface vertex of depth >= minDepth
  * once the path spread exceeds spreadThreshold x the primary spread (Mueller
@@ -566,8 +574,19 @@ typedef struct MRayNrcRenderParams
 
 /* nrc == NULL detaches.                                                      */
 int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderParams* hParams);
+/* One cache per view: count = 1 for a one-scene renderer, the view count
+ * (at most 32) for a batch renderer, whose view k then terminates into, is
+ * answered by and trains caches[k] (queries, training rings and the scene
+ * box are per view).  count = 0 detaches.  No reference counterpart: the
+ * per-viewport caches of a batched renderer.                                */
+int mray_renderer_attach_nrcs(MRayRenderer* r, MRayNrc* const* caches, uint32_t count,
+                              const MRayNrcRenderParams* hParams);
 /* One frame: spp more samples per pixel into the film (bounces until they are
- * all finished), then the cache's training steps when one is attached.       */
+ * all finished), then the caches' training steps when caches are attached.
+ * A batch renderer renders frames in frame order: each frame gives every view
+ * spp samples (the views' own spp are not used); its first frame restarts it
+ * in that order, and mray_renderer_reset returns it to view order (the order
+ * mray_renderer_render_all needs).                                           */
 int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* hStats);
 /* Cache queries answered and training pairs produced since the last reset.  */
 int mray_renderer_cache_stats(MRayRenderer* r, uint64_t* hQueries, uint64_t* hTrainingPairs);

@@ -250,6 +250,7 @@ class MRayNrcRenderParams(C.Structure):
 RENDER_MATERIAL_QUEUES = 1
 RENDER_NO_GRAPH = 2
 RENDER_LIGHT_POWER = 4
+RENDER_LANE_REFILL = 8
 
 
 def render_params(width, height, spp, rr=(8, 8), mode=SAMPLE_NEE_MIS, spectral=False,

@@ -90,6 +90,7 @@ int BuildBatchScene(MRayBatchScene* b, const MRayAccel* const* accels, uint32_t 
         for(int c = 0; c < 3; c++) part.boundaryRad[c] = a->view.boundaryRad[c];
         part.sceneDiameter = a->view.sceneDiameter;
         part.camera = a->camera;
+        MRAY_HIP_CHECK(hipMemcpy(part.box, a->dSceneBox, sizeof(part.box), hipMemcpyDeviceToHost));
         b->parts.push_back(part);
         nodes += a->wideNodes;
         tris += a->nTri;

@@ -74,32 +74,37 @@ struct SlotSoA
     uint32_t* inst;    // instanced scenes: instance of the hit
 };
 
-// Radiance-cache state of the renderer (RGB mode).  One path slot in
-// 2^trainShift is a training path: it never terminates into the cache and
-// records every surface vertex (query record, throughput, radiance so far);
-// when it ends, each vertex yields a training pair (record, reflected
-// radiance = (final - so far) / throughput).  Other paths terminate into the
-// cache once their spread is large (Mueller et al. 2021, sec. 4).
+// Radiance-cache state of the renderer: one cache, or one per view of a
+// batch renderer (queries, training rings and the scene box are per view;
+// a path's view picks them).  One path slot in 2^trainShift is a training
+// path: it never terminates into the cache and records every surface vertex
+// (query record, throughput, radiance so far, four channels: RGB, or the
+// four hero wavelengths); when it ends, each vertex yields a training pair
+// (record, reflected radiance = (final - so far) / throughput, converted to
+// RGB in spectral mode).  Other paths terminate into the cache once their
+// spread is large (Mueller et al. 2021, sec. 4).
 constexpr uint32_t kNrcRec = 11;
-constexpr uint32_t kNrcVert = kNrcRec + 6;   // record, throughput, radiance so far
+constexpr uint32_t kNrcVert = kNrcRec + 8;   // record, throughput, radiance so far
 struct NrcRenderState
 {
     uint32_t on;
     uint32_t trainShift;
     uint32_t maxVerts;
-    uint32_t ringSize;        // training pairs kept (power of two)
+    uint32_t ringSize;        // training pairs kept per view (power of two)
     uint32_t minDepth;        // terminal vertices have depth >= minDepth
     uint32_t neeAtEnd;        // terminal vertices take their NEE; the cache the rest
+    uint32_t views;           // caches (1, or the batch's view count)
+    uint32_t queryCap;        // entries of one view's query queue (N)
     float spreadC;
-    float boxMin[3];
-    float boxInvExtent;
+    const float4* box;        // [views] scene box min xyz, 1 / extent
     float* trainStore;        // [N >> trainShift][maxVerts][kNrcVert]
     uint32_t* trainCount;     // [N >> trainShift]
-    float* queryRec;          // [N][11]
-    uint32_t* querySlot;      // [N]
-    float* queryOut;          // [N][3]
-    float* ringRec;           // [ringSize][11]
-    float* ringTgt;           // [ringSize][3]
+    float* queryRec;          // [views][queryCap][11]
+    uint32_t* querySlot;      // [views][queryCap]
+    float* queryOut;          // [views][queryCap][3]
+    unsigned long long* ringCount;  // [views] training pairs written so far
+    float* ringRec;           // [views][ringSize][11]
+    float* ringTgt;           // [views][ringSize][3]
 };
 
 struct Sched
@@ -115,8 +120,8 @@ struct Sched
     uint32_t fill;
     uint32_t activeCount;
     uint32_t iterations;
-    uint32_t queryCount;      // radiance-cache queries of this iteration
-    unsigned long long ringCount;   // training pairs written so far
+    uint32_t spare;
+    unsigned long long spare2;
     unsigned long long queries;     // radiance-cache queries so far
     unsigned long long startClock;  // wall clock at the first refill
     uint32_t viewsDone;             // batch: views [0, viewsDone) are complete
@@ -134,6 +139,7 @@ struct Sched
 // stays one region of the image.
 constexpr uint32_t kShards = 16;
 constexpr uint32_t kCtrStride = 32;      // uint32 words between counters
+constexpr uint32_t kMaxNrcViews = 32;    // radiance caches per renderer (one per batch view)
 constexpr uint32_t kFetch = 256;         // items a wave takes per fetch atomic
 enum : uint32_t
 {
@@ -142,7 +148,8 @@ enum : uint32_t
     CTR_SHADOW = 2 * kShards,              // shadow-queue shard sizes
     CTR_QFETCH = 3 * kShards,              // [5 material queues][kShards] fetch
     CTR_QCOUNT = 8 * kShards,              // [5 material queues][kShards] sizes
-    CTR_COUNT = 13 * kShards
+    CTR_NRCQ = 13 * kShards,               // [views] radiance-cache queries this bounce
+    CTR_COUNT = 13 * kShards + kMaxNrcViews
 };
 __host__ __device__ constexpr uint32_t ShardCap(uint32_t N)
 {
@@ -261,6 +268,12 @@ struct RenderConst
     const unsigned long long* viewBegin;  // [viewCount + 1] path index ranges
     uint32_t viewCount;
     unsigned long long* viewClock;    // wall clock when a view completed
+    // frame order (mray_renderer_render_frame on a batch): path g is pixel
+    // (g mod pixTotal) of the views laid end to end, so every frame gives
+    // every view its samples at once
+    uint32_t frameMajor;
+    unsigned long long pixTotal;
+    const unsigned long long* viewPix;    // [viewCount + 1] pixel prefix sums
 };
 
 // View whose path range holds global path index g: the last v in [lo, hi]
@@ -470,7 +483,6 @@ KRefillScan(uint4* __restrict__ blockCnt, uint32_t count, Sched* __restrict__ sc
         sched->counter += fill;
         sched->activeCount = carryL + uint32_t(fill);
         sched->deadTotal += accum;
-        sched->queryCount = 0;
         // a bounce after the frame has drained (queued ahead by the host)
         // does nothing and is not counted
         if(sched->activeCount != 0 || accum != 0)
@@ -520,62 +532,61 @@ __device__ __forceinline__ void SplatToFilm(const SlotSoA& S, const RenderConst&
 }
 
 // Training pairs of the training paths that finished in the last bounce:
-// target = (final radiance - radiance before the vertex) / throughput.  One
-// lane per (path, recorded vertex), G lanes per path (G a power of two
-// >= maxVerts, dividing 64); the group's first lane reserves the ring entries.
+// target = (final radiance - radiance before the vertex) / throughput, per
+// channel; in spectral mode the four wavelengths' targets become one RGB
+// value as the film converts a path (SpectrumToRGB).  One lane per (path,
+// recorded vertex), G lanes per path (G a power of two >= maxVerts, dividing
+// 64); each path's first lane reserves its entries in the ring of its view
+// (block-wide per view in LDS, then one global atomic per view and block).
+template<bool SPECTRAL, bool BATCH>
 __global__ void __launch_bounds__(256)
 KNrcEmit(SlotSoA S, RenderConst rc, Sched* __restrict__ sched, uint32_t log2G)
 {
-    __shared__ uint32_t sWave[256 / 64];
-    __shared__ unsigned long long sBase;
+    __shared__ uint32_t sCnt[kMaxNrcViews];
+    __shared__ unsigned long long sBase[kMaxNrcViews];
     const NrcRenderState& nr = rc.nrc;
     const uint32_t G = 1u << log2G;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t lane = threadIdx.x & 63u;
+    if(threadIdx.x < kMaxNrcViews) sCnt[threadIdx.x] = 0u;
+    __syncthreads();
     const uint32_t ts = t >> log2G, v = t & (G - 1u);
     const uint32_t i = ts << nr.trainShift;
     const bool fin = (ts << nr.trainShift) < rc.N && (i >> nr.trainShift) == ts &&
                      MStatus(S.meta[i]) == ST_DEAD;
     const uint32_t nv = fin ? nr.trainCount[ts] : 0u;
-    // ring entries for the whole block with one atomic: an exclusive scan
-    // of the paths' vertex counts (held by each group's first lane)
-    const uint32_t want = (v == 0u) ? nv : 0u;
-    uint32_t x = want;
-    for(uint32_t o = 1; o < 64; o <<= 1)
-    {
-        const uint32_t y = uint32_t(__shfl_up(int(x), o));
-        if(lane >= o) x += y;
-    }
-    if(lane == 63u) sWave[wave] = x;
+    const uint32_t view = (BATCH && fin) ? S.view[i] : 0u;
+    uint32_t mine = 0;
+    if(v == 0u && nv > 0u) mine = atomicAdd(&sCnt[view], nv);
     __syncthreads();
-    if(threadIdx.x == 0)
-    {
-        uint32_t total = 0;
-        for(uint32_t w = 0; w < 256u / 64u; w++)
-        {
-            const uint32_t c = sWave[w];
-            sWave[w] = total;
-            total += c;
-        }
-        sBase = total ? atomicAdd(&sched->ringCount, (unsigned long long)total) : 0ull;
-    }
+    if(threadIdx.x < nr.views && sCnt[threadIdx.x] > 0u)
+        sBase[threadIdx.x] = atomicAdd(&nr.ringCount[threadIdx.x], (unsigned long long)sCnt[threadIdx.x]);
     __syncthreads();
-    const uint32_t mine = sWave[wave] + x - want;              // this path's first entry
     const int src = int(lane & ~(G - 1u));
     const uint32_t first = uint32_t(__shfl(int(mine), src));
     if(!fin || v >= nv) return;
-    const unsigned long long base = sBase + first;
+    const unsigned long long base = sBase[view] + first;
     const float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + v) * kNrcVert;
-    const size_t slot = size_t((base + v) & (nr.ringSize - 1u));
+    const size_t slot = size_t(view) * nr.ringSize + size_t((base + v) & (nr.ringSize - 1u));
     for(uint32_t k = 0; k < kNrcRec; k++) nr.ringRec[slot * kNrcRec + k] = vx[k];
     const float4 fr = S.rad[i];
-    const float fin3[3] = {fr.x, fr.y, fr.z};
-    for(uint32_t c = 0; c < 3; c++)
+    const float fin4[4] = {fr.x, fr.y, fr.z, fr.w};
+    float tgt[4];
+    for(uint32_t c = 0; c < 4; c++)
     {
-        float thr = vx[kNrcRec + c], before = vx[kNrcRec + 3 + c];
-        float tgt = (thr > 1e-12f) ? (fin3[c] - before) / thr : 0.0f;
-        nr.ringTgt[slot * 3 + c] = isfinite(tgt) ? MaxF(tgt, 0.0f) : 0.0f;
+        float thr = vx[kNrcRec + c], before = vx[kNrcRec + 4 + c];
+        float x = (thr > 1e-12f) ? (fin4[c] - before) / thr : 0.0f;
+        tgt[c] = isfinite(x) ? x : 0.0f;
     }
+    if constexpr(SPECTRAL)
+    {
+        const float4 w4 = S.waves[i], p4 = S.wpdf[i];
+        V4 rgb = SpectrumToRGB(rc.spec, V4{tgt[0], tgt[1], tgt[2], tgt[3]},
+                               V4{w4.x, w4.y, w4.z, w4.w}, V4{p4.x, p4.y, p4.z, p4.w},
+                               MDispersed(S.meta[i]));
+        tgt[0] = rgb.x; tgt[1] = rgb.y; tgt[2] = rgb.z;
+    }
+    for(uint32_t c = 0; c < 3; c++) nr.ringTgt[slot * 3 + c] = isfinite(tgt[c]) ? MaxF(tgt[c], 0.0f) : 0.0f;
 }
 
 // Camera path of global pixel `pix` of a W-wide image seen through camera c
@@ -652,14 +663,26 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
             const unsigned long long g = sched->base + rank;
             if constexpr(BATCH)
             {
-                // the view whose path range holds g (at or after this refill's first)
-                const uint32_t lo = ViewOfPath(rc.viewBegin, sched->viewBase, rc.viewCount - 1u, g);
+                uint32_t lo, pix;
+                if(rc.frameMajor)
+                {
+                    // frame order: the view whose pixel range holds g mod pixTotal
+                    const unsigned long long pr = g % rc.pixTotal;
+                    lo = ViewOfPath(rc.viewPix, 0u, rc.viewCount - 1u, pr);
+                    pix = uint32_t(pr - rc.viewPix[lo]);
+                }
+                else
+                {
+                    // the view whose path range holds g (at or after this refill's first)
+                    lo = ViewOfPath(rc.viewBegin, sched->viewBase, rc.viewCount - 1u, g);
+                    pix = uint32_t((g - rc.views[lo].pathBegin) % rc.views[lo].P);
+                }
                 const ViewGPU& v = rc.views[lo];
                 S.view[i] = lo;
-                StartCameraPath<SPECTRAL>(S, v.cam, v.W, i, uint32_t((g - v.pathBegin) % v.P));
+                StartCameraPath<SPECTRAL>(S, v.cam, v.W, i, pix);
             }
             else StartCameraPath<SPECTRAL>(S, rc.cam, rc.W, i, uint32_t(g % rc.P));
-            if(!SPECTRAL && rc.nrc.on)
+            if(rc.nrc.on)
             {
                 S.spread[i] = make_float2(0.0f, 0.0f);
                 if((i & ((1u << rc.nrc.trainShift) - 1u)) == 0u)
@@ -719,6 +742,137 @@ KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __re
         S.hit[i] = make_uint4(h.prim, __float_as_uint(h.t), __float_as_uint(h.a),
                               __float_as_uint(h.b));
     });
+    if(COUNT)
+    {
+        unsigned long long nv = nodes, nt = tris;
+        for(int o = 32; o > 0; o >>= 1)
+        {
+            nv += __shfl_xor(nv, o);
+            nt += __shfl_xor(nt, o);
+        }
+        if((threadIdx.x & 63u) == 0 && (nv | nt))
+        {
+            atomicAdd(&sched->nodeVisits, nv);
+            atomicAdd(&sched->triTests, nt);
+        }
+    }
+}
+
+// Wave-uniform source of 64-item chunks of an n-item list: the chunks of
+// this workgroup's XCD part first, then the other parts (the order FetchLoop
+// hands them out in).
+struct ChunkSource
+{
+    uint32_t* ctr;
+    uint32_t n, C, part, tries;
+    __device__ __forceinline__ ChunkSource(uint32_t* c, uint32_t n_)
+        : ctr(c), n(n_), C((n_ + 63u) / 64u), part(blockIdx.x % kXcds), tries(0) {}
+    // next chunk [begin, end) or false when every part is used up
+    __device__ __forceinline__ bool Next(uint32_t& begin, uint32_t& end)
+    {
+        const uint32_t lane = threadIdx.x & 63u;
+        while(tries < kXcds)
+        {
+            const uint32_t c0 = __builtin_amdgcn_readfirstlane(uint32_t((uint64_t(C) * part) / kXcds));
+            const uint32_t c1 = __builtin_amdgcn_readfirstlane(uint32_t((uint64_t(C) * (part + 1u)) / kXcds));
+            uint32_t* fc = ctr + size_t(part) * kCtrStride;
+            uint32_t c = c1 - c0;
+            if(lane == 0)
+            {
+                c = __hip_atomic_load(fc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if(c < c1 - c0) c = atomicAdd(fc, 1u);
+            }
+            c = __builtin_amdgcn_readfirstlane(c);
+            if(c < c1 - c0)
+            {
+                begin = (c0 + c) * 64u;
+                end = min(begin + 64u, n);
+                return true;
+            }
+            part = (part + 1u) % kXcds;
+            tries++;
+        }
+        return false;
+    }
+};
+
+// Closest hit with lane refill (Aila and Laine 2009, "Understanding the
+// Efficiency of Ray Traversal on GPUs": persistent threads that fetch new
+// rays): a lane whose walk has ended takes the next ray of the wave's chunk
+// once a quarter of the wave is idle, so a wave no longer waits for the
+// longest of its 64 rays.  Each ray's walk is the same as KTrace's, so hits
+// are identical; only the order rays are walked in changes.
+constexpr uint32_t kRefillAt = 16;
+template<bool COUNT, bool BATCH>
+__global__ void __launch_bounds__(kTraceTPB)
+KTraceRefill(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __restrict__ sched,
+             const ViewGPU* __restrict__ views, uint32_t* __restrict__ ctr)
+{
+    __shared__ uint2 lstack[kLdsStack * kTraceTPB];
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t below = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    const uint32_t n = sched->activeCount;
+    if(gtid == 0) sched->extRays += n;
+    ChunkSource src(Ctr(ctr, CTR_TRACE), n);
+    uint32_t pool = 0, poolEnd = 0;        // wave-uniform: items not yet handed out
+    bool more = true;                      // wave-uniform: chunks may remain
+    bool has = false;
+    uint32_t slot = 0, visits = 0;
+    Walk w;
+    uint32_t nodes = 0, tris = 0;
+    for(;;)
+    {
+        uint64_t idle = __ballot(!has);
+        if(more && uint32_t(__popcll(idle)) >= kRefillAt)
+        {
+            while(idle != 0ull)
+            {
+                if(pool >= poolEnd)
+                {
+                    uint32_t b = 0, e = 0;
+                    more = src.Next(b, e);
+                    if(!more) break;
+                    pool = b;
+                    poolEnd = e;
+                    continue;
+                }
+                const uint32_t take = min(poolEnd - pool, uint32_t(__popcll(idle)));
+                if(!has)
+                {
+                    const uint32_t rank = uint32_t(__popcll(idle & below));
+                    if(rank < take)
+                    {
+                        slot = active[pool + rank];
+                        const float4 o = S.rayO[slot], d = S.rayD[slot];
+                        const uint32_t root = BATCH ? views[S.view[slot]].root : 0u;
+                        WalkInit(w, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z}, o.w, d.w, root, 0u);
+                        visits = 0;
+                        has = true;
+                    }
+                }
+                pool += take;
+                idle = __ballot(!has);
+            }
+        }
+        if(__ballot(has) == 0ull)
+        {
+            if(!more) break;
+            continue;
+        }
+        if(has)
+        {
+            // a walk visits at most maxVisits nodes (as TraceW)
+            bool over = WalkStep<4, false, COUNT>(ctx, lstack, gtid, w, 0u, &nodes, &tris);
+            if(++visits >= ctx.maxVisits) over = true;
+            if(over)
+            {
+                S.hit[slot] = make_uint4(w.best.prim, __float_as_uint(w.best.t),
+                                         __float_as_uint(w.best.a), __float_as_uint(w.best.b));
+                has = false;
+            }
+        }
+    }
     if(COUNT)
     {
         unsigned long long nv = nodes, nt = tris;
@@ -1092,6 +1246,7 @@ struct ShadeOut
 {
     bool shadow;
     bool query;
+    uint32_t view;            // the path's view (its cache's query queue)
     float rec[kNrcRec];
 };
 
@@ -1209,67 +1364,69 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
     V3 wO = Normalize(-rdir);
     bool specular = IsSpecular(Specularity(mat));
     bool terminal = false;   // radiance cache: ends after this vertex's NEE
-    if constexpr(!SPECTRAL)
+    if(rc.nrc.on)
     {
-        if(rc.nrc.on)
+        const NrcRenderState& nr = rc.nrc;
+        const uint32_t view = BATCH ? S.view[i] : 0u;
+        // path spread (Mueller et al. 2021, eq. 3-4): a0 at the first
+        // hit, then the sum of sqrt(d^2 / (pdf |cos|)) over later hits
+        float2 sp = S.spread[i];
+        float tHit = __uint_as_float(hh.y);
+        float cosT = MaxF(fabsf(Dot(rdir, sf.geoNormal)), 1e-6f);
+        if(depth == 0u) sp = make_float2(tHit * tHit / (4.0f * kPi * cosT), 0.0f);
+        else
         {
-            const NrcRenderState& nr = rc.nrc;
-            // path spread (Mueller et al. 2021, eq. 3-4): a0 at the first
-            // hit, then the sum of sqrt(d^2 / (pdf |cos|)) over later hits
-            float2 sp = S.spread[i];
-            float tHit = __uint_as_float(hh.y);
-            float cosT = MaxF(fabsf(Dot(rdir, sf.geoNormal)), 1e-6f);
-            if(depth == 0u) sp = make_float2(tHit * tHit / (4.0f * kPi * cosT), 0.0f);
-            else
+            float pdf = S.prevPdf[i];
+            if(pdf > 0.0f && isfinite(pdf)) sp.y += sqrtf(tHit * tHit / (pdf * cosT));
+        }
+        S.spread[i] = sp;
+        // query record: position in the view's scene box, outgoing direction
+        // and normal as (theta, phi), roughness, albedo (RGB)
+        const float4 bx = nr.box[view];
+        float rec[kNrcRec];
+        rec[0] = (sf.position.x - bx.x) * bx.w;
+        rec[1] = (sf.position.y - bx.y) * bx.w;
+        rec[2] = (sf.position.z - bx.z) * bx.w;
+        auto sph = [](V3 d, float& th, float& ph)
+        {
+            th = fm::Acos(ClampF(d.z, -1.0f, 1.0f)) * kInvPi;
+            ph = (fm::Atan2(d.y, d.x) + kPi) * (0.5f * kInvPi);
+        };
+        sph(wO, rec[3], rec[4]);
+        sph(sf.geoNormal, rec[5], rec[6]);
+        rec[7] = (mat.type == MRAY_MAT_UNREAL) ? M.roughness : (mat.type == MRAY_MAT_LAMBERT ? 1.0f : 0.0f);
+        rec[8] = M.albedo[0];
+        rec[9] = M.albedo[1];
+        rec[10] = M.albedo[2];
+        const bool training = (i & ((1u << nr.trainShift) - 1u)) == 0u;
+        if(!training && depth >= nr.minDepth && sp.y * sp.y > nr.spreadC * sp.x)
+        {
+            // terminate into the cache: a query; the path ends here, or
+            // after this vertex's NEE with neeAtEnd
+            out.query = true;
+            out.view = view;
+            for(uint32_t k = 0; k < kNrcRec; k++) out.rec[k] = rec[k];
+            if(!nr.neeAtEnd)
             {
-                float pdf = S.prevPdf[i];
-                if(pdf > 0.0f && isfinite(pdf)) sp.y += sqrtf(tHit * tHit / (pdf * cosT));
+                S.rng[i] = st;
+                S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
+                return;
             }
-            S.spread[i] = sp;
-            // query record: position in the scene box, outgoing direction
-            // and normal as (theta, phi), roughness, albedo
-            float rec[kNrcRec];
-            rec[0] = (sf.position.x - nr.boxMin[0]) * nr.boxInvExtent;
-            rec[1] = (sf.position.y - nr.boxMin[1]) * nr.boxInvExtent;
-            rec[2] = (sf.position.z - nr.boxMin[2]) * nr.boxInvExtent;
-            auto sph = [](V3 d, float& th, float& ph)
+            terminal = true;
+        }
+        if(training)
+        {
+            uint32_t ts = i >> nr.trainShift;
+            uint32_t nv = nr.trainCount[ts];
+            if(nv < nr.maxVerts)
             {
-                th = fm::Acos(ClampF(d.z, -1.0f, 1.0f)) * kInvPi;
-                ph = (fm::Atan2(d.y, d.x) + kPi) * (0.5f * kInvPi);
-            };
-            sph(wO, rec[3], rec[4]);
-            sph(sf.geoNormal, rec[5], rec[6]);
-            rec[7] = (mat.type == MRAY_MAT_UNREAL) ? M.roughness : (mat.type == MRAY_MAT_LAMBERT ? 1.0f : 0.0f);
-            rec[8] = M.albedo[0];
-            rec[9] = M.albedo[1];
-            rec[10] = M.albedo[2];
-            const bool training = (i & ((1u << nr.trainShift) - 1u)) == 0u;
-            if(!training && depth >= nr.minDepth && sp.y * sp.y > nr.spreadC * sp.x)
-            {
-                // terminate into the cache: a query; the path ends here, or
-                // after this vertex's NEE with neeAtEnd
-                out.query = true;
-                for(uint32_t k = 0; k < kNrcRec; k++) out.rec[k] = rec[k];
-                if(!nr.neeAtEnd)
-                {
-                    S.rng[i] = st;
-                    S.meta[i] = MPack(depth, ST_DEAD, type, dispersed, false);
-                    return;
-                }
-                terminal = true;
-            }
-            if(training)
-            {
-                uint32_t ts = i >> nr.trainShift;
-                uint32_t nv = nr.trainCount[ts];
-                if(nv < nr.maxVerts)
-                {
-                    float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + nv) * kNrcVert;
-                    for(uint32_t k = 0; k < kNrcRec; k++) vx[k] = rec[k];
-                    vx[kNrcRec + 0] = thr.x; vx[kNrcRec + 1] = thr.y; vx[kNrcRec + 2] = thr.z;
-                    vx[kNrcRec + 3] = rad.x; vx[kNrcRec + 4] = rad.y; vx[kNrcRec + 5] = rad.z;
-                    nr.trainCount[ts] = nv + 1u;
-                }
+                float* vx = nr.trainStore + (size_t(ts) * nr.maxVerts + nv) * kNrcVert;
+                for(uint32_t k = 0; k < kNrcRec; k++) vx[k] = rec[k];
+                vx[kNrcRec + 0] = thr.x; vx[kNrcRec + 1] = thr.y;
+                vx[kNrcRec + 2] = thr.z; vx[kNrcRec + 3] = thr.w;
+                vx[kNrcRec + 4] = rad.x; vx[kNrcRec + 5] = rad.y;
+                vx[kNrcRec + 6] = rad.z; vx[kNrcRec + 7] = rad.w;
+                nr.trainCount[ts] = nv + 1u;
             }
         }
     }
@@ -1346,7 +1503,7 @@ __device__ __forceinline__ void ShadeSlot(const SlotSoA& S, const RenderConst& r
     {
         V4 nt = DivideByPDF(t3, bs.pdf);
         S.thr[i] = make_float4(nt.x, nt.y, nt.z, nt.w);
-        if(rc.mode == MRAY_SAMPLE_NEE_MIS || (!SPECTRAL && rc.nrc.on)) S.prevPdf[i] = bs.pdf;
+        if(rc.mode == MRAY_SAMPLE_NEE_MIS || rc.nrc.on) S.prevPdf[i] = bs.pdf;
         V3 nn = bs.passedThrough ? -sf.geoNormal : sf.geoNormal;
         V3 np = Nudge(bs.wIPos, nn);
         S.rayO[i] = make_float4(np.x, np.y, np.z, kLargeEps);
@@ -1376,18 +1533,25 @@ __device__ __forceinline__ void ShadeAppend(const ShadeOut& o, uint32_t i, uint3
     }
     if(queries)
     {
-        const uint64_t qm = __ballot(o.query);
-        if(qm)
+        // queries go to their view's queue: one pass per distinct view in
+        // the wave (slots of a wave hold one or two views' paths)
+        uint64_t pending = __ballot(o.query);
+        while(pending)
         {
+            const int leader = __ffsll((unsigned long long)pending) - 1;
+            const uint32_t v = uint32_t(__shfl(int(o.view), leader));
+            const bool mine = o.query && o.view == v;
+            const uint64_t qm = __ballot(mine);
             uint32_t b = 0;
-            if(lane == 0) b = atomicAdd(&sched->queryCount, uint32_t(__popcll(qm)));
-            b = __shfl(b, 0);
-            if(o.query)
+            if(lane == uint32_t(leader)) b = atomicAdd(Ctr(ctr, CTR_NRCQ + v), uint32_t(__popcll(qm)));
+            b = uint32_t(__shfl(int(b), leader));
+            if(mine)
             {
-                const uint32_t q = b + uint32_t(__popcll(qm & lt));
-                for(uint32_t k = 0; k < kNrcRec; k++) nr.queryRec[size_t(q) * kNrcRec + k] = o.rec[k];
+                const size_t q = size_t(v) * nr.queryCap + b + uint32_t(__popcll(qm & lt));
+                for(uint32_t k = 0; k < kNrcRec; k++) nr.queryRec[q * kNrcRec + k] = o.rec[k];
                 nr.querySlot[q] = i;
             }
+            pending &= ~qm;
         }
     }
 }
@@ -1402,7 +1566,7 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
 {
     const uint32_t n = sched->activeCount;
     const uint32_t cap = ShadowShardCap(rc.N);
-    const bool queries = !SPECTRAL && rc.nrc.on;
+    const bool queries = rc.nrc.on;
     for(uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n;
         base += gridDim.x * blockDim.x)
     {
@@ -1410,6 +1574,7 @@ KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
         ShadeOut o;
         o.shadow = false;
         o.query = false;
+        o.view = 0;
         uint32_t i = 0;
         if(k < n)
         {
@@ -1471,7 +1636,7 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues,
         Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ, uint32_t* __restrict__ ctr)
 {
     const uint32_t cap = ShardCap(rc.N), scap = ShadowShardCap(rc.N);
-    const bool queries = !SPECTRAL && rc.nrc.on;
+    const bool queries = rc.nrc.on;
     Shards g;
     LoadShards(ctr, CTR_QCOUNT + uint32_t(KEY) * kShards, g);
     const uint32_t n = g.pre[kShards];
@@ -1488,6 +1653,7 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues,
         ShadeOut o;
         o.shadow = false;
         o.query = false;
+        o.view = 0;
         uint32_t i = 0, sh = 0;
         if(k < n)
         {
@@ -1544,16 +1710,17 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
             S.rad[i] = make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
             // a training path's vertex recorded this bounce (its last) counts
             // its visible NEE as radiance gathered before the cache's part
-            if(!BATCH && nrc.neeAtEnd && (i & ((1u << nrc.trainShift) - 1u)) == 0u)
+            if(nrc.on && nrc.neeAtEnd && (i & ((1u << nrc.trainShift) - 1u)) == 0u)
             {
                 const uint32_t ts = i >> nrc.trainShift;
                 const uint32_t nv = nrc.trainCount[ts];
                 if(nv > 0)
                 {
                     float* vx = nrc.trainStore + (size_t(ts) * nrc.maxVerts + nv - 1u) * kNrcVert;
-                    vx[kNrcRec + 3] += b.x;
-                    vx[kNrcRec + 4] += b.y;
-                    vx[kNrcRec + 5] += b.z;
+                    vx[kNrcRec + 4] += b.x;
+                    vx[kNrcRec + 5] += b.y;
+                    vx[kNrcRec + 6] += b.z;
+                    vx[kNrcRec + 7] += b.w;
                 }
             }
         }
@@ -1574,23 +1741,46 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
     }
 }
 
-// Radiance-cache answers: a terminated path adds throughput x max(L, 0).
+// Radiance-cache answers: a terminated path adds throughput x max(L, 0);
+// in spectral mode the cache's RGB becomes the path's four wavelengths as an
+// emitter's RGB radiance does (ConvRadiance).  Every view's queue in turn.
+template<bool SPECTRAL>
 __global__ void __launch_bounds__(256)
-KNrcApply(SlotSoA S, RenderConst rc, Sched* __restrict__ sched)
+KNrcApply(SlotSoA S, RenderConst rc, Sched* __restrict__ sched, uint32_t* __restrict__ ctr)
 {
-    uint32_t n = sched->queryCount;
-    uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if(q == 0) sched->queries += n;
-    for(; q < n; q += gridDim.x * blockDim.x)
+    const NrcRenderState& nr = rc.nrc;
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long total = 0;
+    for(uint32_t v = 0; v < nr.views; v++)
     {
-        uint32_t i = rc.nrc.querySlot[q];
-        const float* L = rc.nrc.queryOut + size_t(q) * 3u;
-        float4 t = S.thr[i], r = S.rad[i];
-        r.x += t.x * MaxF(L[0], 0.0f);
-        r.y += t.y * MaxF(L[1], 0.0f);
-        r.z += t.z * MaxF(L[2], 0.0f);
-        S.rad[i] = r;
+        const uint32_t n = *Ctr(ctr, CTR_NRCQ + v);
+        total += n;
+        for(uint32_t q = t0; q < n; q += gridDim.x * blockDim.x)
+        {
+            const size_t e = size_t(v) * nr.queryCap + q;
+            const uint32_t i = nr.querySlot[e];
+            const float* L = nr.queryOut + e * 3u;
+            const V3 l = {MaxF(L[0], 0.0f), MaxF(L[1], 0.0f), MaxF(L[2], 0.0f)};
+            float4 t = S.thr[i], r = S.rad[i];
+            if constexpr(SPECTRAL)
+            {
+                const float4 w4 = S.waves[i];
+                const V4 ls = ConvRadiance<true>(rc, l, V4{w4.x, w4.y, w4.z, w4.w});
+                r.x += t.x * ls.x;
+                r.y += t.y * ls.y;
+                r.z += t.z * ls.z;
+                r.w += t.w * ls.w;
+            }
+            else
+            {
+                r.x += t.x * l.x;
+                r.y += t.y * l.y;
+                r.z += t.z * l.z;
+            }
+            S.rad[i] = r;
+        }
     }
+    if(t0 == 0) sched->queries += total;
 }
 
 __global__ void KZeroFloats(float* __restrict__ p, size_t n)
@@ -1624,8 +1814,6 @@ __global__ void KResetSlots(uint32_t* __restrict__ meta, uint32_t* __restrict__ 
         sched->fill = 0;
         sched->activeCount = 0;
         sched->iterations = 0;
-        sched->queryCount = 0;
-        sched->ringCount = 0;
         sched->queries = 0;
         sched->startClock = 0;
         sched->viewsDone = 0;
@@ -1678,7 +1866,6 @@ struct MRayRenderer
     double phaseMs[4] = {0, 0, 0, 0};
     uint64_t phaseLaunches[4] = {0, 0, 0, 0};
     // radiance cache (mray_renderer_attach_nrc)
-    MRayNrc* nrc = nullptr;
     MRayNrcRenderParams nrcParams{};
     DeviceArena nrcArena;
     uint32_t nrcLog2G = 0;        // lanes per training path in KNrcEmit (log2)
@@ -1697,8 +1884,13 @@ struct MRayRenderer
     bool batch = false;
     const MRayBatchScene* bscene = nullptr;
     std::vector<ViewGPU> hViews;
+    std::vector<uint32_t> hViewScene;   // merged batch: each view's scene part
     ViewGPU* dViews = nullptr;
     unsigned long long* dViewBegin = nullptr;
+    unsigned long long* dViewPix = nullptr;
+    // radiance caches: one, or one per batch view (mray_renderer_attach_nrcs)
+    std::vector<MRayNrc*> caches;
+    float4* dNrcBox = nullptr;
     unsigned long long* dViewClock = nullptr;
     uint32_t traceTPB = kTraceTPB;
 };
@@ -1782,6 +1974,8 @@ static void LaunchTrace(MRayRenderer* r, hipStream_t s)
         }
     }
     if(ctx.width == 8) KTrace<8, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
+    else if(r->params.flags & MRAY_RENDER_LANE_REFILL)
+        KTraceRefill<COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
     else KTrace<4, COUNT, BATCH><<<nb, kTraceTPB, 0, s>>>(ctx, r->S, r->dActive, r->dSched, r->dViews, r->dCtr);
 }
 
@@ -1822,14 +2016,19 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
     int mk;
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     KRefillCount<BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S.meta, r->S.view, N, r->dBlockCnt);
+    // per-view latency needs views issued in order: not in frame order
     KRefillScan<<<1, 1024, 0, s>>>(r->dBlockCnt, r->nBlocks, r->dSched, r->rc.viewBegin,
-                                   r->rc.viewCount, BATCH ? r->dViewClock : nullptr, r->dCtr);
-    if(!BATCH && r->nrc)
+                                   r->rc.viewCount,
+                                   (BATCH && !r->rc.frameMajor) ? r->dViewClock : nullptr, r->dCtr);
+    if(r->rc.nrc.on)
     {
         // training pairs of finished training paths, before the refill reuses their slots
         const NrcRenderState& nr = r->rc.nrc;
         const uint64_t lanes = (uint64_t((N - 1u) >> nr.trainShift) + 1u) << r->nrcLog2G;
-        KNrcEmit<<<DivUp(lanes, 256), 256, 0, s>>>(r->S, r->rc, r->dSched, r->nrcLog2G);
+        if(r->rc.spectral)
+            KNrcEmit<true, BATCH><<<DivUp(lanes, 256), 256, 0, s>>>(r->S, r->rc, r->dSched, r->nrcLog2G);
+        else
+            KNrcEmit<false, BATCH><<<DivUp(lanes, 256), 256, 0, s>>>(r->S, r->rc, r->dSched, r->nrcLog2G);
     }
     if(r->rc.spectral)
         KRefillApply<true, BATCH><<<r->nBlocks, kSlotTPB, 0, s>>>(r->S, r->rc, r->dBlockCnt, r->dSched,
@@ -1858,13 +2057,20 @@ static int EnqueueIterationT(MRayRenderer* r, hipStream_t s)
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     if(r->counting) LaunchShadow<true, BATCH, INST>(r, s);
     else LaunchShadow<false, BATCH, INST>(r, s);
-    if(!BATCH && r->nrc)
+    if(r->rc.nrc.on)
     {
-        // radiance-cache answers for the paths that terminated this bounce
-        const uint32_t* dCount = reinterpret_cast<const uint32_t*>(
-            reinterpret_cast<const char*>(r->dSched) + offsetof(Sched, queryCount));
-        MRAY_TRY_IT(mray_nrc_infer(r->nrc, r->rc.nrc.queryRec, 0, dCount, r->rc.nrc.queryOut, s));
-        KNrcApply<<<r->nBlocks, 256, 0, s>>>(r->S, r->rc, r->dSched);
+        // radiance-cache answers for the paths that terminated this bounce:
+        // each view's queue through its own cache, counts read on the device
+        const NrcRenderState& nr = r->rc.nrc;
+        for(uint32_t v = 0; v < nr.views; v++)
+        {
+            const uint32_t* dCount = r->dCtr + size_t(CTR_NRCQ + v) * kCtrStride;
+            const size_t e = size_t(v) * nr.queryCap;
+            MRAY_TRY_IT(mray_nrc_infer(r->caches[v], nr.queryRec + e * kNrcRec, 0, dCount,
+                                       nr.queryOut + e * 3u, s));
+        }
+        if(r->rc.spectral) KNrcApply<true><<<r->nBlocks, 256, 0, s>>>(r->S, r->rc, r->dSched, r->dCtr);
+        else KNrcApply<false><<<r->nBlocks, 256, 0, s>>>(r->S, r->rc, r->dSched, r->dCtr);
     }
     if((mk = Mark(r, s)) != MRAY_OK) return mk;
     MRAY_HIP_CHECK(hipGetLastError());
@@ -1895,6 +2101,12 @@ static int ResetRenderer(MRayRenderer* r)
     {
         const size_t V = r->hViews.size();
         KZeroFloats<<<DivUp(2 * V, 256), 256, 0, s>>>(reinterpret_cast<float*>(r->dViewClock), 2 * V);
+    }
+    if(r->rc.nrc.on)
+    {
+        // training pairs counted since the reset (u64 per view)
+        const size_t V = r->rc.nrc.views;
+        KZeroFloats<<<1, 256, 0, s>>>(reinterpret_cast<float*>(r->rc.nrc.ringCount), 2 * V);
     }
     MRAY_HIP_CHECK(hipGetLastError());
     MRAY_HIP_CHECK(hipStreamSynchronize(s));
@@ -2018,6 +2230,7 @@ static int SizeLaunches(MRayRenderer* r)
     const uint32_t tpb = r->traceTPB;
     const uint32_t maxTraceBlocks = r->ctx.gridThreads / tpb;
     const bool w8 = r->ctx.width == 8;
+    const bool refill = (p->flags & MRAY_RENDER_LANE_REFILL) != 0u;
     if(r->iscene)
     {
         r->traceBlocks = uint32_t(resident(KTrace<4, false, false, kTraceTPB, true>, tpb, maxTraceBlocks));
@@ -2026,7 +2239,8 @@ static int SizeLaunches(MRayRenderer* r)
     else if(!r->batch)
     {
         r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false, false>, tpb, maxTraceBlocks)
-                                     : resident(KTrace<4, false, false>, tpb, maxTraceBlocks));
+                                : refill ? resident(KTraceRefill<false, false>, tpb, maxTraceBlocks)
+                                         : resident(KTrace<4, false, false>, tpb, maxTraceBlocks));
         r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8, false>, tpb, maxTraceBlocks)
                                       : resident(KShadow<4, false>, tpb, maxTraceBlocks));
     }
@@ -2040,7 +2254,8 @@ static int SizeLaunches(MRayRenderer* r)
     else
     {
         r->traceBlocks = uint32_t(w8 ? resident(KTrace<8, false, true>, tpb, maxTraceBlocks)
-                                     : resident(KTrace<4, false, true>, tpb, maxTraceBlocks));
+                                : refill ? resident(KTraceRefill<false, true>, tpb, maxTraceBlocks)
+                                         : resident(KTrace<4, false, true>, tpb, maxTraceBlocks));
         r->shadowBlocks = uint32_t(w8 ? resident(KShadow<8, true>, tpb, maxTraceBlocks)
                                       : resident(KShadow<4, true>, tpb, maxTraceBlocks));
     }
@@ -2221,6 +2436,7 @@ static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayInstS
     {
         const MRayBatchView& v = views[k];
         const MRayBatchScene::Part& part = is ? instPart : b->parts[v.scene];
+        r->hViewScene.push_back(is ? 0u : v.scene);
         ViewGPU& g = r->hViews[k];
         g = ViewGPU{};
         BuildCamera(v.useCamera ? v.camera : part.camera, v.width, v.height, fr, g.cam);
@@ -2269,6 +2485,16 @@ static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayInstS
     }
     r->rc.viewBegin = r->dViewBegin;
     r->rc.viewClock = r->dViewClock;
+    {
+        std::vector<unsigned long long> pix(viewCount + 1, 0ull);
+        for(uint32_t k = 0; k < viewCount; k++) pix[k + 1] = pix[k] + r->hViews[k].P;
+        MRAY_TRY(DevAlloc(ar, (void**)&r->dViewPix, (viewCount + 1) * sizeof(unsigned long long)));
+        MRAY_HIP_CHECK(hipMemcpy(r->dViewPix, pix.data(), pix.size() * sizeof(unsigned long long),
+                                 hipMemcpyHostToDevice));
+        r->rc.viewPix = r->dViewPix;
+        r->rc.pixTotal = pix[viewCount];
+        r->rc.frameMajor = 0;
+    }
     if(is) MRAY_TRY(SetupLightPower(r, {0u}, {is->view.lightCount}, is->view.lightCount));
     else
     {
@@ -2374,9 +2600,13 @@ int mray_renderer_destroy(MRayRenderer* r)
     return MRAY_OK;
 }
 
+static int SetFrameOrder(MRayRenderer* r, bool on);
+
 int mray_renderer_reset(MRayRenderer* r)
 {
     if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }
+    // a batch renderer returns to view order (every view's spp, views in turn)
+    if(r->batch) MRAY_TRY(SetFrameOrder(r, false));
     return ResetRenderer(r);
 }
 
@@ -2488,6 +2718,11 @@ int mray_renderer_stats(MRayRenderer* r, MRayRenderStats* out)
 int mray_renderer_render_all(MRayRenderer* r, MRayRenderStats* hStats)
 {
     if(!r) { SetError("renderer is NULL"); return MRAY_ERR_INVALID; }
+    if(r->rc.frameMajor)
+    {
+        SetError("render_all: this batch renderer renders by frames; reset it first");
+        return MRAY_ERR_INVALID;
+    }
     MRAY_TRY(RunToLimit(r, r->limit, "render_all"));
     MRayRenderStats total{};
     MRAY_TRY(mray_renderer_stats(r, &total));
@@ -2547,26 +2782,59 @@ int mray_renderer_film_device(MRayRenderer* r, float** dFilm)
     return MRAY_OK;
 }
 
-int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderParams* p)
+// Scene box of the view's scene (a one-scene renderer: its scene), mapped to
+// [0,1]^3 with one scale (keeps the aspect): (min xyz, 1 / extent).
+static float4 NrcBox(const MRayRenderer* r, uint32_t view, int& rv)
+{
+    float box[6] = {0, 0, 0, 1, 1, 1};
+    rv = MRAY_OK;
+    if(r->iscene) std::memcpy(box, r->iscene->box, sizeof(box));
+    else if(r->bscene)
+    {
+        const uint32_t part = r->hViewScene[view];
+        std::memcpy(box, r->bscene->parts[part].box, sizeof(box));
+    }
+    else if(hipMemcpy(box, r->accel->dSceneBox, sizeof(box), hipMemcpyDeviceToHost) != hipSuccess)
+    {
+        SetError("attach_nrc: scene box read failed");
+        rv = MRAY_ERR_HIP;
+    }
+    float ext = std::max({box[3] - box[0], box[4] - box[1], box[5] - box[2], 1e-6f});
+    return make_float4(box[0] - 1e-3f * ext, box[1] - 1e-3f * ext, box[2] - 1e-3f * ext,
+                       1.0f / (ext * 1.002f));
+}
+
+int mray_renderer_attach_nrcs(MRayRenderer* r, MRayNrc* const* caches, uint32_t count,
+                              const MRayNrcRenderParams* p)
 {
     if(!r || !p) { SetError("attach_nrc: NULL argument"); return MRAY_ERR_INVALID; }
     MRAY_HIP_CHECK(hipDeviceSynchronize());
     if(r->graph) { (void)hipGraphExecDestroy(r->graph); r->graph = nullptr; }
-    if(!nrc)
+    if(count == 0 || !caches)
     {
-        r->nrc = nullptr;
-        r->rc.nrc.on = 0;
+        r->caches.clear();
+        r->rc.nrc = NrcRenderState{};
         r->nrcArena.Release();
         return MRAY_OK;
     }
-    if(r->batch)
+    const uint32_t views = r->batch ? uint32_t(r->hViews.size()) : 1u;
+    if(count != views)
     {
-        SetError("attach_nrc: a batch renderer has no radiance cache");
+        SetError("attach_nrc: " + std::to_string(count) + " caches for " + std::to_string(views) +
+                 " view(s): one cache per view");
+        return MRAY_ERR_INVALID;
+    }
+    if(views > kMaxNrcViews)
+    {
+        SetError("attach_nrc: at most " + std::to_string(kMaxNrcViews) + " views carry caches");
         return MRAY_ERR_UNSUPPORTED;
     }
-    if(r->rc.spectral)
+    for(uint32_t v = 0; v < count; v++)
+        if(!caches[v]) { SetError("attach_nrc: NULL cache"); return MRAY_ERR_INVALID; }
+    if(r->dQueues)
     {
-        SetError("attach_nrc: the radiance cache runs with PathTracerRGB only");
+        SetError("attach_nrc: the radiance cache runs with the generic shading kernel "
+                 "(no material queues)");
         return MRAY_ERR_UNSUPPORTED;
     }
     if(p->trainShift > 16 || p->trainBatch == 0 || (p->trainBatch % 32u) != 0 || p->trainSteps == 0 ||
@@ -2595,36 +2863,64 @@ int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderP
     nr.spreadC = p->spreadThreshold;
     nr.minDepth = std::max(1u, p->minDepth);
     nr.neeAtEnd = (p->flags & MRAY_NRC_NEE_AT_END) ? 1u : 0u;
+    nr.views = views;
+    nr.queryCap = N;
     auto A = [&](void** ptr, size_t bytes) { return DevAlloc(r->nrcArena, ptr, bytes); };
     MRAY_TRY_IT(A((void**)&nr.trainStore, size_t(slots) * nr.maxVerts * kNrcVert * 4));
     MRAY_TRY_IT(A((void**)&nr.trainCount, size_t(slots) * 4));
-    MRAY_TRY_IT(A((void**)&nr.queryRec, size_t(N) * kNrcRec * 4));
-    MRAY_TRY_IT(A((void**)&nr.querySlot, size_t(N) * 4));
-    MRAY_TRY_IT(A((void**)&nr.queryOut, size_t(N) * 3 * 4));
-    MRAY_TRY_IT(A((void**)&nr.ringRec, size_t(ring) * kNrcRec * 4));
-    MRAY_TRY_IT(A((void**)&nr.ringTgt, size_t(ring) * 3 * 4));
+    MRAY_TRY_IT(A((void**)&nr.queryRec, size_t(views) * N * kNrcRec * 4));
+    MRAY_TRY_IT(A((void**)&nr.querySlot, size_t(views) * N * 4));
+    MRAY_TRY_IT(A((void**)&nr.queryOut, size_t(views) * N * 3 * 4));
+    MRAY_TRY_IT(A((void**)&nr.ringRec, size_t(views) * ring * kNrcRec * 4));
+    MRAY_TRY_IT(A((void**)&nr.ringTgt, size_t(views) * ring * 3 * 4));
+    MRAY_TRY_IT(A((void**)&nr.ringCount, size_t(views) * sizeof(unsigned long long)));
+    MRAY_TRY_IT(A((void**)&r->dNrcBox, size_t(views) * sizeof(float4)));
     MRAY_TRY_IT(A((void**)&r->S.spread, size_t(N) * sizeof(float2)));
     MRAY_HIP_CHECK(hipMemset(nr.trainCount, 0, size_t(slots) * 4));
-    MRAY_HIP_CHECK(hipMemset(nr.ringTgt, 0, size_t(ring) * 3 * 4));
-    MRAY_HIP_CHECK(hipMemset(nr.ringRec, 0, size_t(ring) * kNrcRec * 4));
-    // scene box -> [0,1]^3 with one scale (keeps the aspect)
-    float box[6];
-    if(r->iscene) std::memcpy(box, r->iscene->box, sizeof(box));
-    else MRAY_HIP_CHECK(hipMemcpy(box, r->accel->dSceneBox, sizeof(box), hipMemcpyDeviceToHost));
-    float ext = std::max({box[3] - box[0], box[4] - box[1], box[5] - box[2], 1e-6f});
-    for(int k = 0; k < 3; k++) nr.boxMin[k] = box[k] - 1e-3f * ext;
-    nr.boxInvExtent = 1.0f / (ext * 1.002f);
+    MRAY_HIP_CHECK(hipMemset(nr.ringTgt, 0, size_t(views) * ring * 3 * 4));
+    MRAY_HIP_CHECK(hipMemset(nr.ringRec, 0, size_t(views) * ring * kNrcRec * 4));
+    MRAY_HIP_CHECK(hipMemset(nr.ringCount, 0, size_t(views) * sizeof(unsigned long long)));
+    std::vector<float4> boxes(views);
+    for(uint32_t v = 0; v < views; v++)
+    {
+        int rv;
+        boxes[v] = NrcBox(r, v, rv);
+        if(rv != MRAY_OK) return rv;
+    }
+    MRAY_HIP_CHECK(hipMemcpy(r->dNrcBox, boxes.data(), views * sizeof(float4), hipMemcpyHostToDevice));
+    nr.box = r->dNrcBox;
     nr.on = 1;
-    r->nrc = nrc;
+    r->caches.assign(caches, caches + count);
     r->nrcParams = *p;
+    return MRAY_OK;
+}
+
+int mray_renderer_attach_nrc(MRayRenderer* r, MRayNrc* nrc, const MRayNrcRenderParams* p)
+{
+    return mray_renderer_attach_nrcs(r, nrc ? &nrc : nullptr, nrc ? 1u : 0u, p);
+}
+
+// A batch renderer switches to frame order at its first frame (and back to
+// view order at reset): the captured bursts hold the old constants.
+static int SetFrameOrder(MRayRenderer* r, bool on)
+{
+    if(bool(r->rc.frameMajor) == on) return MRAY_OK;
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    if(r->graph) { (void)hipGraphExecDestroy(r->graph); r->graph = nullptr; }
+    r->rc.frameMajor = on ? 1u : 0u;
     return MRAY_OK;
 }
 
 int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* hStats)
 {
     if(!r || spp == 0) { SetError("render_frame: NULL renderer or spp == 0"); return MRAY_ERR_INVALID; }
-    if(r->batch) { SetError("render_frame: not for a batch renderer"); return MRAY_ERR_UNSUPPORTED; }
     hipStream_t s = r->stream;
+    if(r->batch && !r->rc.frameMajor)
+    {
+        // every frame renders `spp` samples of every view: restart in frame order
+        MRAY_TRY_IT(SetFrameOrder(r, true));
+        MRAY_TRY_IT(ResetRenderer(r));
+    }
     r->frameLimit += (unsigned long long)spp * r->rc.P;
     KSetLimit<<<1, 1, 0, s>>>(r->dSched, r->frameLimit);
     MRAY_HIP_CHECK(hipGetLastError());
@@ -2633,16 +2929,23 @@ int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* h
     MRAY_TRY_IT(mray_renderer_stats(r, &total));
     total.hostWaits = r->hostWaits;
     total.deviceDrains = 1;
-    if(r->nrc)
+    if(r->rc.nrc.on)
     {
-        // online training on the most recent training pairs
+        // online training of every view's cache on its most recent pairs
         const NrcRenderState& nr = r->rc.nrc;
-        unsigned long long have = std::min<unsigned long long>(r->hSched->ringCount, nr.ringSize);
-        uint32_t B = r->nrcParams.trainBatch;
-        uint32_t steps = uint32_t(std::min<unsigned long long>(r->nrcParams.trainSteps, have / B));
-        for(uint32_t k = 0; k < steps; k++)
-            MRAY_TRY_IT(mray_nrc_train(r->nrc, nr.ringRec + size_t(k) * B * kNrcRec,
-                                       nr.ringTgt + size_t(k) * B * 3, B, s));
+        std::vector<unsigned long long> made(nr.views);
+        MRAY_HIP_CHECK(hipMemcpy(made.data(), nr.ringCount, nr.views * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost));
+        const uint32_t B = r->nrcParams.trainBatch;
+        for(uint32_t v = 0; v < nr.views; v++)
+        {
+            unsigned long long have = std::min<unsigned long long>(made[v], nr.ringSize);
+            uint32_t steps = uint32_t(std::min<unsigned long long>(r->nrcParams.trainSteps, have / B));
+            const size_t base = size_t(v) * nr.ringSize;
+            for(uint32_t k = 0; k < steps; k++)
+                MRAY_TRY_IT(mray_nrc_train(r->caches[v], nr.ringRec + (base + size_t(k) * B) * kNrcRec,
+                                           nr.ringTgt + (base + size_t(k) * B) * 3, B, s));
+        }
         MRAY_HIP_CHECK(hipStreamSynchronize(s));
     }
     total.finished = 1;
@@ -2656,7 +2959,15 @@ int mray_renderer_cache_stats(MRayRenderer* r, uint64_t* hQueries, uint64_t* hTr
     MRAY_HIP_CHECK(hipDeviceSynchronize());
     MRAY_HIP_CHECK(hipMemcpy(r->hSched, r->dSched, sizeof(Sched), hipMemcpyDeviceToHost));
     *hQueries = r->hSched->queries;
-    *hTrainingPairs = r->hSched->ringCount;
+    *hTrainingPairs = 0;
+    const NrcRenderState& nr = r->rc.nrc;
+    if(nr.on)
+    {
+        std::vector<unsigned long long> made(nr.views);
+        MRAY_HIP_CHECK(hipMemcpy(made.data(), nr.ringCount, nr.views * sizeof(unsigned long long),
+                                 hipMemcpyDeviceToHost));
+        for(unsigned long long m : made) *hTrainingPairs += m;
+    }
     return MRAY_OK;
 }
 

@@ -120,6 +120,7 @@ struct MRayBatchScene
         float boundaryRad[3] = {0, 0, 0};
         float sceneDiameter = 0;
         MRayCamera camera{};
+        float box[6] = {0, 0, 0, 0, 0, 0};   // scene box (min xyz, max xyz)
         // textured Skysphere of the scene (the merged scene's own copy)
         mray::SceneView env{};
     };

@@ -35,7 +35,8 @@ EXPORTS = [
     "mray_nrc_create", "mray_nrc_destroy", "mray_nrc_infer", "mray_nrc_backward", "mray_nrc_adam",
     "mray_nrc_train", "mray_nrc_last_loss", "mray_nrc_param_counts", "mray_nrc_read",
     "mray_nrc_write", "mray_nrc_zero_grads", "mray_nrc_set_profiling", "mray_nrc_kernel_times",
-    "mray_renderer_attach_nrc", "mray_renderer_render_frame", "mray_renderer_cache_stats",
+    "mray_renderer_attach_nrc", "mray_renderer_attach_nrcs", "mray_renderer_render_frame",
+    "mray_renderer_cache_stats",
     "mray_device_memory",
     "mray_batch_scene_create", "mray_batch_scene_destroy", "mray_batch_scene_info",
     "mray_batch_create", "mray_batch_read_film", "mray_batch_view_times", "mray_batch_autotune",
@@ -112,6 +113,7 @@ def lib():
     L.mray_nrc_set_profiling.argtypes = [P, u32]
     L.mray_nrc_kernel_times.argtypes = [P, P, P]
     L.mray_renderer_attach_nrc.argtypes = [P, P, C.POINTER(abi.MRayNrcRenderParams)]
+    L.mray_renderer_attach_nrcs.argtypes = [P, P, u32, P]
     L.mray_renderer_render_frame.argtypes = [P, u32, C.POINTER(abi.MRayRenderStats)]
     L.mray_renderer_cache_stats.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]
     L.mray_renderer_set_measurement.argtypes = [P, u32, u32]
@@ -427,17 +429,31 @@ class Renderer:
                                               pmf.ctypes.data))
         return prob, alias, pmf
 
-    def attach_nrc(self, nrc, train_shift=4, spread=0.01, batch=1 << 14, steps=4, min_depth=1,
-                   nee_at_end=False):
-        """Terminate paths into `nrc` and train it after every frame (RGB only);
-        nrc=None detaches."""
+    @staticmethod
+    def _nrc_params(train_shift, spread, batch, steps, min_depth, nee_at_end):
         p = abi.MRayNrcRenderParams()
         p.trainShift, p.spreadThreshold, p.trainBatch, p.trainSteps = train_shift, spread, batch, steps
         p.minDepth = min_depth
         p.flags = abi.NRC_NEE_AT_END if nee_at_end else 0
+        return p
+
+    def attach_nrc(self, nrc, train_shift=4, spread=0.01, batch=1 << 14, steps=4, min_depth=1,
+                   nee_at_end=False):
+        """Terminate paths into `nrc` and train it after every frame;
+        nrc=None detaches."""
+        p = self._nrc_params(train_shift, spread, batch, steps, min_depth, nee_at_end)
         self.nrc = nrc
         check(lib().mray_renderer_attach_nrc(self.h, nrc.h if nrc is not None else None,
                                              C.byref(p)))
+
+    def attach_nrcs(self, caches, train_shift=4, spread=0.01, batch=1 << 14, steps=4,
+                    min_depth=1, nee_at_end=False):
+        """One cache per view (a batch renderer's views in order); [] detaches."""
+        p = self._nrc_params(train_shift, spread, batch, steps, min_depth, nee_at_end)
+        self.caches = list(caches)
+        arr = (C.c_void_p * max(1, len(caches)))(*[c.h.value for c in caches])
+        check(lib().mray_renderer_attach_nrcs(self.h, arr if caches else None, len(caches),
+                                              C.byref(p)))
 
     def render_frame(self, spp=1):
         st = abi.MRayRenderStats()

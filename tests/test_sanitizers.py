@@ -12,6 +12,7 @@
 The builds are the `sanitize` targets of mray_amd/host/Makefile and
 oracle/Makefile (g++; no GPU code).
 """
+import fcntl
 import os
 import subprocess
 import sys
@@ -29,10 +30,19 @@ ENV = dict(os.environ, ASAN_OPTIONS="detect_leaks=0:abort_on_error=0",
 REPORT = ("ERROR: AddressSanitizer", "runtime error:", "ERROR: LeakSanitizer")
 
 
+def _make(target_dir):
+    """`make sanitize` under a file lock: parallel test workers share the
+    build directory."""
+    os.makedirs(SAN, exist_ok=True)
+    with open(os.path.join(SAN, ".lock"), "w") as lock:
+        fcntl.flock(lock, fcntl.LOCK_EX)
+        subprocess.run(["make", "-s", "-C", target_dir, "sanitize"], check=True,
+                       capture_output=True, timeout=600)
+
+
 @pytest.fixture(scope="module")
 def san_driver():
-    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "mray_amd", "host"), "sanitize"],
-                   check=True, capture_output=True, timeout=600)
+    _make(os.path.join(ROOT, "mray_amd", "host"))
     return os.path.join(SAN, "tracer_driver"), os.path.join(SAN, "libmray_tracer.so")
 
 
@@ -122,8 +132,7 @@ print("oracle-ok")
 
 
 def test_oracle_under_sanitizers():
-    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "sanitize"], check=True,
-                   capture_output=True, timeout=600)
+    _make(os.path.join(ROOT, "oracle"))
     asan = subprocess.run(["g++", "-print-file-name=libasan.so"], capture_output=True,
                           text=True).stdout.strip()
     ubsan = subprocess.run(["g++", "-print-file-name=libubsan.so"], capture_output=True,
