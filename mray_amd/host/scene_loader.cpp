@@ -246,7 +246,7 @@ SceneIds LoadSceneMRay(TracerI& tracer, const std::string& path)
             // area-weighted face normals (as the reference does without normals)
             for(const Vector3ui& t : m.idx)
             {
-                const float* p0 = m.pos[t.v[0]].v; const float* p1 = m.pos[t.v[1]].v; const float* p2 = m.pos[t.v[2]].v;
+                const float* p0 = m.pos[t.v[0]].v.data(); const float* p1 = m.pos[t.v[1]].v.data(); const float* p2 = m.pos[t.v[2]].v.data();
                 float e0[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
                 float e1[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
                 float c[3] = {e0[1] * e1[2] - e0[2] * e1[1], e0[2] * e1[0] - e0[0] * e1[2], e0[0] * e1[1] - e0[1] * e1[0]};

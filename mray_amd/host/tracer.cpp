@@ -629,7 +629,7 @@ class Tracer final : public TracerI
         for(size_t i = 0; i < batches.size(); i++)
         {
             PrimBatch& pb = CommittedBatch(g, batches[i]);
-            pb.baked = Compose(MakeRigid(xf[i].m), pb.baked);
+            pb.baked = Compose(MakeRigid(xf[i].m.data()), pb.baked);
         }
     }
 
@@ -690,7 +690,7 @@ class Tracer final : public TracerI
             if(t.has_value()) Unsupported("Texture-backed material attributes");
         if(info[attr].isTexturable == MR_TEXTURE_ONLY)
         {
-            if(data.Size() != 0) throw MRayError("Texture-only attribute given constant data");
+            if(data.ByteSize() != 0) throw MRayError("Texture-only attribute given constant data");
             return;   // optional texture left empty
         }
         SetMatConstant(grp, r, attr, data);
@@ -718,7 +718,7 @@ class Tracer final : public TracerI
             {
                 const Vector3& v = Elems<Vector3>(data, n, name.c_str())[k];
                 float* dst = (name == "albedo") ? m.albedo : (name == "cauchyBack") ? m.cauchyBack : m.cauchyFront;
-                std::copy_n(v.v, 3, dst);
+                std::copy_n(v.v.data(), 3, dst);
             }
             else
             {
@@ -776,7 +776,7 @@ class Tracer final : public TracerI
         auto s = Elems<Matrix4x4>(data, n, "Transform");
         for(size_t k = 0; k < n; k++)
         {
-            const float* m = s[k].m;
+            const float* m = s[k].m.data();
             if(m[12] != 0.0f || m[13] != 0.0f || m[14] != 0.0f || m[15] != 1.0f)
                 Unsupported("A projective transform");
             grp.xf[IndexOf(r[0]) + k] = MakeRigid(m, true);
@@ -855,7 +855,7 @@ class Tracer final : public TracerI
         CheckRange(r, U(g), grp.lights.size());
         size_t n = IndexOf(r[1]) - IndexOf(r[0]) + 1;
         auto s = Elems<Vector3>(data, n, "radiance");
-        for(size_t k = 0; k < n; k++) std::copy_n(s[k].v, 3, grp.lights[IndexOf(r[0]) + k].radiance);
+        for(size_t k = 0; k < n; k++) std::copy_n(s[k].v.data(), 3, grp.lights[IndexOf(r[0]) + k].radiance);
     }
     void PushLightAttribute(LightGroupId, CommonIdRange, uint32_t, std::vector<TextureId>) override
     {
@@ -898,7 +898,7 @@ class Tracer final : public TracerI
             CameraEntry& c = grp.cams[IndexOf(r[0]) + k];
             switch(attr)
             {
-                case 0: std::copy_n(Elems<Vector4>(data, n, "FovAndPlanes")[k].v, 4, c.fovPlanes); break;
+                case 0: std::copy_n(Elems<Vector4>(data, n, "FovAndPlanes")[k].v.data(), 4, c.fovPlanes); break;
                 case 1: c.gaze = Elems<Vector3>(data, n, "gaze")[k]; break;
                 case 2: c.position = Elems<Vector3>(data, n, "position")[k]; break;
                 case 3: c.up = Elems<Vector3>(data, n, "up")[k]; break;
@@ -988,11 +988,11 @@ class Tracer final : public TracerI
             for(uint32_t v = 0; v < pb.vertexCount; v++)
             {
                 float p[3], q[4];
-                ApplyPoint(r, pb.pos[v].v, p);
+                ApplyPoint(r, pb.pos[v].v.data(), p);
                 // TransformPrimitives turns the frames with the vertices
                 // (KCApplyTransformsTriangle); a surface's transform does not
                 // (Triangle::GenerateSurface leaves the TBN in mesh space)
-                ApplyTbn(pb.baked, pb.tbn[v].v, q);
+                ApplyTbn(pb.baked, pb.tbn[v].v.v.data(), q);
                 f.pos.insert(f.pos.end(), p, p + 3);
                 f.tbn.insert(f.tbn.end(), q, q + 4);
             }
@@ -1100,8 +1100,8 @@ class Tracer final : public TracerI
         const CameraEntry& c = camGroups[GroupOf(U(cs.cameraId))].cams[IndexOf(U(cs.cameraId))];
         Rigid xf = TransformOf(cs.transformId);
         MRayCamera cam{};
-        ApplyPoint(xf, c.position.v, cam.position);
-        ApplyPoint(xf, c.gaze.v, cam.gaze);
+        ApplyPoint(xf, c.position.v.data(), cam.position);
+        ApplyPoint(xf, c.gaze.v.data(), cam.gaze);
         float up[3] = {c.up.v[0], c.up.v[1], c.up.v[2]};
         if(!xf.identity)
             for(int i = 0; i < 3; i++)
@@ -1117,9 +1117,9 @@ class Tracer final : public TracerI
     {
         MRayCamera c = CameraOf(U(s));
         CameraTransform t;
-        std::copy_n(c.position, 3, t.position.v);
-        std::copy_n(c.gaze, 3, t.gazePoint.v);
-        std::copy_n(c.up, 3, t.up.v);
+        std::copy_n(c.position, 3, t.position.v.data());
+        std::copy_n(c.gaze, 3, t.gazePoint.v.data());
+        std::copy_n(c.up, 3, t.up.v.data());
         return t;
     }
     VolumeId RegisterVolume(VolumeParams) override { Unsupported("Volumes"); }
@@ -1202,8 +1202,8 @@ class Tracer final : public TracerI
            (ip.regionMax[1] != 0 && ip.regionMax[1] != ip.resolution[1]))
             Unsupported("Partial-region rendering");
         if(params.filmFilter.type != FilterType::GAUSSIAN) Unsupported("A non-Gaussian film filter");
-        if(params.samplerType != SamplerType::INDEPENDENT) Unsupported("A non-independent sampler");
-        if(it->second.spectral && params.wavelengthSampleMode != WavelengthSampleMode::HYPERBOLIC_PBRT)
+        if(params.samplerType.e != SamplerType::INDEPENDENT) Unsupported("A non-independent sampler");
+        if(it->second.spectral && params.wavelengthSampleMode.e != WavelengthSampleMode::HYPERBOLIC_PBRT)
             Unsupported("A wavelength sampler other than HYPERBOLIC_PBRT");
         StopRender();
         const RendererRec& rr = it->second;
@@ -1354,7 +1354,7 @@ extern "C" void DestroyTracer(mray_amd::TracerI* tracer)
 // u32 V, T, M, L, boundaryType; f32 boundaryRadiance[3], sceneDiameter;
 // f32 pos[V*3], tbn[V*4]; u32 idx[T*3], keys[T]; u8 cull[T];
 // MRayMaterial[M]; u32 lightTris[L]; f32 lightRad[L*3]; u8 twoSided[L].
-extern "C" int mray_tracer_flatten_to_file(mray_amd::TracerI* tracer, const char* path)
+extern "C" MRAY_AMD_TRACER_API int mray_tracer_flatten_to_file(mray_amd::TracerI* tracer, const char* path)
 {
     auto* t = dynamic_cast<mray_amd::Tracer*>(tracer);
     if(!t) return MRAY_ERR_INVALID;
@@ -1362,7 +1362,7 @@ extern "C" int mray_tracer_flatten_to_file(mray_amd::TracerI* tracer, const char
     {
         t->FlattenOnly();
     }
-    catch(const mray_amd::MRayError& e)
+    catch(const MRayError& e)
     {
         std::fprintf(stderr, "flatten: %s\n", e.what());
         return MRAY_ERR_INVALID;

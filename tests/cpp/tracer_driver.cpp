@@ -107,9 +107,8 @@ TransientData Data(const std::vector<T>& v)
 }
 TransientData Str(std::string_view s)
 {
-    TransientData d(std::in_place_type_t<std::string_view>{}, s.size());
-    auto out = d.AccessAsString();
-    std::copy(s.begin(), s.end(), out.begin());
+    TransientData d(std::in_place_type_t<std::string>{}, s.size());
+    d.PushString(s);
     return d;
 }
 CommonIdRange One(uint32_t id) { return {id, id}; }
@@ -164,8 +163,8 @@ CamSurfaceId Feed(TracerI& t, const SceneFile& s)
         std::vector<Vector2> uv(vs.size());
         for(size_t j = 0; j < vs.size(); j++)
         {
-            std::copy_n(&s.pos[size_t(vs[j]) * 3], 3, p[j].v);
-            std::copy_n(&s.tbn[size_t(vs[j]) * 4], 4, q[j].v);
+            std::copy_n(&s.pos[size_t(vs[j]) * 3], 3, p[j].v.data());
+            std::copy_n(&s.tbn[size_t(vs[j]) * 4], 4, q[j].v.v.data());
         }
         std::vector<Vector3ui> ix(runs[k].count);
         for(uint32_t i = 0; i < runs[k].count; i++)
@@ -195,7 +194,7 @@ CamSurfaceId Feed(TracerI& t, const SceneFile& s)
         const Mat& m = s.mats[i];
         MatGroupId g = matGroups[m.type];
         CommonIdRange r = One(uint32_t(matIds[i]));
-        auto v3 = [](const float* a) { Vector3 v; std::copy_n(a, 3, v.v); return std::vector<Vector3>{v}; };
+        auto v3 = [](const float* a) { Vector3 v; std::copy_n(a, 3, v.v.data()); return std::vector<Vector3>{v}; };
         auto f1 = [](float a) { return std::vector<float>{a}; };
         switch(m.type)
         {
@@ -231,7 +230,7 @@ CamSurfaceId Feed(TracerI& t, const SceneFile& s)
             const Run& r = runs[lr[j]];
             runLight[lr[j]] = ids[j];
             Vector3 rad;
-            std::copy_n(&s.lightRad[size_t(r.key) * 3], 3, rad.v);
+            std::copy_n(&s.lightRad[size_t(r.key) * 3], 3, rad.v.data());
             t.PushLightAttribute(lg, One(uint32_t(ids[j])), 0, Data(std::vector<Vector3>{rad}), texNone);
             TransientData two(std::in_place_type_t<bool>{}, 1);
             bool b = s.twoSided[r.key] != 0;
@@ -246,7 +245,7 @@ CamSurfaceId Feed(TracerI& t, const SceneFile& s)
         LightId sky = t.ReserveLight(sg, {});
         t.CommitLightReservations(sg);
         Vector3 rad;
-        std::copy_n(s.boundaryRad, 3, rad.v);
+        std::copy_n(s.boundaryRad, 3, rad.v.data());
         t.PushLightAttribute(sg, One(uint32_t(sky)), 0, Data(std::vector<Vector3>{rad}), texNone);
         t.SetBoundarySurface(sky, TransformId(0));
     }
