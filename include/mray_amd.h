@@ -142,6 +142,15 @@ typedef struct MRayRenderParams
  * once 16 of the wave's 64 lanes are idle (Aila and Laine 2009), instead of
  * idling until the wave's longest walk ends.  Same hits either way.        */
 #define MRAY_RENDER_LANE_REFILL 8u
+/* Tiled rendering (the reference's ImageTiler, Tracer/RenderImage.cpp:20-136,
+ * and DoLatencyRender, TracerDLL/PathTracerRenderer.cu:1076-1180): the image
+ * is split into tiles of about maxRayCount pixels (0: 2^21) as
+ * ImageTiler::FindOptimumTileSize does, the renderer holds one path slot per
+ * pixel of a tile, and mray_renderer_render_tile renders a tile at a time.
+ * Every pixel keeps its own random state for the whole render (on the
+ * device; the reference caches them on the host, Random.cu:728-774).  One
+ * scene only (mray_renderer_create); no radiance cache.                    */
+#define MRAY_RENDER_TILED 16u
 
 /* Per-iteration analytics (Common/AnalyticStructs.h:33-52). */
 typedef struct MRayRenderStats
@@ -306,6 +315,34 @@ int mray_renderer_read_film(MRayRenderer* r, float* hOut);
  * runner each iteration (Tracer/RenderImage.cpp:164-209), which then
  * accumulates sum / weight (MRay/RunCommand.cpp:609-700).                    */
 int mray_renderer_take_film(MRayRenderer* r, float* hOut);
+
+/* Tiled renderers (MRAY_RENDER_TILED).  Tiles are numbered row-major,
+ * x fastest; edge tiles may be smaller than tileSize.                      */
+typedef struct MRayTileInfo
+{
+    uint32_t tileSize[2];     /* covering tile size, pixels (= path slots)  */
+    uint32_t tileCount[2];
+} MRayTileInfo;
+typedef struct MRayTileResult
+{
+    uint32_t tileIndex;       /* the tile the call rendered                 */
+    uint32_t pixelMin[2];     /* its pixels [pixelMin, pixelMax)            */
+    uint32_t pixelMax[2];
+    uint32_t tileSPP;         /* samples of that tile after the call        */
+} MRayTileResult;
+int mray_renderer_tiling(const MRayRenderer* r, MRayTileInfo* hOut);
+/* One DoLatencyRender: `passCount` more samples of every pixel of the
+ * current tile, run until all of them are in the film, then the next tile
+ * becomes current.  passCount 0 renders nothing and still moves on.  The
+ * tile's random states are loaded when the tile changes and stored back
+ * when it is left (RNGGroupIndependent::SetupRange).  hStats as
+ * mray_renderer_stats.                                                      */
+int mray_renderer_render_tile(MRayRenderer* r, uint32_t passCount, MRayTileResult* hTile,
+                              MRayRenderStats* hStats);
+/* The last rendered tile's sums, packed: hOut [4][h][w] of that tile (R, G,
+ * B, weight; row 0 = the tile's bottom row); those film pixels are then
+ * cleared, as RenderImage::TransferToHost hands over a tile buffer.        */
+int mray_renderer_take_tile(MRayRenderer* r, float* hOut);
 /* Film [4][height][width] (R, G, B sums, weight sums; row 0 = bottom) as an
  * OpenEXR file of per-pixel means, RGB 32-bit float, top row first: the
  * image the reference's run command writes on triggerSave

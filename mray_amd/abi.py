@@ -251,6 +251,16 @@ RENDER_MATERIAL_QUEUES = 1
 RENDER_NO_GRAPH = 2
 RENDER_LIGHT_POWER = 4
 RENDER_LANE_REFILL = 8
+RENDER_TILED = 16
+
+
+class MRayTileInfo(C.Structure):
+    _fields_ = [("tileSize", C.c_uint32 * 2), ("tileCount", C.c_uint32 * 2)]
+
+
+class MRayTileResult(C.Structure):
+    _fields_ = [("tileIndex", C.c_uint32), ("pixelMin", C.c_uint32 * 2),
+                ("pixelMax", C.c_uint32 * 2), ("tileSPP", C.c_uint32)]
 
 
 def render_params(width, height, spp, rr=(8, 8), mode=SAMPLE_NEE_MIS, spectral=False,

@@ -249,6 +249,17 @@ struct ViewGPU
     uint32_t envW, envH;
 };
 
+// The current tile of a tiled renderer (MRAY_RENDER_TILED): the tile's
+// sub-camera (CameraPinhole::GenerateSubCamera, CamerasDefault.hpp:226-247)
+// at the tile's own resolution, and where the tile sits in the film.
+struct TileGPU
+{
+    CameraGPU cam;
+    uint32_t x0, y0;      // first film pixel
+    uint32_t w;           // width of this tile (edge tiles may be narrower)
+    uint32_t P;           // pixels of this tile
+};
+
 struct RenderConst
 {
     SceneView scene;
@@ -274,6 +285,9 @@ struct RenderConst
     uint32_t frameMajor;
     unsigned long long pixTotal;
     const unsigned long long* viewPix;    // [viewCount + 1] pixel prefix sums
+    // tiled renderers: the current tile (device memory, rewritten between
+    // tiles), NULL otherwise
+    const TileGPU* tile;
 };
 
 // View whose path range holds global path index g: the last v in [lo, hi]
@@ -680,6 +694,15 @@ KRefillApply(SlotSoA S, RenderConst rc, const uint4* __restrict__ blockOff,
                 const ViewGPU& v = rc.views[lo];
                 S.view[i] = lo;
                 StartCameraPath<SPECTRAL>(S, v.cam, v.W, i, pix);
+            }
+            else if(rc.tile)
+            {
+                // path g of the tile is its pixel g mod P, rendered through
+                // the tile's sub-camera and splatted at its film position
+                const TileGPU& t = *rc.tile;
+                const uint32_t local = uint32_t(g % t.P);
+                StartCameraPath<SPECTRAL>(S, t.cam, t.w, i, local);
+                S.pixel[i] = (t.y0 + local / t.w) * rc.W + t.x0 + local % t.w;
             }
             else StartCameraPath<SPECTRAL>(S, rc.cam, rc.W, i, uint32_t(g % rc.P));
             if(rc.nrc.on)
@@ -1893,6 +1916,20 @@ struct MRayRenderer
     float4* dNrcBox = nullptr;
     unsigned long long* dViewClock = nullptr;
     uint32_t traceTPB = kTraceTPB;
+    // tiled rendering (MRAY_RENDER_TILED)
+    bool tiled = false;
+    uint32_t tileSize[2] = {0, 0};
+    uint32_t tileCount[2] = {0, 0};
+    TileGPU* dTile = nullptr;
+    uint32_t* dPixRng = nullptr;      // [P] every pixel's random state
+    uint32_t* dPixSeed = nullptr;     // [P] the states at the start of a render
+    std::vector<unsigned long long> tilePaths;   // paths issued per tile
+    std::vector<uint32_t> tileSPP;
+    uint32_t curTile = 0;
+    int loadedTile = -1;              // tile whose states the slots hold
+    unsigned long long pathsDone = 0; // paths of every tile call so far
+    MRayTileResult lastTile{};
+    TileGPU hTile{};                  // source of the current tile's upload
 };
 
 #define MRAY_TRY_IT(expr)               \
@@ -2090,6 +2127,45 @@ __global__ void KSetLimit(Sched* __restrict__ sched, unsigned long long limit)
     if(threadIdx.x == 0 && blockIdx.x == 0) sched->limit = limit;
 }
 
+// A tile call's paths: the tile's global path indices [first, limit).
+__global__ void KSetTileRange(Sched* __restrict__ sched, unsigned long long first,
+                              unsigned long long limit)
+{
+    if(threadIdx.x == 0 && blockIdx.x == 0)
+    {
+        sched->counter = first;
+        sched->limit = limit;
+    }
+}
+
+// Slot j of a tiled renderer holds the random state of the tile's pixel
+// (x0 + j mod w, y0 + j / w): loaded when the tile becomes current, stored
+// back when another tile replaces it (the 2-D copies of
+// RNGGroupIndependent::SetupRange, Random.cu:728-774).
+template<bool LOAD>
+__global__ void KTileRng(uint32_t* __restrict__ slotRng, uint32_t* __restrict__ pixRng, uint32_t W,
+                         uint32_t x0, uint32_t y0, uint32_t w, uint32_t n)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= n) return;
+    const size_t px = size_t(y0 + j / w) * W + x0 + j % w;
+    if constexpr(LOAD) slotRng[j] = pixRng[px];
+    else pixRng[px] = slotRng[j];
+}
+
+// Clears a w x h rectangle of the four film planes.
+__global__ void KZeroRect(float* __restrict__ film, size_t P, uint32_t W, uint32_t x0, uint32_t y0,
+                          uint32_t w, uint32_t n)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if(j >= n) return;
+    const size_t px = size_t(y0 + j / w) * W + x0 + j % w;
+    for(int c = 0; c < 4; c++) film[size_t(c) * P + px] = 0.0f;
+}
+
+static CameraGPU SubCamera(const CameraGPU& c, uint32_t sx, uint32_t sy, uint32_t cx, uint32_t cy,
+                           uint32_t w, uint32_t h);
+
 static int ResetRenderer(MRayRenderer* r)
 {
     hipStream_t s = r->stream;
@@ -2107,6 +2183,29 @@ static int ResetRenderer(MRayRenderer* r)
         // training pairs counted since the reset (u64 per view)
         const size_t V = r->rc.nrc.views;
         KZeroFloats<<<1, 256, 0, s>>>(reinterpret_cast<float*>(r->rc.nrc.ringCount), 2 * V);
+    }
+    if(r->tiled)
+    {
+        // every pixel back to its first random state, no tile rendered; no
+        // path is issued until render_tile sets a tile's range, and the
+        // device tile is tile 0 until then
+        KSetTileRange<<<1, 1, 0, s>>>(r->dSched, 0ull, 0ull);
+        MRAY_HIP_CHECK(hipMemcpyAsync(r->dPixRng, r->dPixSeed, size_t(r->rc.P) * 4,
+                                      hipMemcpyDeviceToDevice, s));
+        r->hTile.cam = SubCamera(r->rc.cam, 0, 0, r->tileCount[0], r->tileCount[1],
+                                 std::min(r->tileSize[0], r->params.width),
+                                 std::min(r->tileSize[1], r->params.height));
+        r->hTile.x0 = 0;
+        r->hTile.y0 = 0;
+        r->hTile.w = r->hTile.cam.width;
+        r->hTile.P = r->hTile.cam.width * r->hTile.cam.height;
+        MRAY_HIP_CHECK(hipMemcpyAsync(r->dTile, &r->hTile, sizeof(TileGPU), hipMemcpyHostToDevice, s));
+        std::fill(r->tilePaths.begin(), r->tilePaths.end(), 0ull);
+        std::fill(r->tileSPP.begin(), r->tileSPP.end(), 0u);
+        r->curTile = 0;
+        r->loadedTile = -1;
+        r->pathsDone = 0;
+        r->lastTile = MRayTileResult{};
     }
     MRAY_HIP_CHECK(hipGetLastError());
     MRAY_HIP_CHECK(hipStreamSynchronize(s));
@@ -2144,6 +2243,65 @@ static void BuildCamera(const MRayCamera& c, uint32_t W, uint32_t H, float filte
 }
 
 static int SizeLaunches(MRayRenderer* r);
+
+// RNGGroupIndependent (Random.cu:661-703): seed32 = hi ^ lo of the 64-bit
+// seed, one std::mt19937 draw per pixel through PCG32's GenerateState.
+static std::vector<uint32_t> PixelSeeds(uint64_t seed, uint32_t n)
+{
+    std::vector<uint32_t> seeds(n);
+    uint32_t seed32 = uint32_t((seed >> 32) ^ (seed & 0xFFFFFFFFull));
+    std::mt19937 mt(seed32);
+    for(uint32_t i = 0; i < n; i++)
+    {
+        uint32_t x = uint32_t(mt());
+        uint32_t st = 0u * 747796405u + 2891336453u;
+        st += x;
+        st = st * 747796405u + 2891336453u;
+        seeds[i] = st;
+    }
+    return seeds;
+}
+
+// ImageTiler::FindOptimumTileSize (RenderImage.cpp:20-52): a tile of about
+// `hint` pixels with the image's aspect ratio, then per axis the count of
+// tiles nearest to the image over that size, and the tile size that covers
+// the axis with that count.  A hint below one pixel per axis is taken as 1.
+static void FindTileSize(uint32_t W, uint32_t H, uint32_t hint, uint32_t out[2])
+{
+    const float aspect = float(W) / float(H);
+    const float factor = std::sqrt(float(hint) / aspect);
+    const uint32_t want[2] = {std::max(1u, uint32_t(std::round(aspect * factor))),
+                              std::max(1u, uint32_t(std::round(factor)))};
+    const uint32_t fb[2] = {W, H};
+    for(int i = 0; i < 2; i++)
+    {
+        if(fb[i] < want[i])
+        {
+            out[i] = fb[i];
+            continue;
+        }
+        const uint32_t count = uint32_t(std::round(float(fb[i]) / float(want[i])));
+        uint32_t residual = fb[i] % count;
+        residual = (residual + count - 1u) / count;
+        out[i] = fb[i] / count + residual;
+    }
+}
+
+// CameraPinhole::GenerateSubCamera (CamerasDefault.hpp:226-247): stratum
+// (sx, sy) of a (cx, cy) split of the image plane, seen at w x h pixels.
+static CameraGPU SubCamera(const CameraGPU& c, uint32_t sx, uint32_t sy, uint32_t cx, uint32_t cy,
+                           uint32_t w, uint32_t h)
+{
+    CameraGPU t = c;
+    const float dx = c.planeSize[0] / float(cx), dy = c.planeSize[1] / float(cy);
+    const float rx = float(sx) * dx, ry = float(sy) * dy;
+    for(int k = 0; k < 3; k++) t.bottomLeft[k] = c.bottomLeft[k] + ((rx * c.right[k]) + (ry * c.up[k]));
+    t.planeSize[0] = dx;
+    t.planeSize[1] = dy;
+    t.width = w;
+    t.height = h;
+    return t;
+}
 
 // Path-slot state, queues, seeds and launch sizes, shared by the one-scene
 // and batch renderers (r->params, r->rc.N, r->ctx and r->batch set).
@@ -2191,17 +2349,7 @@ static int InitSlots(MRayRenderer* r, size_t filmPixels)
     // RNGGroupIndependent (Random.cu:661-703): seed32 = hi ^ lo of the 64-bit
     // seed, one std::mt19937 draw per pixel through GenerateState.
     {
-        std::vector<uint32_t> seeds(N);
-        uint32_t seed32 = uint32_t((p->seed >> 32) ^ (p->seed & 0xFFFFFFFFull));
-        std::mt19937 mt(seed32);
-        for(uint32_t i = 0; i < N; i++)
-        {
-            uint32_t x = uint32_t(mt());
-            uint32_t st = 0u * 747796405u + 2891336453u;
-            st += x;
-            st = st * 747796405u + 2891336453u;
-            seeds[i] = st;
-        }
+        std::vector<uint32_t> seeds = PixelSeeds(p->seed, N);
         MRAY_HIP_CHECK(hipMemcpy(r->dSeeds, seeds.data(), size_t(N) * 4, hipMemcpyHostToDevice));
     }
     return SizeLaunches(r);
@@ -2322,6 +2470,15 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
     const uint64_t P = uint64_t(p->width) * p->height;
     uint32_t N = p->maxRayCount ? p->maxRayCount : uint32_t(P);
     if(N > P) N = uint32_t(P);
+    if(p->flags & MRAY_RENDER_TILED)
+    {
+        // one slot per pixel of a (covering) tile
+        r->tiled = true;
+        FindTileSize(p->width, p->height, p->maxRayCount ? p->maxRayCount : (1u << 21), r->tileSize);
+        r->tileCount[0] = DivUp(p->width, r->tileSize[0]);
+        r->tileCount[1] = DivUp(p->height, r->tileSize[1]);
+        N = r->tileSize[0] * r->tileSize[1];
+    }
     r->rc.N = N;
     r->rc.P = uint32_t(P);
     r->rc.W = p->width;
@@ -2336,6 +2493,18 @@ static int CreateRenderer(MRayRenderer* r, const MRayAccel* a, const MRaySceneDe
     MRAY_TRY(SetupLightPower(r, {0u}, {a->view.lightCount}, a->view.lightCount));
     MRAY_TRY(InitSlots(r, size_t(P)));
     MRAY_TRY(SetupSpectralCoeffs(r, uint32_t(a->hMats.size()), a->view.lightCount));
+    if(r->tiled)
+    {
+        const uint32_t T = r->tileCount[0] * r->tileCount[1];
+        MRAY_TRY(DevAlloc(r->arena, (void**)&r->dTile, sizeof(TileGPU)));
+        MRAY_TRY(DevAlloc(r->arena, (void**)&r->dPixRng, size_t(P) * 4));
+        MRAY_TRY(DevAlloc(r->arena, (void**)&r->dPixSeed, size_t(P) * 4));
+        std::vector<uint32_t> seeds = PixelSeeds(p->seed, uint32_t(P));
+        MRAY_HIP_CHECK(hipMemcpy(r->dPixSeed, seeds.data(), size_t(P) * 4, hipMemcpyHostToDevice));
+        r->tilePaths.assign(T, 0ull);
+        r->tileSPP.assign(T, 0u);
+        r->rc.tile = r->dTile;
+    }
     return ResetRenderer(r);
 }
 
@@ -2357,6 +2526,11 @@ static int SetupLightPower(MRayRenderer* r, const std::vector<uint32_t>& offs,
 
 static int CreateInstanced(MRayRenderer* r, const MRayInstScene* sc, const MRayRenderParams* p)
 {
+    if(p->flags & MRAY_RENDER_TILED)
+    {
+        SetError("renderer_create_instanced: tiled rendering takes one flattened scene");
+        return MRAY_ERR_UNSUPPORTED;
+    }
     r->iscene = sc;
     r->ctx = sc->ctx;
     r->params = *p;
@@ -2416,6 +2590,11 @@ static MRayBatchScene::Part InstancedPart(const MRayInstScene* is)
 static int CreateBatch(MRayRenderer* r, const MRayBatchScene* b, const MRayInstScene* is,
                        const MRayBatchView* views, uint32_t viewCount, const MRayRenderParams* p)
 {
+    if(p->flags & MRAY_RENDER_TILED)
+    {
+        SetError("batch_create: a batch renders whole views; tiling is for one scene");
+        return MRAY_ERR_UNSUPPORTED;
+    }
     r->batch = true;
     r->bscene = b;
     r->iscene = is;
@@ -2723,6 +2902,34 @@ int mray_renderer_render_all(MRayRenderer* r, MRayRenderStats* hStats)
         SetError("render_all: this batch renderer renders by frames; reset it first");
         return MRAY_ERR_INVALID;
     }
+    if(r->tiled)
+    {
+        // every tile up to totalSPP, in tile order from the current one
+        uint32_t waits = 0;
+        MRayRenderStats sum{};
+        for(uint32_t k = 0; k < r->tileCount[0] * r->tileCount[1]; k++)
+        {
+            const uint32_t have = r->tileSPP[r->curTile];
+            MRayRenderStats st{};
+            MRAY_TRY(mray_renderer_render_tile(r, have < r->params.totalSPP ? r->params.totalSPP - have : 0u,
+                                               nullptr, &st));
+            waits += st.hostWaits;
+            sum.deadPaths += st.deadPaths;
+            sum.extensionRays += st.extensionRays;
+            sum.shadowRays += st.shadowRays;
+            sum.iterations += st.iterations;
+            sum.nodeVisits += st.nodeVisits;
+            sum.triTests += st.triTests;
+            sum.shadowNodeVisits += st.shadowNodeVisits;
+            sum.shadowTriTests += st.shadowTriTests;
+            sum.totalDeadPaths = st.totalDeadPaths;
+        }
+        sum.finished = 1;
+        sum.hostWaits = waits;
+        sum.deviceDrains = r->tileCount[0] * r->tileCount[1];
+        if(hStats) *hStats = sum;
+        return MRAY_OK;
+    }
     MRAY_TRY(RunToLimit(r, r->limit, "render_all"));
     MRayRenderStats total{};
     MRAY_TRY(mray_renderer_stats(r, &total));
@@ -2752,6 +2959,112 @@ int mray_renderer_take_film(MRayRenderer* r, float* hOut)
     KZeroFloats<<<DivUp(n, 256 * 8), 256, 0, r->stream>>>(r->rc.film, n);
     MRAY_HIP_CHECK(hipGetLastError());
     MRAY_HIP_CHECK(hipStreamSynchronize(r->stream));
+    return MRAY_OK;
+}
+
+int mray_renderer_tiling(const MRayRenderer* r, MRayTileInfo* hOut)
+{
+    if(!r || !hOut) { SetError("tiling: NULL argument"); return MRAY_ERR_INVALID; }
+    if(!r->tiled) { SetError("tiling: the renderer was not created with MRAY_RENDER_TILED"); return MRAY_ERR_INVALID; }
+    for(int k = 0; k < 2; k++)
+    {
+        hOut->tileSize[k] = r->tileSize[k];
+        hOut->tileCount[k] = r->tileCount[k];
+    }
+    return MRAY_OK;
+}
+
+// PathTracerRendererT::DoLatencyRender (PathTracerRenderer.cu:1076-1180):
+// the tile's states in the slots, the tile's sub-camera, its next
+// passCount x pixels path indices, bursts until all of them are in the film.
+int mray_renderer_render_tile(MRayRenderer* r, uint32_t passCount, MRayTileResult* hTile,
+                              MRayRenderStats* hStats)
+{
+    if(!r) { SetError("render_tile: renderer is NULL"); return MRAY_ERR_INVALID; }
+    if(!r->tiled) { SetError("render_tile: the renderer was not created with MRAY_RENDER_TILED"); return MRAY_ERR_INVALID; }
+    hipStream_t s = r->stream;
+    const uint32_t W = r->params.width, H = r->params.height;
+    auto rect = [&](uint32_t t, uint32_t& x0, uint32_t& y0, uint32_t& w, uint32_t& h)
+    {
+        x0 = (t % r->tileCount[0]) * r->tileSize[0];
+        y0 = (t / r->tileCount[0]) * r->tileSize[1];
+        w = std::min(r->tileSize[0], W - x0);
+        h = std::min(r->tileSize[1], H - y0);
+    };
+    const uint32_t t = r->curTile;
+    uint32_t x0, y0, w, h;
+    rect(t, x0, y0, w, h);
+    if(r->loadedTile != int(t))
+    {
+        if(r->loadedTile >= 0)
+        {
+            uint32_t px0, py0, pw, ph;
+            rect(uint32_t(r->loadedTile), px0, py0, pw, ph);
+            KTileRng<false><<<DivUp(pw * ph, 256), 256, 0, s>>>(r->S.rng, r->dPixRng, W, px0, py0, pw, pw * ph);
+        }
+        KTileRng<true><<<DivUp(w * h, 256), 256, 0, s>>>(r->S.rng, r->dPixRng, W, x0, y0, w, w * h);
+        MRAY_HIP_CHECK(hipGetLastError());
+        r->loadedTile = int(t);
+    }
+    MRAY_HIP_CHECK(hipStreamSynchronize(s));        // the previous upload has been read
+    r->hTile.cam = SubCamera(r->rc.cam, t % r->tileCount[0], t / r->tileCount[0], r->tileCount[0],
+                             r->tileCount[1], w, h);
+    r->hTile.x0 = x0;
+    r->hTile.y0 = y0;
+    r->hTile.w = w;
+    r->hTile.P = w * h;
+    MRAY_HIP_CHECK(hipMemcpyAsync(r->dTile, &r->hTile, sizeof(TileGPU), hipMemcpyHostToDevice, s));
+    const unsigned long long first = r->tilePaths[t];
+    const uint32_t spp = r->tileSPP[t] + passCount;
+    const unsigned long long lim = (unsigned long long)spp * (w * h);
+    KSetTileRange<<<1, 1, 0, s>>>(r->dSched, first, lim);
+    MRAY_HIP_CHECK(hipGetLastError());
+    r->hostWaits = 0;
+    if(lim > first)
+    {
+        r->pathsDone += lim - first;
+        MRAY_TRY_IT(RunToLimit(r, r->pathsDone, "render_tile"));
+    }
+    r->tilePaths[t] = lim;
+    r->tileSPP[t] = spp;
+    r->lastTile.tileIndex = t;
+    r->lastTile.pixelMin[0] = x0;
+    r->lastTile.pixelMin[1] = y0;
+    r->lastTile.pixelMax[0] = x0 + w;
+    r->lastTile.pixelMax[1] = y0 + h;
+    r->lastTile.tileSPP = spp;
+    r->curTile = (t + 1u) % (r->tileCount[0] * r->tileCount[1]);
+    if(hTile) *hTile = r->lastTile;
+    MRayRenderStats total{};
+    MRAY_TRY_IT(mray_renderer_stats(r, &total));
+    total.hostWaits = r->hostWaits;
+    total.deviceDrains = 1;
+    total.finished = 1;
+    for(uint32_t k : r->tileSPP) total.finished &= (k >= r->params.totalSPP) ? 1u : 0u;
+    if(hStats) *hStats = total;
+    return MRAY_OK;
+}
+
+int mray_renderer_take_tile(MRayRenderer* r, float* hOut)
+{
+    if(!r || !hOut) { SetError("take_tile: NULL argument"); return MRAY_ERR_INVALID; }
+    if(!r->tiled) { SetError("take_tile: the renderer was not created with MRAY_RENDER_TILED"); return MRAY_ERR_INVALID; }
+    const MRayTileResult& t = r->lastTile;
+    const uint32_t w = t.pixelMax[0] - t.pixelMin[0], h = t.pixelMax[1] - t.pixelMin[1];
+    if(w * h == 0) { SetError("take_tile: no tile rendered yet"); return MRAY_ERR_INVALID; }
+    hipStream_t s = r->stream;
+    const size_t P = r->rc.P, W = r->params.width;
+    MRAY_HIP_CHECK(hipStreamSynchronize(s));
+    for(size_t c = 0; c < 4; c++)
+    {
+        const float* src = r->rc.film + c * P + size_t(t.pixelMin[1]) * W + t.pixelMin[0];
+        MRAY_HIP_CHECK(hipMemcpy2DAsync(hOut + c * size_t(w) * h, size_t(w) * 4, src, W * 4,
+                                        size_t(w) * 4, h, hipMemcpyDeviceToHost, s));
+    }
+    KZeroRect<<<DivUp(w * h, 256), 256, 0, s>>>(r->rc.film, P, uint32_t(W), t.pixelMin[0], t.pixelMin[1],
+                                                w, w * h);
+    MRAY_HIP_CHECK(hipGetLastError());
+    MRAY_HIP_CHECK(hipStreamSynchronize(s));
     return MRAY_OK;
 }
 
@@ -2816,6 +3129,11 @@ int mray_renderer_attach_nrcs(MRayRenderer* r, MRayNrc* const* caches, uint32_t 
         r->rc.nrc = NrcRenderState{};
         r->nrcArena.Release();
         return MRAY_OK;
+    }
+    if(r->tiled)
+    {
+        SetError("attach_nrc: a tiled renderer takes no radiance cache");
+        return MRAY_ERR_UNSUPPORTED;
     }
     const uint32_t views = r->batch ? uint32_t(r->hViews.size()) : 1u;
     if(count != views)
@@ -2914,6 +3232,7 @@ static int SetFrameOrder(MRayRenderer* r, bool on)
 int mray_renderer_render_frame(MRayRenderer* r, uint32_t spp, MRayRenderStats* hStats)
 {
     if(!r || spp == 0) { SetError("render_frame: NULL renderer or spp == 0"); return MRAY_ERR_INVALID; }
+    if(r->tiled) { SetError("render_frame: a tiled renderer renders with mray_renderer_render_tile"); return MRAY_ERR_INVALID; }
     hipStream_t s = r->stream;
     if(r->batch && !r->rc.frameMajor)
     {

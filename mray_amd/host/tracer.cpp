@@ -291,6 +291,7 @@ struct RendererRec
     uint32_t sampleMode = MRAY_SAMPLE_NEE_MIS;
     uint32_t rr[2] = {4, 20};
     bool powerLights = false;   // neeSamplerType "Power"
+    bool latency = false;       // renderMode "Latency"
 };
 
 uint32_t ParseSampleMode(std::string_view s)
@@ -360,6 +361,12 @@ class Tracer final : public TracerI
     bool saveImage = false;
     uint64_t sampleCounter = 0;
     double lastIterMs = 0;
+    // tile by tile (DoLatencyRender): the renderer's tiles and, per tile,
+    // the samples it has (PathTracerRendererBase::tileSPPs)
+    bool tiled = false;
+    uint32_t tileCount = 1;
+    uint32_t curTile = 0;
+    std::vector<uint32_t> tileSPPs;
 
     // ---- lookups -----------------------------------------------------------
     template<class V> static auto& At(V& v, uint32_t g, const char* what)
@@ -1156,7 +1163,8 @@ class Tracer final : public TracerI
                 std::string_view m = std::as_const(data).AccessAsString();
                 if(m != "Throughput" && m != "Latency")
                     throw MRayError("Unknown render mode \"" + std::string(m) + "\"");
-                break;   // one schedule serves both: every call returns the whole film
+                r.latency = (m == "Latency");
+                break;
             }
             case 3: r.sampleMode = ParseSampleMode(std::as_const(data).AccessAsString()); break;
             case 4:
@@ -1220,18 +1228,40 @@ class Tracer final : public TracerI
         p.filterRadius = params.filmFilter.radius;
         p.flags = rr.powerLights ? MRAY_RENDER_LIGHT_POWER : 0u;
         MRaySceneDesc d = Desc(CameraOf(U(cs)));
-        CheckRc(mray_renderer_create(accel, &d, &p, &renderer), "StartRender");
+        // PathTracerRendererBase::DoRender (PathTracerRendererBase.cu:507-533):
+        // Throughput on one tile with burstSize 1 renders bounce by bounce
+        // over the whole image; everything else renders a tile at a time,
+        // burstSize samples (Throughput) or one sample (Latency) per call.
+        MRayRenderer* tr = nullptr;
+        MRayRenderParams pt = p;
+        pt.flags |= MRAY_RENDER_TILED;
+        CheckRc(mray_renderer_create(accel, &d, &pt, &tr), "StartRender");
+        MRayTileInfo ti{};
+        CheckRc(mray_renderer_tiling(tr, &ti), "StartRender");
+        tileCount = ti.tileCount[0] * ti.tileCount[1];
+        tiled = rr.latency || rr.burstSize > 1 || tileCount > 1;
+        if(tiled) renderer = tr;
+        else
+        {
+            mray_renderer_destroy(tr);
+            CheckRc(mray_renderer_create(accel, &d, &p, &renderer), "StartRender");
+        }
+        curTile = 0;
+        tileSPPs.assign(tileCount, 0u);
         activeRenderer = U(rid);
         imgParams = ip;
-        size_t P = size_t(p.width) * p.height;
+        // a tile's planes, packed (the whole image when it is one tile)
+        size_t P = tiled ? size_t(ti.tileSize[0]) * ti.tileSize[1] : size_t(p.width) * p.height;
         staging.Allocate(4 * P * sizeof(float), importAlignment);
         sampleCounter = 0;
         saveImage = true;
         RenderBufferInfo info{};
         info.data = reinterpret_cast<const std::byte*>(staging.p);
         info.totalSize = staging.bytes;
-        info.renderColorSpace = MRayColorSpaceEnum::MR_REC_709;
+        info.renderColorSpace = params.globalTextureColorSpace;
         info.resolution = ip.resolution;
+        info.curRenderLogic0 = rr.sampleMode;
+        info.curRenderLogic1 = UINT32_MAX;
         return info;
     }
     void SetCameraTransform(RendererId rid, CameraTransform) override
@@ -1247,6 +1277,7 @@ class Tracer final : public TracerI
     {
         if(!renderer) return RendererOutput{};
         const RendererRec& rr = renderers.at(activeRenderer);
+        if(tiled) return DoTileWork(rr);
         auto t0 = std::chrono::steady_clock::now();
         CheckRc(mray_renderer_iterate(renderer, rr.burstSize, nullptr), "DoRenderWork");
         MRayRenderStats st{};
@@ -1257,14 +1288,15 @@ class Tracer final : public TracerI
         size_t P = size_t(imgParams.resolution[0]) * imgParams.resolution[1];
         RendererAnalyticData a{};
         a.throughput = double(st.deadPaths) / (ms * 1e-3) / 1e6;   // PathTracerRendererBase.cu:305-311
-        a.throughputSuffix = "M paths/s";
+        a.throughputSuffix = "M path/s";
         a.workPerPixel = double(sampleCounter) / double(P);
         a.wppLimit = rr.totalSPP;
         a.workPerPixelSuffix = "spp";
         a.iterationTimeMS = float(ms);
         a.renderResolution = imgParams.resolution;
-        a.outputColorSpace = MRayColorSpaceEnum::MR_REC_709;
+        a.outputColorSpace = params.globalTextureColorSpace;
         a.usedGPUMemoryBytes = UsedDeviceMemory();
+        a.customLogicSize0 = 3;                                    // sample modes
         RendererOutput out;
         out.analytics = a;
         // RenderImage::TransferToHost (RenderImage.cpp:164-209): wait until
@@ -1283,7 +1315,7 @@ class Tracer final : public TracerI
         RenderImageSection sec{};
         sec.pixelMin = Vector2ui{{0, 0}};
         sec.pixelMax = imgParams.resolution;
-        sec.globalWeight = 0.0f;
+        sec.globalWeight = 1.0f;                                   // PathTracerRendererBase.cu:260
         sec.waitCounter = waitCounter;
         sec.pixStartOffsets = {0, P * sizeof(float), 2 * P * sizeof(float)};
         sec.weightStartOffset = 3 * P * sizeof(float);
@@ -1291,6 +1323,66 @@ class Tracer final : public TracerI
         // one save request when the sample limit is reached
         // (PathTracerRenderer.cu:1063-1065)
         out.triggerSave = saveImage && st.finished != 0;
+        if(out.triggerSave) saveImage = false;
+        return out;
+    }
+
+    // PathTracerRendererT::DoLatencyRender (PathTracerRenderer.cu:1076-1180):
+    // passCount more samples of the current tile (capped at totalSPP until
+    // the save was requested), that tile's section handed over with global
+    // weight 1, then the next tile; the save is requested once every tile
+    // has totalSPP samples.
+    RendererOutput DoTileWork(const RendererRec& rr)
+    {
+        const uint32_t passCount = rr.latency ? 1u : rr.burstSize;
+        const uint32_t t = curTile;
+        uint32_t sppLimit = tileSPPs[t] + passCount;
+        if(saveImage) sppLimit = std::min(sppLimit, rr.totalSPP);
+        sppLimit = std::max(sppLimit, tileSPPs[t]);
+        auto t0 = std::chrono::steady_clock::now();
+        MRayTileResult tile{};
+        MRayRenderStats st{};
+        CheckRc(mray_renderer_render_tile(renderer, sppLimit - tileSPPs[t], &tile, &st), "DoRenderWork");
+        tileSPPs[t] = sppLimit;
+        curTile = (t + 1u) % tileCount;
+        const uint32_t w = tile.pixelMax[0] - tile.pixelMin[0], h = tile.pixelMax[1] - tile.pixelMin[1];
+        RendererOutput out;
+        if(sem && !sem->Acquire(acquireValue)) return out;
+        CheckRc(mray_renderer_take_tile(renderer, staging.p), "DoRenderWork");
+        uint64_t waitCounter = 0;
+        if(sem)
+        {
+            sem->Release();
+            acquireValue += 2;
+            waitCounter = acquireValue - 1;
+        }
+        double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        lastIterMs = ms;
+        // CalculateAnalyticDataLatency (PathTracerRendererBase.cu:328-356)
+        uint64_t sppSum = 0;
+        for(uint32_t v : tileSPPs) sppSum += v;
+        RendererAnalyticData a{};
+        a.throughput = double(st.deadPaths) / (ms * 1e-3) / 1e6;
+        a.throughputSuffix = "M path/s";
+        a.workPerPixel = double(sppSum) / double(tileCount);
+        a.wppLimit = rr.totalSPP;
+        a.workPerPixelSuffix = "spp";
+        a.iterationTimeMS = float(ms);
+        a.renderResolution = imgParams.resolution;
+        a.outputColorSpace = params.globalTextureColorSpace;
+        a.usedGPUMemoryBytes = UsedDeviceMemory();
+        a.customLogicSize0 = 3;
+        out.analytics = a;
+        const size_t P = size_t(w) * h;
+        RenderImageSection sec{};
+        sec.pixelMin = Vector2ui{{tile.pixelMin[0], tile.pixelMin[1]}};
+        sec.pixelMax = Vector2ui{{tile.pixelMax[0], tile.pixelMax[1]}};
+        sec.globalWeight = 1.0f;
+        sec.waitCounter = waitCounter;
+        sec.pixStartOffsets = {0, P * sizeof(float), 2 * P * sizeof(float)};
+        sec.weightStartOffset = 3 * P * sizeof(float);
+        out.imageOut = sec;
+        out.triggerSave = saveImage && sppSum == uint64_t(rr.totalSPP) * tileCount;
         if(out.triggerSave) saveImage = false;
         return out;
     }

@@ -47,6 +47,7 @@ EXPORTS = [
     "mray_renderer_create_instanced", "mray_instanced_batch_create",
     "mray_accel_set_environment", "mray_accel_environment_cdfs",
     "mray_instanced_scene_set_environment", "mray_renderer_light_table",
+    "mray_renderer_tiling", "mray_renderer_render_tile", "mray_renderer_take_tile",
 ]
 
 
@@ -92,6 +93,10 @@ def lib():
     L.mray_renderer_stats.argtypes = [P, C.POINTER(abi.MRayRenderStats)]
     L.mray_renderer_read_film.argtypes = [P, P]
     L.mray_renderer_take_film.argtypes = [P, P]
+    L.mray_renderer_tiling.argtypes = [P, C.POINTER(abi.MRayTileInfo)]
+    L.mray_renderer_render_tile.argtypes = [P, u32, C.POINTER(abi.MRayTileResult),
+                                            C.POINTER(abi.MRayRenderStats)]
+    L.mray_renderer_take_tile.argtypes = [P, P]
     L.mray_film_write_exr.argtypes = [P, u32, u32, C.c_char_p]
     L.mray_renderer_reset.argtypes = [P]
     L.mray_renderer_film_device.argtypes = [P, C.POINTER(P)]
@@ -419,6 +424,32 @@ class Renderer:
         """Sums splatted since the last take; the device film is cleared."""
         out = np.zeros((4, self.height, self.width), np.float32)
         check(lib().mray_renderer_take_film(self.h, out.ctypes.data))
+        return out
+
+    def tiling(self):
+        """(tile size, tile count) of a tiled renderer (MRAY_RENDER_TILED)."""
+        t = abi.MRayTileInfo()
+        check(lib().mray_renderer_tiling(self.h, C.byref(t)))
+        return tuple(t.tileSize), tuple(t.tileCount)
+
+    def render_tile(self, pass_count=1):
+        """`pass_count` more samples of the current tile, then the next tile
+        is current; returns the tile rendered and the call's statistics."""
+        t = abi.MRayTileResult()
+        st = abi.MRayRenderStats()
+        check(lib().mray_renderer_render_tile(self.h, pass_count, C.byref(t), C.byref(st)))
+        tile = {"index": t.tileIndex, "min": tuple(t.pixelMin), "max": tuple(t.pixelMax),
+                "spp": t.tileSPP}
+        return tile, {"dead_paths": st.deadPaths, "ext_rays": st.extensionRays,
+                      "shadow_rays": st.shadowRays, "iterations": st.iterations,
+                      "finished": bool(st.finished), "host_waits": st.hostWaits}
+
+    def take_tile(self, tile):
+        """The last tile's sums [4][h][w]; those film pixels are cleared."""
+        w = tile["max"][0] - tile["min"][0]
+        h = tile["max"][1] - tile["min"][1]
+        out = np.zeros((4, h, w), np.float32)
+        check(lib().mray_renderer_take_tile(self.h, out.ctypes.data))
         return out
 
     def light_table(self, n):

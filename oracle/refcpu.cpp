@@ -2391,6 +2391,183 @@ static RenderOut Render(const Scene& s, const MRayRenderParams& p, float* film, 
     return out;
 }
 
+// ------------------------------------------------------------------------
+// Tiled rendering: ImageTiler (Tracer/RenderImage.cpp:20-136) and
+// PathTracerRendererT::DoLatencyRender (TracerDLL/PathTracerRenderer.cu:
+// 1076-1180) with RNGGroupIndependent::SetupRange (Random.cu:728-774).
+// ------------------------------------------------------------------------
+// ImageTiler::FindOptimumTileSize (RenderImage.cpp:20-52); a hint below one
+// pixel per axis is taken as 1 (the reference would divide by zero).
+static void TileSize(uint32_t W, uint32_t H, uint32_t hint, uint32_t out[2])
+{
+    float aspectRatio = float(W) / float(H);
+    float factor = std::sqrt(float(hint) / aspectRatio);
+    uint32_t tileHint[2] = {std::max(1u, uint32_t(std::round(aspectRatio * factor))),
+                            std::max(1u, uint32_t(std::round(factor)))};
+    uint32_t fbSize[2] = {W, H};
+    for(int i = 0; i < 2; i++)
+    {
+        if(fbSize[i] < tileHint[i])
+        {
+            out[i] = fbSize[i];
+            continue;
+        }
+        float tileCountF = float(fbSize[i]) / float(tileHint[i]);
+        uint32_t tileCount = uint32_t(std::round(tileCountF));
+        uint32_t result = fbSize[i] / tileCount;
+        uint32_t residual = fbSize[i] % tileCount;
+        residual = (residual + tileCount - 1) / tileCount;
+        out[i] = result + residual;
+    }
+}
+
+// CameraPinhole::GenerateSubCamera (CamerasDefault.hpp:226-247)
+static Camera SubCamera(const Camera& c, uint32_t sx, uint32_t sy, uint32_t cx, uint32_t cy)
+{
+    Camera t = c;
+    V2 delta = {c.planeSize.x / float(cx), c.planeSize.y / float(cy)};
+    V2 dist = {float(sx) * delta.x, float(sy) * delta.y};
+    t.bottomLeft = c.bottomLeft + ((dist.x * c.right) + (dist.y * c.up));
+    t.planeSize = delta;
+    return t;
+}
+
+struct TileCall { uint32_t tile, x0, y0, x1, y1, spp; };
+
+// One call per passCounts[c]: that many more samples of the current tile,
+// run until every one of its paths is in the film, then the next tile
+// (row-major, x fastest).  Slot j holds the random state of the tile's
+// pixel (x0 + j mod w, y0 + j / w): loaded when the tile becomes current,
+// stored back when another tile replaces it; slots past a narrower edge
+// tile keep whatever they held.  Path g of a tile is its pixel g mod (w h),
+// rendered through the tile's sub-camera at w x h.  film [4][H][W]
+// accumulates every call.
+static RenderOut RenderTiled(const Scene& s, const MRayRenderParams& p, const uint32_t* passCounts,
+                             uint32_t calls, float* film, uint32_t threads, TileCall* out)
+{
+    RenderOut ro;
+    const uint32_t W = p.width, H = p.height;
+    const uint64_t P = uint64_t(W) * H;
+    uint32_t ts[2];
+    TileSize(W, H, p.maxRayCount ? p.maxRayCount : (1u << 21), ts);
+    const uint32_t cx = (W + ts[0] - 1) / ts[0], cy = (H + ts[1] - 1) / ts[1];
+    const uint32_t T = cx * cy;
+    const uint32_t N = ts[0] * ts[1];
+    const float filterRadius = (p.filterRadius > 0.0f) ? p.filterRadius : 1.0f;
+    const Camera cam = MakeCamera(s.cam);
+    std::vector<uint32_t> pixRng(P);
+    SeedStates(p.seed, uint32_t(P), pixRng.data());
+    std::vector<uint32_t> slotSeeds(N);
+    SeedStates(p.seed, N, slotSeeds.data());
+    std::vector<Slot> slots(N);
+    for(uint32_t i = 0; i < N; i++)
+    {
+        std::memset(&slots[i], 0, sizeof(Slot));
+        slots[i].status = ST_INVALID;
+        slots[i].rng = slotSeeds[i];
+    }
+    std::memset(film, 0, sizeof(float) * 4 * P);
+    std::vector<uint64_t> tilePaths(T, 0);
+    std::vector<uint32_t> tileSPP(T, 0);
+    uint32_t cur = 0;
+    int loaded = -1;
+    auto rect = [&](uint32_t t, uint32_t& x0, uint32_t& y0, uint32_t& w, uint32_t& h)
+    {
+        x0 = (t % cx) * ts[0];
+        y0 = (t / cx) * ts[1];
+        w = std::min(ts[0], W - x0);
+        h = std::min(ts[1], H - y0);
+    };
+    std::vector<RenderCounters> ctr(std::max(threads, 1u));
+    std::vector<uint32_t> alive;
+    alive.reserve(N);
+    for(uint32_t c = 0; c < calls; c++)
+    {
+        const uint32_t t = cur;
+        uint32_t x0, y0, w, h;
+        rect(t, x0, y0, w, h);
+        if(loaded != int(t))
+        {
+            if(loaded >= 0)
+            {
+                uint32_t a0, b0, aw, bh;
+                rect(uint32_t(loaded), a0, b0, aw, bh);
+                for(uint32_t j = 0; j < aw * bh; j++)
+                    pixRng[size_t(b0 + j / aw) * W + a0 + j % aw] = slots[j].rng;
+            }
+            for(uint32_t j = 0; j < w * h; j++)
+                slots[j].rng = pixRng[size_t(y0 + j / w) * W + x0 + j % w];
+            loaded = int(t);
+        }
+        const Camera sub = SubCamera(cam, t % cx, t / cx, cx, cy);
+        const uint64_t tp = uint64_t(w) * h;
+        const uint32_t spp = tileSPP[t] + passCounts[c];
+        const uint64_t limit = uint64_t(spp) * tp;
+        uint64_t counter = tilePaths[t];
+        for(;;)
+        {
+            // ReloadPaths over the tile's path indices (PathTracerRendererBase.cu:69-203)
+            uint64_t k = 0;
+            for(uint32_t i = 0; i < N; i++)
+            {
+                Slot& sl = slots[i];
+                if(sl.status == ST_ALIVE) continue;
+                if(counter + k < limit)
+                {
+                    uint64_t local = (counter + k) % tp;
+                    uint32_t px = uint32_t(local % w), py = uint32_t(local / w);
+                    float xi0 = ToFloat01(PcgNext(sl.rng));
+                    float xi1 = ToFloat01(PcgNext(sl.rng));
+                    CamRay cr = GenCamRay(sub, px, py, w, h, xi0, xi1, filterRadius);
+                    sl.pos = cr.pos; sl.dir = cr.dir; sl.tMin = cr.tMin; sl.tMax = cr.tMax;
+                    sl.filterWeight = cr.weight;
+                    sl.pixel = (y0 + py) * W + x0 + px;
+                    sl.radiance = Splat4(0.0f);
+                    sl.throughput = Splat4(1.0f);
+                    sl.depth = 0;
+                    sl.status = ST_ALIVE;
+                    sl.type = RT_CAMERA;
+                    sl.dispersed = false;
+                    sl.prevMatPdf = 0.0f;
+                    if(p.spectral)
+                    {
+                        float xw = ToFloat01(PcgNext(sl.rng));
+                        SampleWavelengths(xw, sl.waves, sl.wavePdf);
+                    }
+                    k++;
+                }
+                else sl.status = ST_INVALID;
+            }
+            counter += k;
+            alive.clear();
+            for(uint32_t i = 0; i < N; i++) if(slots[i].status == ST_ALIVE) alive.push_back(i);
+            if(alive.empty()) break;
+            ro.iterations++;
+            for(auto& cc : ctr) cc = RenderCounters{};
+            ParallelFor(uint32_t(alive.size()), threads, [&](uint32_t i, uint32_t th)
+            {
+                BounceSlot(s, p, slots[alive[i]], ctr[th]);
+            });
+            for(auto& cc : ctr) { ro.extRays += cc.extRays; ro.shadowRays += cc.shadowRays; }
+            for(uint32_t i : alive)
+            {
+                Slot& sl = slots[i];
+                if(sl.status != ST_DEAD) continue;
+                V4 rad = sl.radiance;
+                if(p.spectral) rad = SpectrumToRGB(rad, sl.waves, sl.wavePdf, sl.dispersed);
+                FilmAdd(film, W, H, sl.pixel, rad, sl.filterWeight);
+                sl.status = ST_INVALID;
+                ro.deadPaths++;
+            }
+        }
+        tilePaths[t] = limit;
+        tileSPP[t] = spp;
+        if(out) out[c] = TileCall{t, x0, y0, x0 + w, y0 + h, spp};
+        cur = (t + 1) % T;
+    }
+    return ro;
+}
+
 // Batch of views over several scenes (mray_batch_create semantics; no
 // reference counterpart): one slot pool, global path indices taken view by
 // view (view k owns [sum of earlier W*H*spp, + its own)), each slot shaded
@@ -2719,6 +2896,44 @@ int rc_render(void* h, const MRayRenderParams* p, float* film, uint32_t threads,
     if(p->spectral && !gSpec.loaded) return 1;
     if(threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
     RenderOut o = Render(s, *p, film, threads, maxIterations ? maxIterations : 0xFFFFFFFFu, pathLimit);
+    if(stats)
+    {
+        stats[0] = o.deadPaths;
+        stats[1] = o.extRays;
+        stats[2] = o.shadowRays;
+        stats[3] = o.iterations;
+    }
+    return 0;
+}
+
+// Tiled render: calls[c] = passCount of call c; film [4][H][W] accumulates
+// every call; tiles[6 * c] = tile, x0, y0, x1, y1, tile spp of call c;
+// tileInfo[4] = tile size (2), tile count (2).  stats[4] as rc_render.
+int rc_render_tiled(void* h, const MRayRenderParams* p, const uint32_t* calls, uint32_t nCalls,
+                    float* film, uint32_t threads, uint32_t* tiles, uint32_t* tileInfo,
+                    uint64_t* stats)
+{
+    const Scene& s = *static_cast<Scene*>(h);
+    if(p->spectral && !gSpec.loaded) return 1;
+    if(threads == 0) threads = std::max(1u, std::thread::hardware_concurrency());
+    std::vector<TileCall> tc(nCalls);
+    RenderOut o = RenderTiled(s, *p, calls, nCalls, film, threads, tc.data());
+    uint32_t ts[2];
+    TileSize(p->width, p->height, p->maxRayCount ? p->maxRayCount : (1u << 21), ts);
+    if(tileInfo)
+    {
+        tileInfo[0] = ts[0];
+        tileInfo[1] = ts[1];
+        tileInfo[2] = (p->width + ts[0] - 1) / ts[0];
+        tileInfo[3] = (p->height + ts[1] - 1) / ts[1];
+    }
+    if(tiles)
+        for(uint32_t c = 0; c < nCalls; c++)
+        {
+            const TileCall& t = tc[c];
+            uint32_t* o6 = tiles + 6 * size_t(c);
+            o6[0] = t.tile; o6[1] = t.x0; o6[2] = t.y0; o6[3] = t.x1; o6[4] = t.y1; o6[5] = t.spp;
+        }
     if(stats)
     {
         stats[0] = o.deadPaths;

@@ -88,6 +88,9 @@ def lib():
         L.rc_render.restype = C.c_int
         L.rc_render.argtypes = [P, C.POINTER(abi.MRayRenderParams), f32, C.c_uint32,
                                 C.c_uint32, C.c_uint64, u64]
+        L.rc_render_tiled.restype = C.c_int
+        L.rc_render_tiled.argtypes = [P, C.POINTER(abi.MRayRenderParams), u32, C.c_uint32, f32,
+                                      C.c_uint32, u32, u32, u64]
         L.rc_render_batch.restype = C.c_int
         L.rc_render_batch.argtypes = [P, C.c_uint32, P, C.c_uint32, C.POINTER(abi.MRayRenderParams),
                                       P, C.c_uint32, u64]
@@ -306,6 +309,26 @@ class OracleScene:
             raise RuntimeError("rc_render failed (spectral tables not loaded?)")
         return film, {"dead_paths": int(stats[0]), "ext_rays": int(stats[1]),
                       "shadow_rays": int(stats[2]), "iterations": int(stats[3])}
+
+    def render_tiled(self, params, passes, threads=0):
+        """Tiled render (DoLatencyRender): one call per entry of `passes`, each
+        that many samples of the current tile.  Returns the film accumulated
+        over every call, the calls' tiles (index, x0, y0, x1, y1, tile spp),
+        (tile size, tile count) and the statistics."""
+        film = np.zeros((4, params.height, params.width), np.float32)
+        calls = np.ascontiguousarray(passes, np.uint32)
+        tiles = np.zeros((len(calls), 6), np.uint32)
+        info = np.zeros(4, np.uint32)
+        stats = np.zeros(4, np.uint64)
+        rc = lib().rc_render_tiled(self.h, C.byref(params), calls.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   len(calls), abi.fptr(film), threads,
+                                   tiles.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   info.ctypes.data_as(C.POINTER(C.c_uint32)), abi.u64ptr(stats))
+        if rc != 0:
+            raise RuntimeError("rc_render_tiled failed (spectral tables not loaded?)")
+        return film, tiles, (tuple(int(v) for v in info[:2]), tuple(int(v) for v in info[2:])), {
+            "dead_paths": int(stats[0]), "ext_rays": int(stats[1]),
+            "shadow_rays": int(stats[2]), "iterations": int(stats[3])}
 
     def light_probe(self, point, rn):
         out = np.zeros(8, np.float32)

@@ -4,7 +4,9 @@
 // a scene loader makes (SceneLoaderMRay.cpp: groups -> reservations ->
 // commits -> attribute pushes -> surfaces), then either
 //   flatten <out>   writes CommitSurfaces' host-side scene (no GPU), or
-//   render  <out> W H spp spectral mode rr0 rr1 seed slots [exr prefix]
+//   render  <out> W H spp spectral mode rr0 rr1 seed slots renderMode burstSize [exr prefix]
+//                   (slots = parallelizationHint, 0: W x H; renderMode
+//                   Throughput or Latency)
 //                   renders through SetupRenderEnv / CreateRenderer /
 //                   StartRender / DoRenderWork with a consumer thread that
 //                   plays the reference's run command: it waits on the
@@ -396,7 +398,8 @@ int Run(int argc, char** argv)
     uint32_t W = 0, H = 0, spp = 0, spectral = 0, sampleMode = 2, rr0 = 8, rr1 = 8, slots = 0;
     if(mode == "render")
     {
-        if(argc < 14) throw MRayError("render needs: W H spp spectral mode rr0 rr1 seed slots");
+        if(argc < 16)
+            throw MRayError("render needs: W H spp spectral mode rr0 rr1 seed slots renderMode burstSize");
         W = uint32_t(std::atoi(argv[5])); H = uint32_t(std::atoi(argv[6]));
         spp = uint32_t(std::atoi(argv[7])); spectral = uint32_t(std::atoi(argv[8]));
         sampleMode = uint32_t(std::atoi(argv[9]));
@@ -439,8 +442,8 @@ int Run(int argc, char** argv)
             RendererId r = t->CreateRenderer(spectral ? "(R)PathTracerSpectral" : "(R)PathTracerRGB");
             const char* modes[3] = {"Pure", "WithNEE", "WithNEEAndMIS"};
             t->PushRendererAttribute(r, 0, Data(std::vector<uint32_t>{spp}));
-            t->PushRendererAttribute(r, 1, Data(std::vector<uint32_t>{4u}));
-            t->PushRendererAttribute(r, 2, Str("Throughput"));
+            t->PushRendererAttribute(r, 1, Data(std::vector<uint32_t>{uint32_t(std::atoi(argv[15]))}));
+            t->PushRendererAttribute(r, 2, Str(argv[14]));
             t->PushRendererAttribute(r, 3, Str(modes[sampleMode]));
             t->PushRendererAttribute(r, 4, Data(std::vector<Vector2ui>{Vector2ui{{rr0, rr1}}}));
             t->PushRendererAttribute(r, 5, Str("Uniform"));
@@ -475,14 +478,21 @@ int Run(int argc, char** argv)
                     for(int c = 0; c < 3; c++)
                         rgb[c] = reinterpret_cast<const float*>(info.data + sec.pixStartOffsets[c]);
                     const float* w = reinterpret_cast<const float*>(info.data + sec.weightStartOffset);
-                    for(size_t i = 0; i < P; i++)
-                    {
-                        double sOut = acc[3 * P + i], total = sOut + double(w[i]);
-                        for(int c = 0; c < 3; c++)
-                            acc[c * P + i] = (total == 0.0) ? 0.0
-                                : (acc[c * P + i] * sOut + double(rgb[c][i])) / total;
-                        acc[3 * P + i] = total;
-                    }
+                    // the section's pixels [pixelMin, pixelMax), scanlines
+                    // packed at its own width (RunCommand.cpp:609-700)
+                    const uint32_t sw = sec.pixelMax[0] - sec.pixelMin[0];
+                    const uint32_t sh = sec.pixelMax[1] - sec.pixelMin[1];
+                    for(uint32_t y = 0; y < sh; y++)
+                        for(uint32_t x = 0; x < sw; x++)
+                        {
+                            const size_t k = size_t(y) * sw + x;
+                            const size_t i = size_t(sec.pixelMin[1] + y) * W + sec.pixelMin[0] + x;
+                            double sOut = acc[3 * P + i], total = sOut + double(w[k]);
+                            for(int c = 0; c < 3; c++)
+                                acc[c * P + i] = (total == 0.0) ? 0.0
+                                    : (acc[c * P + i] * sOut + double(rgb[c][k])) / total;
+                            acc[3 * P + i] = total;
+                        }
                     consumed++;
                     sem.Release();
                 }
@@ -504,15 +514,12 @@ int Run(int argc, char** argv)
                     }
                     if(calls > 100000) throw MRayError("render did not finish");
                 } while(!out.triggerSave);
-                // one more call after the save request must not ask again
+                // one more call after the save request must not ask again;
+                // its samples go past totalSPP (a tiled renderer keeps
+                // rendering, as the reference's does for an open viewer), so
+                // the saved image does not take them
                 RendererOutput extra = t->DoRenderWork();
                 if(extra.triggerSave) throw MRayError("triggerSave reported twice");
-                if(extra.imageOut)
-                {
-                    std::lock_guard<std::mutex> lk(qm);
-                    queue.push_back(*extra.imageOut);
-                    qc.notify_all();
-                }
             }
             catch(...)
             {
@@ -533,7 +540,7 @@ int Run(int argc, char** argv)
             if(!f) throw MRayError("cannot write film");
             std::fwrite(img.data(), sizeof(float), img.size(), f);
             std::fclose(f);
-            if(argc > 14)
+            if(argc > 16)
             {
                 // RunCommand.cpp:873-930: sums back from mean x weight, then
                 // <prefix>_<wpp>wpp.exr
@@ -544,7 +551,7 @@ int Run(int argc, char** argv)
                 if(!exr) throw MRayError("mray_film_write_exr not found");
                 char name[64];
                 std::snprintf(name, sizeof(name), "_%08.0fwpp.exr", std::round(out.analytics->workPerPixel));
-                std::string path = std::string(argv[14]) + name;
+                std::string path = std::string(argv[16]) + name;
                 if(exr(img.data(), W, H, path.c_str()) != 0) throw MRayError("EXR write failed");
                 std::printf("saved %s\n", path.c_str());
             }

@@ -16,16 +16,18 @@ from tests.test_tracer_api import DRIVER, TRACER_LIB, _built, write_scene_file
 pytestmark = pytest.mark.gpu
 
 
-def _driver_render(scene, tmp_path, w, h, spp, spectral, mode, rr, seed, exr_prefix=None):
+def _driver_render(scene, tmp_path, w, h, spp, spectral, mode, rr, seed, exr_prefix=None,
+                   render_mode="Throughput", burst=4, hint=0):
     """The driver plays the run command: it accumulates the image sections
     handed over through the TimelineSemaphore into means + weights.  Returned
-    as a film (sums = mean x weight) for comparison with the C-ABI's."""
+    as a film (sums = mean x weight) for comparison with the C-ABI's.
+    hint = TracerParameters::parallelizationHint (0: width x height)."""
     _built()
     src = str(tmp_path / "scene.bin")
     out = str(tmp_path / "film.bin")
     write_scene_file(scene, src)
     cmd = [DRIVER, TRACER_LIB, src, "render", out, str(w), str(h), str(spp), str(int(spectral)),
-           str(mode), str(rr[0]), str(rr[1]), str(seed), "0"]
+           str(mode), str(rr[0]), str(rr[1]), str(seed), str(hint), render_mode, str(burst)]
     if exr_prefix:
         cmd.append(exr_prefix)
     print("run:", " ".join(cmd[2:]), flush=True)
@@ -86,7 +88,7 @@ def test_json_scene_renders_like_cabi(torch_gpu, tmp_path):
     scenes.to_mray_json(scene, src)
     out = str(tmp_path / "film.bin")
     cmd = [DRIVER, TRACER_LIB, src, "render", out, "64", "64", "4", "0", str(abi.SAMPLE_NEE_MIS),
-           "8", "8", "5", "0"]
+           "8", "8", "5", "0", "Throughput", "4"]
     r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
     print(r.stdout, r.stderr, flush=True)
     assert r.returncode == 0, r.stderr
@@ -116,3 +118,46 @@ def test_tracer_saves_exr_like_the_run_command(torch_gpu, tmp_path):
     b = _cabi_render(scene, abi.render_params(48, 32, 4, rr=(8, 8), mode=abi.SAMPLE_NEE_MIS,
                                               seed=3))
     assert relmse(img, film_image(b)) < 1e-6
+
+
+def _tiled_cabi(scene, params, passes):
+    from mray_amd import native
+    acc = native.Accel(scene)
+    rnd = native.Renderer(acc, params)
+    for n in passes:
+        rnd.render_tile(n)
+    film = rnd.film()
+    tiles = rnd.tiling()
+    rnd.close()
+    acc.close()
+    return film, tiles
+
+
+@pytest.mark.parametrize("render_mode,burst", [("Latency", 4), ("Throughput", 2)])
+def test_tracer_api_tile_by_tile(torch_gpu, tmp_path, render_mode, burst):
+    """A parallelization hint below the image size splits it into tiles
+    (ImageTiler); Latency renders one sample of one tile per DoRenderWork,
+    Throughput burstSize samples (PathTracerRendererBase::DoRender).  The
+    consumer's image, assembled from the tile sections, equals the C-ABI's
+    tiled renderer driven with the same calls."""
+    W, H, spp, hint = 90, 50, 4, 1200
+    scene = scenes.cornell(aspect=W / H)
+    a = _driver_render(scene, tmp_path, W, H, spp, False, abi.SAMPLE_NEE_MIS, (8, 8), 2,
+                       render_mode=render_mode, burst=burst, hint=hint)
+    params = abi.render_params(W, H, spp, rr=(8, 8), mode=abi.SAMPLE_NEE_MIS, seed=2,
+                               max_ray_count=hint, flags=abi.RENDER_TILED)
+    per_call = 1 if render_mode == "Latency" else burst
+    from mray_amd import native
+    acc = native.Accel(scene)
+    rnd = native.Renderer(acc, params)
+    (tw, th), (cx, cy) = rnd.tiling()
+    rnd.close()
+    acc.close()
+    assert cx * cy > 1
+    passes = [per_call] * (cx * cy * (spp // per_call))
+    b, _ = _tiled_cabi(scene, params, passes)
+    assert np.allclose(a[3], float(spp)) and np.allclose(b[3], float(spp))
+    e = relmse(film_image(a), film_image(b))
+    print(f"{render_mode} burst {burst}: tiles {tw}x{th} x {cx}x{cy}, relMSE TracerI vs C-ABI "
+          f"{e:.3e}", flush=True)
+    assert e < 1e-8
