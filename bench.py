@@ -761,12 +761,22 @@ def main():
     scene = scenes.terrain_spheres(grid=args.grid, spheres=args.spheres, width=args.width,
                                    height=args.height)
     log(f"scene: {scene.triangle_count} triangles in {time.perf_counter() - t0:.1f} s")
+    # build times of the two builders: the all-GPU LBVH (Karras tree,
+    # collapse and soup on the device) and the binned SAH tree (host)
+    build_times = {}
     t0 = time.perf_counter()
     accel = native.Accel(scene, builder=args.builder, width=args.bvh_width)
     torch.cuda.synchronize()
     build_s = time.perf_counter() - t0
+    build_times[args.builder] = round(build_s, 3)
+    if args.builder != "lbvh":
+        t0 = time.perf_counter()
+        probe = native.Accel(scene, builder="lbvh", width=4)
+        torch.cuda.synchronize()
+        build_times["lbvh_gpu"] = round(time.perf_counter() - t0, 3)
+        probe.close()
     info = accel.info()
-    log(f"accel: {info} in {build_s:.2f} s")
+    log(f"accel: {info} in {build_s:.2f} s; build times {build_times}")
     params = abi.render_params(args.width, args.height, args.spp, rr=(10, 10),
                                mode=abi.SAMPLE_NEE_MIS, spectral=spectral, seed=0,
                                flags=(abi.RENDER_MATERIAL_QUEUES if args.queues else 0)
@@ -827,6 +837,7 @@ def main():
         "lane_refill": bool(args.lane_refill),
         "profiled_ms_per_step": round(prof_elapsed * 1e3 / args.steps, 2),
         "accel_build_s": round(build_s, 3),
+        "accel_build_s_by_builder": build_times,
         "phase_ms_per_step": {k: round(v[0] / args.steps, 3) for k, v in phases.items()},
         "roofline": {
             "bound": "hbm",

@@ -237,12 +237,19 @@ int mray_accel_create(const MRaySceneDesc* hScene, MRayAccel** out);
  * uses a binned surface-area-heuristic tree over the same triangle boxes,
  * LBVH collapses the Karras tree itself.  Hits are identical either way.   */
 enum { MRAY_BUILDER_SAH = 0, MRAY_BUILDER_LBVH = 1 };
+/* The LBVH builder with 4-wide nodes collapses the Karras tree and gathers
+ * the triangle soup on the GPU (the whole build stays in HBM); the SAH tree
+ * and 8-wide nodes are collapsed on the host.  This flag collapses the LBVH
+ * on the host as well (the same tree, its nodes numbered depth first rather
+ * than level by level).                                                    */
+#define MRAY_ACCEL_HOST_COLLAPSE 1u
 typedef struct MRayAccelOptions
 {
     uint32_t builder;
     uint32_t width;            /* 4: fp32 4-wide nodes; 8: compressed 8-wide
                                   nodes (8-bit quantised boxes); 0 = 4     */
-    uint32_t reserved[2];
+    uint32_t flags;            /* MRAY_ACCEL_*                              */
+    uint32_t reserved;
 } MRayAccelOptions;
 int mray_accel_create_ex(const MRaySceneDesc* hScene, const MRayAccelOptions* hOptions,
                          MRayAccel** out);
@@ -256,6 +263,13 @@ int mray_accel_info(const MRayAccel* accel, uint32_t* hBinaryNodes,
  * [n-1][3] and node boxes [n-1][6].  Any pointer may be NULL.             */
 int mray_accel_read_lbvh(const MRayAccel* accel, uint64_t* hCodes, uint32_t* hLeafOrder,
                          uint32_t* hNodes, float* hNodeAabbs);
+/* Copy out the 4-wide traversal tree (4-wide accelerators): hNodes
+ * [wideNodes][32] words per node = child boxes lox[4] hix[4] loy[4] hiy[4]
+ * loz[4] hiz[4] (floats), child[4] (node index, or 0x80000000 |
+ * (count-1) << 26 | first soup slot; 0xFFFFFFFF empty), 4 pad words; and
+ * hSoupPrims [triangleCount], the triangle of each soup slot.  Either
+ * pointer may be NULL.                                                      */
+int mray_accel_read_wide(const MRayAccel* accel, uint32_t* hNodes, uint32_t* hSoupPrims);
 /* Closest hit (BaseAcceleratorI::CastRays).  dRays [n][8] = RayGMem
  * {pos xyz, tMin, dir xyz, tMax}.  dPrim [n] triangle index or 0xFFFFFFFF;
  * dT [n]; dBary [n][2] = MetaHit (a, b).  Ties in t resolve to the lower

@@ -30,7 +30,8 @@ BUILDER_LBVH = 1
 
 
 class MRayAccelOptions(C.Structure):
-    _fields_ = [("builder", C.c_uint32), ("width", C.c_uint32), ("reserved", C.c_uint32 * 2)]
+    _fields_ = [("builder", C.c_uint32), ("width", C.c_uint32), ("flags", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class MRayMaterial(C.Structure):
@@ -252,6 +253,7 @@ RENDER_NO_GRAPH = 2
 RENDER_LIGHT_POWER = 4
 RENDER_LANE_REFILL = 8
 RENDER_TILED = 16
+ACCEL_HOST_COLLAPSE = 1
 
 
 class MRayTileInfo(C.Structure):

@@ -106,7 +106,8 @@ static int ValidateScene(const MRaySceneDesc* s)
 // AcceleratorGroupLBVH::Construct + BaseAcceleratorLBVH::InternalConstruct
 // (AcceleratorLBVH.hpp:433-582, AcceleratorLBVH.cu:537-740): K1..K7 on the
 // GPU, then the 4-wide collapse and the triangle soup.
-static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder, uint32_t width)
+static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder, uint32_t width,
+                      bool hostCollapse)
 {
     hipStream_t st = nullptr;
     const uint32_t n = s->triangleCount;
@@ -138,18 +139,12 @@ static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder, ui
                              a->dBinBoxes, a->dBinRange, dTemp, tempBytes, st));
     MRAY_HIP_CHECK(hipStreamSynchronize(st));
 
-    // Collapse to the 4-wide layout (host) and gather the triangle soup.
+    // The binary tree's topology, checked on the host: every node and leaf
+    // reached once from the root, each split at one leaf-order point.
     std::vector<uint32_t> binNodes(size_t(nodeCount) * 3), binRange(size_t(nodeCount) * 2);
-    std::vector<float> binBoxes(size_t(nodeCount) * 6), aabbs(size_t(n) * 6);
-    std::vector<uint32_t> leafOrder(n);
     MRAY_HIP_CHECK(hipMemcpy(binNodes.data(), a->dBinNodes, binNodes.size() * 4,
                              hipMemcpyDeviceToHost));
     MRAY_HIP_CHECK(hipMemcpy(binRange.data(), a->dBinRange, binRange.size() * 4,
-                             hipMemcpyDeviceToHost));
-    MRAY_HIP_CHECK(hipMemcpy(binBoxes.data(), a->dBinBoxes, binBoxes.size() * 4,
-                             hipMemcpyDeviceToHost));
-    MRAY_HIP_CHECK(hipMemcpy(aabbs.data(), a->dAabbs, aabbs.size() * 4, hipMemcpyDeviceToHost));
-    MRAY_HIP_CHECK(hipMemcpy(leafOrder.data(), a->dLeafOrder, size_t(n) * 4,
                              hipMemcpyDeviceToHost));
     if(!ValidateBinaryTree(binNodes, binRange, n))
     {
@@ -162,8 +157,6 @@ static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder, ui
                                  hipMemcpyDeviceToHost));
         MRAY_HIP_CHECK(hipMemcpy(binRange.data(), a->dBinRange, binRange.size() * 4,
                                  hipMemcpyDeviceToHost));
-        MRAY_HIP_CHECK(hipMemcpy(binBoxes.data(), a->dBinBoxes, binBoxes.size() * 4,
-                                 hipMemcpyDeviceToHost));
         a->strictTie = 1;
         if(!ValidateBinaryTree(binNodes, binRange, n))
         {
@@ -171,61 +164,90 @@ static int BuildAccel(MRayAccel* a, const MRaySceneDesc* s, uint32_t builder, ui
             return MRAY_ERR_INVALID;
         }
     }
-    if(builder == MRAY_BUILDER_SAH && n > 1)
-    {
-        // traversal tree: binned SAH over the same triangle boxes (the LBVH
-        // above stays on the device for the reference-layout products)
-        std::vector<uint32_t> sNodes, sRange, sOrder;
-        std::vector<float> sBoxes;
-        if(!BuildSahBinary(aabbs, n, sNodes, sBoxes, sRange, sOrder))
-        {
-            SetError("SAH construction produced a malformed tree");
-            return MRAY_ERR_INVALID;
-        }
-        binNodes.swap(sNodes);
-        binBoxes.swap(sBoxes);
-        binRange.swap(sRange);
-        leafOrder.swap(sOrder);
-    }
     a->builder = builder;
     a->width = width;
-    if(width == 8)
+    if(builder == MRAY_BUILDER_LBVH && width == 4 && n > 1 && !hostCollapse)
     {
-        CBvh8 w;
-        if(!CollapseToCBvh8(binNodes, binBoxes, binRange, aabbs, leafOrder, w))
+        // the whole build on the device: collapse, then the soup in leaf order
+        MRAY_TRY(CollapseToBvh4Device(a->dBinNodes, a->dBinBoxes, a->dBinRange, a->dAabbs, n, ar,
+                                      &a->dNodes, &a->wideNodes, &a->leafSlots, &a->maxDepth, st));
+        MRAY_TRY(DevAlloc(ar, (void**)&a->dSoup, size_t(n) * sizeof(SoupTri)));
+        uint8_t* dCull = nullptr;
+        if(s->hCullBackFace)
         {
-            SetError("compressed 8-wide BVH: a child box has no conservative 8-bit encoding");
-            return MRAY_ERR_INVALID;
+            MRAY_HIP_CHECK(hipMalloc((void**)&dCull, n));
+            MRAY_HIP_CHECK(hipMemcpy(dCull, s->hCullBackFace, n, hipMemcpyHostToDevice));
         }
-        a->wideNodes = uint32_t(w.nodes.size());
-        a->leafSlots = w.leafSlots;
-        a->maxDepth = w.maxDepth;
-        MRAY_TRY(Upload(ar, &a->dCNodes, w.nodes.data(), w.nodes.size()));
+        int rg = LaunchGatherSoup(a->dPos, a->dIdx, dCull, a->dLeafOrder, n, a->dSoup, st);
+        MRAY_HIP_CHECK(hipStreamSynchronize(st));
+        if(dCull) MRAY_HIP_CHECK(hipFree(dCull));
+        MRAY_TRY(rg);
     }
     else
     {
-        WideBvh w = CollapseToBvh4(binNodes, binBoxes, binRange, aabbs, leafOrder);
-        a->wideNodes = uint32_t(w.nodes.size());
-        a->leafSlots = w.leafSlots;
-        a->maxDepth = w.maxDepth;
-        MRAY_TRY(Upload(ar, &a->dNodes, w.nodes.data(), w.nodes.size()));
-    }
-    std::vector<SoupTri> soup(n);
-    for(uint32_t p = 0; p < n; p++)
-    {
-        uint32_t t = leafOrder[p];
-        SoupTri& o = soup[p];
-        for(int v = 0; v < 3; v++)
+        // Collapse on the host (SAH tree, 8-wide nodes, or on request) and
+        // gather the triangle soup there.
+        std::vector<float> binBoxes(size_t(nodeCount) * 6), aabbs(size_t(n) * 6);
+        std::vector<uint32_t> leafOrder(n);
+        MRAY_HIP_CHECK(hipMemcpy(binBoxes.data(), a->dBinBoxes, binBoxes.size() * 4,
+                                 hipMemcpyDeviceToHost));
+        MRAY_HIP_CHECK(hipMemcpy(aabbs.data(), a->dAabbs, aabbs.size() * 4, hipMemcpyDeviceToHost));
+        MRAY_HIP_CHECK(hipMemcpy(leafOrder.data(), a->dLeafOrder, size_t(n) * 4,
+                                 hipMemcpyDeviceToHost));
+        if(builder == MRAY_BUILDER_SAH && n > 1)
         {
-            const float* src = s->hPositions + 3 * size_t(s->hIndices[3 * size_t(t) + v]);
-            float* dst = (v == 0) ? o.v0 : (v == 1) ? o.v1 : o.v2;
-            dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
+            // traversal tree: binned SAH over the same triangle boxes (the LBVH
+            // above stays on the device for the reference-layout products)
+            std::vector<uint32_t> sNodes, sRange, sOrder;
+            std::vector<float> sBoxes;
+            if(!BuildSahBinary(aabbs, n, sNodes, sBoxes, sRange, sOrder))
+            {
+                SetError("SAH construction produced a malformed tree");
+                return MRAY_ERR_INVALID;
+            }
+            binNodes.swap(sNodes);
+            binBoxes.swap(sBoxes);
+            binRange.swap(sRange);
+            leafOrder.swap(sOrder);
         }
-        o.prim = t;
-        o.cull = s->hCullBackFace ? s->hCullBackFace[t] : 0u;
-        o.pad = 0;
+        if(width == 8)
+        {
+            CBvh8 w;
+            if(!CollapseToCBvh8(binNodes, binBoxes, binRange, aabbs, leafOrder, w))
+            {
+                SetError("compressed 8-wide BVH: a child box has no conservative 8-bit encoding");
+                return MRAY_ERR_INVALID;
+            }
+            a->wideNodes = uint32_t(w.nodes.size());
+            a->leafSlots = w.leafSlots;
+            a->maxDepth = w.maxDepth;
+            MRAY_TRY(Upload(ar, &a->dCNodes, w.nodes.data(), w.nodes.size()));
+        }
+        else
+        {
+            WideBvh w = CollapseToBvh4(binNodes, binBoxes, binRange, aabbs, leafOrder);
+            a->wideNodes = uint32_t(w.nodes.size());
+            a->leafSlots = w.leafSlots;
+            a->maxDepth = w.maxDepth;
+            MRAY_TRY(Upload(ar, &a->dNodes, w.nodes.data(), w.nodes.size()));
+        }
+        std::vector<SoupTri> soup(n);
+        for(uint32_t p = 0; p < n; p++)
+        {
+            uint32_t t = leafOrder[p];
+            SoupTri& o = soup[p];
+            for(int v = 0; v < 3; v++)
+            {
+                const float* src = s->hPositions + 3 * size_t(s->hIndices[3 * size_t(t) + v]);
+                float* dst = (v == 0) ? o.v0 : (v == 1) ? o.v1 : o.v2;
+                dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2];
+            }
+            o.prim = t;
+            o.cull = s->hCullBackFace ? s->hCullBackFace[t] : 0u;
+            o.pad = 0;
+        }
+        MRAY_TRY(Upload(ar, &a->dSoup, soup.data(), soup.size()));
     }
-    MRAY_TRY(Upload(ar, &a->dSoup, soup.data(), soup.size()));
 
     // Traversal stack spill area: <= width-1 entries per level below the root.
     int dev = 0;
@@ -339,7 +361,7 @@ int mray_accel_create_ex(const MRaySceneDesc* hScene, const MRayAccelOptions* hO
     MRayAccel* a = new MRayAccel();
     a->nTri = hScene->triangleCount;
     a->nVert = hScene->vertexCount;
-    int r = BuildAccel(a, hScene, hOpt->builder, width);
+    int r = BuildAccel(a, hScene, hOpt->builder, width, (hOpt->flags & MRAY_ACCEL_HOST_COLLAPSE) != 0u);
     if(r == MRAY_OK) r = UploadSceneData(a, hScene);
     if(r != MRAY_OK)
     {
@@ -357,6 +379,20 @@ int mray_accel_destroy(MRayAccel* accel)
     (void)hipDeviceSynchronize();
     accel->arena.Release();
     delete accel;
+    return MRAY_OK;
+}
+
+int mray_accel_read_wide(const MRayAccel* a, uint32_t* hNodes, uint32_t* hSoupPrims)
+{
+    if(!a) { SetError("accel is NULL"); return MRAY_ERR_INVALID; }
+    if(a->width != 4 || !a->dNodes) { SetError("read_wide: not a 4-wide accelerator"); return MRAY_ERR_INVALID; }
+    MRAY_HIP_CHECK(hipDeviceSynchronize());
+    if(hNodes)
+        MRAY_HIP_CHECK(hipMemcpy(hNodes, a->dNodes, size_t(a->wideNodes) * sizeof(BNode4),
+                                 hipMemcpyDeviceToHost));
+    if(hSoupPrims && a->nTri)
+        MRAY_HIP_CHECK(hipMemcpy2D(hSoupPrims, 4, reinterpret_cast<const char*>(a->dSoup) + 12,
+                                   sizeof(SoupTri), 4, a->nTri, hipMemcpyDeviceToHost));
     return MRAY_OK;
 }
 

@@ -65,6 +65,15 @@ struct alignas(16) SoupTri
 };
 static_assert(sizeof(SoupTri) == 48, "SoupTri layout");
 
+struct DeviceArena;
+// collapse.hip: the 4-wide tree of an n >= 2 leaf binary tree built on the
+// device (nodes allocated from `ar`), and the triangle soup in leaf order.
+int CollapseToBvh4Device(const uint32_t* dBinNodes, const float* dBinBoxes, const uint32_t* dBinRange,
+                         const float* dTriAabbs, uint32_t n, DeviceArena& ar, BNode4** dNodesOut,
+                         uint32_t* nodeCount, uint32_t* leafSlots, uint32_t* maxDepth, hipStream_t s);
+int LaunchGatherSoup(const float* dPos, const uint32_t* dIdx, const uint8_t* dCull,
+                     const uint32_t* dLeafOrder, uint32_t n, SoupTri* dSoup, hipStream_t s);
+
 // Compressed 8-wide node (one 128-byte line): child boxes quantised to 8
 // bits per plane on a per-node grid, decoded as fmaf(q, 2^e, origin) -- the
 // builder checks every decoded box contains the exact one, so traversal stays

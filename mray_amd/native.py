@@ -48,6 +48,7 @@ EXPORTS = [
     "mray_accel_set_environment", "mray_accel_environment_cdfs",
     "mray_instanced_scene_set_environment", "mray_renderer_light_table",
     "mray_renderer_tiling", "mray_renderer_render_tile", "mray_renderer_take_tile",
+    "mray_accel_read_wide",
 ]
 
 
@@ -97,6 +98,7 @@ def lib():
     L.mray_renderer_render_tile.argtypes = [P, u32, C.POINTER(abi.MRayTileResult),
                                             C.POINTER(abi.MRayRenderStats)]
     L.mray_renderer_take_tile.argtypes = [P, P]
+    L.mray_accel_read_wide.argtypes = [P, P, P]
     L.mray_film_write_exr.argtypes = [P, u32, u32, C.c_char_p]
     L.mray_renderer_reset.argtypes = [P]
     L.mray_renderer_film_device.argtypes = [P, C.POINTER(P)]
@@ -279,7 +281,7 @@ def spectral_tables():
 class Accel:
     """Device scene + LBVH + 4-wide BVH (MRayAccel)."""
 
-    def __init__(self, scene, builder="sah", width=4):
+    def __init__(self, scene, builder="sah", width=4, host_collapse=False):
         require_gpu()
         self.scene = scene
         self.desc = scene.desc()
@@ -287,6 +289,7 @@ class Accel:
         opt = abi.MRayAccelOptions()
         opt.builder = {"sah": abi.BUILDER_SAH, "lbvh": abi.BUILDER_LBVH}[builder]
         opt.width = width
+        opt.flags = abi.ACCEL_HOST_COLLAPSE if host_collapse else 0
         check(lib().mray_accel_create_ex(C.byref(self.desc), C.byref(opt), C.byref(h)))
         self.h = h
         self.n = scene.triangle_count
@@ -314,6 +317,14 @@ class Accel:
             self.close()
         except Exception:
             pass
+
+    def read_wide(self):
+        """The 4-wide tree: (nodes [wide][32] uint32 words, soup triangle ids)."""
+        wide = self.info()["wide_nodes"]
+        nodes = np.zeros((wide, 32), np.uint32)
+        prims = np.zeros(self.n, np.uint32)
+        check(lib().mray_accel_read_wide(self.h, nodes.ctypes.data, prims.ctypes.data))
+        return nodes, prims
 
     def info(self):
         a, b, c = C.c_uint32(), C.c_uint32(), C.c_uint32()

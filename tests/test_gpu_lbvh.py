@@ -137,3 +137,50 @@ def test_stage_lbvh_build_degenerate_codes(torch_gpu):
     # children are written by their own node's thread (deterministic); in a
     # malformed tree several nodes claim one child, so parent words race
     assert np.array_equal(nodes[:, :2], rnodes[:, :2])
+
+
+def _canonical(nodes, prims):
+    """The 4-wide tree as nested tuples from the root: each slot's box bits
+    and either its leaf word or its subtree (node numbering removed)."""
+    out = []
+    stack = [0]
+    seen = 0
+    while stack:
+        v = stack.pop()
+        seen += 1
+        w = nodes[v]
+        row = [tuple(w[:24].tolist())]
+        for k in range(4):
+            c = int(w[24 + k])
+            if c == 0xFFFFFFFF or c & 0x80000000:
+                row.append(c)
+            else:
+                row.append("node")
+                stack.append(c)
+        out.append(tuple(row))
+    return out, seen
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2, 3])
+def test_device_collapse_is_host_collapse(torch_gpu, idx):
+    """The LBVH builder's 4-wide tree and triangle soup built on the GPU
+    (collapse.hip) against the same collapse on the host: the same nodes in
+    the same depth-first order of their children, the same leaf words and
+    the same soup."""
+    from mray_amd import native
+    scene = (_scenes() + [scenes.random_triangles(300001, seed=9)])[idx]
+    dev = native.Accel(scene, builder="lbvh")
+    host = native.Accel(scene, builder="lbvh", host_collapse=True)
+    dn, dp = dev.read_wide()
+    hn, hp = host.read_wide()
+    di, hi = dev.info(), host.info()
+    dev.close()
+    host.close()
+    print(f"scene {scene.name}: {scene.triangle_count} tris, wide nodes {di['wide_nodes']}", flush=True)
+    assert di["wide_nodes"] == hi["wide_nodes"] == len(dn) == len(hn)
+    assert np.array_equal(dp, hp)
+    assert np.array_equal(np.sort(dp), np.arange(scene.triangle_count, dtype=np.uint32))
+    cd, nd = _canonical(dn, dp)
+    ch, nh = _canonical(hn, hp)
+    assert nd == nh == len(dn)
+    assert cd == ch
