@@ -229,6 +229,10 @@ int BuildInstanced(MRayInstScene* sc, const MRayAccel* const* meshes, uint32_t m
     c.tlasOrder = dOrder;
     c.inst = dInst;
     c.tlasVisits = sc->tlasNodes + 1u;
+    // a walk enters each instance at most once and visits each node of its
+    // tree at most once while inside
+    c.instWalkVisits = uint32_t(std::min<uint64_t>(0xFFFFFFFFull,
+                                                   uint64_t(c.tlasVisits) + uint64_t(n) * c.maxVisits));
     return MRAY_OK;
 }
 } // namespace

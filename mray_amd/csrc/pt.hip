@@ -752,7 +752,7 @@ KTrace(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ active, Sched* __re
         if constexpr(INST)
         {
             uint32_t instId;
-            h = TraceInst<false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+            h = TraceInstFlat<false, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
                                         o.w, d.w, instId, &nodes, &tris);
             S.inst[i] = instId;
         }
@@ -1718,7 +1718,7 @@ KShadow(TraceCtx ctx, SlotSoA S, const uint32_t* __restrict__ shadowQ, Sched* __
         if constexpr(INST)
         {
             uint32_t instId;
-            h = TraceInst<true, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
+            h = TraceInstFlat<true, COUNT>(ctx, lstack, gtid, V3{o.x, o.y, o.z}, V3{d.x, d.y, d.z},
                                        o.w, d.w, instId, &nodes, &tris);
         }
         else
