@@ -13,8 +13,10 @@
 // Parity pinning (what checks each part):
 //   Morton 63/30       reference golden words T_GraphicsFunctions.cpp:147-305,
 //                      brute-force bit interleave (tests/test_oracle_morton.py)
-//   LBVH topology      Karras invariants + brute force on tiny inputs
-//                      (tests/test_oracle_lbvh.py); no reference vector exists
+//   LBVH topology      Karras invariants, and node for node (children,
+//                      parents, numbering) an independent top-down Karras
+//                      construction (tests/test_oracle_lbvh.py); no
+//                      reference vector exists
 //   closest / any hit  brute force over all triangles (same Moller-Trumbore)
 //   PCG32 / seeding    std::mt19937 standard values + closed form
 //   camera / filter    closed forms (tests/test_oracle_sampling.py)

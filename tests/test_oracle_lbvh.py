@@ -4,7 +4,10 @@ No reference test pins LBVH topology (there are no LBVH tests upstream), so
 the construction is pinned by what Karras 2012 fixes: each internal node
 covers a contiguous range of sorted leaves split at the first position where
 the highest differing bit of (code, index) changes, every node/leaf has one
-parent, and every box is the exact union of its children.
+parent, and every box is the exact union of its children.  The node
+numbering, which the reference's traversal order and node arrays follow
+(internal node i has i at one end of its range), is pinned node for node
+against a top-down construction written independently here.
 """
 import numpy as np
 import pytest
@@ -74,6 +77,35 @@ def _check_tree(nodes, boxes, codes, order, aabbs):
         assert ((key[split] ^ key[split + 1]).bit_length()) == hb
 
 
+def _karras_top_down(codes, order):
+    """Karras 2012's tree built top down, independently of the oracle's
+    bottom-up per-node search: a range [lo, hi] of sorted leaves splits after
+    the last leaf whose (code, index) key shares the range's highest
+    differing bit with key[lo]; the node of a range is numbered by the end
+    that is not its parent's split (root 0), so the left child of a split at
+    g is internal node g and the right child internal node g + 1, each a leaf
+    when its range is one leaf (AcceleratorLBVH.cu:268-291)."""
+    n = len(codes)
+    key = [(int(c) << 32) | i for i, c in enumerate(codes)]
+    nodes = np.zeros((max(n - 1, 1), 3), np.uint32)
+    nodes[0][2] = 0xFFFFFFFF
+    todo = [(0, 0, n - 1)]
+    while todo:
+        v, lo, hi = todo.pop()
+        hb = (key[lo] ^ key[hi]).bit_length() - 1
+        g = lo
+        while g + 1 < hi and not ((key[g + 1] ^ key[lo]) >> hb):
+            g += 1
+        for side, (a, b, idx) in enumerate(((lo, g, g), (g + 1, hi, g + 1))):
+            if a == b:
+                nodes[v][side] = LEAF | int(order[a])
+            else:
+                nodes[v][side] = idx
+                nodes[idx][2] = v
+                todo.append((idx, a, b))
+    return nodes
+
+
 @pytest.mark.parametrize("n,seed,dups", [(2, 0, False), (3, 1, False), (8, 2, False),
                                          (37, 3, False), (200, 4, True), (1000, 5, True)])
 def test_karras_invariants(n, seed, dups):
@@ -88,6 +120,29 @@ def test_karras_invariants(n, seed, dups):
     assert refcpu.lbvh_tree_valid(codes, order)
     nodes, boxes = refcpu.lbvh_from_sorted(codes, order, aabbs)
     _check_tree(nodes, boxes, codes, order, aabbs)
+    # the node numbering too: node for node, the top-down construction
+    assert np.array_equal(nodes, _karras_top_down(codes, order))
+
+
+def test_karras_numbering_fails_on_planted_mistakes():
+    """The top-down comparison catches a swapped child pair and a split moved
+    by one leaf (both keep the tree's shape invariants plausible)."""
+    rng = np.random.default_rng(9)
+    n = 64
+    codes = np.sort(rng.integers(0, 1 << 63, size=n, dtype=np.uint64))
+    order = rng.permutation(n).astype(np.uint32)
+    good = _karras_top_down(codes, order)
+    nodes, _ = refcpu.lbvh_from_sorted(codes, order, np.zeros((n, 6), np.float32))
+    assert np.array_equal(nodes, good)
+    swapped = good.copy()
+    swapped[0][[0, 1]] = swapped[0][[1, 0]]
+    assert not np.array_equal(nodes, swapped)
+    # moving the root's split one leaf left changes the numbering below it
+    shifted = codes.copy()
+    hb = int(codes[0] ^ codes[-1]).bit_length() - 1
+    g = max(i for i in range(n - 1) if not (int(codes[i] ^ codes[0]) >> hb))
+    shifted[g] = codes[g + 1]  # leaf g now sorts with the right half
+    assert not np.array_equal(_karras_top_down(shifted, order), good)
 
 
 def test_reference_delta_malformed_on_degenerate_codes():
@@ -109,6 +164,7 @@ def test_scene_lbvh_consistency():
     assert np.array_equal(np.sort(codes), codes[order])
     assert np.array_equal(aabb, np.concatenate([aabbs[:, :3].min(0), aabbs[:, 3:].max(0)]))
     _check_tree(nodes, boxes, codes[order], order, aabbs)
+    assert np.array_equal(nodes, _karras_top_down(codes[order], order))
 
 
 def _rays(n, seed):
