@@ -188,6 +188,35 @@ __device__ __forceinline__ void SlabTest8(const CNode8* __restrict__ np, const R
     }
 }
 
+// Children in ascending tNear (optimal networks: 5 / 19 comparators), each
+// compare-exchange as selects (no exec-mask branch per comparator).  Any-hit
+// walks near-first as well: occluders of a shadow ray cluster near its
+// origin, and the order cut the shadow pass by about 30 %.
+template<int W>
+__device__ __forceinline__ void SortNear(float (&tn)[W], uint32_t (&c)[W])
+{
+    auto cs = [&](int i, int j)
+    {
+        const bool s = tn[j] < tn[i];
+        const float a = s ? tn[j] : tn[i], b = s ? tn[i] : tn[j];
+        const uint32_t ca = s ? c[j] : c[i], cb = s ? c[i] : c[j];
+        tn[i] = a; tn[j] = b; c[i] = ca; c[j] = cb;
+    };
+    if constexpr(W == 8)
+    {
+        cs(0, 2); cs(1, 3); cs(4, 6); cs(5, 7);
+        cs(0, 4); cs(1, 5); cs(2, 6); cs(3, 7);
+        cs(0, 1); cs(2, 3); cs(4, 5); cs(6, 7);
+        cs(2, 4); cs(3, 5);
+        cs(1, 4); cs(3, 6);
+        cs(1, 2); cs(3, 4); cs(5, 6);
+    }
+    else
+    {
+        cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+    }
+}
+
 template<bool ANY_HIT>
 __device__ __forceinline__ bool TestLeafRange(const SoupTri* __restrict__ soup, uint32_t leaf,
                                               const RayPre& r, float tMin, float tMax,
@@ -271,30 +300,7 @@ __device__ __forceinline__ bool WalkStep(const TraceCtx& ctx, uint2* lstack, uin
         SlabTest4(ctx.nodes + w.node, w.r, w.tMin, curMax, tn, ch);
         c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
     }
-    auto cs = [&](int i, int j)
-    {
-        if(tn[j] < tn[i])
-        {
-            float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;
-            uint32_t cc = c[i]; c[i] = c[j]; c[j] = cc;
-        }
-    };
-    // ascending tNear (optimal networks: 5 / 19 comparators); any-hit
-    // walks near-first as well: occluders of a shadow ray cluster near
-    // its origin, and the order cut the shadow pass by about 30 %
-    if constexpr(W == 8)
-    {
-        cs(0, 2); cs(1, 3); cs(4, 6); cs(5, 7);
-        cs(0, 4); cs(1, 5); cs(2, 6); cs(3, 7);
-        cs(0, 1); cs(2, 3); cs(4, 5); cs(6, 7);
-        cs(2, 4); cs(3, 5);
-        cs(1, 4); cs(3, 6);
-        cs(1, 2); cs(3, 4); cs(5, 6);
-    }
-    else
-    {
-        cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
-    }
+    SortNear<W>(tn, c);
     uint32_t inner[W];
     float innerT[W];
     int innerCount = 0;
@@ -381,56 +387,46 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
             SlabTest4(ctx.nodes + node, r, tMin, curMax, tn, ch);
             c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
         }
-        auto cs = [&](int i, int j)
-        {
-            if(tn[j] < tn[i])
-            {
-                float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;
-                uint32_t cc = c[i]; c[i] = c[j]; c[j] = cc;
-            }
-        };
-        // ascending tNear (optimal networks: 5 / 19 comparators); any-hit
-        // walks near-first as well: occluders of a shadow ray cluster near
-        // its origin, and the order cut the shadow pass by about 30 %
-        {
-            if constexpr(W == 8)
-            {
-                cs(0, 2); cs(1, 3); cs(4, 6); cs(5, 7);
-                cs(0, 4); cs(1, 5); cs(2, 6); cs(3, 7);
-                cs(0, 1); cs(2, 3); cs(4, 5); cs(6, 7);
-                cs(2, 4); cs(3, 5);
-                cs(1, 4); cs(3, 6);
-                cs(1, 2); cs(3, 4); cs(5, 6);
-            }
-            else
-            {
-                cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
-            }
-        }
+        SortNear<W>(tn, c);
         uint32_t inner[W];
         float innerT[W];
         int innerCount = 0;
+        uint32_t leafMask = 0;
         #pragma unroll
         for(int k = 0; k < W; k++)
         {
             if(tn[k] == INFINITY) continue;
-            if(c[k] & kLeafFlag)
-            {
-                float lim = (best.prim == kInvalid) ? tMax : best.t;
-                if(!ANY_HIT && tn[k] > lim * kGrow) continue;
-                bool h = TestLeafRange<ANY_HIT>(ctx.soup, c[k], r, tMin, tMax, best,
-                                                COUNT ? triTests : nullptr);
-                if(ANY_HIT && h)
-                {
-                    best.prim = 0;
-                    return best;
-                }
-            }
+            if(c[k] & kLeafFlag) leafMask |= 1u << k;
             else
             {
                 inner[innerCount] = c[k];
                 innerT[innerCount] = tn[k];
                 innerCount++;
+            }
+        }
+        // the node's leaves, nearest first: a wave runs as many triangle
+        // tests as its lane with the most leaves here, not one per child
+        // position that any lane has a leaf in
+        while(leafMask)
+        {
+            const int k = __builtin_ctz(leafMask);
+            leafMask &= leafMask - 1u;
+            uint32_t ck = c[0];
+            float tk = tn[0];
+            #pragma unroll
+            for(int j = 1; j < W; j++)
+            {
+                ck = (k == j) ? c[j] : ck;
+                tk = (k == j) ? tn[j] : tk;
+            }
+            const float lim = (best.prim == kInvalid) ? tMax : best.t;
+            if(!ANY_HIT && tk > lim * kGrow) continue;
+            bool h = TestLeafRange<ANY_HIT>(ctx.soup, ck, r, tMin, tMax, best,
+                                            COUNT ? triTests : nullptr);
+            if(ANY_HIT && h)
+            {
+                best.prim = 0;
+                return best;
             }
         }
         uint32_t next = kInvalid;
@@ -503,15 +499,7 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
         uint4 ch;
         SlabTest4(tree + node, r, tMin, curMax, tn, ch);
         uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
-        auto cs = [&](int i, int j)
-        {
-            if(tn[j] < tn[i])
-            {
-                float tt = tn[i]; tn[i] = tn[j]; tn[j] = tt;
-                uint32_t cc = c[i]; c[i] = c[j]; c[j] = cc;
-            }
-        };
-        cs(0, 1); cs(2, 3); cs(0, 2); cs(1, 3); cs(1, 2);
+        SortNear<4>(tn, c);
         uint32_t inner[4];
         float innerT[4];
         int innerCount = 0;
