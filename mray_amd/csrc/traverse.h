@@ -301,44 +301,38 @@ __device__ __forceinline__ bool WalkStep(const TraceCtx& ctx, uint2* lstack, uin
         c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
     }
     SortNear<W>(tn, c);
-    uint32_t inner[W];
-    float innerT[W];
-    int innerCount = 0;
-    #pragma unroll
-    for(int k = 0; k < W; k++)
-    {
-        if(tn[k] == INFINITY) continue;
-        if(c[k] & kLeafFlag)
-        {
-            float lim = (best.prim == kInvalid) ? w.tMax : best.t;
-            if(!ANY_HIT && tn[k] > lim * kGrow) continue;
-            bool h = TestLeafRange<ANY_HIT>(ctx.soup, c[k], w.r, w.tMin, w.tMax, best,
-                                            COUNT ? triTests : nullptr);
-            if(ANY_HIT && h)
-            {
-                best.prim = 0;
-                return true;
-            }
-        }
-        else
-        {
-            inner[innerCount] = c[k];
-            innerT[innerCount] = tn[k];
-            innerCount++;
-        }
-    }
+    // as in TraceW: inner children farthest first, each pushing the one
+    // found before it; then the leaves nearest first
     uint32_t next = kInvalid;
+    float nextT = 0.0f;
     uint32_t sp = w.sp;
-    if(innerCount > 0)
+    #pragma unroll
+    for(int k = W - 1; k >= 0; k--)
     {
-        next = inner[0];
-        for(int k = innerCount - 1; k >= 1; k--)
+        if(tn[k] == INFINITY || (c[k] & kLeafFlag)) continue;
+        if(next != kInvalid && sp < spMax)   // sp < spMax: sized (W-1) per level
         {
-            uint2 v = make_uint2(inner[k], __float_as_uint(innerT[k]));
-            if(sp >= spMax) break;   // unreachable: sized (W-1) per level
+            const uint2 v = make_uint2(next, __float_as_uint(nextT));
             if(sp < kLdsStack) lstack[sp * B + tid] = v;
             else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
             sp++;
+        }
+        next = c[k];
+        nextT = tn[k];
+    }
+    #pragma unroll
+    for(int k = 0; k < W; k++)
+    {
+        if(tn[k] == INFINITY || !(c[k] & kLeafFlag)) continue;
+        float lim = (best.prim == kInvalid) ? w.tMax : best.t;
+        if(!ANY_HIT && tn[k] > lim * kGrow) continue;
+        bool h = TestLeafRange<ANY_HIT>(ctx.soup, c[k], w.r, w.tMin, w.tMax, best,
+                                        COUNT ? triTests : nullptr);
+        if(ANY_HIT && h)
+        {
+            best.prim = 0;
+            w.sp = sp;
+            return true;
         }
     }
     while(next == kInvalid && sp > sp0)
@@ -388,21 +382,29 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
             c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
         }
         SortNear<W>(tn, c);
-        uint32_t inner[W];
-        float innerT[W];
-        int innerCount = 0;
+        // inner children, farthest first: each pushes the one found before
+        // it, so the nearest is left in next and the stack pops near-first
         uint32_t leafMask = 0;
+        uint32_t next = kInvalid;
+        float nextT = 0.0f;
         #pragma unroll
-        for(int k = 0; k < W; k++)
+        for(int k = W - 1; k >= 0; k--)
         {
             if(tn[k] == INFINITY) continue;
-            if(c[k] & kLeafFlag) leafMask |= 1u << k;
-            else
+            if(c[k] & kLeafFlag)
             {
-                inner[innerCount] = c[k];
-                innerT[innerCount] = tn[k];
-                innerCount++;
+                leafMask |= 1u << k;
+                continue;
             }
+            if(next != kInvalid && sp < spMax)   // sp < spMax: sized (W-1) per level
+            {
+                const uint2 v = make_uint2(next, __float_as_uint(nextT));
+                if(sp < kLdsStack) lstack[sp * B + tid] = v;
+                else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
+                sp++;
+            }
+            next = c[k];
+            nextT = tn[k];
         }
         // the node's leaves, nearest first: a wave runs as many triangle
         // tests as its lane with the most leaves here, not one per child
@@ -427,19 +429,6 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
             {
                 best.prim = 0;
                 return best;
-            }
-        }
-        uint32_t next = kInvalid;
-        if(innerCount > 0)
-        {
-            next = inner[0];
-            for(int k = innerCount - 1; k >= 1; k--)
-            {
-                uint2 v = make_uint2(inner[k], __float_as_uint(innerT[k]));
-                if(sp >= spMax) break;   // unreachable: sized (W-1) per level
-                if(sp < kLdsStack) lstack[sp * B + tid] = v;
-                else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
-                sp++;
             }
         }
         while(next == kInvalid && sp > sp0)
@@ -500,9 +489,27 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
         SlabTest4(tree + node, r, tMin, curMax, tn, ch);
         uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
         SortNear<4>(tn, c);
-        uint32_t inner[4];
-        float innerT[4];
-        int innerCount = 0;
+        // candidates farthest first (an inner node of either tree, or an
+        // instance of the instance tree: kInstEnter | its leaf position):
+        // each pushes the one found before it, so the nearest is left in
+        // next and the stack pops near-first
+        uint32_t next = kInvalid;
+        float nextT = 0.0f;
+        #pragma unroll
+        for(int k = 3; k >= 0; k--)
+        {
+            if(tn[k] == INFINITY) continue;
+            if((c[k] & kLeafFlag) && inst != kInvalid) continue;
+            if(next != kInvalid && sp < spMax)   // sp < spMax: sized for both trees and the mark
+            {
+                const uint2 v = make_uint2(next, __float_as_uint(nextT));
+                if(sp < kLdsStack) lstack[sp * B + tid] = v;
+                else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
+                sp++;
+            }
+            next = c[k];
+            nextT = tn[k];
+        }
         #pragma unroll
         for(int k = 0; k < 4; k++)
         {
@@ -523,27 +530,6 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
                     return best;
                 }
                 if(best.prim != before || best.t != beforeT) instOut = inst;
-            }
-            else
-            {
-                // an inner node of either tree, or an instance of the
-                // instance tree (kInstEnter | its leaf position)
-                inner[innerCount] = c[k];
-                innerT[innerCount] = tn[k];
-                innerCount++;
-            }
-        }
-        uint32_t next = kInvalid;
-        if(innerCount > 0)
-        {
-            next = inner[0];
-            for(int k = innerCount - 1; k >= 1; k--)
-            {
-                uint2 v = make_uint2(inner[k], __float_as_uint(innerT[k]));
-                if(sp >= spMax) break;   // unreachable: sized for both trees and the mark
-                if(sp < kLdsStack) lstack[sp * B + tid] = v;
-                else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
-                sp++;
             }
         }
         while(next == kInvalid && sp > 0)
