@@ -188,6 +188,40 @@ __device__ __forceinline__ void SlabTest8(const CNode8* __restrict__ np, const R
     }
 }
 
+// One traversal-stack entry: the first kLdsStack levels in LDS (entry sp of
+// lane tid at sp * B + tid), deeper ones in the lane's global spill area.
+// While no lane of the wave is past the LDS levels (nearly always) the wave
+// takes a branch-free LDS path, chosen by one wave-uniform test; otherwise
+// each lane picks its memory.  The empty asm statement (a compiler barrier;
+// it emits nothing) keeps that slow path's LDS and global accesses apart:
+// merged, they would become one access through a generic pointer.  B and
+// tid come from the caller, read once outside its loop.
+__device__ __forceinline__ void StackPut(const TraceCtx& ctx, uint2* lstack, uint32_t sp,
+                                         uint32_t B, uint32_t tid, uint32_t gtid, uint2 v)
+{
+    if(__builtin_expect(__ballot(sp >= kLdsStack) == 0ull, 1)) lstack[sp * B + tid] = v;
+    else if(sp < kLdsStack) lstack[sp * B + tid] = v;
+    else
+    {
+        ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
+        asm volatile("" ::: "memory");
+    }
+}
+
+__device__ __forceinline__ uint2 StackGet(const TraceCtx& ctx, const uint2* lstack, uint32_t sp,
+                                          uint32_t B, uint32_t tid, uint32_t gtid)
+{
+    uint2 v;
+    if(__builtin_expect(__ballot(sp >= kLdsStack) == 0ull, 1)) v = lstack[sp * B + tid];
+    else if(sp < kLdsStack) v = lstack[sp * B + tid];
+    else
+    {
+        v = ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
+        asm volatile("" ::: "memory");
+    }
+    return v;
+}
+
 // Children in ascending tNear (optimal networks: 5 / 19 comparators), each
 // compare-exchange as selects (no exec-mask branch per comparator).  Any-hit
 // walks near-first as well: occluders of a shadow ray cluster near its
@@ -313,8 +347,7 @@ __device__ __forceinline__ bool WalkStep(const TraceCtx& ctx, uint2* lstack, uin
         if(next != kInvalid && sp < spMax)   // sp < spMax: sized (W-1) per level
         {
             const uint2 v = make_uint2(next, __float_as_uint(nextT));
-            if(sp < kLdsStack) lstack[sp * B + tid] = v;
-            else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
+            StackPut(ctx, lstack, sp, B, tid, gtid, v);
             sp++;
         }
         next = c[k];
@@ -339,8 +372,7 @@ __device__ __forceinline__ bool WalkStep(const TraceCtx& ctx, uint2* lstack, uin
     {
         sp--;
         uint2 v;
-        if(sp < kLdsStack) v = lstack[sp * B + tid];
-        else v = ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
+        v = StackGet(ctx, lstack, sp, B, tid, gtid);
         // drop subtrees that start beyond the closest hit so far
         if(!ANY_HIT && best.prim != kInvalid && __uint_as_float(v.y) > best.t * kGrow) continue;
         next = v.x;
@@ -362,10 +394,10 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
 {
     HitRec best{kInvalid, INFINITY, 0.0f, 0.0f};
     RayPre r = MakeRayPre(o, d);
-    const uint32_t tid = threadIdx.x;
-    const uint32_t B = blockDim.x;
     uint32_t sp = sp0;
     uint32_t node = root;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t B = blockDim.x;
     const uint32_t spMax = kLdsStack + ctx.gstackDepth;
     const float kGrow = 1.0000005f;
     for(uint32_t visits = 0; visits < ctx.maxVisits; visits++)
@@ -399,8 +431,7 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
             if(next != kInvalid && sp < spMax)   // sp < spMax: sized (W-1) per level
             {
                 const uint2 v = make_uint2(next, __float_as_uint(nextT));
-                if(sp < kLdsStack) lstack[sp * B + tid] = v;
-                else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
+                StackPut(ctx, lstack, sp, B, tid, gtid, v);
                 sp++;
             }
             next = c[k];
@@ -434,8 +465,7 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
         while(next == kInvalid && sp > sp0)
         {
             sp--;
-            uint2 v = (sp < kLdsStack) ? lstack[sp * B + tid]
-                                       : ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
+            uint2 v = StackGet(ctx, lstack, sp, B, tid, gtid);
             // drop subtrees that start beyond the closest hit so far
             if(!ANY_HIT && best.prim != kInvalid && __uint_as_float(v.y) > best.t * kGrow) continue;
             next = v.x;
@@ -474,10 +504,10 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
     RayPre r = world;
     uint32_t inst = kInvalid;                 // instance whose mesh tree is walked
     const BNode4* tree = ctx.tlas;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t B = blockDim.x;
     uint32_t sp = 0;
     uint32_t node = 0;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t B = blockDim.x;
     const uint32_t spMax = kLdsStack + ctx.gstackDepth;
     const float kGrow = 1.0000005f;
     for(uint32_t visits = 0; visits < ctx.instWalkVisits; visits++)
@@ -503,8 +533,7 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
             if(next != kInvalid && sp < spMax)   // sp < spMax: sized for both trees and the mark
             {
                 const uint2 v = make_uint2(next, __float_as_uint(nextT));
-                if(sp < kLdsStack) lstack[sp * B + tid] = v;
-                else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
+                StackPut(ctx, lstack, sp, B, tid, gtid, v);
                 sp++;
             }
             next = c[k];
@@ -536,8 +565,7 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
         {
             sp--;
             uint2 v;
-            if(sp < kLdsStack) v = lstack[sp * B + tid];
-            else v = ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
+            v = StackGet(ctx, lstack, sp, B, tid, gtid);
             if(v.x == kInstReturn)
             {
                 // back from an instance: world ray, instance tree
@@ -558,8 +586,7 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
             const InstGPU& I = ctx.inst[id];
             if(sp >= spMax) break;       // unreachable: the mark has its slot
             const uint2 mark = make_uint2(kInstReturn, 0u);
-            if(sp < kLdsStack) lstack[sp * B + tid] = mark;
-            else ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = mark;
+            StackPut(ctx, lstack, sp, B, tid, gtid, mark);
             sp++;
             const float4 a = I.inv[0], b = I.inv[1], e = I.inv[2];
             const V3 lo = {a.x * o.x + a.y * o.y + a.z * o.z + a.w,
