@@ -1580,32 +1580,33 @@ __device__ __forceinline__ void ShadeAppend(const ShadeOut& o, uint32_t i, uint3
 }
 
 // Generic shading over the whole active list: static, one 64-item chunk per
-// wave step (shading is short per item, a fetch atomic would cost as much as
-// the work it hands out); the chunk's shard picks the shadow-queue shard.
+// wave (shading is short per item, a fetch atomic would cost as much as the
+// work it hands out); the chunk's shard picks the shadow-queue shard.  The
+// grid covers N slots and has no loop: a grid-stride loop made the compiler
+// hold the scene's many pointers in SGPRs for the loop's length, and spill
+// them (74 SGPRs) and 40 more VGPRs (150 -> 110, 3 -> 4 waves per SIMD).
 template<bool SPECTRAL, bool BATCH, bool INST = false>
 __global__ void __launch_bounds__(kSlotTPB)
 KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ, uint32_t* __restrict__ ctr)
 {
     const uint32_t n = sched->activeCount;
+    const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+    if(base >= n) return;
     const uint32_t cap = ShadowShardCap(rc.N);
     const bool queries = rc.nrc.on;
-    for(uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u; base < n;
-        base += gridDim.x * blockDim.x)
+    const uint32_t k = base + (threadIdx.x & 63u);
+    ShadeOut o;
+    o.shadow = false;
+    o.query = false;
+    o.view = 0;
+    uint32_t i = 0;
+    if(k < n)
     {
-        const uint32_t k = base + (threadIdx.x & 63u);
-        ShadeOut o;
-        o.shadow = false;
-        o.query = false;
-        o.view = 0;
-        uint32_t i = 0;
-        if(k < n)
-        {
-            i = active[k];
-            ShadeSlot<SPECTRAL, -1, BATCH, INST>(S, rc, i, o);
-        }
-        ShadeAppend(o, i, ShardOfChunk(base >> 6, n), cap, ctr, shadowQ, sched, rc.nrc, queries);
+        i = active[k];
+        ShadeSlot<SPECTRAL, -1, BATCH, INST>(S, rc, i, o);
     }
+    ShadeAppend(o, i, ShardOfChunk(base >> 6, n), cap, ctr, shadowQ, sched, rc.nrc, queries);
 }
 
 // Material-keyed queues (replacing the reference's work-key sort and
@@ -2420,18 +2421,8 @@ static int SizeLaunches(MRayRenderer* r)
         else if(p->spectral) ShadeQueueOccupancy<true, false>(r, resident);
         else ShadeQueueOccupancy<false, false>(r, resident);
     }
-    if(r->iscene && r->batch)
-        r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, true, true>, kSlotTPB, 1u << 20)
-                                              : resident(KShade<false, true, true>, kSlotTPB, 1u << 20));
-    else if(r->iscene)
-        r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, false, true>, kSlotTPB, 1u << 20)
-                                              : resident(KShade<false, false, true>, kSlotTPB, 1u << 20));
-    else if(r->batch)
-        r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, true>, kSlotTPB, 1u << 20)
-                                              : resident(KShade<false, true>, kSlotTPB, 1u << 20));
-    else
-        r->shadeBlocks = uint32_t(p->spectral ? resident(KShade<true, false>, kSlotTPB, 1u << 20)
-                                              : resident(KShade<false, false>, kSlotTPB, 1u << 20));
+    // one wave per 64-item chunk of the active list, N slots at most
+    r->shadeBlocks = uint32_t((r->rc.N + kSlotTPB - 1u) / kSlotTPB);
     return MRAY_OK;
 }
 
