@@ -188,19 +188,26 @@ __device__ __forceinline__ void SlabTest8(const CNode8* __restrict__ np, const R
     }
 }
 
-// One traversal-stack entry: the first kLdsStack levels in LDS (entry sp of
-// lane tid at sp * B + tid), deeper ones in the lane's global spill area.
+// LDS stack layout: each wave owns kLdsStack rows of 64 entries, entry sp
+// of a lane at sp * 64 + StackLane() (a shift, no multiply by the block
+// size; a row is 64 consecutive 8-byte entries, conflict-free).
+__device__ __forceinline__ uint32_t StackLane()
+{
+    return (threadIdx.x >> 6) * (kLdsStack * 64u) + (threadIdx.x & 63u);
+}
+
+// One traversal-stack entry: the first kLdsStack levels in LDS, deeper ones
+// in the lane's global spill area.
 // While no lane of the wave is past the LDS levels (nearly always) the wave
 // takes a branch-free LDS path, chosen by one wave-uniform test; otherwise
 // each lane picks its memory.  The empty asm statement (a compiler barrier;
 // it emits nothing) keeps that slow path's LDS and global accesses apart:
-// merged, they would become one access through a generic pointer.  B and
-// tid come from the caller, read once outside its loop.
+// merged, they would become one access through a generic pointer.
 __device__ __forceinline__ void StackPut(const TraceCtx& ctx, uint2* lstack, uint32_t sp,
-                                         uint32_t B, uint32_t tid, uint32_t gtid, uint2 v)
+                                         uint32_t lb, uint32_t gtid, uint2 v)
 {
-    if(__builtin_expect(__ballot(sp >= kLdsStack) == 0ull, 1)) lstack[sp * B + tid] = v;
-    else if(sp < kLdsStack) lstack[sp * B + tid] = v;
+    if(__builtin_expect(__ballot(sp >= kLdsStack) == 0ull, 1)) lstack[sp * 64u + lb] = v;
+    else if(sp < kLdsStack) lstack[sp * 64u + lb] = v;
     else
     {
         ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid] = v;
@@ -209,17 +216,63 @@ __device__ __forceinline__ void StackPut(const TraceCtx& ctx, uint2* lstack, uin
 }
 
 __device__ __forceinline__ uint2 StackGet(const TraceCtx& ctx, const uint2* lstack, uint32_t sp,
-                                          uint32_t B, uint32_t tid, uint32_t gtid)
+                                          uint32_t lb, uint32_t gtid)
 {
     uint2 v;
-    if(__builtin_expect(__ballot(sp >= kLdsStack) == 0ull, 1)) v = lstack[sp * B + tid];
-    else if(sp < kLdsStack) v = lstack[sp * B + tid];
+    if(__builtin_expect(__ballot(sp >= kLdsStack) == 0ull, 1)) v = lstack[sp * 64u + lb];
+    else if(sp < kLdsStack) v = lstack[sp * 64u + lb];
     else
     {
         v = ctx.gstack[size_t(sp - kLdsStack) * ctx.gridThreads + gtid];
         asm volatile("" ::: "memory");
     }
     return v;
+}
+
+// A node's stack candidates (bit k of cand: child k, in ascending tNear
+// order), farthest first: each pushes the one found before it, so the
+// nearest is left in next and the stack pops near-first.  While every lane
+// of the wave has room for W-1 more entries in LDS (one wave-uniform test)
+// the pushes are branch-free: each candidate writes the free slot and the
+// stack grows only on a real push (a write above the top is dead).
+template<int W, bool BRANCH_FREE = true>
+__device__ __forceinline__ void PushFarFirst(const TraceCtx& ctx, uint2* lstack, uint32_t& sp,
+                                             uint32_t spMax, uint32_t lb, uint32_t gtid, const float (&tn)[W],
+                                             const uint32_t (&c)[W], uint32_t cand,
+                                             uint32_t& next, float& nextT)
+{
+    next = kInvalid;
+    nextT = 0.0f;
+    if(BRANCH_FREE && __builtin_expect(__ballot(sp + uint32_t(W - 1) > kLdsStack) == 0ull, 1))
+    {
+        #pragma unroll
+        for(int k = W - 1; k >= 0; k--)
+        {
+            const bool ck = (cand >> k) & 1u;
+            if(k < W - 1)
+            {
+                lstack[sp * 64u + lb] = make_uint2(next, __float_as_uint(nextT));
+                sp += (ck && next != kInvalid) ? 1u : 0u;
+            }
+            next = ck ? c[k] : next;
+            nextT = ck ? tn[k] : nextT;
+        }
+    }
+    else
+    {
+        #pragma unroll
+        for(int k = W - 1; k >= 0; k--)
+        {
+            if(!((cand >> k) & 1u)) continue;
+            if(next != kInvalid && sp < spMax)   // sp < spMax: sized for every level
+            {
+                StackPut(ctx, lstack, sp, lb, gtid, make_uint2(next, __float_as_uint(nextT)));
+                sp++;
+            }
+            next = c[k];
+            nextT = tn[k];
+        }
+    }
 }
 
 // Children in ascending tNear (optimal networks: 5 / 19 comparators), each
@@ -318,8 +371,7 @@ template<int W, bool ANY_HIT, bool COUNT>
 __device__ __forceinline__ bool WalkStep(const TraceCtx& ctx, uint2* lstack, uint32_t gtid, Walk& w,
                                          uint32_t sp0, uint32_t* nodeVisits, uint32_t* triTests)
 {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t B = blockDim.x;
+    const uint32_t lb = StackLane();
     const uint32_t spMax = kLdsStack + ctx.gstackDepth;
     const float kGrow = 1.0000005f;
     if(COUNT) (*nodeVisits)++;
@@ -347,7 +399,7 @@ __device__ __forceinline__ bool WalkStep(const TraceCtx& ctx, uint2* lstack, uin
         if(next != kInvalid && sp < spMax)   // sp < spMax: sized (W-1) per level
         {
             const uint2 v = make_uint2(next, __float_as_uint(nextT));
-            StackPut(ctx, lstack, sp, B, tid, gtid, v);
+            StackPut(ctx, lstack, sp, lb, gtid, v);
             sp++;
         }
         next = c[k];
@@ -372,7 +424,7 @@ __device__ __forceinline__ bool WalkStep(const TraceCtx& ctx, uint2* lstack, uin
     {
         sp--;
         uint2 v;
-        v = StackGet(ctx, lstack, sp, B, tid, gtid);
+        v = StackGet(ctx, lstack, sp, lb, gtid);
         // drop subtrees that start beyond the closest hit so far
         if(!ANY_HIT && best.prim != kInvalid && __uint_as_float(v.y) > best.t * kGrow) continue;
         next = v.x;
@@ -396,8 +448,7 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
     RayPre r = MakeRayPre(o, d);
     uint32_t sp = sp0;
     uint32_t node = root;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t B = blockDim.x;
+    const uint32_t lb = StackLane();
     const uint32_t spMax = kLdsStack + ctx.gstackDepth;
     const float kGrow = 1.0000005f;
     for(uint32_t visits = 0; visits < ctx.maxVisits; visits++)
@@ -414,29 +465,18 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
             c[0] = ch.x; c[1] = ch.y; c[2] = ch.z; c[3] = ch.w;
         }
         SortNear<W>(tn, c);
-        // inner children, farthest first: each pushes the one found before
-        // it, so the nearest is left in next and the stack pops near-first
-        uint32_t leafMask = 0;
-        uint32_t next = kInvalid;
-        float nextT = 0.0f;
+        // hit children: leaves are tested below, inner ones go on the stack
+        uint32_t leafMask = 0, cand = 0;
         #pragma unroll
-        for(int k = W - 1; k >= 0; k--)
+        for(int k = 0; k < W; k++)
         {
-            if(tn[k] == INFINITY) continue;
-            if(c[k] & kLeafFlag)
-            {
-                leafMask |= 1u << k;
-                continue;
-            }
-            if(next != kInvalid && sp < spMax)   // sp < spMax: sized (W-1) per level
-            {
-                const uint2 v = make_uint2(next, __float_as_uint(nextT));
-                StackPut(ctx, lstack, sp, B, tid, gtid, v);
-                sp++;
-            }
-            next = c[k];
-            nextT = tn[k];
+            const bool hit = tn[k] != INFINITY, leaf = (c[k] & kLeafFlag) != 0u;
+            leafMask |= (hit && leaf) ? (1u << k) : 0u;
+            cand |= (hit && !leaf) ? (1u << k) : 0u;
         }
+        uint32_t next;
+        float nextT;
+        PushFarFirst<W>(ctx, lstack, sp, spMax, lb, gtid, tn, c, cand, next, nextT);
         // the node's leaves, nearest first: a wave runs as many triangle
         // tests as its lane with the most leaves here, not one per child
         // position that any lane has a leaf in
@@ -465,7 +505,7 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
         while(next == kInvalid && sp > sp0)
         {
             sp--;
-            uint2 v = StackGet(ctx, lstack, sp, B, tid, gtid);
+            uint2 v = StackGet(ctx, lstack, sp, lb, gtid);
             // drop subtrees that start beyond the closest hit so far
             if(!ANY_HIT && best.prim != kInvalid && __uint_as_float(v.y) > best.t * kGrow) continue;
             next = v.x;
@@ -506,8 +546,7 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
     const BNode4* tree = ctx.tlas;
     uint32_t sp = 0;
     uint32_t node = 0;
-    const uint32_t tid = threadIdx.x;
-    const uint32_t B = blockDim.x;
+    const uint32_t lb = StackLane();
     const uint32_t spMax = kLdsStack + ctx.gstackDepth;
     const float kGrow = 1.0000005f;
     for(uint32_t visits = 0; visits < ctx.instWalkVisits; visits++)
@@ -519,26 +558,19 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
         SlabTest4(tree + node, r, tMin, curMax, tn, ch);
         uint32_t c[4] = {ch.x, ch.y, ch.z, ch.w};
         SortNear<4>(tn, c);
-        // candidates farthest first (an inner node of either tree, or an
-        // instance of the instance tree: kInstEnter | its leaf position):
-        // each pushes the one found before it, so the nearest is left in
-        // next and the stack pops near-first
-        uint32_t next = kInvalid;
-        float nextT = 0.0f;
+        // stack candidates: an inner node of either tree, or an instance of
+        // the instance tree (kInstEnter | its leaf position); mesh leaves
+        // are tested below
+        uint32_t cand = 0;
         #pragma unroll
-        for(int k = 3; k >= 0; k--)
+        for(int k = 0; k < 4; k++)
         {
-            if(tn[k] == INFINITY) continue;
-            if((c[k] & kLeafFlag) && inst != kInvalid) continue;
-            if(next != kInvalid && sp < spMax)   // sp < spMax: sized for both trees and the mark
-            {
-                const uint2 v = make_uint2(next, __float_as_uint(nextT));
-                StackPut(ctx, lstack, sp, B, tid, gtid, v);
-                sp++;
-            }
-            next = c[k];
-            nextT = tn[k];
+            const bool meshLeaf = (c[k] & kLeafFlag) && inst != kInvalid;
+            cand |= (tn[k] != INFINITY && !meshLeaf) ? (1u << k) : 0u;
         }
+        uint32_t next;
+        float nextT;
+        PushFarFirst<4, false>(ctx, lstack, sp, spMax, lb, gtid, tn, c, cand, next, nextT);
         #pragma unroll
         for(int k = 0; k < 4; k++)
         {
@@ -565,7 +597,7 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
         {
             sp--;
             uint2 v;
-            v = StackGet(ctx, lstack, sp, B, tid, gtid);
+            v = StackGet(ctx, lstack, sp, lb, gtid);
             if(v.x == kInstReturn)
             {
                 // back from an instance: world ray, instance tree
@@ -586,7 +618,7 @@ __device__ __forceinline__ HitRec TraceInstFlat(const TraceCtx& ctx, uint2* lsta
             const InstGPU& I = ctx.inst[id];
             if(sp >= spMax) break;       // unreachable: the mark has its slot
             const uint2 mark = make_uint2(kInstReturn, 0u);
-            StackPut(ctx, lstack, sp, B, tid, gtid, mark);
+            StackPut(ctx, lstack, sp, lb, gtid, mark);
             sp++;
             const float4 a = I.inv[0], b = I.inv[1], e = I.inv[2];
             const V3 lo = {a.x * o.x + a.y * o.y + a.z * o.z + a.w,
