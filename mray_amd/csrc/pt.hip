@@ -1665,30 +1665,25 @@ KShadeQ(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ queues,
     LoadShards(ctr, CTR_QCOUNT + uint32_t(KEY) * kShards, g);
     const uint32_t n = g.pre[kShards];
     const uint32_t* q = queues + size_t(KEY) * kShards * cap;
-    const uint32_t lane = threadIdx.x & 63u;
-    uint32_t* counter = Ctr(ctr, CTR_QFETCH + uint32_t(KEY) * kShards);
-    for(;;)
+    // one 64-item chunk of the queue per wave over a grid of N slots (no
+    // loop: a loop keeps the scene's pointers live in SGPRs and spills them)
+    const uint32_t base = (blockIdx.x * blockDim.x + threadIdx.x) & ~63u;
+    if(base >= n) return;
+    const uint32_t k = base + (threadIdx.x & 63u);
+    ShadeOut o;
+    o.shadow = false;
+    o.query = false;
+    o.view = 0;
+    uint32_t i = 0, sh = 0;
+    if(k < n)
     {
-        uint32_t base = 0;
-        if(lane == 0) base = atomicAdd(counter, 64u);
-        base = __builtin_amdgcn_readfirstlane(base);
-        if(base >= n) break;
-        const uint32_t k = base + lane;
-        ShadeOut o;
-        o.shadow = false;
-        o.query = false;
-        o.view = 0;
-        uint32_t i = 0, sh = 0;
-        if(k < n)
-        {
-            i = q[ShardItem(g, k, cap, sh)];
-            ShadeSlot<SPECTRAL, KEY, BATCH>(S, rc, i, o);
-        }
-        // the wave's first item names the shadow shard (a wave straddling two
-        // queue shards adds at most 63 items to it: ShadowShardCap's slack)
-        ShadeAppend(o, i, __builtin_amdgcn_readfirstlane(sh), scap, ctr, shadowQ, sched, rc.nrc,
-                    queries);
+        i = q[ShardItem(g, k, cap, sh)];
+        ShadeSlot<SPECTRAL, KEY, BATCH>(S, rc, i, o);
     }
+    // the wave's first item names the shadow shard (a wave straddling two
+    // queue shards adds at most 63 items to it: ShadowShardCap's slack)
+    ShadeAppend(o, i, __builtin_amdgcn_readfirstlane(sh), scap, ctr, shadowQ, sched, rc.nrc,
+                queries);
 }
 
 // ---------------------------------------------------------------------------
@@ -1983,11 +1978,9 @@ static void LaunchShadeQueues(MRayRenderer* r, hipStream_t s)
 template<bool SPECTRAL, bool BATCH, typename F>
 static void ShadeQueueOccupancy(MRayRenderer* r, F&& resident)
 {
-    r->shadeQBlocks[0] = uint32_t(resident(KShadeQ<SPECTRAL, 0, BATCH>, kSlotTPB, 1u << 20));
-    r->shadeQBlocks[1] = uint32_t(resident(KShadeQ<SPECTRAL, 1, BATCH>, kSlotTPB, 1u << 20));
-    r->shadeQBlocks[2] = uint32_t(resident(KShadeQ<SPECTRAL, 2, BATCH>, kSlotTPB, 1u << 20));
-    r->shadeQBlocks[3] = uint32_t(resident(KShadeQ<SPECTRAL, 3, BATCH>, kSlotTPB, 1u << 20));
-    r->shadeQBlocks[4] = uint32_t(resident(KShadeQ<SPECTRAL, 4, BATCH>, kSlotTPB, 1u << 20));
+    // a queue holds N items at most: one wave per 64-item chunk
+    for(uint32_t q = 0; q < kMatQueues; q++)
+        r->shadeQBlocks[q] = uint32_t((r->rc.N + kSlotTPB - 1u) / kSlotTPB);
 }
 
 // Closest-hit launch; a batch may use 128-thread traversal workgroups.
