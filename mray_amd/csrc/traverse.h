@@ -234,7 +234,9 @@ __device__ __forceinline__ uint2 StackGet(const TraceCtx& ctx, const uint2* lsta
 // nearest is left in next and the stack pops near-first.  While every lane
 // of the wave has room for W-1 more entries in LDS (one wave-uniform test)
 // the pushes are branch-free: each candidate writes the free slot and the
-// stack grows only on a real push (a write above the top is dead).
+// stack grows only on a real push (a write above the top is dead).  The
+// 8-wide walk and the two-level walk use the guarded pushes only (their
+// branch-free form spilled SGPRs, and measured slower, respectively).
 template<int W, bool BRANCH_FREE = true>
 __device__ __forceinline__ void PushFarFirst(const TraceCtx& ctx, uint2* lstack, uint32_t& sp,
                                              uint32_t spMax, uint32_t lb, uint32_t gtid, const float (&tn)[W],
@@ -476,7 +478,7 @@ __device__ __forceinline__ HitRec TraceW(const TraceCtx& ctx, uint2* lstack, uin
         }
         uint32_t next;
         float nextT;
-        PushFarFirst<W>(ctx, lstack, sp, spMax, lb, gtid, tn, c, cand, next, nextT);
+        PushFarFirst<W, W == 4>(ctx, lstack, sp, spMax, lb, gtid, tn, c, cand, next, nextT);
         // the node's leaves, nearest first: a wave runs as many triangle
         // tests as its lane with the most leaves here, not one per child
         // position that any lane has a leaf in
