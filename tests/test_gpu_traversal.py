@@ -149,3 +149,81 @@ def test_rays_in_box_planes(torch_gpu, width):
     neg[:, 5] = -0.0
     for r in (rays, neg):
         _compare(acc, o, r, brute=True)
+
+
+def _plane_stack(n_planes):
+    """n_planes parallel squares [-1, 1]^2 at z = k / n_planes, two triangles
+    each: every box of the tree spans the whole square, so a ray along +z
+    enters every child of every node it visits."""
+    pos, idx = [], []
+    for k in range(n_planes):
+        z = k / n_planes
+        b = len(pos)
+        pos += [[-1, -1, z], [1, -1, z], [1, 1, z], [-1, 1, z]]
+        idx += [[b, b + 1, b + 2], [b, b + 2, b + 3]]
+    pos = np.array(pos, np.float32)
+    idx = np.array(idx, np.uint32)
+    n = len(idx)
+    return scenes.Scene(positions=pos, tbn=np.tile(np.array([[1, 0, 0, 0]], np.float32), (len(pos), 1)),
+                        indices=idx, surface_keys=np.zeros(n, np.uint32), cull=np.zeros(n, np.uint8),
+                        materials=[{"type": abi.MAT_LAMBERT, "albedo": (1, 1, 1)}],
+                        light_tris=np.zeros(0, np.uint32), light_radiance=np.zeros((0, 3), np.float32),
+                        light_two_sided=np.zeros(0, np.uint8),
+                        camera={"position": (0, 0, -2), "gaze": (0, 0, 0), "up": (0, 1, 0),
+                                "fov": (1.0, 1.0), "nearFar": (0.1, 10.0)})
+
+
+def _first_descent_depth(nodes, x, y):
+    """Stack depth of a +z ray at (x, y) when its near-first walk of the
+    4-wide tree reaches its first leaf: every node pushes its hit inner
+    children but the nearest (words 0-23: lo/hi x, y, z of the 4 children,
+    24-27: child words, leaf flag 0x80000000)."""
+    node, depth = 0, 0
+    while True:
+        w = nodes[node]
+        inner = []
+        for k in range(4):
+            lx, hx = w[0 + k].view(np.float32), w[4 + k].view(np.float32)
+            ly, hy = w[8 + k].view(np.float32), w[12 + k].view(np.float32)
+            lz = w[16 + k].view(np.float32)
+            c = int(w[24 + k])
+            if not (lx <= x <= hx and ly <= y <= hy) or (c & 0x80000000):
+                continue
+            inner.append((float(lz), c))
+        if not inner:
+            return depth
+        inner.sort()
+        depth += len(inner) - 1
+        node = inner[0][1]
+
+
+@pytest.mark.parametrize("builder", ["sah", "lbvh"])
+def test_deep_stack_spills_past_lds(torch_gpu, builder):
+    """A walk whose stack outgrows the 12 LDS levels: the deeper entries go
+    to the lane's global spill area, and waves mix lanes on either side of
+    the boundary.  Hits, t, barycentrics and visibility equal brute force."""
+    from mray_amd import native
+    s = _plane_stack(4096)
+    acc = native.Accel(s, builder=builder, width=4)
+    nodes, _ = acc.read_wide()
+    depth = _first_descent_depth(nodes, np.float32(0.1), np.float32(-0.2))
+    print(f"{builder}: {len(nodes)} nodes, first-descent stack depth {depth}", flush=True)
+    assert depth > 12
+    rng = np.random.default_rng(3)
+    n = 4096
+    rays = np.zeros((n, 8), np.float32)
+    # +z rays from below (deep stacks), oblique rays and rays from between
+    # the planes (shallow stacks) interleaved within every wave
+    rays[:, 0:2] = rng.uniform(-0.9, 0.9, size=(n, 2))
+    rays[:, 2] = -1.0
+    rays[:, 6] = 1.0
+    odd = np.arange(n) % 3 == 1
+    d = rng.normal(size=(int(odd.sum()), 3))
+    d[:, 2] = np.abs(d[:, 2]) + 0.2
+    rays[odd, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    mid = np.arange(n) % 3 == 2
+    rays[mid, 2] = rng.uniform(0.0, 1.0, size=int(mid.sum()))
+    rays[mid, 6] = -1.0
+    rays[:, 3] = 1e-4
+    rays[:, 7] = 3.4e38
+    _compare(acc, refcpu.OracleScene(s), rays, brute=True)
