@@ -224,3 +224,37 @@ def test_instanced_batch_errors(torch_gpu):
     isc.close()
     for a in accs:
         a.close()
+
+
+def test_identity_instance_deep_stack(torch_gpu):
+    """The two-level walk with stacks past the 12 LDS levels: one identity
+    instance of 4096 parallel planes seen edge-on to their stack (camera on
+    the stack's axis, constant sky).  Its walks hold the instance tree's
+    entries, the return mark and 18 or more mesh-tree entries, so they
+    spill to the global area; the image and counts equal the one-level
+    renderer's on the same mesh."""
+    from mray_amd import native
+    from tests.test_gpu_traversal import _plane_stack
+    sc = _plane_stack(4096)
+    sc.boundary_type = abi.BOUNDARY_SKYSPHERE_COOCTA
+    sc.boundary_radiance = (0.5, 0.5, 0.5)
+    env = {"boundary_type": sc.boundary_type, "boundary_radiance": sc.boundary_radiance,
+           "scene_diameter": sc.diameter, "camera": sc.camera}
+    ident = [{"mesh": 0, "rotation": (1.0, 0.0, 0.0, 0.0), "scale": 1.0,
+              "translation": (0.0, 0.0, 0.0)}]
+    params = abi.render_params(48, 32, 2, rr=(3, 6), seed=4, max_ray_count=1536)
+    film, st, info = _gpu_instanced([sc], ident, env, params)
+    acc = native.Accel(sc)
+    r = native.Renderer(acc, params)
+    fst = r.render_all()
+    ffilm = r.film()
+    r.close()
+    acc.close()
+    print(f"deep stack: instanced {st} one level {fst}", flush=True)
+    assert st["dead_paths"] == fst["dead_paths"] == 48 * 32 * 2
+    assert st["ext_rays"] == fst["ext_rays"] and st["shadow_rays"] == fst["shadow_rays"]
+    assert film[:3].any()
+    assert np.array_equal(film[3], ffilm[3])
+    e = relmse(film_image(film), film_image(ffilm))
+    print(f"deep stack identity instance vs one level: relMSE={e:.3e}", flush=True)
+    assert e <= 1e-8
