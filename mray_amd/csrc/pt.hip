@@ -1585,8 +1585,10 @@ __device__ __forceinline__ void ShadeAppend(const ShadeOut& o, uint32_t i, uint3
 // grid covers N slots and has no loop: a grid-stride loop made the compiler
 // hold the scene's many pointers in SGPRs for the loop's length, and spill
 // them (74 SGPRs) and 40 more VGPRs (150 -> 110, 3 -> 4 waves per SIMD).
+// The one-scene kernels run at 5 waves per SIMD (96 VGPRs, a few spilled in
+// the spectral one); the batch kernels would spill more and stay at 4.
 template<bool SPECTRAL, bool BATCH, bool INST = false>
-__global__ void __launch_bounds__(kSlotTPB)
+__global__ void __launch_bounds__(kSlotTPB) __attribute__((amdgpu_waves_per_eu(BATCH ? 4 : 5, 8)))
 KShade(SlotSoA S, RenderConst rc, const uint32_t* __restrict__ active,
        Sched* __restrict__ sched, uint32_t* __restrict__ shadowQ, uint32_t* __restrict__ ctr)
 {
