@@ -2,7 +2,7 @@
 // with fp32 boxes (BNode4) or compressed 8-wide nodes (CNode8).
 //
 // One lane per ray.  The traversal stack holds (node, tNear) pairs: its first
-// kLdsStack entries live in LDS (thread-strided, conflict-free), deeper ones
+// kLdsStack entries live in LDS (rows of 64 per wave, conflict-free), deeper ones
 // spill to a global per-lane area, so any tree depth is handled without
 // scratch.  A popped entry whose tNear is beyond the closest hit found so far
 // is dropped without fetching the node.
